@@ -1,0 +1,160 @@
+/*
+ * ppm_api.h — C ABI of the MI355X (gfx950) probe-particle AFM library
+ *             (libppm_hip.so, built from spammm_amd/csrc/).
+ *
+ * Drop-in boundary for the probe-particle-model AFM path of SPAMMM.  Every
+ * entry point below replaces one OpenCL kernel launch (or host NumPy stage) of
+ * the reference; the replaced interface is cited on each declaration.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer (HIP global memory) owned by the
+ *     caller; the library allocates nothing on these paths and keeps no state
+ *     between calls apart from compiled code.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Calls are
+ *     stream-ordered and return after the launch; they never synchronize the
+ *     device.  Calls on different streams may run concurrently.
+ *   - Return value: PPM_OK (0) or a PPM_ERR_* code; ppm_last_error() returns a
+ *     thread-local message describing the last failure on the calling thread.
+ *   - float4 arrays are packed as 4 consecutive floats (x,y,z,w), 16-byte aligned.
+ *   - Grid layout: (nz, ny, nx, 4) float, x fastest — the texel order of the
+ *     reference's RGBA32F image3d (kernels/AFM.cl:989, write_imagef at (ia,ib,ic)).
+ *   - Scan layout: point index k = ix*ny + iy (ix-major, spammm/SPM/AFM.py:977-980);
+ *     per-(point, z) outputs at index k*nz + iz, iz=0 = first (highest) tip position.
+ */
+#ifndef PPM_API_H
+#define PPM_API_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PPM_API_VERSION 1
+
+/* status codes */
+#define PPM_OK                 0
+#define PPM_ERR_INVALID_ARG    1
+#define PPM_ERR_HIP            2
+#define PPM_ERR_UNSUPPORTED    3
+
+/* pair models */
+#define PPM_MODEL_LJ     0   /* cpar row = (c6, c12)            getLJ     kernels/AFM.cl:738-745 */
+#define PPM_MODEL_MORSE  1   /* cpar row = (R0, E0, K<0, unused) getMorse kernels/AFM.cl:747-757 */
+
+/* Grid geometry.  Voxel (ia,ib,ic) sits at p0 + dA*ia + dB*ib + dC*ic
+ * (kernels/AFM.cl:959).  The normalized sampling coordinate of a world point pos
+ * is s_a = dot((pos,1), dinvA) etc. (interpFE, kernels/AFM.cl:100-105); the .w
+ * entries carry -origin/L + 0.5/n (spammm/SPM/AFM.py:275-294). */
+typedef struct PpmGridSpec {
+    int   nx, ny, nz, reserved;
+    float p0[4], dA[4], dB[4], dC[4];
+    float dinvA[4], dinvB[4], dinvC[4];
+} PpmGridSpec;
+
+/* Tip / probe parameter record (AFMulator.DEFAULT_*, spammm/SPM/AFM.py:296-304). */
+typedef struct PpmTip {
+    float tipA[4], tipB[4], tipC[4];  /* tip frame rows; tipC.w = dtip (z step per slice) */
+    float stiffness[4];               /* (k_x, k_y, k_z, k_rad), negative = restoring [eV/A^2] */
+    float dpos0[4];                   /* probe rest offset (x,y,z) in tip frame, .w = bond length L */
+    float relax[4];                   /* (dt, damp, unused, unused) */
+    float surfFF[4];                  /* .x = constant force along tipC */
+} PpmTip;
+
+/* Relaxation control (compile-time N_RELAX_STEP_MAX / F2CONV of kernels/AFM.cl:121-122
+ * made runtime). */
+typedef struct PpmRelaxCtl {
+    int   max_iter;       /* FIRE iterations per z slice (reference: 128) */
+    float f2conv;         /* stop when |F|^2 < f2conv (reference: 1e-8) */
+    int   check_before;   /* 0: test after the update (relaxStrokesTilted, kernels/AFM.cl:544);
+                             1: test before it (relaxStrokesTiltedMorseDirect, kernels/AFM.cl:1478) */
+    int   scan_ny;        /* >0: points form an ix-major raster with this many iy per ix
+                             (enables 8x8 pixel tiles per wavefront); 0: no raster */
+} PpmRelaxCtl;
+
+/* Library version (PPM_API_VERSION). */
+int ppm_version(void);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* ppm_last_error(void);
+
+/* Force-field grid projection.
+ * Replaces evalLJC_QZs_toImg (kernels/AFM.cl:916-991) and evalMorseC_QZs_toImg
+ * (kernels/AFM.cl:996-1056) as launched by AFMulator.make_forcefield
+ * (spammm/SPM/AFM.py:897-946), extended with periodic x,y images in the
+ * convention of make_MorseFF (kernels/gridFF.cl:938-996).
+ *   atoms    (na,4)  x,y,z,q                     [A, e]
+ *   cpar     (na,2) for PPM_MODEL_LJ, (na,4) for PPM_MODEL_MORSE
+ *   Qs, QZs  host float[4]: tip charges [e] and their z offsets [A]
+ *            (COULOMB_CONST is applied inside, kernels/AFM.cl:961)
+ *   npbc_a, npbc_b  image range [-n,n] along lvA, lvB (host float[3]); 0 = no images
+ *   cap      nonzero: rescale all four channels by 100/|E| when |E|>100
+ *            (kernels/AFM.cl:984-985)
+ *   grid     (nz,ny,nx,4) output. */
+int ppm_grid_project(int model, const float* atoms, const float* cpar, int na,
+                     const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
+                     int npbc_a, int npbc_b, const float lvA[3], const float lvB[3],
+                     int cap, float* grid, void* stream);
+
+/* Relaxed scan with a tilted tip frame.
+ * Replaces relaxStrokesTilted (kernels/AFM.cl:478-562) as launched by
+ * AFMulator.run_scan (spammm/SPM/AFM.py:950-1010).
+ *   points (n_scan,4) tip start positions; FEs (n_scan,nz,4) tip-rotated sample FE.
+ *   PPs    (n_scan,nz,4) or NULL: probe position and +iterations (converged) /
+ *          -iterations (not converged or non-finite), the telemetry convention of
+ *          relaxStrokesTiltedMorseDirect (kernels/AFM.cl:1494-1505). */
+int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec,
+                             const float* points, int n_scan, int nz,
+                             const PpmTip* tip, const PpmRelaxCtl* ctl,
+                             float* FEs, float* PPs, void* stream);
+
+/* Relaxed scan, untilted tip, with displacement output.
+ * Replaces relaxStrokes (kernels/AFM.cl:345-394) as launched by
+ * AFMulator.scan_fdbm (spammm/SPM/AFM.py:493-642).
+ *   dTip, stiffness, dpos0, relax: host float[4].
+ *   FEs (n_scan,nz,4): sample taken at the start of the last FIRE iteration;
+ *   disps (n_scan,nz,4) or NULL: pos - (tip + dpos0); PPs as above or NULL. */
+int ppm_relax_strokes(const float* grid, const PpmGridSpec* spec,
+                      const float* points, int n_scan, int nz,
+                      const float dTip[4], const float stiffness[4], const float dpos0[4],
+                      const float relax[4], const PpmRelaxCtl* ctl,
+                      float* FEs, float* disps, float* PPs, void* stream);
+
+/* Unrelaxed sampling along strokes.
+ * Replaces getFEinStrokesTilted (kernels/AFM.cl:210-233) as launched by
+ * AFMulator.get_raw_FE (spammm/SPM/AFM.py:2523-2580).  tip supplies tipA/B/C;
+ * dTip and dpos0 are host float[4]. */
+int ppm_fe_in_strokes_tilted(const float* grid, const PpmGridSpec* spec,
+                             const float* points, int n_scan, int nz,
+                             const PpmTip* tip, const float dTip[4], const float dpos0[4],
+                             float* FEs, void* stream);
+
+/* Linear operator along z, one output per (pixel, z_out):
+ *   out[p*nz_out + j] = sum_{t<taps} W[j*taps + t] * in[(p*nz_in + start[j] + t)*in_stride + in_offset]
+ * Replaces the host NumPy Fz -> df conversions compute_df / compute_df_amp /
+ * get_df (spammm/SPM/AFM.py:3048-3089, :2591-2599); the host builds W/start for
+ * the chosen definition (finite difference, finite-amplitude mean, Giessibl).
+ *   W (nz_out*taps) float, start (nz_out) int: device arrays; indices start[j]+t are
+ *   clamped into [0, nz_in-1]. */
+int ppm_z_stencil(const float* in, int in_stride, int in_offset, int n_pix, int nz_in,
+                  const float* W, const int* start, int taps, int nz_out,
+                  float* out, void* stream);
+
+/* General banded linear operator along one axis of a strided float volume:
+ *   out[(o*n_axis_out + j)*n_inner + i] = alpha * sum_{t<taps} W[j*taps+t] * v(o, start[j]+t, i)
+ *                                         (+ previous out value if accumulate != 0)
+ *   v(o, m, i) = sum_{c<nch} chw[c] * in[o*in_outer_stride + m*in_axis_stride + i*in_inner_stride + c]
+ * (strides in floats).  Used for the directional derivatives of compute_df_dir /
+ * compute_df_amp_dir (spammm/SPM/AFM.py:3105-3123, :3203-3236), which project F onto
+ * the oscillation direction (chw = n) and differentiate along x, y or z, and for the
+ * transpose of the df operator in df_loss_seed (spammm/SPM/AFM.py:3125-3200).
+ * Indices start[j]+t are clamped into [0, n_axis_in-1]. */
+int ppm_axis_operator(const float* in, long long in_outer_stride, long long in_axis_stride,
+                      long long in_inner_stride, const float chw[4], int nch,
+                      int n_outer, int n_axis_in, int n_inner,
+                      const float* W, const int* start, int taps, int n_axis_out,
+                      float alpha, int accumulate, float* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PPM_API_H */

@@ -1,0 +1,236 @@
+"""CPU oracle for the probe-particle AFM path — TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
+leg import this module, and only as the checker (or, for ``cpu_baseline``, as
+the timed CPU restatement).  The product package ``spammm_amd`` never imports
+it; the product path fails loudly when its HIP library is missing.
+
+ctypes wrapper over ``oracle/_build/libppm_oracle.so`` (built from
+``oracle/ppm_oracle.cpp`` by ``oracle/Makefile``).  Every function restates the
+reference (file:line cited in ``ppm_oracle.cpp`` and below).
+
+Parity pins (what checks each oracle function against something other than its
+own formula; the tests live in ``tests/test_oracle_*.py``):
+
+* ``pair_fe`` (getLJ / getMorse / getCoulombAFM, kernels/AFM.cl:730-757):
+  F = -grad E by central finite difference; single-atom closed forms (LJ and
+  Morse minimum at r=R0 with E=-E0, Coulomb q/r); translation, permutation and
+  charge-linearity invariants (tests/SPM/test_afm_morse.py:694-857);
+  periodic images checked against explicitly replicated atom lists.
+* ``grid_project`` (evalLJC_QZs_toImg / evalMorseC_QZs_toImg,
+  kernels/AFM.cl:916-1056): equals ``pair_fe`` at the voxel positions
+  (p0 + dA*ia + dB*ib + dC*ic) followed by the |E|>100 rescale; voxel layout
+  x-fastest; FP32 vs FP64 agreement.
+* ``interp`` (sampler_1 trilinear REPEAT, kernels/AFM.cl:49,100-105): exact
+  on affine fields, reproduces voxel values at texel centres
+  (tests/SPM/test_texel_centers.py:17-94), REPEAT wrap on every axis.  The
+  reference's hardware filter weight precision is not pinned by any reference
+  test: parity unpinned for that property only.
+* ``relax_tilted`` / ``relax_strokes`` (kernels/AFM.cl:478-562, :345-394):
+  equilibrium residual |F_total(PP)| < 1e-4 eV/A at every converged state
+  (tests/SPM/test_afm_morse.py:907-941); harmonic-field closed form (linear
+  force field => analytic equilibrium); on-axis symmetric pixel => zero
+  lateral displacement; step-trace reproducibility; FP32 and FP64 modes agree
+  at tight convergence.
+* ``fe_in_strokes_tilted`` (kernels/AFM.cl:210-233): equals ``interp`` at the
+  stroke positions, rotated into the tip frame.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libppm_oracle.so")
+_lib = None
+
+_f = ctypes.POINTER(ctypes.c_float)
+_d = ctypes.POINTER(ctypes.c_double)
+_i = ctypes.POINTER(ctypes.c_int)
+_l = ctypes.POINTER(ctypes.c_long)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle library with its Makefile (g++, OpenMP)."""
+    if force and os.path.exists(_LIB_PATH):
+        os.remove(_LIB_PATH)
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c_int, c_float = ctypes.c_int, ctypes.c_float
+        L.oracle_pair_fe_f64.argtypes = [c_int, _f, _f, c_int, c_int, _f, _f, c_int, c_int, _f, _f, _d, c_int, _d]
+        L.oracle_pair_fe_f64d.argtypes = [c_int, _d, _d, c_int, c_int, _d, _d, c_int, c_int, _d, _d, _d, c_int, _d]
+        for nm in ("oracle_grid_project_f64", "oracle_grid_project_f32"):
+            getattr(L, nm).argtypes = [c_int, _f, _f, c_int, c_int, _f, _f, c_int, c_int, _f, _f, _i,
+                                       _f, _f, _f, _f, c_int, _f]
+        L.oracle_grid_voxels_f64.argtypes = [c_int, _f, _f, c_int, c_int, _f, _f, c_int, c_int, _f, _f, _i,
+                                             _f, _f, _f, _f, c_int, _l, c_int, _d]
+        L.oracle_interp_f64.argtypes = [_f, _i, _f, _f, _f, _d, c_int, _d]
+        for nm in ("oracle_relax_tilted_f64", "oracle_relax_tilted_f32"):
+            getattr(L, nm).argtypes = [_f, _i, _f, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, _f, _f, _f,
+                                       c_int, c_float, c_int, _f, _f]
+        for nm in ("oracle_relax_strokes_f64", "oracle_relax_strokes_f32"):
+            getattr(L, nm).argtypes = [_f, _i, _f, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, c_int, c_float,
+                                       _f, _f, _f]
+        for nm in ("oracle_fe_in_strokes_tilted_f64", "oracle_fe_in_strokes_tilted_f32"):
+            getattr(L, nm).argtypes = [_f, _i, _f, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, _f, _f]
+        _lib = L
+    return _lib
+
+
+def _cf(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    if n is not None:
+        a = np.ascontiguousarray(np.resize(a.ravel(), n).astype(np.float32)) if a.size != n else a.ravel()
+    return a
+
+
+def _cd(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _vec(a, n):
+    out = np.zeros(n, dtype=np.float32)
+    a = np.asarray(a, dtype=np.float32).ravel()
+    out[: min(n, a.size)] = a[: min(n, a.size)]
+    return out
+
+
+def pair_fe(model, atoms, cpar, queries, Qs, QZs, npbc=(0, 0), lvA=(0, 0, 0), lvB=(0, 0, 0)):
+    """Direct sample FE (Fx,Fy,Fz,E) at points, FP64, uncapped.  model 0=LJ, 1=Morse."""
+    atoms = _cf(atoms)
+    cpar = _cf(cpar)
+    na = atoms.shape[0]
+    cs = cpar.shape[1] if cpar.ndim == 2 else 0
+    q = _cd(queries).reshape(-1, 3)
+    out = np.zeros((q.shape[0], 4), dtype=np.float64)
+    Qs, QZs, lvA, lvB = _vec(Qs, 4), _vec(QZs, 4), _vec(lvA, 3), _vec(lvB, 3)
+    lib().oracle_pair_fe_f64(int(model), _p(atoms, _f), _p(cpar, _f), cs, na, _p(Qs, _f), _p(QZs, _f),
+                             int(npbc[0]), int(npbc[1]), _p(lvA, _f), _p(lvB, _f), _p(q, _d), q.shape[0],
+                             _p(out, _d))
+    return out
+
+
+def pair_fe_d(model, atoms, cpar, queries, Qs, QZs, npbc=(0, 0), lvA=(0, 0, 0), lvB=(0, 0, 0)):
+    """As ``pair_fe`` with double-precision atoms/params (for finite differences)."""
+    atoms = _cd(atoms)
+    cpar = _cd(cpar)
+    na = atoms.shape[0]
+    cs = cpar.shape[1]
+    q = _cd(queries).reshape(-1, 3)
+    out = np.zeros((q.shape[0], 4), dtype=np.float64)
+    Qs = _cd(np.resize(np.asarray(Qs, float), 4))
+    QZs = _cd(np.resize(np.asarray(QZs, float), 4))
+    lvA = _cd(np.asarray(lvA, float).reshape(3))
+    lvB = _cd(np.asarray(lvB, float).reshape(3))
+    lib().oracle_pair_fe_f64d(int(model), _p(atoms, _d), _p(cpar, _d), cs, na, _p(Qs, _d), _p(QZs, _d),
+                              int(npbc[0]), int(npbc[1]), _p(lvA, _d), _p(lvB, _d), _p(q, _d), q.shape[0],
+                              _p(out, _d))
+    return out
+
+
+def grid_project(model, atoms, cpar, n, p0, dA, dB, dC, Qs, QZs, npbc=(0, 0), lvA=(0, 0, 0), lvB=(0, 0, 0),
+                 cap=True, precision="f64"):
+    """Full grid (nz,ny,nx,4) float32 output, computed in ``precision``."""
+    atoms = _cf(atoms)
+    cpar = _cf(cpar)
+    cs = cpar.shape[1]
+    nn = np.asarray(n, dtype=np.int32).ravel()[:3].copy()
+    out = np.zeros((int(nn[2]), int(nn[1]), int(nn[0]), 4), dtype=np.float32)
+    args = [_vec(x, 3) for x in (p0, dA, dB, dC)]
+    Qs, QZs, lvA, lvB = _vec(Qs, 4), _vec(QZs, 4), _vec(lvA, 3), _vec(lvB, 3)
+    fn = lib().oracle_grid_project_f64 if precision == "f64" else lib().oracle_grid_project_f32
+    fn(int(model), _p(atoms, _f), _p(cpar, _f), cs, atoms.shape[0], _p(Qs, _f), _p(QZs, _f), int(npbc[0]),
+       int(npbc[1]), _p(lvA, _f), _p(lvB, _f), _p(nn, _i), *[_p(a, _f) for a in args], int(bool(cap)),
+       _p(out, _f))
+    return out
+
+
+def grid_voxels(model, atoms, cpar, n, p0, dA, dB, dC, Qs, QZs, idx, npbc=(0, 0), lvA=(0, 0, 0),
+                lvB=(0, 0, 0), cap=True):
+    """FP64 values (k,4) of the grid at flat voxel indices ``idx``."""
+    atoms = _cf(atoms)
+    cpar = _cf(cpar)
+    cs = cpar.shape[1]
+    nn = np.asarray(n, dtype=np.int32).ravel()[:3].copy()
+    idx = np.ascontiguousarray(np.asarray(idx, dtype=np.int64))
+    out = np.zeros((idx.size, 4), dtype=np.float64)
+    args = [_vec(x, 3) for x in (p0, dA, dB, dC)]
+    Qs, QZs, lvA, lvB = _vec(Qs, 4), _vec(QZs, 4), _vec(lvA, 3), _vec(lvB, 3)
+    lib().oracle_grid_voxels_f64(int(model), _p(atoms, _f), _p(cpar, _f), cs, atoms.shape[0], _p(Qs, _f),
+                                 _p(QZs, _f), int(npbc[0]), int(npbc[1]), _p(lvA, _f), _p(lvB, _f), _p(nn, _i),
+                                 *[_p(a, _f) for a in args], int(bool(cap)), _p(idx, _l), idx.size, _p(out, _d))
+    return out
+
+
+def _grid_args(grid, n, dinvA, dinvB, dinvC):
+    grid = _cf(grid)
+    nn = np.asarray(n, dtype=np.int32).ravel()[:3].copy()
+    assert grid.size == 4 * int(np.prod(nn.astype(np.int64))), "grid size does not match n"
+    return grid, nn, _vec(dinvA, 4), _vec(dinvB, 4), _vec(dinvC, 4)
+
+
+def interp(grid, n, dinvA, dinvB, dinvC, pos):
+    g, nn, a, b, c = _grid_args(grid, n, dinvA, dinvB, dinvC)
+    pos = _cd(pos).reshape(-1, 3)
+    out = np.zeros((pos.shape[0], 4), dtype=np.float64)
+    lib().oracle_interp_f64(_p(g, _f), _p(nn, _i), _p(a, _f), _p(b, _f), _p(c, _f), _p(pos, _d), pos.shape[0],
+                            _p(out, _d))
+    return out
+
+
+def relax_tilted(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF,
+                 max_iter=128, f2conv=1e-8, check_before=False, precision="f64"):
+    """Returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4)) float32."""
+    g, nn, a, b, c = _grid_args(grid, n, dinvA, dinvB, dinvC)
+    pts = _cf(pts).reshape(-1, 4)
+    ns = pts.shape[0]
+    FEs = np.zeros((ns, nz, 4), dtype=np.float32)
+    PPs = np.zeros((ns, nz, 4), dtype=np.float32)
+    vs = [_vec(x, 4) for x in (tipA, tipB, tipC, stiffness, dpos0, relax, surfFF)]
+    fn = lib().oracle_relax_tilted_f64 if precision == "f64" else lib().oracle_relax_tilted_f32
+    fn(_p(g, _f), _p(nn, _i), _p(a, _f), _p(b, _f), _p(c, _f), _p(pts, _f), ns, int(nz),
+       *[_p(v, _f) for v in vs], int(max_iter), float(f2conv), int(bool(check_before)), _p(FEs, _f), _p(PPs, _f))
+    return FEs, PPs
+
+
+def relax_strokes(grid, n, dinvA, dinvB, dinvC, pts, nz, dTip, stiffness, dpos0, relax, max_iter=128,
+                  f2conv=1e-8, precision="f64"):
+    """Returns (FEs, disps, PPs), each (n_scan,nz,4) float32."""
+    g, nn, a, b, c = _grid_args(grid, n, dinvA, dinvB, dinvC)
+    pts = _cf(pts).reshape(-1, 4)
+    ns = pts.shape[0]
+    FEs = np.zeros((ns, nz, 4), dtype=np.float32)
+    disps = np.zeros((ns, nz, 4), dtype=np.float32)
+    PPs = np.zeros((ns, nz, 4), dtype=np.float32)
+    vs = [_vec(x, 4) for x in (dTip, stiffness, dpos0, relax)]
+    fn = lib().oracle_relax_strokes_f64 if precision == "f64" else lib().oracle_relax_strokes_f32
+    fn(_p(g, _f), _p(nn, _i), _p(a, _f), _p(b, _f), _p(c, _f), _p(pts, _f), ns, int(nz), *[_p(v, _f) for v in vs],
+       int(max_iter), float(f2conv), _p(FEs, _f), _p(disps, _f), _p(PPs, _f))
+    return FEs, disps, PPs
+
+
+def fe_in_strokes_tilted(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC, dTip, dpos0, precision="f64"):
+    g, nn, a, b, c = _grid_args(grid, n, dinvA, dinvB, dinvC)
+    pts = _cf(pts).reshape(-1, 4)
+    ns = pts.shape[0]
+    FEs = np.zeros((ns, nz, 4), dtype=np.float32)
+    vs = [_vec(x, 4) for x in (tipA, tipB, tipC, dTip, dpos0)]
+    fn = lib().oracle_fe_in_strokes_tilted_f64 if precision == "f64" else lib().oracle_fe_in_strokes_tilted_f32
+    fn(_p(g, _f), _p(nn, _i), _p(a, _f), _p(b, _f), _p(c, _f), _p(pts, _f), ns, int(nz), *[_p(v, _f) for v in vs],
+       _p(FEs, _f))
+    return FEs

@@ -1,0 +1,95 @@
+"""ctypes binding of libppm_hip.so (the C ABI declared in include/ppm_api.h).
+
+The library is built in-tree by ``python -m spammm_amd.build``.  There is no
+fallback: if the library or a GPU is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libppm_hip.so")
+
+PPM_OK = 0
+PPM_ERR_INVALID_ARG = 1
+PPM_ERR_HIP = 2
+PPM_ERR_UNSUPPORTED = 3
+PPM_MODEL_LJ = 0
+PPM_MODEL_MORSE = 1
+
+_F4 = ctypes.c_float * 4
+
+
+class PpmGridSpec(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("p0", _F4), ("dA", _F4), ("dB", _F4), ("dC", _F4),
+                ("dinvA", _F4), ("dinvB", _F4), ("dinvC", _F4)]
+
+
+class PpmTip(ctypes.Structure):
+    _fields_ = [("tipA", _F4), ("tipB", _F4), ("tipC", _F4), ("stiffness", _F4), ("dpos0", _F4),
+                ("relax", _F4), ("surfFF", _F4)]
+
+
+class PpmRelaxCtl(ctypes.Structure):
+    _fields_ = [("max_iter", ctypes.c_int), ("f2conv", ctypes.c_float), ("check_before", ctypes.c_int),
+                ("scan_ny", ctypes.c_int)]
+
+
+EXPORTS = (
+    "ppm_version", "ppm_last_error", "ppm_grid_project", "ppm_relax_strokes_tilted", "ppm_relax_strokes",
+    "ppm_fe_in_strokes_tilted", "ppm_z_stencil", "ppm_axis_operator",
+)
+
+_lib = None
+
+
+def f4(v) -> _F4:
+    a = [float(x) for x in list(v)[:4]]
+    a += [0.0] * (4 - len(a))
+    return _F4(*a)
+
+
+def load():
+    """Load libppm_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m spammm_amd.build` "
+                           "(the AFM path has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+    P = ctypes.POINTER
+    L.ppm_version.restype = i
+    L.ppm_version.argtypes = []
+    L.ppm_last_error.restype = ctypes.c_char_p
+    L.ppm_last_error.argtypes = []
+    L.ppm_grid_project.restype = i
+    L.ppm_grid_project.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
+                                   P(ctypes.c_float * 3), i, vp, vp]
+    L.ppm_relax_strokes_tilted.restype = i
+    L.ppm_relax_strokes_tilted.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), P(PpmRelaxCtl), vp, vp, vp]
+    L.ppm_relax_strokes.restype = i
+    L.ppm_relax_strokes.argtypes = [vp, P(PpmGridSpec), vp, i, i, _F4, _F4, _F4, _F4, P(PpmRelaxCtl), vp, vp, vp,
+                                    vp]
+    L.ppm_fe_in_strokes_tilted.restype = i
+    L.ppm_fe_in_strokes_tilted.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), _F4, _F4, vp, vp]
+    L.ppm_z_stencil.restype = i
+    L.ppm_z_stencil.argtypes = [vp, i, i, i, i, vp, vp, i, i, vp, vp]
+    ll = ctypes.c_longlong
+    L.ppm_axis_operator.restype = i
+    L.ppm_axis_operator.argtypes = [vp, ll, ll, ll, _F4, i, i, i, i, vp, vp, i, i, f, i, vp, vp]
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    """Map a PPM status code to the exception type the reference raises for that failure."""
+    if rc == PPM_OK:
+        return
+    msg = load().ppm_last_error().decode(errors="replace")
+    if rc == PPM_ERR_INVALID_ARG:
+        raise ValueError(msg)
+    raise RuntimeError(msg)

@@ -1,0 +1,180 @@
+// ppm_api.hip — extern "C" entry points of libppm_hip.so (declared in include/ppm_api.h).
+//
+// Each entry point validates its arguments on the host, then enqueues the kernels on
+// the caller's stream.  Nothing is allocated and nothing is synchronized.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "ppm_common.hpp"
+
+namespace ppm {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int check_launch(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("%s: kernel launch failed: %s", what, hipGetErrorString(e));
+        return PPM_ERR_HIP;
+    }
+    return PPM_OK;
+}
+
+static bool finite4(const float* v, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(v[i])) return false;
+    return true;
+}
+
+static int check_spec(const char* fn, const PpmGridSpec* s) {
+    if (!s) { set_error("%s: spec is NULL", fn); return PPM_ERR_INVALID_ARG; }
+    if (s->nx < 1 || s->ny < 1 || s->nz < 1) {
+        set_error("%s: invalid grid dims (%d,%d,%d)", fn, s->nx, s->ny, s->nz);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if ((long long)s->nx * s->ny > (1LL << 30) || (long long)s->nx * s->ny * s->nz > (1LL << 34)) {
+        set_error("%s: grid (%d,%d,%d) too large", fn, s->nx, s->ny, s->nz);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (!finite4(s->dinvA, 4) || !finite4(s->dinvB, 4) || !finite4(s->dinvC, 4) || !finite4(s->p0, 3) ||
+        !finite4(s->dA, 3) || !finite4(s->dB, 3) || !finite4(s->dC, 3)) {
+        set_error("%s: non-finite grid geometry", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return PPM_OK;
+}
+
+static int check_scan(const char* fn, const float* pts, int n_scan, int nz, const float* FEs) {
+    if (n_scan < 1 || nz < 1) { set_error("%s: invalid n_scan=%d nz=%d", fn, n_scan, nz); return PPM_ERR_INVALID_ARG; }
+    if ((long long)n_scan * nz > (1LL << 31)) { set_error("%s: n_scan*nz too large", fn); return PPM_ERR_INVALID_ARG; }
+    if (!pts || !FEs) { set_error("%s: NULL points/FEs", fn); return PPM_ERR_INVALID_ARG; }
+    return PPM_OK;
+}
+
+static int check_ctl(const char* fn, const PpmRelaxCtl* c, int n_scan) {
+    if (!c) { set_error("%s: ctl is NULL", fn); return PPM_ERR_INVALID_ARG; }
+    if (c->max_iter < 1 || c->max_iter > (1 << 20)) {
+        set_error("%s: max_iter=%d out of range [1, 2^20]", fn, c->max_iter);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (!std::isfinite(c->f2conv) || c->f2conv < 0.f) { set_error("%s: invalid f2conv", fn); return PPM_ERR_INVALID_ARG; }
+    if (c->scan_ny < 0 || (c->scan_ny > 0 && n_scan % c->scan_ny != 0)) {
+        set_error("%s: scan_ny=%d does not divide n_scan=%d", fn, c->scan_ny, n_scan);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return PPM_OK;
+}
+
+}  // namespace ppm
+
+using namespace ppm;
+
+extern "C" {
+
+int ppm_version(void) { return PPM_API_VERSION; }
+
+const char* ppm_last_error(void) { return g_err; }
+
+int ppm_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
+                     const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                     const float lvB[3], int cap, float* grid, void* stream) {
+    const char* fn = "ppm_grid_project";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (model != PPM_MODEL_LJ && model != PPM_MODEL_MORSE) { set_error("%s: unknown model %d", fn, model); return PPM_ERR_INVALID_ARG; }
+    if (na < 1 || na > (1 << 20)) { set_error("%s: invalid atom count %d", fn, na); return PPM_ERR_INVALID_ARG; }
+    if (!atoms || !cpar || !grid || !Qs || !QZs) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    if (npbc_a < 0 || npbc_b < 0 || npbc_a > 64 || npbc_b > 64) {
+        set_error("%s: image range (%d,%d) out of [0,64]", fn, npbc_a, npbc_b);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if ((npbc_a > 0 && !lvA) || (npbc_b > 0 && !lvB)) { set_error("%s: images requested without lattice vectors", fn); return PPM_ERR_INVALID_ARG; }
+    if ((long long)na * (2 * npbc_a + 1) * (2 * npbc_b + 1) > (1LL << 28)) { set_error("%s: too many image sites", fn); return PPM_ERR_INVALID_ARG; }
+    if (!finite4(Qs, 4) || !finite4(QZs, 4)) { set_error("%s: non-finite tip charges", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_grid_project(model, atoms, cpar, na, *spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, grid,
+                               (hipStream_t)stream);
+}
+
+int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
+                             const PpmTip* tip, const PpmRelaxCtl* ctl, float* FEs, float* PPs, void* stream) {
+    const char* fn = "ppm_relax_strokes_tilted";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (int r = check_ctl(fn, ctl, n_scan)) return r;
+    if (!grid || !tip) { set_error("%s: NULL grid/tip", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_relax_tilted(grid, *spec, points, n_scan, nz, *tip, *ctl, FEs, PPs, (hipStream_t)stream);
+}
+
+int ppm_relax_strokes(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
+                      const float dTip[4], const float stiffness[4], const float dpos0[4], const float relax[4],
+                      const PpmRelaxCtl* ctl, float* FEs, float* disps, float* PPs, void* stream) {
+    const char* fn = "ppm_relax_strokes";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (int r = check_ctl(fn, ctl, n_scan)) return r;
+    if (!grid || !dTip || !stiffness || !dpos0 || !relax) { set_error("%s: NULL argument", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_relax_strokes(grid, *spec, points, n_scan, nz, dTip, stiffness, dpos0, relax, *ctl, FEs, disps, PPs,
+                                (hipStream_t)stream);
+}
+
+int ppm_fe_in_strokes_tilted(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
+                             const PpmTip* tip, const float dTip[4], const float dpos0[4], float* FEs, void* stream) {
+    const char* fn = "ppm_fe_in_strokes_tilted";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (!grid || !tip || !dTip || !dpos0) { set_error("%s: NULL argument", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_fe_in_strokes(grid, *spec, points, n_scan, nz, *tip, dTip, dpos0, FEs, (hipStream_t)stream);
+}
+
+int ppm_z_stencil(const float* in, int in_stride, int in_offset, int n_pix, int nz_in, const float* W,
+                  const int* start, int taps, int nz_out, float* out, void* stream) {
+    const char* fn = "ppm_z_stencil";
+    g_err[0] = 0;
+    if (!in || !W || !start || !out) { set_error("%s: NULL argument", fn); return PPM_ERR_INVALID_ARG; }
+    if (n_pix < 1 || nz_in < 1 || nz_out < 1 || taps < 1 || taps > nz_in) {
+        set_error("%s: invalid sizes n_pix=%d nz_in=%d nz_out=%d taps=%d", fn, n_pix, nz_in, nz_out, taps);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (in_stride < 1 || in_offset < 0 || in_offset >= in_stride) {
+        set_error("%s: invalid stride/offset (%d,%d)", fn, in_stride, in_offset);
+        return PPM_ERR_INVALID_ARG;
+    }
+    const float chw[4] = {1.f, 0.f, 0.f, 0.f};
+    return launch_axis_operator(in + in_offset, (long long)nz_in * in_stride, in_stride, 0, chw, 1, n_pix, nz_in, 1,
+                                W, start, taps, nz_out, 1.0f, 0, out, (hipStream_t)stream);
+}
+
+int ppm_axis_operator(const float* in, long long in_outer_stride, long long in_axis_stride,
+                      long long in_inner_stride, const float chw[4], int nch, int n_outer, int n_axis_in,
+                      int n_inner, const float* W, const int* start, int taps, int n_axis_out, float alpha,
+                      int accumulate, float* out, void* stream) {
+    const char* fn = "ppm_axis_operator";
+    g_err[0] = 0;
+    if (!in || !W || !start || !out || !chw) { set_error("%s: NULL argument", fn); return PPM_ERR_INVALID_ARG; }
+    if (n_outer < 1 || n_axis_in < 1 || n_inner < 1 || n_axis_out < 1 || taps < 1 || taps > n_axis_in) {
+        set_error("%s: invalid sizes outer=%d axis_in=%d inner=%d axis_out=%d taps=%d", fn, n_outer, n_axis_in,
+                  n_inner, n_axis_out, taps);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (nch < 1 || nch > 4 || in_outer_stride < 0 || in_axis_stride < 0 || in_inner_stride < 0) {
+        set_error("%s: invalid channel count or strides", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if ((long long)n_outer * n_axis_out * n_inner > (1LL << 40)) { set_error("%s: output too large", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_axis_operator(in, in_outer_stride, in_axis_stride, in_inner_stride, chw, nch, n_outer, n_axis_in,
+                                n_inner, W, start, taps, n_axis_out, alpha, accumulate, out, (hipStream_t)stream);
+}
+
+}  // extern "C"

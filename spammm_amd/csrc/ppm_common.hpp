@@ -1,0 +1,49 @@
+// ppm_common.hpp — shared definitions of the gfx950 probe-particle AFM kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "ppm_api.h"
+
+namespace ppm {
+
+// Physical constants of kernels/common.cl:28-29.
+constexpr float kR2SAFE = 1e-4f;
+constexpr float kCOULOMB = 14.3996448915f;
+
+// Algorithmic FLOP per pair evaluation (grid projection roofline unit), counted on
+// the reference pair laws with add/sub/mul = 1, FMA = 2, rcp/sqrt/rsqrt/exp = 1:
+//   getLJ (kernels/AFM.cl:738-745) 27 + one getCoulombAFM site (kernels/AFM.cl:730-736) 25 = 52
+//   getMorse (kernels/AFM.cl:747-757) 29 + one getCoulombAFM site 25 = 54
+constexpr int kFlopPairLJ = 52;
+constexpr int kFlopPairMorse = 54;
+
+struct f3 {
+    float x, y, z;
+};
+__host__ __device__ inline f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__host__ __device__ inline f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
+
+// Launch helpers shared by the translation units.
+void set_error(const char* fmt, ...);
+int check_launch(const char* what);
+
+// Per-translation-unit launchers (host side).
+int launch_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
+                        const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                        const float lvB[3], int cap, float* grid, hipStream_t stream);
+int launch_relax_tilted(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
+                        const PpmTip& tip, const PpmRelaxCtl& ctl, float* FEs, float* PPs, hipStream_t stream);
+int launch_relax_strokes(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
+                         const float dTip[4], const float stiffness[4], const float dpos0[4], const float relax[4],
+                         const PpmRelaxCtl& ctl, float* FEs, float* disps, float* PPs, hipStream_t stream);
+int launch_fe_in_strokes(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
+                         const PpmTip& tip, const float dTip[4], const float dpos0[4], float* FEs,
+                         hipStream_t stream);
+int launch_axis_operator(const float* in, long long so, long long sa, long long si, const float chw[4], int nch,
+                         int n_outer, int n_axis_in, int n_inner, const float* W, const int* start, int taps,
+                         int n_axis_out, float alpha, int accumulate, float* out, hipStream_t stream);
+
+}  // namespace ppm

@@ -1,0 +1,229 @@
+"""Device-tensor wrappers over the C ABI (include/ppm_api.h).
+
+Every function takes and returns float32 CUDA (HIP) tensors, enqueues on the
+current torch stream of the tensor's device and returns without synchronizing.
+There is no CPU path: a missing GPU or library raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+
+def require_gpu(device=None):
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("spammm_amd needs a HIP GPU (torch.cuda.is_available() is False); "
+                           "the AFM path has no CPU fallback")
+    _lib.load()
+    return torch.device(device if device is not None else "cuda:0")
+
+
+def _stream(t):
+    import torch
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def _f32dev(t, name, shape_last=None):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float32:
+        raise ValueError(f"{name} must be a float32 CUDA tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if shape_last is not None and t.shape[-1] != shape_last:
+        raise ValueError(f"{name} must have last dimension {shape_last}, got {tuple(t.shape)}")
+    return t
+
+
+@dataclass
+class GridSpec:
+    """Grid geometry: voxel (ia,ib,ic) at p0 + dA*ia + dB*ib + dC*ic; sampling rows dinvA/B/C."""
+    n: tuple
+    p0: np.ndarray
+    dA: np.ndarray
+    dB: np.ndarray
+    dC: np.ndarray
+    dinvA: np.ndarray
+    dinvB: np.ndarray
+    dinvC: np.ndarray
+
+    def c(self) -> _lib.PpmGridSpec:
+        s = _lib.PpmGridSpec()
+        s.nx, s.ny, s.nz = (int(x) for x in self.n)
+        s.reserved = 0
+        s.p0, s.dA, s.dB, s.dC = (_lib.f4(v) for v in (self.p0, self.dA, self.dB, self.dC))
+        s.dinvA, s.dinvB, s.dinvC = (_lib.f4(v) for v in (self.dinvA, self.dinvB, self.dinvC))
+        return s
+
+    @property
+    def nvox(self) -> int:
+        return int(self.n[0]) * int(self.n[1]) * int(self.n[2])
+
+
+def dinv_axis_aligned(L, origin, n):
+    """(dinvA, dinvB, dinvC) with the half-texel offset (AFMulator._make_dinv_axis_aligned,
+    spammm/SPM/AFM.py:275-285): dot((pos,1), dinvA) = (pos.x - origin.x)/L.x + 0.5/nx."""
+    nx, ny, nz = n
+    dinvA = np.array([1. / L[0], 0., 0., -origin[0] / L[0] + 0.5 / nx], dtype=np.float32)
+    dinvB = np.array([0., 1. / L[1], 0., -origin[1] / L[1] + 0.5 / ny], dtype=np.float32)
+    dinvC = np.array([0., 0., 1. / L[2], -origin[2] / L[2] + 0.5 / nz], dtype=np.float32)
+    return dinvA, dinvB, dinvC
+
+
+def dinv_lvec(invMT, n):
+    """(dinvA, dinvB, dinvC) of a skew cell (AFMulator._make_dinv_lvec, spammm/SPM/AFM.py:287-294)."""
+    nx, ny, nz = n
+    dinvA = np.array([invMT[0, 0], invMT[0, 1], invMT[0, 2], 0.5 / nx], dtype=np.float32)
+    dinvB = np.array([invMT[1, 0], invMT[1, 1], invMT[1, 2], 0.5 / ny], dtype=np.float32)
+    dinvC = np.array([invMT[2, 0], invMT[2, 1], invMT[2, 2], 0.5 / nz], dtype=np.float32)
+    return dinvA, dinvB, dinvC
+
+
+def axis_aligned_spec(p0, L, n) -> GridSpec:
+    p0 = np.asarray(p0, dtype=np.float32).reshape(3)
+    L = np.asarray(L, dtype=np.float32).reshape(3)
+    nx, ny, nz = (int(v) for v in n)
+    dA = np.array([L[0] / nx, 0., 0.], dtype=np.float32)
+    dB = np.array([0., L[1] / ny, 0.], dtype=np.float32)
+    dC = np.array([0., 0., L[2] / nz], dtype=np.float32)
+    a, b, c = dinv_axis_aligned(L, p0, (nx, ny, nz))
+    return GridSpec((nx, ny, nz), p0, dA, dB, dC, a, b, c)
+
+
+@dataclass
+class Tip:
+    """Tip / probe parameter record (AFMulator.DEFAULT_*, spammm/SPM/AFM.py:296-304)."""
+    tipA: np.ndarray = field(default_factory=lambda: np.array([1., 0., 0., 0.], np.float32))
+    tipB: np.ndarray = field(default_factory=lambda: np.array([0., 1., 0., 0.], np.float32))
+    tipC: np.ndarray = field(default_factory=lambda: np.array([0., 0., 1., -0.1], np.float32))
+    stiffness: np.ndarray = field(default_factory=lambda: np.array([-0.03, -0.03, -0.03, -1.0], np.float32))
+    dpos0: np.ndarray = field(default_factory=lambda: np.array([0., 0., -4.0, 4.0], np.float32))
+    relax: np.ndarray = field(default_factory=lambda: np.array([0.5, 0.1, 0.02, 0.5], np.float32))
+    surfFF: np.ndarray = field(default_factory=lambda: np.zeros(4, np.float32))
+
+    def c(self) -> _lib.PpmTip:
+        t = _lib.PpmTip()
+        t.tipA, t.tipB, t.tipC = _lib.f4(self.tipA), _lib.f4(self.tipB), _lib.f4(self.tipC)
+        t.stiffness, t.dpos0 = _lib.f4(self.stiffness), _lib.f4(self.dpos0)
+        t.relax, t.surfFF = _lib.f4(self.relax), _lib.f4(self.surfFF)
+        return t
+
+
+def relax_ctl(max_iter=128, f2conv=1e-8, check_before=False, scan_ny=0) -> _lib.PpmRelaxCtl:
+    c = _lib.PpmRelaxCtl()
+    c.max_iter = int(max_iter)
+    c.f2conv = float(f2conv)
+    c.check_before = int(bool(check_before))
+    c.scan_ny = int(scan_ny)
+    return c
+
+
+def grid_project(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 0), lvA=(0, 0, 0), lvB=(0, 0, 0),
+                 cap=True, out=None):
+    """Force-field grid (nz,ny,nx,4) on the device (evalLJC_QZs_toImg / evalMorseC_QZs_toImg)."""
+    import torch
+    _f32dev(atoms, "atoms", 4)
+    _f32dev(cpar, "cpar", 2 if model == _lib.PPM_MODEL_LJ else 4)
+    if atoms.shape[0] != cpar.shape[0]:
+        raise ValueError("atoms and cpar must have the same number of rows")
+    nx, ny, nz = (int(v) for v in spec.n)
+    if out is None:
+        out = torch.empty((nz, ny, nx, 4), dtype=torch.float32, device=atoms.device)
+    else:
+        _f32dev(out, "out", 4)
+        if out.numel() != 4 * nx * ny * nz:
+            raise ValueError("out has the wrong size")
+    L = _lib.load()
+    lva = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvA, float).reshape(3)])
+    lvb = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvB, float).reshape(3)])
+    s = spec.c()
+    rc = L.ppm_grid_project(int(model), atoms.data_ptr(), cpar.data_ptr(), int(atoms.shape[0]), ctypes.byref(s),
+                            _lib.f4(Qs), _lib.f4(QZs), int(npbc[0]), int(npbc[1]), ctypes.byref(lva),
+                            ctypes.byref(lvb), int(bool(cap)), out.data_ptr(), _stream(atoms))
+    _lib.check(rc)
+    return out
+
+
+def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, f2conv=1e-8, check_before=False,
+                 scan_ny=0, FEs=None, PPs=None, want_pps=True):
+    """relaxStrokesTilted on the device: returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None)."""
+    import torch
+    _f32dev(grid, "grid", 4)
+    _f32dev(points, "points", 4)
+    if grid.numel() != 4 * spec.nvox:
+        raise ValueError("grid size does not match spec")
+    ns = int(points.shape[0])
+    if FEs is None:
+        FEs = torch.empty((ns, nz, 4), dtype=torch.float32, device=grid.device)
+    if PPs is None and want_pps:
+        PPs = torch.empty((ns, nz, 4), dtype=torch.float32, device=grid.device)
+    for nm, t in (("FEs", FEs), ("PPs", PPs)):
+        if t is not None:
+            _f32dev(t, nm, 4)
+            if t.numel() != ns * nz * 4:
+                raise ValueError(f"{nm} has the wrong size")
+    L = _lib.load()
+    s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, scan_ny)
+    rc = L.ppm_relax_strokes_tilted(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz),
+                                    ctypes.byref(tp), ctypes.byref(ctl), FEs.data_ptr(),
+                                    PPs.data_ptr() if PPs is not None else None, _stream(grid))
+    _lib.check(rc)
+    return FEs, PPs
+
+
+def relax_strokes(grid, spec: GridSpec, points, nz: int, dTip, stiffness, dpos0, relax, max_iter=128, f2conv=1e-8,
+                  scan_ny=0, want_disps=True, want_pps=True):
+    """relaxStrokes on the device: returns (FEs, disps or None, PPs or None), each (n_scan,nz,4)."""
+    import torch
+    _f32dev(grid, "grid", 4)
+    _f32dev(points, "points", 4)
+    if grid.numel() != 4 * spec.nvox:
+        raise ValueError("grid size does not match spec")
+    ns = int(points.shape[0])
+    dev = grid.device
+    FEs = torch.empty((ns, nz, 4), dtype=torch.float32, device=dev)
+    disps = torch.empty((ns, nz, 4), dtype=torch.float32, device=dev) if want_disps else None
+    PPs = torch.empty((ns, nz, 4), dtype=torch.float32, device=dev) if want_pps else None
+    L = _lib.load()
+    s, ctl = spec.c(), relax_ctl(max_iter, f2conv, False, scan_ny)
+    rc = L.ppm_relax_strokes(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz), _lib.f4(dTip),
+                             _lib.f4(stiffness), _lib.f4(dpos0), _lib.f4(relax), ctypes.byref(ctl), FEs.data_ptr(),
+                             disps.data_ptr() if disps is not None else None,
+                             PPs.data_ptr() if PPs is not None else None, _stream(grid))
+    _lib.check(rc)
+    return FEs, disps, PPs
+
+
+def fe_in_strokes_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, dTip, dpos0):
+    """getFEinStrokesTilted on the device: (n_scan,nz,4)."""
+    import torch
+    _f32dev(grid, "grid", 4)
+    _f32dev(points, "points", 4)
+    if grid.numel() != 4 * spec.nvox:
+        raise ValueError("grid size does not match spec")
+    ns = int(points.shape[0])
+    FEs = torch.empty((ns, nz, 4), dtype=torch.float32, device=grid.device)
+    L = _lib.load()
+    s, tp = spec.c(), tip.c()
+    rc = L.ppm_fe_in_strokes_tilted(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz),
+                                    ctypes.byref(tp), _lib.f4(dTip), _lib.f4(dpos0), FEs.data_ptr(), _stream(grid))
+    _lib.check(rc)
+    return FEs
+
+
+def scan_points(scan_p0, scan_da, scan_db, nx: int, ny: int) -> np.ndarray:
+    """(nx*ny,4) float32 ix-major scan starts: p0 + da*ix + db*iy in float32, the arithmetic of the
+    reference loop (spammm/SPM/AFM.py:977-980) and build_scan_xy_points_vectorized (:82-96)."""
+    p0 = np.asarray(scan_p0, dtype=np.float32).reshape(3)
+    da = np.asarray(scan_da, dtype=np.float32).reshape(3)
+    db = np.asarray(scan_db, dtype=np.float32).reshape(3)
+    ix = np.arange(int(nx), dtype=np.float32)[:, None, None]
+    iy = np.arange(int(ny), dtype=np.float32)[None, :, None]
+    xyz = (p0 + da * ix) + db * iy
+    pts = np.zeros((int(nx) * int(ny), 4), dtype=np.float32)
+    pts[:, :3] = xyz.reshape(-1, 3)
+    return pts

@@ -1,0 +1,298 @@
+"""GPU parity of the HIP path (libppm_hip.so through its C ABI) against the CPU oracle.
+
+Bars:
+  * grid projection (FP32 on the GPU vs FP64 oracle): per voxel
+    |d(Fx,Fy,Fz,E)| <= 1e-5 * |(Fx,Fy,Fz,E)| + 1e-6 (terms of opposite sign cancel,
+    so the bound is relative to the voxel's vector magnitude);
+  * relaxation / sampling (FP32 on the GPU vs the FP32 op-order-faithful oracle on the
+    same grid): bit-exact (probe positions, iteration counts, FEs);
+  * relaxation vs the FP64 oracle at tightened convergence (|F| < 1e-6 eV/A):
+    probe positions <= 1e-4 A and Fz <= 1e-4 eV/A;
+  * df operators (FP32 GPU vs FP64 NumPy/SciPy oracle): <= 1e-5 relative to max|df|.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O          # noqa: E402
+from oracle import df_ref as R          # noqa: E402
+from spammm_amd import device as D      # noqa: E402
+from spammm_amd import workloads as W   # noqa: E402
+
+
+def dev():
+    return D.require_gpu()
+
+
+def spec_of(w):
+    return D.axis_aligned_spec(w.p0, w.L, w.n)
+
+
+def grid_gpu(w, model=0, cpar=None, Qs=None, QZs=None, npbc=None, cap=True):
+    d = dev()
+    atoms = torch.as_tensor(w.atoms, device=d)
+    cp = torch.as_tensor(w.cLJ if cpar is None else cpar, device=d)
+    sp = spec_of(w)
+    G = D.grid_project(atoms, cp, model, sp, w.Qs if Qs is None else Qs, w.QZs if QZs is None else QZs,
+                       w.npbc if npbc is None else npbc, w.lvA, w.lvB, cap=cap)
+    torch.cuda.synchronize()
+    return G, sp
+
+
+def check_grid(Gg, Go):
+    Go = Go.astype(np.float64)
+    mag = np.linalg.norm(Go, axis=-1, keepdims=True)
+    err = np.abs(Gg.astype(np.float64) - Go)
+    worst = float((err / (mag + 1e-30)).max())
+    assert (err <= 1e-5 * mag + 1e-6).all(), f"grid mismatch, worst relative {worst:.3e}"
+    return worst
+
+
+def morse_params(w):
+    c = np.zeros((w.atoms.shape[0], 4), np.float32)
+    R0 = np.where(np.array(w.enames) == "H", 1.487, 1.908) + 1.66
+    c[:, 0] = R0
+    c[:, 1] = np.sqrt(0.0091 * np.where(np.array(w.enames) == "H", 0.00068, 0.0037))
+    c[:, 2] = -1.8
+    return c
+
+
+@pytest.mark.parametrize("model", [0, 1])
+@pytest.mark.parametrize("case", ["pbc_q0", "nopbc_offsets", "pbc_offsets_nocap"])
+def test_grid_project_small_vs_oracle(model, case):
+    w = W.small()
+    cpar = w.cLJ if model == 0 else morse_params(w)
+    Qs, QZs, npbc, cap = w.Qs, w.QZs, w.npbc, True
+    if case == "nopbc_offsets":
+        Qs, QZs, npbc = np.array([0., -0.1, 0.1, 0.], np.float32), np.array([0., 1.8, 3.6, 0.], np.float32), (0, 0)
+    elif case == "pbc_offsets_nocap":
+        Qs, QZs, cap = np.array([-0.05, 0.2, 0., 0.], np.float32), np.array([0., 1.2, 0., 0.], np.float32), False
+    print(f"[grid] model={model} case={case} n={w.n}", flush=True)
+    G, sp = grid_gpu(w, model, cpar, Qs, QZs, npbc, cap)
+    Go = O.grid_project(model, w.atoms, cpar, w.n, sp.p0, sp.dA, sp.dB, sp.dC, Qs, QZs, npbc, w.lvA, w.lvB,
+                        cap=cap)
+    worst = check_grid(G.cpu().numpy(), Go)
+    print(f"[grid] worst relative error {worst:.3e}")
+
+
+def test_grid_project_full_naphthalene_sampled():
+    w = W.naphthalene()
+    print(f"[grid] naphthalene full grid {w.n}", flush=True)
+    G, sp = grid_gpu(w)
+    Gh = G.cpu().numpy().reshape(-1, 4)
+    assert np.isfinite(Gh).all()
+    idx = np.random.default_rng(5).choice(Gh.shape[0], 3000, replace=False)
+    Go = O.grid_voxels(0, w.atoms, w.cLJ, w.n, sp.p0, sp.dA, sp.dB, sp.dC, w.Qs, w.QZs, idx, w.npbc, w.lvA, w.lvB)
+    worst = check_grid(Gh[idx], Go)
+    print(f"[grid] worst relative error {worst:.3e}")
+
+
+def tip_of(w, dtip=None):
+    return D.Tip(stiffness=w.stiffness, dpos0=w.dpos0,
+                 tipC=np.array([0., 0., 1., w.dtip if dtip is None else dtip], np.float32))
+
+
+def relax_both(w, G, sp, precision="f32", max_iter=None, f2conv=None, check_before=False, scan_ny=0, pts=None):
+    d = dev()
+    if pts is None:
+        pts = D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy)
+    tip = tip_of(w)
+    mi = w.max_iter if max_iter is None else max_iter
+    f2 = w.f2conv if f2conv is None else f2conv
+    FEs, PPs = D.relax_tilted(G, sp, torch.as_tensor(pts, device=d), w.nz, tip, mi, f2, check_before, scan_ny)
+    torch.cuda.synchronize()
+    Gh = G.cpu().numpy()
+    FEo, PPo = O.relax_tilted(Gh, sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pts, w.nz, tip.tipA, tip.tipB, tip.tipC,
+                              tip.stiffness, tip.dpos0, tip.relax, tip.surfFF, mi, f2, check_before, precision)
+    return FEs.cpu().numpy(), PPs.cpu().numpy(), FEo, PPo
+
+
+@pytest.mark.parametrize("check_before", [False, True])
+@pytest.mark.parametrize("tiled", [False, True])
+def test_relax_tilted_bitexact_vs_fp32_oracle(check_before, tiled):
+    w = W.small()
+    G, sp = grid_gpu(w)
+    print(f"[relax] small check_before={check_before} tiled={tiled}", flush=True)
+    FEs, PPs, FEo, PPo = relax_both(w, G, sp, "f32", check_before=check_before, scan_ny=w.nxy[1] if tiled else 0)
+    np.testing.assert_array_equal(PPs, PPo)
+    np.testing.assert_array_equal(FEs, FEo)
+    assert (PPs[..., 3] != 0).all()
+
+
+def test_relax_tilted_tilted_frame_bitexact():
+    """A rotated tip frame and a surface force exercise rotMat/rotMatT and surfFF."""
+    w = W.small()
+    G, sp = grid_gpu(w)
+    d = dev()
+    th = 0.2
+    A = np.array([np.cos(th), 0., -np.sin(th), 0.], np.float32)
+    B = np.array([0., 1., 0., 0.], np.float32)
+    C = np.array([np.sin(th), 0., np.cos(th), -0.1], np.float32)
+    tip = D.Tip(A, B, C, w.stiffness, w.dpos0, np.array([0.5, 0.1, 0, 0], np.float32),
+                np.array([0.01, 0, 0, 0], np.float32))
+    pts = D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy)
+    FEs, PPs = D.relax_tilted(G, sp, torch.as_tensor(pts, device=d), w.nz, tip, 100, 1e-8)
+    FEo, PPo = O.relax_tilted(G.cpu().numpy(), sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pts, w.nz, A, B, C,
+                              tip.stiffness, tip.dpos0, tip.relax, tip.surfFF, 100, 1e-8, False, "f32")
+    np.testing.assert_array_equal(PPs.cpu().numpy(), PPo)
+    np.testing.assert_array_equal(FEs.cpu().numpy(), FEo)
+
+
+def test_relax_vs_fp64_oracle_tight_convergence():
+    """FP32 GPU vs FP64 oracle with |F| < 1e-6 eV/A: probe positions within 1e-4 A, Fz within 1e-4 eV/A."""
+    w = W.small()
+    G, sp = grid_gpu(w)
+    FEs, PPs, FEo, PPo = relax_both(w, G, sp, "f64", max_iter=4000, f2conv=1e-12, check_before=True)
+    conv = (PPs[..., 3] > 0) & (PPo[..., 3] > 0)
+    assert conv.mean() > 0.98, f"only {conv.mean():.3f} of states converged at |F|<1e-6"
+    dpos = np.abs(PPs[..., :3] - PPo[..., :3]).max(-1)[conv]
+    dFz = np.abs(FEs[..., 2] - FEo[..., 2])[conv]
+    print(f"[relax f64] PP max {dpos.max():.2e} median {np.median(dpos):.2e}; Fz max {dFz.max():.2e}")
+    assert dpos.max() <= 1e-4
+    assert dFz.max() <= 1e-4
+
+
+def test_relax_reference_settings_error_vs_fp64_reported():
+    """Reference settings (|F|<1e-4, check after update): FP32 GPU vs FP64 oracle error is reported;
+    two correct implementations may stop up to |F|/k apart, so only the median is bounded."""
+    w = W.small()
+    G, sp = grid_gpu(w)
+    FEs, PPs, FEo, PPo = relax_both(w, G, sp, "f64")
+    dpos = np.abs(PPs[..., :3] - PPo[..., :3]).max(-1)
+    dFz = np.abs(FEs[..., 2] - FEo[..., 2])
+    print(f"[relax ref] PP max {dpos.max():.2e} median {np.median(dpos):.2e}; "
+          f"Fz max {dFz.max():.2e} median {np.median(dFz):.2e}")
+    assert np.median(dpos) < 1e-2
+
+
+def test_relax_residual_at_converged_states():
+    """check_before mode: every converged PP satisfies |F_total| < 1e-4 eV/A (tests/SPM/test_afm_morse.py:907-941)."""
+    w = W.small()
+    G, sp = grid_gpu(w)
+    FEs, PPs, _, _ = relax_both(w, G, sp, "f32", check_before=True)
+    pts = D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy)
+    Gh = G.cpu().numpy()
+    conv = PPs[..., 3] > 0
+    pp = PPs[..., :3].reshape(-1, 3)
+    fe = O.interp(Gh, sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pp)
+    tip = pts[:, None, :3] + np.arange(w.nz)[None, :, None] * np.array([0, 0, w.dtip])
+    d = pp - tip.reshape(-1, 3)
+    r = np.linalg.norm(d, axis=1)
+    k = w.stiffness.astype(np.float64)
+    d0 = w.dpos0.astype(np.float64)
+    ftip = (d - d0[:3]) * k[:3] + d * (k[3] * (r - d0[3]) / r)[:, None]
+    res = np.linalg.norm(fe[:, :3] + ftip, axis=1).reshape(conv.shape)
+    assert conv.mean() > 0.95
+    assert res[conv].max() < 1.0001e-4, f"residual {res[conv].max():.3e}"
+
+
+def test_relax_strokes_and_raw_fe_bitexact():
+    w = W.small()
+    G, sp = grid_gpu(w)
+    d = dev()
+    pts = D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy)
+    dTip = np.array([0., 0., -0.1, 0.], np.float32)
+    st, d0, rl = w.stiffness, np.array([0., 0., -3.0, 3.0], np.float32), np.array([0.5, 0.1, 0, 0], np.float32)
+    FEs, disps, PPs = D.relax_strokes(G, sp, torch.as_tensor(pts, device=d), w.nz, dTip, st, d0, rl, 100, 1e-8)
+    FEo, dio, PPo = O.relax_strokes(G.cpu().numpy(), sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pts, w.nz, dTip, st, d0,
+                                    rl, 100, 1e-8, "f32")
+    np.testing.assert_array_equal(FEs.cpu().numpy(), FEo)
+    np.testing.assert_array_equal(disps.cpu().numpy(), dio)
+    np.testing.assert_array_equal(PPs.cpu().numpy(), PPo)
+    tip = tip_of(w)
+    raw = D.fe_in_strokes_tilted(G, sp, torch.as_tensor(pts, device=d), w.nz, tip, dTip, w.dpos0)
+    rawo = O.fe_in_strokes_tilted(G.cpu().numpy(), sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pts, w.nz, tip.tipA,
+                                  tip.tipB, tip.tipC, dTip, w.dpos0, "f32")
+    np.testing.assert_array_equal(raw.cpu().numpy(), rawo)
+
+
+def test_full_naphthalene_scan_subsample_bitexact():
+    """Full-size 200^3 grid and 100x100x40 scan: all states finite; 256 pixels bit-exact vs the FP32 oracle."""
+    w = W.naphthalene()
+    G, sp = grid_gpu(w)
+    d = dev()
+    pts = D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy)
+    FEs, PPs = D.relax_tilted(G, sp, torch.as_tensor(pts, device=d), w.nz, tip_of(w), w.max_iter, w.f2conv,
+                              scan_ny=w.nxy[1])
+    FEs, PPs = FEs.cpu().numpy(), PPs.cpu().numpy()
+    assert np.isfinite(FEs).all() and np.isfinite(PPs).all()
+    conv = (PPs[..., 3] > 0).mean()
+    print(f"[full] converged fraction {conv:.4f}, mean iters {np.abs(PPs[..., 3]).mean():.2f}")
+    sel = np.random.default_rng(3).choice(w.n_scan, 256, replace=False)
+    tip = tip_of(w)
+    FEo, PPo = O.relax_tilted(G.cpu().numpy(), sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pts[sel], w.nz, tip.tipA,
+                              tip.tipB, tip.tipC, tip.stiffness, tip.dpos0, tip.relax, tip.surfFF, w.max_iter,
+                              w.f2conv, False, "f32")
+    np.testing.assert_array_equal(PPs[sel], PPo)
+    np.testing.assert_array_equal(FEs[sel], FEo)
+
+
+# ── df operators ─────────────────────────────────────────────────────────────
+
+def _cmp(a, b, rel=1e-5):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    scale = max(np.abs(b).max(), 1e-30)
+    assert np.abs(a - b).max() <= rel * scale, f"max err {np.abs(a - b).max():.3e} vs scale {scale:.3e}"
+
+
+def test_df_gradient_and_amp9_vs_oracle():
+    from spammm_amd import afm
+    rng = np.random.default_rng(42)
+    Fz = rng.normal(size=(5, 6, 41)).astype(np.float32)
+    _cmp(afm.compute_df(Fz, 0.1), R.compute_df(Fz.astype(np.float64), 0.1))
+    _cmp(afm.compute_df_amp(Fz, 0.1, 1.0), R.compute_df_amp(Fz, 0.1, 1.0))
+    _cmp(afm.compute_df_amp(Fz, -0.1, 0.37), R.compute_df_amp(Fz, -0.1, 0.37))
+
+
+def test_df_giessibl_vs_quadrature():
+    from spammm_amd import afm
+    rng = np.random.default_rng(1)
+    z = 6.0 - 0.1 * np.arange(40)
+    Fz = (0.3 * np.exp(-(z[None, None, :] - 3.0)) + 0.05 * rng.normal(size=(3, 2, 40))).astype(np.float32)
+    df = afm.df_giessibl(Fz, -0.1, 1.0, 30000.0, 1800.0)
+    ref = R.giessibl_df_piecewise(Fz.astype(np.float64), z, 1.0, 30000.0, 1800.0)
+    _cmp(df, ref, 2e-5)
+
+
+def test_df_loss_seed_vs_oracle():
+    from spammm_amd import afm
+    rng = np.random.default_rng(7)
+    FEs = rng.normal(size=(4, 3, 9, 4))
+    tgt = rng.normal(size=(4, 3, 9))
+    w = rng.uniform(0.5, 1.5, size=(4, 3, 9))
+    L, df, dL = afm.df_loss_seed(FEs, tgt, -0.1, w)
+    Lo, dfo, dLo = R.df_loss_seed(FEs.astype(np.float32).astype(np.float64), tgt, -0.1, w)
+    assert abs(L - Lo) <= 1e-5 * abs(Lo)
+    _cmp(df, dfo)
+    _cmp(dL, dLo)
+
+
+def test_df_directional_and_dissipation_vs_oracle():
+    from spammm_amd import afm
+    rng = np.random.default_rng(9)
+    FEs = rng.normal(size=(6, 5, 11, 4)).astype(np.float32)
+    n = np.array([1., 2., 3.]) / np.linalg.norm([1., 2., 3.])
+    sp = (0.4, 0.3, 0.2)
+    _cmp(afm.compute_df_dir(FEs, sp, n), R.compute_df_dir(FEs.astype(np.float64), sp, n))
+    _cmp(afm.compute_df_amp_dir(FEs, sp, n, 0.5), R.compute_df_amp_dir(FEs, sp, n, 0.5))
+    _cmp(afm.compute_df_amp_dir(FEs, sp, (0., 0., 1.), 0.5), R.compute_df_amp_dir(FEs, sp, (0., 0., 1.), 0.5))
+    h = 3.0 + 0.2 * np.arange(11)
+    bwd = FEs + rng.normal(scale=0.1, size=FEs.shape).astype(np.float32)
+    _cmp(afm.compute_dissipation(FEs, bwd, h, h[2:9:3], 0.4), R.compute_dissipation(FEs, bwd, h, h[2:9:3], 0.4))
+
+
+def test_errors_are_raised_not_swallowed():
+    d = dev()
+    w = W.small()
+    G, sp = grid_gpu(w)
+    pts = torch.zeros((4, 4), device=d)
+    with pytest.raises(ValueError):
+        D.relax_tilted(G, sp, pts, 3, tip_of(w), max_iter=0)
+    with pytest.raises(ValueError):
+        D.relax_tilted(G, sp, pts.double(), 3, tip_of(w))
+    bad = D.axis_aligned_spec(w.p0, w.L, (0, 4, 4))
+    with pytest.raises(ValueError):
+        D.grid_project(torch.as_tensor(w.atoms, device=d), torch.as_tensor(w.cLJ, device=d), 0, bad, w.Qs, w.QZs)

@@ -1,0 +1,99 @@
+"""Host-side checks that need no GPU: the C ABI library and the facade's host logic."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from spammm_amd import _lib, device as D, params as P, workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "ppm_api.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ppm_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    assert os.path.exists(_lib.LIB_PATH), "libppm_hip.so not built (python -m spammm_amd.build)"
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = header_functions()
+    assert len(names) >= 8
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in ppm_api.h but not exported"
+    assert set(names) == set(_lib.EXPORTS)
+
+
+def test_version_and_error_string_without_gpu():
+    L = _lib.load()
+    assert L.ppm_version() == 1
+    assert isinstance(L.ppm_last_error(), bytes)
+
+
+def test_invalid_arguments_rejected_on_host():
+    """Argument validation runs before any device work, so it is testable without a GPU."""
+    L = _lib.load()
+    s = _lib.PpmGridSpec()
+    s.nx, s.ny, s.nz = 0, 4, 4
+    ctl = D.relax_ctl()
+    tip = D.Tip().c()
+    rc = L.ppm_relax_strokes_tilted(1, ctypes.byref(s), 1, 4, 3, ctypes.byref(tip), ctypes.byref(ctl), 1, None, None)
+    assert rc == _lib.PPM_ERR_INVALID_ARG
+    assert b"grid dims" in L.ppm_last_error()
+    s.nx = 4
+    ctl.max_iter = 0
+    rc = L.ppm_relax_strokes_tilted(1, ctypes.byref(s), 1, 4, 3, ctypes.byref(tip), ctypes.byref(ctl), 1, None, None)
+    assert rc == _lib.PPM_ERR_INVALID_ARG and b"max_iter" in L.ppm_last_error()
+    with pytest.raises(ValueError):
+        _lib.check(rc)
+
+
+def test_struct_layouts_match_header():
+    assert ctypes.sizeof(_lib.PpmGridSpec) == 16 + 7 * 16
+    assert ctypes.sizeof(_lib.PpmTip) == 7 * 16
+    assert ctypes.sizeof(_lib.PpmRelaxCtl) == 16
+
+
+def test_scan_points_match_reference_loop():
+    """Vectorized scan starts equal the reference's double loop (spammm/SPM/AFM.py:977-980) bit for bit."""
+    p0 = np.array([0.3, -1.7, 9.1], np.float32)
+    da = np.array([0.21, 0.0, 0.0], np.float32)
+    db = np.array([0.0, 0.19, 0.01], np.float32)
+    pts = D.scan_points(p0, da, db, 7, 5)
+    k = 0
+    for ix in range(7):
+        for iy in range(5):
+            np.testing.assert_array_equal(pts[k, :3], p0 + da * ix + db * iy)
+            k += 1
+    assert (pts[:, 3] == 0).all()
+
+
+def test_mixing_rules():
+    c = P.mix_lj(["C", "H"], P.ELEMENTS_BENCH, 1.66, 0.0091)
+    R0, E0 = 1.66 + 1.908, np.sqrt(0.0091 * 0.0037)
+    np.testing.assert_allclose(c[0], [2 * E0 * R0 ** 6, E0 * R0 ** 12], rtol=1e-6)
+    m = P.mix_morse(["H"], P.ELEMENTS_REFERENCE, 1.452, 0.0006808, -1.8)
+    np.testing.assert_allclose(m[0], [1.452 + 1.443, np.sqrt(0.0006808 * 0.00190802059), -1.8, 0], rtol=1e-6)
+    assert abs(P.stiffness_Nm_to_eVA2(0.25) - 0.25 / 16.02176634) < 1e-15
+
+
+def test_workloads_match_configuration():
+    w = W.naphthalene()
+    assert w.enames.count("C") == 10 and w.enames.count("H") == 8
+    assert abs(w.atoms[:, 3].sum()) < 1e-6 and np.abs(w.atoms[:, 3]).max() <= 0.3   # U(-0.15,0.15) - mean
+    C = w.atoms[:10, :3]
+    d = np.linalg.norm(C[:, None] - C[None], axis=-1)
+    assert np.isclose(np.sort(d[d > 0])[0], 1.40, atol=1e-6)
+    assert (np.sum(np.isclose(d, 1.40, atol=1e-6), axis=1) >= 2).all()
+    H = w.atoms[10:, :3]
+    dh = np.linalg.norm(H[:, None] - C[None], axis=-1).min(1)
+    np.testing.assert_allclose(dh, 1.08, atol=1e-6)
+    assert w.n == (200, 200, 200) and w.nxy == (100, 100) and w.nz == 40
+    f = W.flake()
+    assert f.atoms.shape[0] == 200 and abs(f.atoms[:, 3].sum()) < 1e-5
+    heavy = [e for e in f.enames if e != "H"]
+    assert abs(sum(e in ("N", "O") for e in heavy) - 0.1 * len(heavy)) <= 1
+    assert f.n == (300, 300, 240) and f.nxy == (256, 256) and f.nz == 60

@@ -1,0 +1,170 @@
+"""Pins of the oracle's df functions and of the host-built df operators (CPU only).
+
+Oracle pins (reference checks restated, tests/SPM/test_afm_morse.py:61-114, :1376-1457):
+  * Fx = x*z with x oscillation => df = -z on every slice (also with finite amplitude, interior);
+  * Fx depending only on z has zero x derivative;
+  * F = n (n.r) => directional df = -1 for any unit n, and is invariant under n -> -n;
+  * vertical directional df equals compute_df / compute_df_amp;
+  * df_loss_seed: explicit stencil matrix, exact transpose, finite differences, errors.
+Giessibl pins (mathematics):
+  * a force linear in z gives -(f0/2k) dF/dz for every amplitude;
+  * the small-amplitude limit is -(f0/2k) dFz/dz;
+  * an exponential force matches the Bessel-function closed form.
+Host operators (spammm_amd.df, product code, applied here on the CPU to check the weights):
+  every banded operator equals the oracle function it replaces.
+"""
+import numpy as np
+import pytest
+
+from oracle import df_ref as R
+from spammm_amd import df as DF
+
+C = DF.EV_A2_TO_N_M
+
+
+def test_df_dir_closed_forms():
+    dx, dy, dz = 0.4, 0.3, 0.2
+    x = (np.arange(7) * dx)[:, None, None]
+    z = (np.arange(9) * dz)[None, None, :]
+    FEs = np.zeros((7, 5, 9, 4))
+    FEs[..., 0] = x * z
+    expected = -np.broadcast_to(z, FEs.shape[:3])
+    assert np.allclose(R.compute_df_dir(FEs, (dx, dy, dz), (1., 0., 0.)), expected, atol=1e-12)
+    assert np.allclose(R.compute_df_amp_dir(FEs, (dx, dy, dz), (1., 0., 0.), amp=dx)[2:-2], expected[2:-2], atol=1e-6)
+    F2 = np.zeros((5, 3, 7, 4))
+    F2[..., 0] = np.arange(7)[None, None, :] ** 2
+    assert np.abs(R.compute_df_dir(F2, (0.5, 0.5, 0.2), (1., 0., 0.))).max() < 1e-12
+    n = np.array([1., 2., 3.]) / np.linalg.norm([1., 2., 3.])
+    xs, ys, zs = (np.arange(6) * dx)[:, None, None], (np.arange(5) * dy)[None, :, None], (np.arange(7) * dz)[None, None, :]
+    s = n[0] * xs + n[1] * ys + n[2] * zs
+    F3 = np.zeros((6, 5, 7, 4))
+    F3[..., :3] = s[..., None] * n
+    assert np.allclose(R.compute_df_dir(F3, (dx, dy, dz), n), -1.0, atol=1e-12)
+    assert np.allclose(R.compute_df_dir(F3, (dx, dy, dz), -n), -1.0, atol=1e-12)
+
+
+def test_df_dir_vertical_equals_vertical_functions():
+    rng = np.random.default_rng(42)
+    Fz = rng.normal(size=(4, 5, 21)).astype(np.float32)
+    FEs = np.zeros(Fz.shape + (4,), np.float32)
+    FEs[..., 2] = Fz
+    assert np.allclose(R.compute_df_dir(FEs, (0.3, 0.2, 0.1), (0, 0, 1.)), R.compute_df(Fz, 0.1), atol=1e-6)
+    assert np.allclose(R.compute_df_amp_dir(FEs, (0.3, 0.2, 0.1), (0, 0, 1.), 0.4), R.compute_df_amp(Fz, 0.1, 0.4),
+                       atol=1e-6)
+
+
+def test_df_loss_seed_stencil_transpose_and_fd():
+    rng = np.random.default_rng(3)
+    nx, ny, nz, dtip = 3, 2, 8, -0.1
+    FEs = rng.normal(size=(nx, ny, nz, 4))
+    tgt = rng.normal(size=(nx, ny, nz))
+    w = rng.uniform(0.2, 1.0, (nx, ny, nz))
+    w[:, :, 0] = w[:, :, -1] = 0.0     # boundary slices carry no weight (the reference default)
+    L, df, dL = R.df_loss_seed(FEs, tgt, dtip, w)
+    D = np.zeros((nz, nz))
+    for i in range(1, nz - 1):
+        D[i, i + 1], D[i, i - 1] = -1 / (2 * 0.1), 1 / (2 * 0.1)
+    np.testing.assert_allclose(df, FEs[..., 2] @ D.T, atol=1e-12)
+    g = w * (df - tgt) / w.sum()
+    np.testing.assert_allclose(dL[..., 2], g @ D, atol=1e-12)
+    assert np.all(dL[..., [0, 1, 3]] == 0)
+    h = 1e-6
+    for (i, j, k) in [(0, 0, 0), (1, 1, 4), (2, 0, 7)]:
+        Fp, Fm = FEs.copy(), FEs.copy()
+        Fp[i, j, k, 2] += h
+        Fm[i, j, k, 2] -= h
+        fd = (R.df_loss_seed(Fp, tgt, dtip, w)[0] - R.df_loss_seed(Fm, tgt, dtip, w)[0]) / (2 * h)
+        assert abs(fd - dL[i, j, k, 2]) < 1e-8
+    with pytest.raises(ValueError):
+        R.df_loss_seed(FEs, tgt, 0.1)
+    with pytest.raises(ValueError):
+        R.df_loss_seed(FEs[:, :, :2], tgt[:, :, :2], -0.1)
+
+
+def test_df_loss_seed_boundary_weight_quirk():
+    """With nonzero weights on the boundary slices the reference seed (spammm/SPM/AFM.py:3185-3194) is not
+    the exact transpose: it also propagates the (constant, zero) boundary df residuals into Fz[1] and
+    Fz[nz-2].  The difference is exactly those two terms."""
+    rng = np.random.default_rng(4)
+    nz = 7
+    FEs = rng.normal(size=(2, 2, nz, 4))
+    tgt = rng.normal(size=(2, 2, nz))
+    w = rng.uniform(0.2, 1.0, (2, 2, nz))
+    _, df, dL = R.df_loss_seed(FEs, tgt, -0.1, w)
+    g = w * (df - tgt) / w.sum()
+    exact = g @ DF.central_matrix(nz, -0.1)
+    diff = dL[..., 2] - exact
+    c = 1 / (2 * 0.1)
+    np.testing.assert_allclose(diff[..., 1], -c * g[..., 0], atol=1e-12)
+    np.testing.assert_allclose(diff[..., nz - 2], c * g[..., nz - 1], atol=1e-12)
+    diff[..., [1, nz - 2]] = 0
+    assert np.abs(diff).max() < 1e-12
+    np.testing.assert_allclose(g @ DF.central_seed_matrix(nz, -0.1).T, dL[..., 2], atol=1e-12)
+
+
+def test_giessibl_continuous_limits():
+    f0, k = 30000.0, 1800.0
+    for A in (0.1, 1.0, 3.0):
+        val = R.giessibl_df_continuous(lambda z: 0.7 - 0.25 * z, np.array([2.0, 5.0]), A, f0, k)
+        np.testing.assert_allclose(val, -(f0 / (2 * k)) * (-0.25) * C, rtol=1e-10)
+    A = 1e-3
+    val = R.giessibl_df_continuous(lambda z: np.exp(-z), np.array([1.5]), A, f0, k)
+    np.testing.assert_allclose(val, -(f0 / (2 * k)) * (-np.exp(-1.5)) * C, rtol=1e-6)
+
+
+def test_giessibl_exponential_closed_form():
+    """F = F0 exp(-z/l): int_{-1}^{1} e^{-A u/l} u/sqrt(1-u^2) du = -pi I1(A/l)."""
+    from scipy.special import iv
+    f0, k, F0, lam, A, zc = 30000.0, 1800.0, 0.8, 0.6, 1.0, 4.0
+    val = R.giessibl_df_continuous(lambda z: F0 * np.exp(-z / lam), np.array([zc]), A, f0, k)
+    ref = -(f0 / (np.pi * k * A)) * C * F0 * np.exp(-zc / lam) * (-np.pi * iv(1, A / lam))
+    np.testing.assert_allclose(val, ref, rtol=1e-9)
+
+
+# ── host-built operators (spammm_amd.df) vs the oracle functions ──────────────
+
+def test_host_gradient_and_amp9_operators_match_oracle():
+    rng = np.random.default_rng(5)
+    Fz = rng.normal(size=(3, 4, 33))
+    for dz in (0.1, -0.1, 0.07):
+        np.testing.assert_allclose(Fz @ (-DF.gradient_matrix(33, abs(dz))).T, R.compute_df(Fz, dz), atol=1e-12)
+        for amp in (1.0, 0.35, 0.05):
+            np.testing.assert_allclose(Fz @ DF.amp9_matrix(33, dz, amp).T, R.compute_df_amp(Fz, dz, amp),
+                                       rtol=1e-5, atol=1e-5)
+
+
+def test_host_central_operator_matches_df_loss_seed():
+    rng = np.random.default_rng(6)
+    FEs = rng.normal(size=(2, 3, 10, 4))
+    _, df, _ = R.df_loss_seed(FEs, np.zeros((2, 3, 10)), -0.2)
+    np.testing.assert_allclose(FEs[..., 2] @ DF.central_matrix(10, -0.2).T, df, atol=1e-12)
+
+
+@pytest.mark.parametrize("dz,amp", [(-0.1, 1.0), (0.1, 1.0), (-0.1, 0.33), (-0.05, 2.0)])
+def test_host_giessibl_operator_matches_quadrature(dz, amp):
+    rng = np.random.default_rng(8)
+    nz = 40
+    z = 6.0 + dz * np.arange(nz)
+    Fz = 0.5 * np.exp(-(z - 2.5) / 0.7) + 0.02 * rng.normal(size=(2, nz))
+    M = DF.giessibl_matrix(nz, dz, amp, 30000.0, 1800.0)
+    ref = R.giessibl_df_piecewise(Fz, z, amp, 30000.0, 1800.0)
+    np.testing.assert_allclose(Fz @ M.T, ref, rtol=1e-8, atol=1e-8 * np.abs(ref).max())
+
+
+def test_host_giessibl_linear_force_exact_in_interior():
+    nz, dz, amp = 50, -0.1, 1.0
+    z = 6.0 + dz * np.arange(nz)
+    Fz = 0.3 - 0.2 * z
+    df = DF.giessibl_matrix(nz, dz, amp, 30000.0, 1800.0) @ Fz
+    inner = slice(11, nz - 11)
+    np.testing.assert_allclose(df[inner], -(30000.0 / 3600.0) * (-0.2) * C, rtol=1e-12)
+
+
+def test_band_compression_roundtrip():
+    M = DF.giessibl_matrix(30, -0.1, 1.0, 30000.0, 1800.0)
+    start, W = DF.band(M)
+    assert (start >= 0).all() and (start + W.shape[1] <= 30).all()
+    R2 = np.zeros_like(M)
+    for j in range(30):
+        R2[j, start[j]:start[j] + W.shape[1]] = W[j]
+    np.testing.assert_array_equal(R2, M)
