@@ -24,6 +24,8 @@
 #ifndef PPM_API_H
 #define PPM_API_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -126,6 +128,37 @@ int ppm_fe_in_strokes_tilted(const float* grid, const PpmGridSpec* spec,
                              const float* points, int n_scan, int nz,
                              const PpmTip* tip, const float dTip[4], const float dpos0[4],
                              float* FEs, void* stream);
+
+/* Implicit-function adjoint of relaxed states (no counterpart on the reference's grid path;
+ * it plays the role of pass 1 of the direct-Morse VJP, morseDirectStateAdjoint,
+ * kernels/AFM.cl:1711-1845, for a trilinear-sampled grid).
+ * At each stored probe position PPs[s].xyz the total force vanishes; for the upstream
+ * cotangent dL_dFEs[s] (float4, w.r.t. the tip-rotated FE output of
+ * ppm_relax_strokes_tilted) it solves J^T l = (dO/dq)^T u with the 3x3 Jacobian J of the
+ * total force (FP64) and ADDS the resulting cotangent of every touched grid value into
+ * grid_cot ((nz,ny,nx,4), caller-zeroed; float atomics).  lambdas (n_scan,nz,4): l and a
+ * status word in .w (bit 1: state not converged, 2: non-finite, 4: singular J; states with
+ * 2 or 4 add nothing).  points/tip must be those of the forward call. */
+int ppm_relax_adjoint(const float* grid, const PpmGridSpec* spec,
+                      const float* points, int n_scan, int nz, const PpmTip* tip,
+                      const float* PPs, const float* dL_dFEs,
+                      float* grid_cot, float* lambdas, void* stream);
+
+/* Bytes of device workspace ppm_grid_adjoint needs for na atoms on this grid (0 on bad input). */
+size_t ppm_grid_adjoint_workspace_size(int na, const PpmGridSpec* spec);
+
+/* Grid -> atom adjoint of ppm_grid_project (no reference counterpart; the reference's only
+ * adjoint is the gridless morseDirectParamPartials/reduce pair, kernels/AFM.cl:1850-2029).
+ * Given grid_cot = dL/dgrid, writes grad (na,6) FP64 = dL/d(x, y, z, q, p0, p1) per atom,
+ * with (p0,p1) = (c6,c12) for PPM_MODEL_LJ and (R0,E0) for PPM_MODEL_MORSE, summed over
+ * periodic images.  grid is the forward output (used to chain through the |E|>100
+ * rescale); the other arguments are those of the forward call.  workspace: device memory
+ * of at least ppm_grid_adjoint_workspace_size(na, spec) bytes. */
+int ppm_grid_adjoint(int model, const float* atoms, const float* cpar, int na,
+                     const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
+                     int npbc_a, int npbc_b, const float lvA[3], const float lvB[3], int cap,
+                     const float* grid, const float* grid_cot, void* workspace,
+                     size_t workspace_bytes, double* grad, void* stream);
 
 /* Linear operator along z, one output per (pixel, z_out):
  *   out[p*nz_out + j] = sum_{t<taps} W[j*taps + t] * in[(p*nz_in + start[j] + t)*in_stride + in_offset]

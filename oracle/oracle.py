@@ -84,6 +84,10 @@ def lib():
                                        _f, _f, _f]
         for nm in ("oracle_fe_in_strokes_tilted_f64", "oracle_fe_in_strokes_tilted_f32"):
             getattr(L, nm).argtypes = [_f, _i, _f, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, _f, _f]
+        L.oracle_grid_project_f64g.argtypes = [c_int, _d, _d, c_int, c_int, _d, _d, c_int, c_int, _d, _d, _i,
+                                               _d, _d, _d, _d, c_int, _d]
+        L.oracle_relax_tilted_f64g.argtypes = [_d, _i, _d, _d, _d, _d, c_int, c_int, _d, _d, _d, _d, _d, _d, _d,
+                                               c_int, ctypes.c_double, c_int, _d, _d]
         _lib = L
     return _lib
 
@@ -234,3 +238,45 @@ def fe_in_strokes_tilted(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC
     fn(_p(g, _f), _p(nn, _i), _p(a, _f), _p(b, _f), _p(c, _f), _p(pts, _f), ns, int(nz), *[_p(v, _f) for v in vs],
        _p(FEs, _f))
     return FEs
+
+
+# ── FP64 end to end (double inputs, double grid) ─────────────────────────────────
+
+def _dvec(a, n):
+    out = np.zeros(n, dtype=np.float64)
+    a = np.asarray(a, dtype=np.float64).ravel()
+    out[: min(n, a.size)] = a[: min(n, a.size)]
+    return out
+
+
+def grid_project_d(model, atoms, cpar, n, p0, dA, dB, dC, Qs, QZs, npbc=(0, 0), lvA=(0, 0, 0), lvB=(0, 0, 0),
+                   cap=True):
+    """Grid (nz,ny,nx,4) float64 from float64 atoms/params (every operation in double)."""
+    atoms = _cd(atoms).reshape(-1, 4)
+    cpar = _cd(cpar)
+    nn = np.asarray(n, dtype=np.int32).ravel()[:3].copy()
+    out = np.zeros((int(nn[2]), int(nn[1]), int(nn[0]), 4), dtype=np.float64)
+    g = [_dvec(x, 3) for x in (p0, dA, dB, dC)]
+    Qs, QZs, lvA, lvB = _dvec(Qs, 4), _dvec(QZs, 4), _dvec(lvA, 3), _dvec(lvB, 3)
+    lib().oracle_grid_project_f64g(int(model), _p(atoms, _d), _p(cpar, _d), cpar.shape[1], atoms.shape[0],
+                                   _p(Qs, _d), _p(QZs, _d), int(npbc[0]), int(npbc[1]), _p(lvA, _d), _p(lvB, _d),
+                                   _p(nn, _i), *[_p(x, _d) for x in g], int(bool(cap)), _p(out, _d))
+    return out
+
+
+def relax_tilted_d(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF,
+                   max_iter=128, f2conv=1e-8, check_before=False):
+    """relaxStrokesTilted in double on a double grid: (FEs, PPs) float64 (n_scan,nz,4)."""
+    grid = _cd(grid)
+    nn = np.asarray(n, dtype=np.int32).ravel()[:3].copy()
+    assert grid.size == 4 * int(np.prod(nn.astype(np.int64)))
+    pts = _cd(pts).reshape(-1, 4)
+    ns = pts.shape[0]
+    FEs = np.zeros((ns, nz, 4))
+    PPs = np.zeros((ns, nz, 4))
+    vs = [_dvec(x, 4) for x in (dinvA, dinvB, dinvC)]
+    ts = [_dvec(x, 4) for x in (tipA, tipB, tipC, stiffness, dpos0, relax, surfFF)]
+    lib().oracle_relax_tilted_f64g(_p(grid, _d), _p(nn, _i), *[_p(v, _d) for v in vs], _p(pts, _d), ns, int(nz),
+                                   *[_p(t, _d) for t in ts], int(max_iter), float(f2conv), int(bool(check_before)),
+                                   _p(FEs, _d), _p(PPs, _d))
+    return FEs, PPs

@@ -54,10 +54,10 @@ template <class R> inline V3<R> rotMat(V3<R> v, V3<R> a, V3<R> b, V3<R> c) {
 template <class R> inline V3<R> rotMatT(V3<R> v, V3<R> a, V3<R> b, V3<R> c) {
     return add(add(mul(a, v.x), mul(b, v.y)), mul(c, v.z));
 }
-template <class R> inline V3<R> ld3(const float* p) { return v3<R>((R)p[0], (R)p[1], (R)p[2]); }
+template <class R, class P> inline V3<R> ld3(const P* p) { return v3<R>((R)p[0], (R)p[1], (R)p[2]); }
 
 // tipForce   kernels/AFM.cl:53-58
-template <class R> inline V3<R> tipForce(V3<R> dpos, const float* k, const float* d0) {
+template <class R, class P> inline V3<R> tipForce(V3<R> dpos, const P* k, const P* d0) {
     R r = std::sqrt(dot(dpos, dpos));
     r = std::max(r, (R)1e-10f);
     V3<R> a = mulv(sub(dpos, ld3<R>(d0)), ld3<R>(k));
@@ -66,11 +66,12 @@ template <class R> inline V3<R> tipForce(V3<R> dpos, const float* k, const float
 }
 
 // Grid + sampling description shared by the relax functions.
-struct Grid {
-    const float* data;   // (nz,ny,nx,4) float, x fastest
+template <class GT> struct GridT {
+    const GT* data;      // (nz,ny,nx,4), x fastest
     int n[3];
-    float dinv[3][4];    // world -> normalized coordinate rows (.w carries -origin/L + 0.5/n)
+    double dinv[3][4];   // world -> normalized coordinate rows (.w carries -origin/L + 0.5/n)
 };
+using Grid = GridT<float>;
 
 // One axis of the OpenCL linear-filter REPEAT addressing (OpenCL 1.2 spec, 8.2):
 //   u = (s - floor(s))*n;  i0 = floor(u-0.5);  a = frac(u-0.5);  i1 = i0+1;
@@ -95,7 +96,7 @@ template <class R> struct FE { R x, y, z, w; };
 
 // interpFE  kernels/AFM.cl:100-105 (coordinate = dot((pos,1), dinv)); trilinear in the
 // nested order of read_imagef_trilin_  kernels/AFM.cl:81-97.
-template <class R> inline FE<R> interp(const Grid& g, V3<R> p) {
+template <class R, class GT> inline FE<R> interp(const GridT<GT>& g, V3<R> p) {
     R s[3];
     for (int k = 0; k < 3; ++k)
         s[k] = p.x * (R)g.dinv[k][0] + p.y * (R)g.dinv[k][1] + p.z * (R)g.dinv[k][2] + (R)g.dinv[k][3];
@@ -106,9 +107,9 @@ template <class R> inline FE<R> interp(const Grid& g, V3<R> p) {
     axis_repeat<R>(s[2], g.n[2], k0, k1, c);
     const R ma = (R)1.0f - a, mb = (R)1.0f - b, mc = (R)1.0f - c;
     const int nx = g.n[0], ny = g.n[1];
-    auto T = [&](int i, int j, int k) -> const float* { return g.data + 4 * (((size_t)k * ny + j) * nx + i); };
-    const float *c000 = T(i0, j0, k0), *c100 = T(i1, j0, k0), *c010 = T(i0, j1, k0), *c110 = T(i1, j1, k0);
-    const float *c001 = T(i0, j0, k1), *c101 = T(i1, j0, k1), *c011 = T(i0, j1, k1), *c111 = T(i1, j1, k1);
+    auto T = [&](int i, int j, int k) -> const GT* { return g.data + 4 * (((size_t)k * ny + j) * nx + i); };
+    const GT *c000 = T(i0, j0, k0), *c100 = T(i1, j0, k0), *c010 = T(i0, j1, k0), *c110 = T(i1, j1, k0);
+    const GT *c001 = T(i0, j0, k1), *c101 = T(i1, j0, k1), *c011 = T(i0, j1, k1), *c111 = T(i1, j1, k1);
     R o[4];
     for (int q = 0; q < 4; ++q) {
         R lo = ((R)c000[q] * ma + (R)c100[q] * a) * mb + ((R)c010[q] * ma + (R)c110[q] * a) * b;
@@ -140,10 +141,10 @@ template <class R> inline V3<R> update_FIRE(V3<R> f, V3<R> v, R& dt, R& damp, R 
 // convergence / -iters otherwise) in the convention of relaxStrokesTiltedMorseDirect
 // (kernels/AFM.cl:1494-1505).  check_before=1 moves the |f|^2 test before the update
 // as relaxStrokesTiltedMorseDirect does (kernels/AFM.cl:1478).
-template <class R>
-void relax_tilted(const Grid& g, const float* pts, int n_scan, int nz, const float* tipA, const float* tipB,
-                  const float* tipC, const float* stiffness, const float* dpos0, const float* relax,
-                  const float* surfFF, int max_iter, float f2conv_f, int check_before, float* FEs, float* PPs) {
+template <class R, class GT, class P>
+void relax_tilted(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P* tipA, const P* tipB, const P* tipC,
+                  const P* stiffness, const P* dpos0, const P* relax, const P* surfFF, int max_iter, double f2conv_f,
+                  int check_before, P* FEs, P* PPs) {
     const V3<R> A = ld3<R>(tipA), B = ld3<R>(tipB), C = ld3<R>(tipC);
     const R f2conv = (R)f2conv_f;
 #pragma omp parallel for schedule(dynamic, 8)
@@ -177,12 +178,12 @@ void relax_tilted(const Grid& g, const float* pts, int n_scan, int nz, const flo
             FE<R> fe = interp<R>(g, pos);
             V3<R> fr = rotMat(v3<R>(fe.x, fe.y, fe.z), A, B, C);
             const size_t o = 4 * ((size_t)ip * nz + iz);
-            FEs[o + 0] = (float)fr.x; FEs[o + 1] = (float)fr.y; FEs[o + 2] = (float)fr.z; FEs[o + 3] = (float)fe.w;
+            FEs[o + 0] = (P)fr.x; FEs[o + 1] = (P)fr.y; FEs[o + 2] = (P)fr.z; FEs[o + 3] = (P)fe.w;
             bool finite = std::isfinite(fe.x) && std::isfinite(fe.y) && std::isfinite(fe.z) && std::isfinite(fe.w) &&
                           std::isfinite(pos.x) && std::isfinite(pos.y) && std::isfinite(pos.z);
             if (PPs) {
-                PPs[o + 0] = (float)pos.x; PPs[o + 1] = (float)pos.y; PPs[o + 2] = (float)pos.z;
-                PPs[o + 3] = (float)((conv && finite) ? iters : -iters);
+                PPs[o + 0] = (P)pos.x; PPs[o + 1] = (P)pos.y; PPs[o + 2] = (P)pos.z;
+                PPs[o + 3] = (P)((conv && finite) ? iters : -iters);
             }
             tipPos = add(tipPos, dTip);
             pos = add(pos, dTip);
@@ -192,10 +193,9 @@ void relax_tilted(const Grid& g, const float* pts, int n_scan, int nz, const flo
 
 // relaxStrokes   kernels/AFM.cl:345-394.  FE output is the sample taken at the start
 // of the last iteration (before the final update); disps = pos - (tipPos + dpos0).
-template <class R>
-void relax_strokes(const Grid& g, const float* pts, int n_scan, int nz, const float* dTip_, const float* stiffness,
-                   const float* dpos0, const float* relax, int max_iter, float f2conv_f, float* FEs, float* disps,
-                   float* PPs) {
+template <class R, class GT, class P>
+void relax_strokes(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P* dTip_, const P* stiffness,
+                   const P* dpos0, const P* relax, int max_iter, double f2conv_f, P* FEs, P* disps, P* PPs) {
     const R f2conv = (R)f2conv_f;
 #pragma omp parallel for schedule(dynamic, 8)
     for (int ip = 0; ip < n_scan; ++ip) {
@@ -222,16 +222,16 @@ void relax_strokes(const Grid& g, const float* pts, int n_scan, int nz, const fl
             }
             int iters = conv ? it + 1 : max_iter;
             const size_t o = 4 * ((size_t)ip * nz + iz);
-            FEs[o + 0] = (float)fe.x; FEs[o + 1] = (float)fe.y; FEs[o + 2] = (float)fe.z; FEs[o + 3] = (float)fe.w;
+            FEs[o + 0] = (P)fe.x; FEs[o + 1] = (P)fe.y; FEs[o + 2] = (P)fe.z; FEs[o + 3] = (P)fe.w;
             V3<R> dd = sub(pos, add(tipPos, d0));
             if (disps) {
-                disps[o + 0] = (float)dd.x; disps[o + 1] = (float)dd.y; disps[o + 2] = (float)dd.z; disps[o + 3] = 0.f;
+                disps[o + 0] = (P)dd.x; disps[o + 1] = (P)dd.y; disps[o + 2] = (P)dd.z; disps[o + 3] = 0.f;
             }
             bool finite = std::isfinite(fe.x) && std::isfinite(fe.y) && std::isfinite(fe.z) && std::isfinite(fe.w) &&
                           std::isfinite(pos.x) && std::isfinite(pos.y) && std::isfinite(pos.z);
             if (PPs) {
-                PPs[o + 0] = (float)pos.x; PPs[o + 1] = (float)pos.y; PPs[o + 2] = (float)pos.z;
-                PPs[o + 3] = (float)((conv && finite) ? iters : -iters);
+                PPs[o + 0] = (P)pos.x; PPs[o + 1] = (P)pos.y; PPs[o + 2] = (P)pos.z;
+                PPs[o + 3] = (P)((conv && finite) ? iters : -iters);
             }
             tipPos = add(tipPos, dTip);
             pos = add(pos, dTip);
@@ -240,9 +240,9 @@ void relax_strokes(const Grid& g, const float* pts, int n_scan, int nz, const fl
 }
 
 // getFEinStrokesTilted   kernels/AFM.cl:210-233 (dpos0 is NOT rotated here).
-template <class R>
-void fe_in_strokes_tilted(const Grid& g, const float* pts, int n_scan, int nz, const float* tipA, const float* tipB,
-                          const float* tipC, const float* dTip_, const float* dpos0, float* FEs) {
+template <class R, class GT, class P>
+void fe_in_strokes_tilted(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P* tipA, const P* tipB,
+                          const P* tipC, const P* dTip_, const P* dpos0, P* FEs) {
     const V3<R> A = ld3<R>(tipA), B = ld3<R>(tipB), C = ld3<R>(tipC);
 #pragma omp parallel for schedule(static)
     for (int ip = 0; ip < n_scan; ++ip) {
@@ -252,7 +252,7 @@ void fe_in_strokes_tilted(const Grid& g, const float* pts, int n_scan, int nz, c
             FE<R> fe = interp<R>(g, pos);
             V3<R> fr = rotMat(v3<R>(fe.x, fe.y, fe.z), A, B, C);
             const size_t o = 4 * ((size_t)ip * nz + iz);
-            FEs[o + 0] = (float)fr.x; FEs[o + 1] = (float)fr.y; FEs[o + 2] = (float)fr.z; FEs[o + 3] = (float)fe.w;
+            FEs[o + 0] = (P)fr.x; FEs[o + 1] = (P)fr.y; FEs[o + 2] = (P)fr.z; FEs[o + 3] = (P)fe.w;
             pos = add(pos, dTip);
         }
     }
@@ -292,25 +292,26 @@ inline void pair_atom(int model, V3<R> p, V3<R> a, R q, const R* cp, const R* Qs
     }
 }
 
-struct Atoms {
+template <class P> struct AtomsT {
     int model;
     int na;
-    const float* atoms;  // na*4 (x,y,z,q)
-    const float* cpar;   // na*cstride
+    const P* atoms;  // na*4 (x,y,z,q)
+    const P* cpar;   // na*cstride
     int cstride;
-    float Qs[4], QZs[4];
+    double Qs[4], QZs[4];
     int npbc[2];
-    float lvA[3], lvB[3];
+    double lvA[3], lvB[3];
 };
+using Atoms = AtomsT<float>;
 
 // Total sample FE at world point p: sum over atoms, periodic images
 // (ix in [-npbc0,npbc0], iy in [-npbc1,npbc1], atom shifted by ix*lvA + iy*lvB).
-template <class R> inline void sample_fe(const Atoms& S, V3<R> p, R acc[4]) {
+template <class R, class P> inline void sample_fe(const AtomsT<P>& S, V3<R> p, R acc[4]) {
     R Qs[4], QZs[4];
     for (int k = 0; k < 4; ++k) { Qs[k] = (R)S.Qs[k] * (R)kCOULOMB; QZs[k] = (R)S.QZs[k]; }
     acc[0] = acc[1] = acc[2] = acc[3] = 0;
     for (int j = 0; j < S.na; ++j) {
-        const float* at = S.atoms + 4 * (size_t)j;
+        const P* at = S.atoms + 4 * (size_t)j;
         R cp[4] = {0, 0, 0, 0};
         for (int c = 0; c < S.cstride && c < 4; ++c) cp[c] = (R)S.cpar[(size_t)j * S.cstride + c];
         for (int iy = -S.npbc[1]; iy <= S.npbc[1]; ++iy)
@@ -479,5 +480,46 @@ RELAX_STROKES(oracle_relax_strokes_f32, float)
     }
 FE_STROKES(oracle_fe_in_strokes_tilted_f64, double)
 FE_STROKES(oracle_fe_in_strokes_tilted_f32, float)
+
+// Fully double-precision variants (double inputs, double grid, double outputs): the
+// reference forward in FP64 end to end, used for finite-difference gradients.
+void oracle_grid_project_f64g(int model, const double* atoms, const double* cpar, int cstride, int na,
+                              const double* Qs, const double* QZs, int npbc_a, int npbc_b, const double* lvA,
+                              const double* lvB, const int* n, const double* p0, const double* dA, const double* dB,
+                              const double* dC, int cap, double* out) {
+    AtomsT<double> S{model, na, atoms, cpar, cstride, {}, {}, {npbc_a, npbc_b}, {}, {}};
+    for (int k = 0; k < 4; ++k) { S.Qs[k] = Qs[k]; S.QZs[k] = QZs[k]; }
+    for (int k = 0; k < 3; ++k) { S.lvA[k] = lvA[k]; S.lvB[k] = lvB[k]; }
+    const long nx = n[0], ny = n[1], nzz = n[2];
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+    for (long ic = 0; ic < nzz; ++ic)
+        for (long ib = 0; ib < ny; ++ib)
+            for (long ia = 0; ia < nx; ++ia) {
+                V3<double> p = v3<double>(p0[0] + dA[0] * ia + dB[0] * ib + dC[0] * ic,
+                                          p0[1] + dA[1] * ia + dB[1] * ib + dC[1] * ic,
+                                          p0[2] + dA[2] * ia + dB[2] * ib + dC[2] * ic);
+                double acc[4];
+                sample_fe<double>(S, p, acc);
+                if (cap) {
+                    double renorm = 100.0 / std::fabs(acc[3]);
+                    if (renorm < 1.0) for (int q = 0; q < 4; ++q) acc[q] *= renorm;
+                }
+                double* o = out + 4 * ((ic * ny + ib) * nx + ia);
+                for (int q = 0; q < 4; ++q) o[q] = acc[q];
+            }
+}
+
+void oracle_relax_tilted_f64g(const double* grid, const int* n, const double* dinvA, const double* dinvB,
+                              const double* dinvC, const double* pts, int n_scan, int nz, const double* tipA,
+                              const double* tipB, const double* tipC, const double* stiffness, const double* dpos0,
+                              const double* relax, const double* surfFF, int max_iter, double f2conv,
+                              int check_before, double* FEs, double* PPs) {
+    GridT<double> g;
+    g.data = grid;
+    for (int k = 0; k < 3; ++k) g.n[k] = n[k];
+    for (int q = 0; q < 4; ++q) { g.dinv[0][q] = dinvA[q]; g.dinv[1][q] = dinvB[q]; g.dinv[2][q] = dinvC[q]; }
+    relax_tilted<double>(g, pts, n_scan, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF, max_iter, f2conv,
+                         check_before, FEs, PPs);
+}
 
 }  // extern "C"

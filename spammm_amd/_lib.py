@@ -39,7 +39,8 @@ class PpmRelaxCtl(ctypes.Structure):
 
 EXPORTS = (
     "ppm_version", "ppm_last_error", "ppm_grid_project", "ppm_relax_strokes_tilted", "ppm_relax_strokes",
-    "ppm_fe_in_strokes_tilted", "ppm_z_stencil", "ppm_axis_operator",
+    "ppm_fe_in_strokes_tilted", "ppm_z_stencil", "ppm_axis_operator", "ppm_relax_adjoint",
+    "ppm_grid_adjoint_workspace_size", "ppm_grid_adjoint",
 )
 
 _lib = None
@@ -81,6 +82,13 @@ def load():
     ll = ctypes.c_longlong
     L.ppm_axis_operator.restype = i
     L.ppm_axis_operator.argtypes = [vp, ll, ll, ll, _F4, i, i, i, i, vp, vp, i, i, f, i, vp, vp]
+    L.ppm_relax_adjoint.restype = i
+    L.ppm_relax_adjoint.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), vp, vp, vp, vp, vp]
+    L.ppm_grid_adjoint_workspace_size.restype = ctypes.c_size_t
+    L.ppm_grid_adjoint_workspace_size.argtypes = [i, P(PpmGridSpec)]
+    L.ppm_grid_adjoint.restype = i
+    L.ppm_grid_adjoint.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
+                                   P(ctypes.c_float * 3), i, vp, vp, vp, ctypes.c_size_t, vp, vp]
     _lib = L
     return L
 

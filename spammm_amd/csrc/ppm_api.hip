@@ -138,6 +138,47 @@ int ppm_fe_in_strokes_tilted(const float* grid, const PpmGridSpec* spec, const f
     return launch_fe_in_strokes(grid, *spec, points, n_scan, nz, *tip, dTip, dpos0, FEs, (hipStream_t)stream);
 }
 
+int ppm_relax_adjoint(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
+                      const PpmTip* tip, const float* PPs, const float* dL_dFEs, float* grid_cot, float* lambdas,
+                      void* stream) {
+    const char* fn = "ppm_relax_adjoint";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (int r = check_scan(fn, points, n_scan, nz, PPs)) return r;
+    if (!grid || !tip || !dL_dFEs || !grid_cot || !lambdas) { set_error("%s: NULL argument", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_relax_adjoint(grid, *spec, points, n_scan, nz, *tip, PPs, dL_dFEs, grid_cot, lambdas,
+                                (hipStream_t)stream);
+}
+
+size_t ppm_grid_adjoint_workspace_size(int na, const PpmGridSpec* spec) {
+    if (!spec || na < 1 || check_spec("ppm_grid_adjoint_workspace_size", spec) != PPM_OK) return 0;
+    return grid_adjoint_workspace(na, *spec);
+}
+
+int ppm_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
+                     const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                     const float lvB[3], int cap, const float* grid, const float* grid_cot, void* workspace,
+                     size_t workspace_bytes, double* grad, void* stream) {
+    const char* fn = "ppm_grid_adjoint";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (model != PPM_MODEL_LJ && model != PPM_MODEL_MORSE) { set_error("%s: unknown model %d", fn, model); return PPM_ERR_INVALID_ARG; }
+    if (na < 1 || na > (1 << 20)) { set_error("%s: invalid atom count %d", fn, na); return PPM_ERR_INVALID_ARG; }
+    if (!atoms || !cpar || !grid || !grid_cot || !grad || !workspace || !Qs || !QZs) {
+        set_error("%s: NULL pointer argument", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (npbc_a < 0 || npbc_b < 0 || npbc_a > 64 || npbc_b > 64) { set_error("%s: image range out of [0,64]", fn); return PPM_ERR_INVALID_ARG; }
+    if ((npbc_a > 0 && !lvA) || (npbc_b > 0 && !lvB)) { set_error("%s: images requested without lattice vectors", fn); return PPM_ERR_INVALID_ARG; }
+    const size_t need = grid_adjoint_workspace(na, *spec);
+    if (workspace_bytes < need) {
+        set_error("%s: workspace of %zu bytes < required %zu (ppm_grid_adjoint_workspace_size)", fn, workspace_bytes, need);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return launch_grid_adjoint(model, atoms, cpar, na, *spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, grid, grid_cot,
+                               workspace, grad, (hipStream_t)stream);
+}
+
 int ppm_z_stencil(const float* in, int in_stride, int in_offset, int n_pix, int nz_in, const float* W,
                   const int* start, int taps, int nz_out, float* out, void* stream) {
     const char* fn = "ppm_z_stencil";

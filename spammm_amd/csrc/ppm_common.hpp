@@ -39,6 +39,14 @@ int launch_relax_tilted(const float* grid, const PpmGridSpec& g, const float* pt
 int launch_relax_strokes(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
                          const float dTip[4], const float stiffness[4], const float dpos0[4], const float relax[4],
                          const PpmRelaxCtl& ctl, float* FEs, float* disps, float* PPs, hipStream_t stream);
+int launch_relax_adjoint(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
+                         const PpmTip& tip, const float* PPs, const float* dL_dFEs, float* grid_cot, float* lambdas,
+                         hipStream_t stream);
+size_t grid_adjoint_workspace(int na, const PpmGridSpec& g);
+int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
+                        const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                        const float lvB[3], int cap, const float* grid, const float* grid_cot, void* workspace,
+                        double* grad, hipStream_t stream);
 int launch_fe_in_strokes(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
                          const PpmTip& tip, const float dTip[4], const float dpos0[4], float* FEs,
                          hipStream_t stream);

@@ -174,6 +174,236 @@ __global__ __launch_bounds__(256) void k_grid_project(const float4* __restrict__
     }
 }
 
+// ── grid -> atom adjoint ─────────────────────────────────────────────────────────────
+//
+// Given the cotangent g = dL/dG of every voxel (float4, same layout as the grid), accumulates
+// per atom j:  dL/d(x,y,z)_j, dL/dq_j and dL/d(p0,p1)_j with (p0,p1) = (c6,c12) for LJ and
+// (R0,E0) for Morse, summed over voxels and periodic images.  For one pair with
+// d = v - a, s = |d|^2 + R2SAFE, F = fr(s) d, E = E(s) (dE/ds = -fr/2):
+//   dL/da = -[ fr gF + (2 (gF.d) dfr/ds - gE fr) d ]
+// and the parameter derivatives follow from the pair law.  Capped voxels (|E| = 100 after
+// the rescale) are differentiated through the rescale: g_raw = s g - s (g.fe_raw)/E_raw e_w,
+// s = 100/|E_raw|, with fe_raw recomputed for the workgroup.
+// Workgroups whose voxels all have zero cotangent exit at once (flag 0); the others write
+// one partial sum per atom, reduced over workgroups by k_reduce_partials in FP64.
+template <int MODEL>
+__device__ __forceinline__ void pair_fe_raw(float dx, float dy, float dz, float dxy2, float q, float p0, float p1,
+                                            float K, const GridArgs& a, float4& fe) {
+    const float r2 = fmaf(dz, dz, dxy2);
+    const float ir = rsq(r2), ir2 = ir * ir;
+    float fr, e;
+    if (MODEL == PPM_MODEL_LJ) {
+        const float ir6 = ir2 * ir2 * ir2;
+        e = fmaf(p1, ir6, -p0) * ir6;
+        fr = fmaf(12.f * p1, ir6, -6.f * p0) * ir6 * ir2;
+    } else {
+        const float r = r2 * ir;
+        const float ex = __expf(K * (r - p0));
+        e = p1 * ex * (ex - 2.f);
+        fr = -2.f * K * p1 * ex * (ex - 1.f) * ir;
+    }
+    const float ec = q * a.q0 * ir;
+    fr = fmaf(ec, ir2, fr);
+    e += ec;
+    fe.x = fmaf(fr, dx, fe.x); fe.y = fmaf(fr, dy, fe.y); fe.z = fmaf(fr, dz, fe.z); fe.w += e;
+    for (int o = 0; o < a.noff; ++o) {
+        const float dzo = dz + a.zoff[o];
+        const float iro = rsq(fmaf(dzo, dzo, dxy2));
+        const float eo = q * a.qoff[o] * iro;
+        const float fro = eo * iro * iro;
+        fe.x = fmaf(fro, dx, fe.x); fe.y = fmaf(fro, dy, fe.y); fe.z = fmaf(fro, dzo, fe.z); fe.w += eo;
+    }
+}
+
+// contributions of one pair to the 6 per-atom accumulators (x,y,z,q,p0,p1)
+template <int MODEL>
+__device__ __forceinline__ void pair_adjoint(float dx, float dy, float dz, float dxy2, float q, float p0, float p1,
+                                             float K, const GridArgs& a, float4 g, float acc[6]) {
+    const float s = fmaf(dz, dz, dxy2);
+    const float ir = rsq(s), ir2 = ir * ir;
+    const float gd = g.x * dx + g.y * dy + g.z * dz;
+    float fr, dfr, dp0, dp1;
+    if (MODEL == PPM_MODEL_LJ) {
+        const float ir6 = ir2 * ir2 * ir2, ir8 = ir6 * ir2;
+        fr = (12.f * p1 * ir6 - 6.f * p0) * ir8;
+        dfr = (-84.f * p1 * ir6 + 24.f * p0) * ir8 * ir2;
+        dp0 = -ir6 * (6.f * gd * ir2 + g.w);              // d/dc6
+        dp1 = ir6 * ir6 * (12.f * gd * ir2 + g.w);        // d/dc12
+    } else {
+        const float r = s * ir;
+        const float ex = __expf(K * (r - p0));
+        const float A = p1 * ex;
+        fr = -2.f * K * A * (ex - 1.f) * ir;
+        // d fr / d r, then / (2 r) for d/ds
+        const float dfr_dr = -2.f * K * A * (K * (2.f * ex - 1.f) * ir - (ex - 1.f) * ir2);
+        dfr = dfr_dr * 0.5f * ir;
+        const float dE_dR0 = -2.f * K * A * (ex - 1.f);
+        const float dfr_dR0 = 2.f * K * K * A * (2.f * ex - 1.f) * ir;
+        dp0 = gd * dfr_dR0 + g.w * dE_dR0;                                   // d/dR0
+        dp1 = gd * (-2.f * K * ex * (ex - 1.f) * ir) + g.w * ex * (ex - 2.f); // d/dE0
+    }
+    float cq = 0.f;
+    if (a.q0 != 0.f) {
+        const float ec = q * a.q0 * ir;          // Coulomb at z offset 0 shares d with the pair law
+        const float frc = ec * ir2;
+        fr += frc;
+        dfr += -1.5f * frc * ir2;
+        cq += a.q0 * ir * (gd * ir2 + g.w);
+    }
+    const float coef = 2.f * gd * dfr - g.w * fr;
+    acc[0] -= fr * g.x + coef * dx;
+    acc[1] -= fr * g.y + coef * dy;
+    acc[2] -= fr * g.z + coef * dz;
+    acc[4] += dp0;
+    acc[5] += dp1;
+    for (int o = 0; o < a.noff; ++o) {
+        const float dzo = dz + a.zoff[o];
+        const float so = fmaf(dzo, dzo, dxy2);
+        const float iro = rsq(so), iro2 = iro * iro;
+        const float gdo = g.x * dx + g.y * dy + g.z * dzo;
+        const float eo = q * a.qoff[o] * iro;
+        const float fro = eo * iro2;
+        const float co = 2.f * gdo * (-1.5f * fro * iro2) - g.w * fro;
+        acc[0] -= fro * g.x + co * dx;
+        acc[1] -= fro * g.y + co * dy;
+        acc[2] -= fro * g.z + co * dzo;
+        cq += a.qoff[o] * iro * (gdo * iro2 + g.w);
+    }
+    acc[3] += cq;
+}
+
+template <int MODEL, int V, bool DCZ>
+__global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__ atoms, const float* __restrict__ cpar,
+                                                      GridArgs a, const float4* __restrict__ grid,
+                                                      const float4* __restrict__ gcot, float* __restrict__ partials,
+                                                      int* __restrict__ active) {
+    __shared__ float red[4][6];
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    const bool colok = col < a.ncol;
+    const int cc = colok ? col : a.ncol - 1;
+    const int ia = cc % a.nx, ib = cc / a.nx;
+    const int ic0 = blockIdx.y * V;
+    float4 g[V];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const bool ok = colok && ic0 + k < a.nz;
+        g[k] = ok ? gcot[(size_t)(ic0 + k) * a.ncol + col] : make_float4(0.f, 0.f, 0.f, 0.f);
+        any |= (g[k].x != 0.f) | (g[k].y != 0.f) | (g[k].z != 0.f) | (g[k].w != 0.f);
+    }
+    if (!__syncthreads_or(any)) {
+        if (threadIdx.x == 0) active[bid] = 0;
+        return;
+    }
+    if (threadIdx.x == 0) active[bid] = 1;
+
+    float px[DCZ ? 1 : V], py[DCZ ? 1 : V], pz[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const float fic = (float)(ic0 + k), fia = (float)ia, fib = (float)ib;
+        pz[k] = a.p0[2] + a.dA[2] * fia + a.dB[2] * fib + a.dC[2] * fic;
+        if (!DCZ || k == 0) {
+            px[k] = a.p0[0] + a.dA[0] * fia + a.dB[0] * fib + a.dC[0] * fic;
+            py[k] = a.p0[1] + a.dA[1] * fia + a.dB[1] * fib + a.dC[1] * fic;
+        }
+    }
+    const int cs = MODEL == PPM_MODEL_LJ ? 2 : 4;
+
+    // voxels clipped by the |E|>100 rescale: recompute the raw sum, chain through the rescale
+    bool capped = false;
+    if (a.cap) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            const bool nzk = (g[k].x != 0.f) | (g[k].y != 0.f) | (g[k].z != 0.f) | (g[k].w != 0.f);
+            if (nzk && colok && ic0 + k < a.nz) {
+                if (fabsf(grid[(size_t)(ic0 + k) * a.ncol + col].w) >= 99.999f) capped = true;
+            }
+        }
+    }
+    if (__syncthreads_or(capped)) {
+        float4 fe[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) fe[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int j = 0; j < a.na; ++j) {
+            const float4 at = atoms[j];
+            const float p0 = cpar[j * cs], p1 = cpar[j * cs + 1], K = MODEL == PPM_MODEL_LJ ? 0.f : cpar[j * cs + 2];
+            for (int iy = -a.npbc_b; iy <= a.npbc_b; ++iy)
+                for (int ix = -a.npbc_a; ix <= a.npbc_a; ++ix) {
+                    const float sx = at.x + ix * a.lvA[0] + iy * a.lvB[0];
+                    const float sy = at.y + ix * a.lvA[1] + iy * a.lvB[1];
+                    const float sz = at.z + ix * a.lvA[2] + iy * a.lvB[2];
+#pragma unroll
+                    for (int k = 0; k < V; ++k) {
+                        const int kk = DCZ ? 0 : k;
+                        const float dx = px[kk] - sx, dy = py[kk] - sy;
+                        pair_fe_raw<MODEL>(dx, dy, pz[k] - sz, fmaf(dx, dx, fmaf(dy, dy, kR2SAFE)), at.w, p0, p1, K,
+                                           a, fe[k]);
+                    }
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            if (fabsf(fe[k].w) > 100.f) {
+                const float sc = 100.f / fabsf(fe[k].w);
+                const float gdf = g[k].x * fe[k].x + g[k].y * fe[k].y + g[k].z * fe[k].z + g[k].w * fe[k].w;
+                g[k].x *= sc; g[k].y *= sc; g[k].z *= sc;
+                g[k].w = sc * g[k].w - sc * gdf / fe[k].w;
+            }
+        }
+    }
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int j = 0; j < a.na; ++j) {
+        const float4 at = atoms[j];
+        const float p0 = cpar[j * cs], p1 = cpar[j * cs + 1], K = MODEL == PPM_MODEL_LJ ? 0.f : cpar[j * cs + 2];
+        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        for (int iy = -a.npbc_b; iy <= a.npbc_b; ++iy)
+            for (int ix = -a.npbc_a; ix <= a.npbc_a; ++ix) {
+                const float sx = at.x + ix * a.lvA[0] + iy * a.lvB[0];
+                const float sy = at.y + ix * a.lvA[1] + iy * a.lvB[1];
+                const float sz = at.z + ix * a.lvA[2] + iy * a.lvB[2];
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    const int kk = DCZ ? 0 : k;
+                    const float dx = px[kk] - sx, dy = py[kk] - sy;
+                    pair_adjoint<MODEL>(dx, dy, pz[k] - sz, fmaf(dx, dx, fmaf(dy, dy, kR2SAFE)), at.w, p0, p1, K,
+                                        a, g[k], acc);
+                }
+            }
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            float v = acc[c];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (lane == 0) red[wave][c] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 6)
+            partials[((size_t)bid * a.na + j) * 6 + threadIdx.x] =
+                red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// grad[j*6 + c] = sum over active workgroups b of partials[(b*na + j)*6 + c], in FP64.
+__global__ __launch_bounds__(256) void k_reduce_partials(const float* __restrict__ partials,
+                                                         const int* __restrict__ active, int nblocks, int na,
+                                                         double* __restrict__ grad) {
+    __shared__ double red[256];
+    const int j = blockIdx.x / 6, c = blockIdx.x - (blockIdx.x / 6) * 6;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256)
+        if (active[b]) acc += (double)partials[((size_t)b * na + j) * 6 + c];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad[j * 6 + c] = red[0];
+}
+
 template <int MODEL, bool HASQ0>
 int dispatch(const float4* atoms, const float* cpar, const GridArgs& a, float4* out, hipStream_t st) {
     const bool dcz = (a.dC[0] == 0.f && a.dC[1] == 0.f);
@@ -195,9 +425,8 @@ int dispatch(const float4* atoms, const float* cpar, const GridArgs& a, float4* 
 
 }  // namespace
 
-int launch_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
-                        const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
-                        const float lvB[3], int cap, float* grid, hipStream_t stream) {
+static GridArgs make_args(int na, const PpmGridSpec& g, const float Qs[4], const float QZs[4], int npbc_a,
+                          int npbc_b, const float lvA[3], const float lvB[3], int cap) {
     GridArgs a{};
     a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.ncol = g.nx * g.ny;
     for (int k = 0; k < 3; ++k) {
@@ -221,6 +450,20 @@ int launch_grid_project(int model, const float* atoms, const float* cpar, int na
             ++a.noff;
         }
     }
+    return a;
+}
+
+// workgroup grid of both the projection and its adjoint
+static dim3 grid_blocks(const PpmGridSpec& g, int V) {
+    return dim3((unsigned)((g.nx * g.ny + 255) / 256), (unsigned)((g.nz + V - 1) / V));
+}
+
+static int adjoint_V(const PpmGridSpec& g) { return (g.dC[0] == 0.f && g.dC[1] == 0.f) ? 8 : 4; }
+
+int launch_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
+                        const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                        const float lvB[3], int cap, float* grid, hipStream_t stream) {
+    const GridArgs a = make_args(na, g, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap);
     const float4* at = reinterpret_cast<const float4*>(atoms);
     float4* o = reinterpret_cast<float4*>(grid);
     const bool hasq0 = a.q0 != 0.f;
@@ -229,6 +472,46 @@ int launch_grid_project(int model, const float* atoms, const float* cpar, int na
                      : dispatch<PPM_MODEL_LJ, false>(at, cpar, a, o, stream);
     return hasq0 ? dispatch<PPM_MODEL_MORSE, true>(at, cpar, a, o, stream)
                  : dispatch<PPM_MODEL_MORSE, false>(at, cpar, a, o, stream);
+}
+
+size_t grid_adjoint_workspace(int na, const PpmGridSpec& g) {
+    const dim3 b = grid_blocks(g, adjoint_V(g));
+    const size_t nb = (size_t)b.x * b.y;
+    return nb * (size_t)na * 6 * sizeof(float) + nb * sizeof(int) + 256;
+}
+
+int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
+                        const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                        const float lvB[3], int cap, const float* grid, const float* grid_cot, void* workspace,
+                        double* grad, hipStream_t stream) {
+    const GridArgs a = make_args(na, g, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap);
+    const int V = adjoint_V(g);
+    const dim3 blocks = grid_blocks(g, V);
+    const size_t nb = (size_t)blocks.x * blocks.y;
+    float* partials = reinterpret_cast<float*>(workspace);
+    int* active = reinterpret_cast<int*>(reinterpret_cast<char*>(workspace) + nb * (size_t)na * 6 * sizeof(float));
+    const float4* at = reinterpret_cast<const float4*>(atoms);
+    const float4* G = reinterpret_cast<const float4*>(grid);
+    const float4* gc = reinterpret_cast<const float4*>(grid_cot);
+    if (model == PPM_MODEL_LJ) {
+        if (V == 8)
+            hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_LJ, 8, true>), blocks, dim3(256), 0, stream, at, cpar, a, G,
+                               gc, partials, active);
+        else
+            hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_LJ, 4, false>), blocks, dim3(256), 0, stream, at, cpar, a,
+                               G, gc, partials, active);
+    } else {
+        if (V == 8)
+            hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_MORSE, 8, true>), blocks, dim3(256), 0, stream, at, cpar, a,
+                               G, gc, partials, active);
+        else
+            hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_MORSE, 4, false>), blocks, dim3(256), 0, stream, at, cpar,
+                               a, G, gc, partials, active);
+    }
+    if (int r = check_launch("grid_adjoint")) return r;
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)(na * 6)), dim3(256), 0, stream, partials, active, (int)nb,
+                       na, grad);
+    return check_launch("grid_adjoint_reduce");
 }
 
 }  // namespace ppm

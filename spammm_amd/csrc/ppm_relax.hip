@@ -281,6 +281,154 @@ __global__ __launch_bounds__(64) void k_fe_in_strokes(GridDev g, const float4* _
     }
 }
 
+// ── implicit-function adjoint of one relaxed state ──────────────────────────────────
+//
+// At the stored probe position q* the total force G(q) = F(q) + R^T g(R(q - tip)) + c*surf
+// vanishes (F: trilinear sample of the grid, g: tipForce in the tip frame R = rows A,B,C).
+// The output O = (R F(q*), E(q*)).  For an upstream cotangent u = dL/dO:
+//   b      = (dO/dq)^T u = J_F^T R^T u_xyz + u_w grad E
+//   J^T l  = b,   J = J_F + R^T (dg/dd) R           (3x3, solved in FP64)
+//   dL/dG_c.xyz += w_c (R^T u_xyz - l),   dL/dG_c.w += w_c u_w     for the 8 corners c
+// (dL/dG = u . dO/dG - l^T dG/dG.)  J_F is the derivative of the trilinear interpolant
+// inside the cell that contains q*.  Status bits: 1 not converged, 2 non-finite, 4 singular J.
+// States with status 2 or 4 contribute nothing.
+struct AdjArgs {
+    int n_scan, nz;
+    float A[3], B[3], C[3], dtip;
+    float k[4], d0[4];
+};
+
+__device__ __forceinline__ void atomic_add4(float* p, float4 v) {
+    atomicAdd(p + 0, v.x);
+    atomicAdd(p + 1, v.y);
+    atomicAdd(p + 2, v.z);
+    atomicAdd(p + 3, v.w);
+}
+
+__global__ __launch_bounds__(256) void k_relax_adjoint(GridDev g, const float4* __restrict__ pts, AdjArgs p,
+                                                       const float4* __restrict__ PPs, const float4* __restrict__ U,
+                                                       float* __restrict__ gcot, float4* __restrict__ lam) {
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (size_t)p.n_scan * p.nz) return;
+    const int ip = (int)(s / p.nz);
+    const int iz = (int)(s - (size_t)ip * p.nz);
+    const f3 A = ld3(p.A), B = ld3(p.B), C = ld3(p.C);
+    const f3 dTip = mk3(C.x * p.dtip, C.y * p.dtip, C.z * p.dtip);
+    const float4 P0 = pts[ip];
+    f3 tip = mk3(P0.x, P0.y, P0.z);
+    for (int k = 0; k < iz; ++k) tip = mk3(tip.x + dTip.x, tip.y + dTip.y, tip.z + dTip.z);   // as the forward
+    const float4 PP = PPs[s];
+    const float4 u = U[s];
+    int status = PP.w > 0.f ? 0 : 1;
+    // cell, weights and their derivatives
+    const float qx = PP.x, qy = PP.y, qz = PP.z;
+    const float sa = qx * g.da[0] + qy * g.da[1] + qz * g.da[2] + g.da[3];
+    const float sb = qx * g.db[0] + qy * g.db[1] + qz * g.db[2] + g.db[3];
+    const float sc = qx * g.dc[0] + qy * g.dc[1] + qz * g.dc[2] + g.dc[3];
+    int i0, i1, j0, j1, k0, k1;
+    float a, b, c;
+    axis_repeat(sa, g.nx, i0, i1, a);
+    axis_repeat(sb, g.ny, j0, j1, b);
+    axis_repeat(sc, g.nz, k0, k1, c);
+    const size_t sx = (size_t)g.nx, sxy = (size_t)g.nx * g.ny;
+    size_t idx[8];
+    idx[0] = (size_t)k0 * sxy + (size_t)j0 * sx + i0; idx[1] = (size_t)k0 * sxy + (size_t)j0 * sx + i1;
+    idx[2] = (size_t)k0 * sxy + (size_t)j1 * sx + i0; idx[3] = (size_t)k0 * sxy + (size_t)j1 * sx + i1;
+    idx[4] = (size_t)k1 * sxy + (size_t)j0 * sx + i0; idx[5] = (size_t)k1 * sxy + (size_t)j0 * sx + i1;
+    idx[6] = (size_t)k1 * sxy + (size_t)j1 * sx + i0; idx[7] = (size_t)k1 * sxy + (size_t)j1 * sx + i1;
+    float4 cn[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) cn[m] = g.data[idx[m]];
+    const float ma = 1.f - a, mb = 1.f - b, mc = 1.f - c;
+    float w[8];
+    w[0] = ma * mb * mc; w[1] = a * mb * mc; w[2] = ma * b * mc; w[3] = a * b * mc;
+    w[4] = ma * mb * c;  w[5] = a * mb * c;  w[6] = ma * b * c;  w[7] = a * b * c;
+    // d(value)/d(fraction) per channel
+    double dFa[4], dFb[4], dFc[4];
+    {
+        const float* f[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) f[m] = &cn[m].x;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            dFa[q] = (double)(((f[1][q] - f[0][q]) * mb + (f[3][q] - f[2][q]) * b) * mc +
+                              ((f[5][q] - f[4][q]) * mb + (f[7][q] - f[6][q]) * b) * c);
+            dFb[q] = (double)(((f[2][q] - f[0][q]) * ma + (f[3][q] - f[1][q]) * a) * mc +
+                              ((f[6][q] - f[4][q]) * ma + (f[7][q] - f[5][q]) * a) * c);
+            dFc[q] = (double)(((f[4][q] - f[0][q]) * ma + (f[5][q] - f[1][q]) * a) * mb +
+                              ((f[6][q] - f[2][q]) * ma + (f[7][q] - f[3][q]) * a) * b);
+        }
+    }
+    // d(fraction)/dq = n * dinv row (xyz)
+    double ta[3], tb[3], tc[3];
+    for (int j = 0; j < 3; ++j) {
+        ta[j] = (double)g.nx * (double)g.da[j];
+        tb[j] = (double)g.ny * (double)g.db[j];
+        tc[j] = (double)g.nz * (double)g.dc[j];
+    }
+    double JF[3][3], gE[3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) JF[i][j] = dFa[i] * ta[j] + dFb[i] * tb[j] + dFc[i] * tc[j];
+    for (int j = 0; j < 3; ++j) gE[j] = dFa[3] * ta[j] + dFb[3] * tb[j] + dFc[3] * tc[j];
+    // tip spring Jacobian in the tip frame, rotated to the world frame
+    const double R[3][3] = {{A.x, A.y, A.z}, {B.x, B.y, B.z}, {C.x, C.y, C.z}};
+    double dw[3] = {(double)qx - tip.x, (double)qy - tip.y, (double)qz - tip.z};
+    double d[3];
+    for (int i = 0; i < 3; ++i) d[i] = R[i][0] * dw[0] + R[i][1] * dw[1] + R[i][2] * dw[2];
+    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    r = r > 1e-10 ? r : 1e-10;
+    const double phi = (double)p.k[3] * (1.0 - (double)p.d0[3] / r);
+    const double beta = (double)p.k[3] * (double)p.d0[3] / (r * r * r);
+    double Jt[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Jt[i][j] = beta * d[i] * d[j] + (i == j ? (double)p.k[i] + phi : 0.0);
+    double J[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double acc = JF[i][j];
+            for (int m = 0; m < 3; ++m)
+                for (int n = 0; n < 3; ++n) acc += R[m][i] * Jt[m][n] * R[n][j];
+            J[i][j] = acc;
+        }
+    // b = J_F^T R^T u_xyz + u_w grad E
+    double ru[3];
+    for (int j = 0; j < 3; ++j) ru[j] = R[0][j] * u.x + R[1][j] * u.y + R[2][j] * u.z;
+    double bv[3];
+    for (int j = 0; j < 3; ++j) bv[j] = JF[0][j] * ru[0] + JF[1][j] * ru[1] + JF[2][j] * ru[2] + (double)u.w * gE[j];
+    // solve J^T l = b by cofactors: (J^T)^{-1} = adj(J)^T / det(J)
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double c10 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    const double c11 = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    const double c12 = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+    const double c20 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    const double c21 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    const double c22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    double scale = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) scale = fmax(scale, fabs(J[i][j]));
+    double l[3] = {0.0, 0.0, 0.0};
+    if (!(fabs(det) > 1e-12 * scale * scale * scale) || !(scale > 0.0)) {
+        status |= 4;
+    } else {
+        // (J^T)^{-1} = adj(J^T) / det = cof(J) / det,  so  l_i = sum_j cof(J)_ij b_j / det
+        l[0] = (c00 * bv[0] + c01 * bv[1] + c02 * bv[2]) / det;
+        l[1] = (c10 * bv[0] + c11 * bv[1] + c12 * bv[2]) / det;
+        l[2] = (c20 * bv[0] + c21 * bv[1] + c22 * bv[2]) / det;
+    }
+    if (!(isfinite(l[0]) && isfinite(l[1]) && isfinite(l[2]) && isfinite(qx) && isfinite(qy) && isfinite(qz) &&
+          isfinite(bv[0]) && isfinite(bv[1]) && isfinite(bv[2])))
+        status |= 2;
+    lam[s] = make_float4((float)l[0], (float)l[1], (float)l[2], (float)status);
+    if (status & 6) return;
+    const float gx = (float)(ru[0] - l[0]), gy = (float)(ru[1] - l[1]), gz = (float)(ru[2] - l[2]);
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+        atomic_add4(gcot + 4 * idx[m], make_float4(w[m] * gx, w[m] * gy, w[m] * gz, w[m] * u.w));
+}
+
 GridDev make_grid(const float* grid, const PpmGridSpec& s) {
     GridDev g;
     g.data = reinterpret_cast<const float4*>(grid);
@@ -328,6 +476,21 @@ int launch_relax_strokes(const float* grid, const PpmGridSpec& s, const float* p
                        reinterpret_cast<const float4*>(pts), p, reinterpret_cast<float4*>(FEs),
                        reinterpret_cast<float4*>(disps), reinterpret_cast<float4*>(PPs));
     return check_launch("relax_strokes");
+}
+
+int launch_relax_adjoint(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
+                         const PpmTip& tip, const float* PPs, const float* dL_dFEs, float* grid_cot, float* lambdas,
+                         hipStream_t stream) {
+    AdjArgs p{};
+    p.n_scan = n_scan; p.nz = nz;
+    for (int q = 0; q < 3; ++q) { p.A[q] = tip.tipA[q]; p.B[q] = tip.tipB[q]; p.C[q] = tip.tipC[q]; }
+    p.dtip = tip.tipC[3];
+    for (int q = 0; q < 4; ++q) { p.k[q] = tip.stiffness[q]; p.d0[q] = tip.dpos0[q]; }
+    const size_t n = (size_t)n_scan * nz;
+    hipLaunchKernelGGL(k_relax_adjoint, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, make_grid(grid, s),
+                       reinterpret_cast<const float4*>(pts), p, reinterpret_cast<const float4*>(PPs),
+                       reinterpret_cast<const float4*>(dL_dFEs), grid_cot, reinterpret_cast<float4*>(lambdas));
+    return check_launch("relax_adjoint");
 }
 
 int launch_fe_in_strokes(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,

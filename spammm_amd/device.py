@@ -215,6 +215,59 @@ def fe_in_strokes_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, dTip, 
     return FEs
 
 
+def relax_adjoint(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, dL_dFEs, grid_cot=None):
+    """Implicit-function adjoint of relaxed states: accumulates dL/dgrid into ``grid_cot`` (zeros if None).
+    Returns (grid_cot, lambdas (n_scan,nz,4) with the status word in .w)."""
+    import torch
+    _f32dev(grid, "grid", 4)
+    _f32dev(points, "points", 4)
+    _f32dev(PPs, "PPs", 4)
+    _f32dev(dL_dFEs, "dL_dFEs", 4)
+    ns = int(points.shape[0])
+    if PPs.numel() != ns * nz * 4 or dL_dFEs.numel() != ns * nz * 4:
+        raise ValueError("PPs / dL_dFEs must be (n_scan, nz, 4)")
+    if grid_cot is None:
+        grid_cot = torch.zeros_like(grid)
+    else:
+        _f32dev(grid_cot, "grid_cot", 4)
+        if grid_cot.numel() != grid.numel():
+            raise ValueError("grid_cot must have the grid's shape")
+    lam = torch.empty((ns, nz, 4), dtype=torch.float32, device=grid.device)
+    L = _lib.load()
+    s, tp = spec.c(), tip.c()
+    rc = L.ppm_relax_adjoint(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz), ctypes.byref(tp),
+                             PPs.data_ptr(), dL_dFEs.data_ptr(), grid_cot.data_ptr(), lam.data_ptr(), _stream(grid))
+    _lib.check(rc)
+    return grid_cot, lam
+
+
+def grid_adjoint(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, grid, grid_cot, npbc=(0, 0), lvA=(0, 0, 0),
+                 lvB=(0, 0, 0), cap=True, workspace=None):
+    """dL/d(x,y,z,q,p0,p1) per atom (na,6) float64 from dL/dgrid (p0,p1 = c6,c12 for LJ, R0,E0 for Morse)."""
+    import torch
+    _f32dev(atoms, "atoms", 4)
+    _f32dev(cpar, "cpar", 2 if model == _lib.PPM_MODEL_LJ else 4)
+    _f32dev(grid, "grid", 4)
+    _f32dev(grid_cot, "grid_cot", 4)
+    if grid.numel() != 4 * spec.nvox or grid_cot.numel() != grid.numel():
+        raise ValueError("grid / grid_cot size does not match spec")
+    na = int(atoms.shape[0])
+    L = _lib.load()
+    s = spec.c()
+    need = int(L.ppm_grid_adjoint_workspace_size(na, ctypes.byref(s)))
+    if workspace is None or workspace.numel() < need:
+        workspace = torch.empty(need, dtype=torch.uint8, device=atoms.device)
+    grad = torch.empty((na, 6), dtype=torch.float64, device=atoms.device)
+    lva = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvA, float).reshape(3)])
+    lvb = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvB, float).reshape(3)])
+    rc = L.ppm_grid_adjoint(int(model), atoms.data_ptr(), cpar.data_ptr(), na, ctypes.byref(s), _lib.f4(Qs),
+                            _lib.f4(QZs), int(npbc[0]), int(npbc[1]), ctypes.byref(lva), ctypes.byref(lvb),
+                            int(bool(cap)), grid.data_ptr(), grid_cot.data_ptr(), workspace.data_ptr(),
+                            workspace.numel(), grad.data_ptr(), _stream(atoms))
+    _lib.check(rc)
+    return grad
+
+
 def scan_points(scan_p0, scan_da, scan_db, nx: int, ny: int) -> np.ndarray:
     """(nx*ny,4) float32 ix-major scan starts: p0 + da*ix + db*iy in float32, the arithmetic of the
     reference loop (spammm/SPM/AFM.py:977-980) and build_scan_xy_points_vectorized (:82-96)."""

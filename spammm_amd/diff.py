@@ -1,0 +1,152 @@
+"""Differentiable df image: a torch.autograd.Function over the HIP forward and adjoint.
+
+Forward  (atoms xyz, charges q, per-atom LJ R0/E0) -> LJ (c6,c12) -> grid projection ->
+         relaxed scan -> df operator along z  (all on the GPU).
+Backward dL/ddf -> transposed df operator -> dL/dFEs (Fz channel) ->
+         ppm_relax_adjoint (implicit-function theorem at each converged probe position,
+         per-state 3x3 solve in FP64, trilinear-adjoint scatter into dL/dgrid) ->
+         ppm_grid_adjoint (grid -> atom pair-law adjoint, FP64 reduction) ->
+         chain (c6,c12) -> (R0,E0):  c6 = 2 E0 R0^6, c12 = E0 R0^12.
+The relaxation trajectory is never stored: the backward needs only the converged probe
+positions, which it either keeps from the forward (16 B per state) or recomputes by
+re-running the deterministic relaxation (``recompute=True``).
+
+There is no reference counterpart for the grid path (the reference differentiates only its
+gridless direct-Morse path, spammm/SPM/AFM.py:2241-2519); the loss convention follows
+df_loss_seed (spammm/SPM/AFM.py:3125-3200).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib, df as DF
+from . import device as D
+
+
+@dataclass
+class Problem:
+    """Everything of the forward that is not differentiated."""
+    spec: D.GridSpec
+    pts: object                 # (n_scan,4) float32 device tensor
+    nxy: tuple
+    nz: int
+    tip: D.Tip
+    Qs: np.ndarray
+    QZs: np.ndarray
+    npbc: tuple = (0, 0)
+    lvA: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
+    lvB: np.ndarray = field(default_factory=lambda: np.zeros(3, np.float32))
+    cap: bool = True
+    max_iter: int = 1000
+    f2conv: float = 1e-12
+    check_before: bool = True
+    df_kind: str = "giessibl"
+    amp: float = 1.0
+    f0: float = 30000.0
+    k_cant: float = 1800.0
+    recompute: bool = False     # backward re-runs the relaxation instead of keeping probe positions
+    strict: bool = True         # raise if any state is singular / non-finite in the adjoint
+    last_status: dict = field(default_factory=dict)
+
+    @property
+    def n_scan(self):
+        return int(self.nxy[0]) * int(self.nxy[1])
+
+    @property
+    def dz(self):
+        return float(self.tip.tipC[3])
+
+    @classmethod
+    def from_workload(cls, w, device, **kw):
+        import torch
+        spec = D.axis_aligned_spec(w.p0, w.L, w.n)
+        pts = torch.as_tensor(D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy), device=device)
+        tip = D.Tip(stiffness=w.stiffness, dpos0=w.dpos0, tipC=np.array([0., 0., 1., w.dtip], np.float32))
+        args = dict(spec=spec, pts=pts, nxy=tuple(w.nxy), nz=int(w.nz), tip=tip, Qs=w.Qs, QZs=w.QZs,
+                    npbc=tuple(w.npbc), lvA=w.lvA, lvB=w.lvB, amp=w.amp, f0=w.f0, k_cant=w.k_cant)
+        args.update(kw)
+        return cls(**args)
+
+
+def lj_cpar(R0, E0):
+    """(na,2) float32 (c6, c12) = (2 E0 R0^6, E0 R0^12), evaluated in FP64."""
+    import torch
+    R0d, E0d = R0.double(), E0.double()
+    r6 = R0d ** 6
+    return torch.stack([2.0 * E0d * r6, E0d * r6 * r6], dim=1).float().contiguous()
+
+
+def _forward_parts(atoms, cpar, prob: Problem):
+    import torch
+    grid = D.grid_project(atoms, cpar, _lib.PPM_MODEL_LJ, prob.spec, prob.Qs, prob.QZs, prob.npbc, prob.lvA,
+                          prob.lvB, cap=prob.cap)
+    FEs, PPs = D.relax_tilted(grid, prob.spec, prob.pts, prob.nz, prob.tip, prob.max_iter, prob.f2conv,
+                              prob.check_before, scan_ny=int(prob.nxy[1]))
+    op = DF.z_operator(prob.df_kind, prob.nz, prob.dz, atoms.device, prob.amp, prob.f0, prob.k_cant)
+    df = op.apply(FEs, (prob.nz * 4, 4, 0), prob.n_scan, 1, offset=2)
+    return grid, FEs, PPs, df.reshape(int(prob.nxy[0]), int(prob.nxy[1]), prob.nz)
+
+
+class DfImage:
+    """Namespace for the autograd Function (built lazily so that importing needs no torch)."""
+    _fn = None
+
+    @classmethod
+    def fn(cls):
+        if cls._fn is None:
+            cls._fn = _make_function()
+        return cls._fn
+
+
+def _make_function():
+    import torch
+
+    class _DfImage(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, xyz, q, R0, E0, prob):
+            atoms = torch.cat([xyz.detach().float(), q.detach().float()[:, None]], dim=1).contiguous()
+            cpar = lj_cpar(R0.detach(), E0.detach())
+            grid, FEs, PPs, df = _forward_parts(atoms, cpar, prob)
+            ctx.prob = prob
+            ctx.dtypes = (xyz.dtype, q.dtype, R0.dtype, E0.dtype)
+            ctx.save_for_backward(atoms, cpar, grid, R0.detach(), E0.detach(),
+                                  PPs if not prob.recompute else torch.empty(0, device=xyz.device))
+            return df
+
+        @staticmethod
+        def backward(ctx, gdf):
+            prob = ctx.prob
+            atoms, cpar, grid, R0, E0, PPs = ctx.saved_tensors
+            dev = atoms.device
+            if prob.recompute:
+                _, PPs = D.relax_tilted(grid, prob.spec, prob.pts, prob.nz, prob.tip, prob.max_iter, prob.f2conv,
+                                        prob.check_before, scan_ny=int(prob.nxy[1]))
+            g = gdf.detach().float().contiguous().reshape(prob.n_scan, prob.nz)
+            opT = DF.z_operator(prob.df_kind, prob.nz, prob.dz, dev, prob.amp, prob.f0, prob.k_cant, transpose=True)
+            U = torch.zeros((prob.n_scan, prob.nz, 4), dtype=torch.float32, device=dev)
+            u_z = opT.apply(g, (prob.nz, 1, 0), prob.n_scan, 1)
+            U[..., 2] = u_z[..., 0]
+            gG, lam = D.relax_adjoint(grid, prob.spec, prob.pts, prob.nz, prob.tip, PPs, U)
+            st = lam[..., 3].to(torch.int32)
+            n_bad = int(((st & 6) != 0).sum())
+            prob.last_status = {"nonconverged": int(((st & 1) != 0).sum()), "bad": n_bad}
+            if prob.strict and n_bad:
+                raise RuntimeError(f"relax adjoint: {n_bad} states singular or non-finite")
+            g6 = D.grid_adjoint(atoms, cpar, _lib.PPM_MODEL_LJ, prob.spec, prob.Qs, prob.QZs, grid, gG, prob.npbc,
+                                prob.lvA, prob.lvB, cap=prob.cap)
+            R0d, E0d = R0.double(), E0.double()
+            r5 = R0d ** 5
+            r6 = r5 * R0d
+            dR0 = g6[:, 4] * 12.0 * E0d * r5 + g6[:, 5] * 12.0 * E0d * r6 * r5
+            dE0 = g6[:, 4] * 2.0 * r6 + g6[:, 5] * r6 * r6
+            tx, tq, tr, te = ctx.dtypes
+            return g6[:, :3].to(tx), g6[:, 3].to(tq), dR0.to(tr), dE0.to(te), None
+
+    return _DfImage
+
+
+def df_image(xyz, q, R0, E0, prob: Problem):
+    """Differentiable df image (nx, ny, nz) on the GPU."""
+    return DfImage.fn().apply(xyz, q, R0, E0, prob)

@@ -1,0 +1,113 @@
+"""GPU adjoint (implicit-function relax adjoint + grid->atom adjoint, torch.autograd.Function)
+against central finite differences of the FP64 oracle forward.
+
+Forward for the finite differences: FP64 grid from FP64 atoms/parameters, FP64 relaxation
+on that grid converged to |F| < 1e-12 eV/A (check before the update), df by the Giessibl
+operator (FP64 matrix, pinned against quadrature in tests/test_oracle_df.py).
+Bar: every checked gradient component within 1e-3 relative of its FD value (denominator
+floored at 1e-2 of the largest |FD| component so that near-zero components are judged on
+the gradient's scale).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O            # noqa: E402
+from spammm_amd import device as D        # noqa: E402
+from spammm_amd import df as DF           # noqa: E402
+from spammm_amd import diff               # noqa: E402
+from spammm_amd import params as P        # noqa: E402
+from spammm_amd import workloads as W     # noqa: E402
+
+
+def setup_case(seed=7):
+    w = W.small(seed)
+    dev = D.require_gpu()
+    prob = diff.Problem.from_workload(w, dev, max_iter=3000, f2conv=1e-12, check_before=True)
+    R0, E0 = P.mix_R0E0(w.enames, P.ELEMENTS_BENCH, 1.66, 0.0091)
+    theta = {
+        "xyz": w.atoms[:, :3].astype(np.float64), "q": w.atoms[:, 3].astype(np.float64),
+        "R0": R0, "E0": E0,
+    }
+    return w, prob, theta
+
+
+def gpu_df(prob, th, requires_grad=False):
+    dev = prob.pts.device
+    t = {k: torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=requires_grad) for k, v in th.items()}
+    df = diff.df_image(t["xyz"], t["q"], t["R0"], t["E0"], prob)
+    return df, t
+
+
+def fp64_df(w, prob, th):
+    atoms = np.concatenate([th["xyz"], th["q"][:, None]], 1)
+    cpar = np.stack([2 * th["E0"] * th["R0"] ** 6, th["E0"] * th["R0"] ** 12], 1)
+    sp = prob.spec
+    G = O.grid_project_d(0, atoms, cpar, sp.n, sp.p0, sp.dA, sp.dB, sp.dC, w.Qs, w.QZs, w.npbc, w.lvA, w.lvB)
+    tip = prob.tip
+    pts = prob.pts.cpu().numpy()
+    FEs, PPs = O.relax_tilted_d(G, sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pts, prob.nz, tip.tipA, tip.tipB, tip.tipC,
+                                tip.stiffness, tip.dpos0, tip.relax, tip.surfFF, max_iter=40000, f2conv=1e-24,
+                                check_before=True)
+    assert (PPs[..., 3] > 0).all(), "FP64 reference relaxation did not converge"
+    M = DF.giessibl_matrix(prob.nz, prob.dz, prob.amp, prob.f0, prob.k_cant)
+    return (FEs[..., 2] @ M.T).reshape(prob.nxy[0], prob.nxy[1], prob.nz)
+
+
+def test_adjoint_matches_fp64_finite_differences():
+    w, prob, th = setup_case()
+    rng = np.random.default_rng(3)
+    # target: df of the same molecule with positions perturbed by N(0, 0.1 A)
+    th_t = {k: v.copy() for k, v in th.items()}
+    th_t["xyz"] = th["xyz"] + rng.normal(0, 0.1, th["xyz"].shape)
+    with torch.no_grad():
+        target = gpu_df(prob, th_t)[0].double()
+    df, t = gpu_df(prob, th, requires_grad=True)
+    loss = 0.5 * ((df.double() - target) ** 2).sum()
+    loss.backward()
+    print(f"[adjoint] loss {float(loss):.6e} status {prob.last_status}")
+    assert prob.last_status["bad"] == 0
+    grads = {k: t[k].grad.cpu().numpy() for k in t}
+    tgt = target.cpu().numpy()
+
+    def L64(thx):
+        d = fp64_df(w, prob, thx)
+        return 0.5 * ((d - tgt) ** 2).sum()
+
+    na = th["q"].size
+    picks = []
+    for k in ("xyz", "q", "R0", "E0"):
+        for _ in range({"xyz": 14, "q": 6, "R0": 6, "E0": 6}[k]):
+            i = int(rng.integers(na))
+            picks.append((k, (i, int(rng.integers(3))) if k == "xyz" else (i,)))
+    fd, an = [], []
+    for k, idx in picks:
+        h = {"xyz": 1e-3, "q": 1e-4, "R0": 1e-3, "E0": 1e-3 * th["E0"][idx[0]]}[k]
+        tp = {kk: v.copy() for kk, v in th.items()}
+        tm = {kk: v.copy() for kk, v in th.items()}
+        tp[k][idx] += h
+        tm[k][idx] -= h
+        fd.append((L64(tp) - L64(tm)) / (2 * h))
+        an.append(grads[k][idx])
+    fd, an = np.array(fd), np.array(an)
+    floor = 1e-2 * np.abs(fd).max()
+    rel = np.abs(an - fd) / np.maximum(np.abs(fd), floor)
+    for (k, idx), a, f, r in zip(picks, an, fd, rel):
+        print(f"  {k}{idx}: adjoint {a:+.6e}  fd {f:+.6e}  rel {r:.2e}")
+    assert rel.max() < 1e-3, f"worst relative error {rel.max():.3e}"
+
+
+def test_adjoint_recompute_equals_stored():
+    """Recomputing the probe positions in the backward gives the same gradient as keeping them."""
+    w, prob, th = setup_case(5)
+    out = []
+    for rec in (False, True):
+        prob.recompute = rec
+        df, t = gpu_df(prob, th, requires_grad=True)
+        (0.5 * (df.double() ** 2).sum()).backward()
+        out.append(np.concatenate([t[k].grad.cpu().numpy().ravel() for k in ("xyz", "q", "R0", "E0")]))
+    scale = np.abs(out[0]).max()
+    assert np.abs(out[0] - out[1]).max() <= 1e-4 * scale
