@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--workload", default="flake", choices=["flake", "naphthalene"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy leg")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")
+    ap.add_argument("--adam-steps", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (every box has one MI355X); "
@@ -163,11 +165,92 @@ def main():
         },
     }
 
+    if not args.no_diff:
+        out["differentiable"] = diff_leg(dev, args.adam_steps)
     if not args.no_check:
         out["accuracy"] = accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(w, sp, grid, pts_h, tip, states)
     print(json.dumps(out), flush=True)
+
+
+def diff_leg(dev, adam_steps=50):
+    """Differentiable df image of the flake molecule, 128x128 pixels x 40 z-steps: forward and adjoint
+    timed with HIP events (adjoint/forward ratio), then Adam on the atom positions towards a target
+    made from positions perturbed by N(0, 0.1 A)."""
+    import torch
+    from spammm_amd import diff, params as P, workloads as WL
+    w = WL.flake()
+    step = 30.0 / 128
+    w.nxy, w.nz = (128, 128), 40
+    w.scan_p0 = np.array([-15. + step / 2, -15. + step / 2, 10.0], np.float32)
+    w.scan_da = np.array([step, 0., 0.], np.float32)
+    w.scan_db = np.array([0., step, 0.], np.float32)
+    prob = diff.Problem.from_workload(w, dev, max_iter=1000, f2conv=1e-12, check_before=True, polish=3,
+                                      strict=False)
+    R0, E0 = P.mix_R0E0(w.enames, P.ELEMENTS_BENCH, 1.66, 0.0091)
+    xyz0 = torch.tensor(w.atoms[:, :3], dtype=torch.float64, device=dev)
+    q = torch.tensor(w.atoms[:, 3], dtype=torch.float64, device=dev)
+    R0 = torch.tensor(R0, device=dev)
+    E0 = torch.tensor(E0, device=dev)
+    rng = np.random.default_rng(3)
+    xyz_t = xyz0 + torch.tensor(rng.normal(0, 0.1, w.atoms[:, :3].shape), device=dev)
+    with torch.no_grad():
+        target = diff.df_image(xyz_t, q, R0, E0, prob).double()
+    stream = torch.cuda.current_stream(dev)
+    t_f, t_b = [], []
+    for rep in range(4):
+        x = xyz0.clone().requires_grad_(True)
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(stream)
+        dfi = diff.df_image(x, q, R0, E0, prob)
+        loss = 0.5 * ((dfi.double() - target) ** 2).sum()
+        e1.record(stream)
+        loss.backward()
+        e2.record(stream)
+        torch.cuda.synchronize()
+        if rep > 0:
+            t_f.append(e0.elapsed_time(e1))
+            t_b.append(e1.elapsed_time(e2))
+    tf, tb = float(np.mean(t_f)), float(np.mean(t_b))
+    # directional finite difference of the loss along -grad (GPU forward), eps = 1e-3 A
+    g = x.grad.detach()
+    dvec = -g / g.norm()
+
+    def loss_at(xx):
+        with torch.no_grad():
+            return float(0.5 * ((diff.df_image(xx, q, R0, E0, prob).double() - target) ** 2).sum())
+    eps = 1e-3
+    slope = (loss_at(xyz0 + eps * dvec) - loss_at(xyz0 - eps * dvec)) / (2 * eps)
+    # Adam on positions
+    x = xyz0.clone().requires_grad_(True)
+    opt = torch.optim.Adam([x], lr=0.003)
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(adam_steps):
+        opt.zero_grad()
+        loss = 0.5 * ((diff.df_image(x, q, R0, E0, prob).double() - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+        hist.append(float(loss))
+    torch.cuda.synchronize()
+    t_adam = time.perf_counter() - t0
+    rmsd0 = float(((xyz0 - xyz_t) ** 2).sum(1).mean().sqrt())
+    rmsd1 = float(((x.detach() - xyz_t) ** 2).sum(1).mean().sqrt())
+    res = {
+        "workload": "flake 200 atoms, 300x300x240 grid, 128x128 pixels x 40 z, Giessibl df, L2 loss",
+        "forward_ms": tf, "adjoint_ms": tb, "adjoint_over_forward": tb / tf,
+        "grad_check_directional_fd": {"slope_fd": slope, "minus_grad_norm": -float(g.norm()),
+                                      "rel_err": abs(slope + float(g.norm())) / float(g.norm())},
+        "pixel_z_steps_per_s_forward": w.n_scan * w.nz / (tf * 1e-3),
+        "adam_steps": adam_steps, "adam_wall_s": t_adam,
+        "loss_first": hist[0] if hist else None, "loss_last": hist[-1] if hist else None,
+        "loss_history_every5": hist[::5],
+        "position_rmsd_to_target_A": {"start": rmsd0, "end": rmsd1},
+        "adjoint_status": prob.last_status,
+    }
+    log(f"[bench] differentiable: {res}")
+    return res
 
 
 def accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip):

@@ -144,6 +144,16 @@ int ppm_relax_adjoint(const float* grid, const PpmGridSpec* spec,
                       const float* PPs, const float* dL_dFEs,
                       float* grid_cot, float* lambdas, void* stream);
 
+/* Newton polish of converged states (no reference counterpart; used by the differentiable
+ * path so that the forward sits on the equilibrium the implicit-function adjoint assumes).
+ * For every state with PPs[s].w > 0, up to `steps` FP64 Newton steps q <- q - J^{-1} G(q) on
+ * the trilinear model (J as in ppm_relax_adjoint), each kept only if it lowers |G|; PPs.xyz
+ * is updated in place and, if FEs is not NULL, FEs[s] is resampled there (tip-rotated).
+ * points/tip must be those of the ppm_relax_strokes_tilted call that produced PPs. */
+int ppm_relax_polish(const float* grid, const PpmGridSpec* spec,
+                     const float* points, int n_scan, int nz, const PpmTip* tip,
+                     int steps, float* PPs, float* FEs, void* stream);
+
 /* Bytes of device workspace ppm_grid_adjoint needs for na atoms on this grid (0 on bad input). */
 size_t ppm_grid_adjoint_workspace_size(int na, const PpmGridSpec* spec);
 

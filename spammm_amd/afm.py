@@ -463,7 +463,7 @@ def _axis_layout(shape, axis):
     """(n_outer, n_axis, n_inner) and strides (in elements) of a C-ordered (nx,ny,nz) volume."""
     nx, ny, nz = shape
     if axis == 0:
-        return nx, 1, ny * nz
+        return 1, nx, ny * nz
     if axis == 1:
         return nx, ny, nz
     return nx * ny, nz, 1

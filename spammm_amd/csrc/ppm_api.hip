@@ -150,6 +150,17 @@ int ppm_relax_adjoint(const float* grid, const PpmGridSpec* spec, const float* p
                                 (hipStream_t)stream);
 }
 
+int ppm_relax_polish(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
+                     const PpmTip* tip, int steps, float* PPs, float* FEs, void* stream) {
+    const char* fn = "ppm_relax_polish";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (int r = check_scan(fn, points, n_scan, nz, PPs)) return r;
+    if (!grid || !tip) { set_error("%s: NULL grid/tip", fn); return PPM_ERR_INVALID_ARG; }
+    if (steps < 0 || steps > 64) { set_error("%s: steps=%d out of [0,64]", fn, steps); return PPM_ERR_INVALID_ARG; }
+    return launch_relax_polish(grid, *spec, points, n_scan, nz, *tip, steps, PPs, FEs, (hipStream_t)stream);
+}
+
 size_t ppm_grid_adjoint_workspace_size(int na, const PpmGridSpec* spec) {
     if (!spec || na < 1 || check_spec("ppm_grid_adjoint_workspace_size", spec) != PPM_OK) return 0;
     return grid_adjoint_workspace(na, *spec);

@@ -42,6 +42,8 @@ int launch_relax_strokes(const float* grid, const PpmGridSpec& g, const float* p
 int launch_relax_adjoint(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
                          const PpmTip& tip, const float* PPs, const float* dL_dFEs, float* grid_cot, float* lambdas,
                          hipStream_t stream);
+int launch_relax_polish(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
+                        const PpmTip& tip, int steps, float* PPs, float* FEs, hipStream_t stream);
 size_t grid_adjoint_workspace(int na, const PpmGridSpec& g);
 int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],

@@ -281,23 +281,171 @@ __global__ __launch_bounds__(64) void k_fe_in_strokes(GridDev g, const float4* _
     }
 }
 
-// ── implicit-function adjoint of one relaxed state ──────────────────────────────────
+// ── relaxed-state calculus: total force, its Jacobian, Newton polish, IFT adjoint ──────
 //
-// At the stored probe position q* the total force G(q) = F(q) + R^T g(R(q - tip)) + c*surf
-// vanishes (F: trilinear sample of the grid, g: tipForce in the tip frame R = rows A,B,C).
-// The output O = (R F(q*), E(q*)).  For an upstream cotangent u = dL/dO:
-//   b      = (dO/dq)^T u = J_F^T R^T u_xyz + u_w grad E
-//   J^T l  = b,   J = J_F + R^T (dg/dd) R           (3x3, solved in FP64)
-//   dL/dG_c.xyz += w_c (R^T u_xyz - l),   dL/dG_c.w += w_c u_w     for the 8 corners c
-// (dL/dG = u . dO/dG - l^T dG/dG.)  J_F is the derivative of the trilinear interpolant
-// inside the cell that contains q*.  Status bits: 1 not converged, 2 non-finite, 4 singular J.
-// States with status 2 or 4 contribute nothing.
-struct AdjArgs {
-    int n_scan, nz;
+// At a probe position q the total force is G(q) = F(q) + R^T g(R(q - tip)) + C*surf, with F the
+// trilinear sample of the grid, g = tipForce in the tip frame R = rows (A,B,C).  Inside the grid
+// cell that contains q, F is trilinear, so J = dG/dq = J_F + R^T (dg/dd) R is available in
+// closed form (J_F from the 8 corners, dg/dd = diag(k) + k_w(1 - L/r) I + (k_w L / r^3) d d^T).
+struct StateArgs {
+    int n_scan, nz, steps;
     float A[3], B[3], C[3], dtip;
     float k[4], d0[4];
+    float surf;
 };
 
+struct StateEval {
+    size_t idx[8];
+    float w[8];
+    float4 fe;          // trilinear sample at (float)q
+    double JF[3][3];    // dF_xyz/dq inside the cell
+    double gE[3];       // dE/dq inside the cell
+    double G[3];        // total force
+    double J[3][3];     // dG/dq
+};
+
+__device__ __forceinline__ f3 tip_anchor(const float4* pts, const StateArgs& p, int ip, int iz) {
+    const f3 C = ld3(p.C);
+    const f3 dTip = mk3(C.x * p.dtip, C.y * p.dtip, C.z * p.dtip);
+    const float4 P0 = pts[ip];
+    f3 tip = mk3(P0.x, P0.y, P0.z);
+    for (int k = 0; k < iz; ++k) tip = mk3(tip.x + dTip.x, tip.y + dTip.y, tip.z + dTip.z);   // as the forward
+    return tip;
+}
+
+__device__ void eval_state(const GridDev& g, const StateArgs& p, f3 tip, double qx, double qy, double qz,
+                           StateEval& e) {
+    const float fx = (float)qx, fy = (float)qy, fz = (float)qz;
+    const float sa = fx * g.da[0] + fy * g.da[1] + fz * g.da[2] + g.da[3];
+    const float sb = fx * g.db[0] + fy * g.db[1] + fz * g.db[2] + g.db[3];
+    const float sc = fx * g.dc[0] + fy * g.dc[1] + fz * g.dc[2] + g.dc[3];
+    int i0, i1, j0, j1, k0, k1;
+    float a, b, c;
+    axis_repeat(sa, g.nx, i0, i1, a);
+    axis_repeat(sb, g.ny, j0, j1, b);
+    axis_repeat(sc, g.nz, k0, k1, c);
+    const size_t sx = (size_t)g.nx, sxy = (size_t)g.nx * g.ny;
+    e.idx[0] = (size_t)k0 * sxy + (size_t)j0 * sx + i0; e.idx[1] = (size_t)k0 * sxy + (size_t)j0 * sx + i1;
+    e.idx[2] = (size_t)k0 * sxy + (size_t)j1 * sx + i0; e.idx[3] = (size_t)k0 * sxy + (size_t)j1 * sx + i1;
+    e.idx[4] = (size_t)k1 * sxy + (size_t)j0 * sx + i0; e.idx[5] = (size_t)k1 * sxy + (size_t)j0 * sx + i1;
+    e.idx[6] = (size_t)k1 * sxy + (size_t)j1 * sx + i0; e.idx[7] = (size_t)k1 * sxy + (size_t)j1 * sx + i1;
+    Cell C;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) C.c[m] = g.data[e.idx[m]];
+    e.fe = lerp8(C, a, b, c);
+    const float ma = 1.f - a, mb = 1.f - b, mc = 1.f - c;
+    e.w[0] = ma * mb * mc; e.w[1] = a * mb * mc; e.w[2] = ma * b * mc; e.w[3] = a * b * mc;
+    e.w[4] = ma * mb * c;  e.w[5] = a * mb * c;  e.w[6] = ma * b * c;  e.w[7] = a * b * c;
+    double dFa[4], dFb[4], dFc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float* f[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) f[m] = &C.c[m].x;
+        dFa[q] = (double)(((f[1][q] - f[0][q]) * mb + (f[3][q] - f[2][q]) * b) * mc +
+                          ((f[5][q] - f[4][q]) * mb + (f[7][q] - f[6][q]) * b) * c);
+        dFb[q] = (double)(((f[2][q] - f[0][q]) * ma + (f[3][q] - f[1][q]) * a) * mc +
+                          ((f[6][q] - f[4][q]) * ma + (f[7][q] - f[5][q]) * a) * c);
+        dFc[q] = (double)(((f[4][q] - f[0][q]) * ma + (f[5][q] - f[1][q]) * a) * mb +
+                          ((f[6][q] - f[2][q]) * ma + (f[7][q] - f[3][q]) * a) * b);
+    }
+    double ta[3], tb[3], tc[3];
+    for (int j = 0; j < 3; ++j) {
+        ta[j] = (double)g.nx * (double)g.da[j];
+        tb[j] = (double)g.ny * (double)g.db[j];
+        tc[j] = (double)g.nz * (double)g.dc[j];
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) e.JF[i][j] = dFa[i] * ta[j] + dFb[i] * tb[j] + dFc[i] * tc[j];
+    for (int j = 0; j < 3; ++j) e.gE[j] = dFa[3] * ta[j] + dFb[3] * tb[j] + dFc[3] * tc[j];
+    const double R[3][3] = {{p.A[0], p.A[1], p.A[2]}, {p.B[0], p.B[1], p.B[2]}, {p.C[0], p.C[1], p.C[2]}};
+    const double dw[3] = {qx - (double)tip.x, qy - (double)tip.y, qz - (double)tip.z};
+    double d[3];
+    for (int i = 0; i < 3; ++i) d[i] = R[i][0] * dw[0] + R[i][1] * dw[1] + R[i][2] * dw[2];
+    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    r = r > 1e-10 ? r : 1e-10;
+    const double phi = (double)p.k[3] * (1.0 - (double)p.d0[3] / r);
+    const double beta = (double)p.k[3] * (double)p.d0[3] / (r * r * r);
+    double gt[3], Jt[3][3];
+    for (int i = 0; i < 3; ++i) {
+        gt[i] = (d[i] - (double)p.d0[i]) * (double)p.k[i] + d[i] * phi;
+        for (int j = 0; j < 3; ++j) Jt[i][j] = beta * d[i] * d[j] + (i == j ? (double)p.k[i] + phi : 0.0);
+    }
+    const double fs[3] = {e.fe.x, e.fe.y, e.fe.z};
+    for (int i = 0; i < 3; ++i) {
+        e.G[i] = fs[i] + R[0][i] * gt[0] + R[1][i] * gt[1] + R[2][i] * gt[2] + (double)p.C[i] * (double)p.surf;
+        for (int j = 0; j < 3; ++j) {
+            double acc = e.JF[i][j];
+            for (int m = 0; m < 3; ++m)
+                for (int n = 0; n < 3; ++n) acc += R[m][i] * Jt[m][n] * R[n][j];
+            e.J[i][j] = acc;
+        }
+    }
+}
+
+// cof[i][j] = cofactor of J; returns det(J) and whether J is numerically regular
+__device__ __forceinline__ bool cofactors(const double J[3][3], double cof[3][3], double& det) {
+    cof[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    cof[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    cof[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    cof[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    cof[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    cof[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+    cof[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    cof[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    cof[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    det = J[0][0] * cof[0][0] + J[0][1] * cof[0][1] + J[0][2] * cof[0][2];
+    double scale = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) scale = fmax(scale, fabs(J[i][j]));
+    return scale > 0.0 && fabs(det) > 1e-12 * scale * scale * scale;
+}
+
+__device__ __forceinline__ double norm3(const double v[3]) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+// Newton polish of converged probe positions (FP64 steps on the trilinear model): a step
+// q <- q - J^{-1} G is kept only if it lowers |G|.  FEs are resampled at the final position.
+__global__ __launch_bounds__(256) void k_relax_polish(GridDev g, const float4* __restrict__ pts, StateArgs p,
+                                                      float4* __restrict__ PPs, float4* __restrict__ FEs) {
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (size_t)p.n_scan * p.nz) return;
+    const float4 PP = PPs[s];
+    if (!(PP.w > 0.f)) return;
+    const int ip = (int)(s / p.nz);
+    const int iz = (int)(s - (size_t)ip * p.nz);
+    const f3 tip = tip_anchor(pts, p, ip, iz);
+    double q[3] = {PP.x, PP.y, PP.z};
+    StateEval e;
+    eval_state(g, p, tip, q[0], q[1], q[2], e);
+    double gn = norm3(e.G);
+    for (int it = 0; it < p.steps; ++it) {
+        double cof[3][3], det;
+        if (!cofactors(e.J, cof, det)) break;
+        double dq[3];
+        for (int i = 0; i < 3; ++i) dq[i] = -(cof[0][i] * e.G[0] + cof[1][i] * e.G[1] + cof[2][i] * e.G[2]) / det;
+        const double qt[3] = {q[0] + dq[0], q[1] + dq[1], q[2] + dq[2]};
+        StateEval et;
+        eval_state(g, p, tip, qt[0], qt[1], qt[2], et);
+        const double gt = norm3(et.G);
+        if (!(gt < gn)) break;
+        q[0] = qt[0]; q[1] = qt[1]; q[2] = qt[2];
+        e = et;
+        gn = gt;
+    }
+    PPs[s] = make_float4((float)q[0], (float)q[1], (float)q[2], PP.w);
+    if (FEs) {
+        const f3 fr = rotMat(mk3(e.fe.x, e.fe.y, e.fe.z), ld3(p.A), ld3(p.B), ld3(p.C));
+        FEs[s] = make_float4(fr.x, fr.y, fr.z, e.fe.w);
+    }
+}
+
+// Implicit-function adjoint of one relaxed state.  The output O = (R F(q*), E(q*)); for an
+// upstream cotangent u = dL/dO:
+//   b      = (dO/dq)^T u = J_F^T R^T u_xyz + u_w grad E
+//   J^T l  = b                                  (3x3, FP64)
+//   dL/dG_c.xyz += w_c (R^T u_xyz - l),   dL/dG_c.w += w_c u_w     for the 8 corners c
+// (dL/dG = u . dO/dG - l^T dG/dG).  Status bits: 1 not converged, 2 non-finite, 4 singular J.
+// States with status 2 or 4 contribute nothing.
 __device__ __forceinline__ void atomic_add4(float* p, float4 v) {
     atomicAdd(p + 0, v.x);
     atomicAdd(p + 1, v.y);
@@ -305,128 +453,42 @@ __device__ __forceinline__ void atomic_add4(float* p, float4 v) {
     atomicAdd(p + 3, v.w);
 }
 
-__global__ __launch_bounds__(256) void k_relax_adjoint(GridDev g, const float4* __restrict__ pts, AdjArgs p,
+__global__ __launch_bounds__(256) void k_relax_adjoint(GridDev g, const float4* __restrict__ pts, StateArgs p,
                                                        const float4* __restrict__ PPs, const float4* __restrict__ U,
                                                        float* __restrict__ gcot, float4* __restrict__ lam) {
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= (size_t)p.n_scan * p.nz) return;
     const int ip = (int)(s / p.nz);
     const int iz = (int)(s - (size_t)ip * p.nz);
-    const f3 A = ld3(p.A), B = ld3(p.B), C = ld3(p.C);
-    const f3 dTip = mk3(C.x * p.dtip, C.y * p.dtip, C.z * p.dtip);
-    const float4 P0 = pts[ip];
-    f3 tip = mk3(P0.x, P0.y, P0.z);
-    for (int k = 0; k < iz; ++k) tip = mk3(tip.x + dTip.x, tip.y + dTip.y, tip.z + dTip.z);   // as the forward
+    const f3 tip = tip_anchor(pts, p, ip, iz);
     const float4 PP = PPs[s];
     const float4 u = U[s];
     int status = PP.w > 0.f ? 0 : 1;
-    // cell, weights and their derivatives
-    const float qx = PP.x, qy = PP.y, qz = PP.z;
-    const float sa = qx * g.da[0] + qy * g.da[1] + qz * g.da[2] + g.da[3];
-    const float sb = qx * g.db[0] + qy * g.db[1] + qz * g.db[2] + g.db[3];
-    const float sc = qx * g.dc[0] + qy * g.dc[1] + qz * g.dc[2] + g.dc[3];
-    int i0, i1, j0, j1, k0, k1;
-    float a, b, c;
-    axis_repeat(sa, g.nx, i0, i1, a);
-    axis_repeat(sb, g.ny, j0, j1, b);
-    axis_repeat(sc, g.nz, k0, k1, c);
-    const size_t sx = (size_t)g.nx, sxy = (size_t)g.nx * g.ny;
-    size_t idx[8];
-    idx[0] = (size_t)k0 * sxy + (size_t)j0 * sx + i0; idx[1] = (size_t)k0 * sxy + (size_t)j0 * sx + i1;
-    idx[2] = (size_t)k0 * sxy + (size_t)j1 * sx + i0; idx[3] = (size_t)k0 * sxy + (size_t)j1 * sx + i1;
-    idx[4] = (size_t)k1 * sxy + (size_t)j0 * sx + i0; idx[5] = (size_t)k1 * sxy + (size_t)j0 * sx + i1;
-    idx[6] = (size_t)k1 * sxy + (size_t)j1 * sx + i0; idx[7] = (size_t)k1 * sxy + (size_t)j1 * sx + i1;
-    float4 cn[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) cn[m] = g.data[idx[m]];
-    const float ma = 1.f - a, mb = 1.f - b, mc = 1.f - c;
-    float w[8];
-    w[0] = ma * mb * mc; w[1] = a * mb * mc; w[2] = ma * b * mc; w[3] = a * b * mc;
-    w[4] = ma * mb * c;  w[5] = a * mb * c;  w[6] = ma * b * c;  w[7] = a * b * c;
-    // d(value)/d(fraction) per channel
-    double dFa[4], dFb[4], dFc[4];
-    {
-        const float* f[8];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) f[m] = &cn[m].x;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            dFa[q] = (double)(((f[1][q] - f[0][q]) * mb + (f[3][q] - f[2][q]) * b) * mc +
-                              ((f[5][q] - f[4][q]) * mb + (f[7][q] - f[6][q]) * b) * c);
-            dFb[q] = (double)(((f[2][q] - f[0][q]) * ma + (f[3][q] - f[1][q]) * a) * mc +
-                              ((f[6][q] - f[4][q]) * ma + (f[7][q] - f[5][q]) * a) * c);
-            dFc[q] = (double)(((f[4][q] - f[0][q]) * ma + (f[5][q] - f[1][q]) * a) * mb +
-                              ((f[6][q] - f[2][q]) * ma + (f[7][q] - f[3][q]) * a) * b);
-        }
-    }
-    // d(fraction)/dq = n * dinv row (xyz)
-    double ta[3], tb[3], tc[3];
-    for (int j = 0; j < 3; ++j) {
-        ta[j] = (double)g.nx * (double)g.da[j];
-        tb[j] = (double)g.ny * (double)g.db[j];
-        tc[j] = (double)g.nz * (double)g.dc[j];
-    }
-    double JF[3][3], gE[3];
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) JF[i][j] = dFa[i] * ta[j] + dFb[i] * tb[j] + dFc[i] * tc[j];
-    for (int j = 0; j < 3; ++j) gE[j] = dFa[3] * ta[j] + dFb[3] * tb[j] + dFc[3] * tc[j];
-    // tip spring Jacobian in the tip frame, rotated to the world frame
-    const double R[3][3] = {{A.x, A.y, A.z}, {B.x, B.y, B.z}, {C.x, C.y, C.z}};
-    double dw[3] = {(double)qx - tip.x, (double)qy - tip.y, (double)qz - tip.z};
-    double d[3];
-    for (int i = 0; i < 3; ++i) d[i] = R[i][0] * dw[0] + R[i][1] * dw[1] + R[i][2] * dw[2];
-    double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    r = r > 1e-10 ? r : 1e-10;
-    const double phi = (double)p.k[3] * (1.0 - (double)p.d0[3] / r);
-    const double beta = (double)p.k[3] * (double)p.d0[3] / (r * r * r);
-    double Jt[3][3];
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) Jt[i][j] = beta * d[i] * d[j] + (i == j ? (double)p.k[i] + phi : 0.0);
-    double J[3][3];
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            double acc = JF[i][j];
-            for (int m = 0; m < 3; ++m)
-                for (int n = 0; n < 3; ++n) acc += R[m][i] * Jt[m][n] * R[n][j];
-            J[i][j] = acc;
-        }
-    // b = J_F^T R^T u_xyz + u_w grad E
+    StateEval e;
+    eval_state(g, p, tip, PP.x, PP.y, PP.z, e);
+    const double R[3][3] = {{p.A[0], p.A[1], p.A[2]}, {p.B[0], p.B[1], p.B[2]}, {p.C[0], p.C[1], p.C[2]}};
     double ru[3];
     for (int j = 0; j < 3; ++j) ru[j] = R[0][j] * u.x + R[1][j] * u.y + R[2][j] * u.z;
     double bv[3];
-    for (int j = 0; j < 3; ++j) bv[j] = JF[0][j] * ru[0] + JF[1][j] * ru[1] + JF[2][j] * ru[2] + (double)u.w * gE[j];
-    // solve J^T l = b by cofactors: (J^T)^{-1} = adj(J)^T / det(J)
-    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    const double c10 = J[0][2] * J[2][1] - J[0][1] * J[2][2];
-    const double c11 = J[0][0] * J[2][2] - J[0][2] * J[2][0];
-    const double c12 = J[0][1] * J[2][0] - J[0][0] * J[2][1];
-    const double c20 = J[0][1] * J[1][2] - J[0][2] * J[1][1];
-    const double c21 = J[0][2] * J[1][0] - J[0][0] * J[1][2];
-    const double c22 = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-    double scale = 0.0;
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) scale = fmax(scale, fabs(J[i][j]));
+    for (int j = 0; j < 3; ++j)
+        bv[j] = e.JF[0][j] * ru[0] + e.JF[1][j] * ru[1] + e.JF[2][j] * ru[2] + (double)u.w * e.gE[j];
+    double cof[3][3], det;
     double l[3] = {0.0, 0.0, 0.0};
-    if (!(fabs(det) > 1e-12 * scale * scale * scale) || !(scale > 0.0)) {
+    if (!cofactors(e.J, cof, det)) {
         status |= 4;
     } else {
-        // (J^T)^{-1} = adj(J^T) / det = cof(J) / det,  so  l_i = sum_j cof(J)_ij b_j / det
-        l[0] = (c00 * bv[0] + c01 * bv[1] + c02 * bv[2]) / det;
-        l[1] = (c10 * bv[0] + c11 * bv[1] + c12 * bv[2]) / det;
-        l[2] = (c20 * bv[0] + c21 * bv[1] + c22 * bv[2]) / det;
+        // J^T l = b:  (J^T)^{-1} = adj(J^T)/det = cof(J)/det
+        for (int i = 0; i < 3; ++i) l[i] = (cof[i][0] * bv[0] + cof[i][1] * bv[1] + cof[i][2] * bv[2]) / det;
     }
-    if (!(isfinite(l[0]) && isfinite(l[1]) && isfinite(l[2]) && isfinite(qx) && isfinite(qy) && isfinite(qz) &&
-          isfinite(bv[0]) && isfinite(bv[1]) && isfinite(bv[2])))
+    if (!(isfinite(l[0]) && isfinite(l[1]) && isfinite(l[2]) && isfinite(PP.x) && isfinite(PP.y) &&
+          isfinite(PP.z) && isfinite(bv[0]) && isfinite(bv[1]) && isfinite(bv[2])))
         status |= 2;
     lam[s] = make_float4((float)l[0], (float)l[1], (float)l[2], (float)status);
     if (status & 6) return;
     const float gx = (float)(ru[0] - l[0]), gy = (float)(ru[1] - l[1]), gz = (float)(ru[2] - l[2]);
 #pragma unroll
     for (int m = 0; m < 8; ++m)
-        atomic_add4(gcot + 4 * idx[m], make_float4(w[m] * gx, w[m] * gy, w[m] * gz, w[m] * u.w));
+        atomic_add4(gcot + 4 * e.idx[m], make_float4(e.w[m] * gx, e.w[m] * gy, e.w[m] * gz, e.w[m] * u.w));
 }
 
 GridDev make_grid(const float* grid, const PpmGridSpec& s) {
@@ -478,19 +540,34 @@ int launch_relax_strokes(const float* grid, const PpmGridSpec& s, const float* p
     return check_launch("relax_strokes");
 }
 
-int launch_relax_adjoint(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
-                         const PpmTip& tip, const float* PPs, const float* dL_dFEs, float* grid_cot, float* lambdas,
-                         hipStream_t stream) {
-    AdjArgs p{};
-    p.n_scan = n_scan; p.nz = nz;
+static StateArgs state_args(int n_scan, int nz, const PpmTip& tip, int steps) {
+    StateArgs p{};
+    p.n_scan = n_scan; p.nz = nz; p.steps = steps;
     for (int q = 0; q < 3; ++q) { p.A[q] = tip.tipA[q]; p.B[q] = tip.tipB[q]; p.C[q] = tip.tipC[q]; }
     p.dtip = tip.tipC[3];
     for (int q = 0; q < 4; ++q) { p.k[q] = tip.stiffness[q]; p.d0[q] = tip.dpos0[q]; }
+    p.surf = tip.surfFF[0];
+    return p;
+}
+
+int launch_relax_adjoint(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
+                         const PpmTip& tip, const float* PPs, const float* dL_dFEs, float* grid_cot, float* lambdas,
+                         hipStream_t stream) {
     const size_t n = (size_t)n_scan * nz;
     hipLaunchKernelGGL(k_relax_adjoint, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, make_grid(grid, s),
-                       reinterpret_cast<const float4*>(pts), p, reinterpret_cast<const float4*>(PPs),
-                       reinterpret_cast<const float4*>(dL_dFEs), grid_cot, reinterpret_cast<float4*>(lambdas));
+                       reinterpret_cast<const float4*>(pts), state_args(n_scan, nz, tip, 0),
+                       reinterpret_cast<const float4*>(PPs), reinterpret_cast<const float4*>(dL_dFEs), grid_cot,
+                       reinterpret_cast<float4*>(lambdas));
     return check_launch("relax_adjoint");
+}
+
+int launch_relax_polish(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
+                        const PpmTip& tip, int steps, float* PPs, float* FEs, hipStream_t stream) {
+    const size_t n = (size_t)n_scan * nz;
+    hipLaunchKernelGGL(k_relax_polish, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, make_grid(grid, s),
+                       reinterpret_cast<const float4*>(pts), state_args(n_scan, nz, tip, steps),
+                       reinterpret_cast<float4*>(PPs), reinterpret_cast<float4*>(FEs));
+    return check_launch("relax_polish");
 }
 
 int launch_fe_in_strokes(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,

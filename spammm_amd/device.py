@@ -241,6 +241,24 @@ def relax_adjoint(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, dL_dFEs,
     return grid_cot, lam
 
 
+def relax_polish(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, FEs=None, steps=3):
+    """FP64 Newton polish of converged probe positions in place (and FEs resampled there)."""
+    _f32dev(grid, "grid", 4)
+    _f32dev(points, "points", 4)
+    _f32dev(PPs, "PPs", 4)
+    ns = int(points.shape[0])
+    if PPs.numel() != ns * nz * 4 or (FEs is not None and FEs.numel() != ns * nz * 4):
+        raise ValueError("PPs / FEs must be (n_scan, nz, 4)")
+    if FEs is not None:
+        _f32dev(FEs, "FEs", 4)
+    L = _lib.load()
+    s, tp = spec.c(), tip.c()
+    rc = L.ppm_relax_polish(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz), ctypes.byref(tp),
+                            int(steps), PPs.data_ptr(), FEs.data_ptr() if FEs is not None else None, _stream(grid))
+    _lib.check(rc)
+    return PPs, FEs
+
+
 def grid_adjoint(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, grid, grid_cot, npbc=(0, 0), lvA=(0, 0, 0),
                  lvB=(0, 0, 0), cap=True, workspace=None):
     """dL/d(x,y,z,q,p0,p1) per atom (na,6) float64 from dL/dgrid (p0,p1 = c6,c12 for LJ, R0,E0 for Morse)."""

@@ -183,8 +183,18 @@ class AxisOperator:
         """Apply to a float32 device tensor ``x`` (any layout given by strides in floats)."""
         import torch
         so, sa, si = axis_strides
+        if x.dtype != torch.float32 or not x.is_cuda or not x.is_contiguous():
+            raise ValueError("AxisOperator.apply expects a contiguous float32 CUDA tensor")
+        if min(so, sa, si) < 0 or not 1 <= nch <= 4 or offset < 0:
+            raise ValueError("invalid strides / channels / offset")
+        last = offset + (n_outer - 1) * so + (self.n_in - 1) * sa + (n_inner - 1) * si + (nch - 1)
+        if last >= x.numel():
+            raise ValueError(f"operator would read element {last} of a tensor with {x.numel()}")
         if out is None:
             out = torch.empty((n_outer, self.n_out, n_inner), dtype=torch.float32, device=x.device)
+        elif (out.dtype != torch.float32 or not out.is_contiguous() or out.device != x.device
+              or out.numel() != n_outer * self.n_out * n_inner):
+            raise ValueError("out must be a contiguous float32 tensor of n_outer*n_out*n_inner elements")
         L = _lib.load()
         stream = torch.cuda.current_stream(x.device).cuda_stream
         rc = L.ppm_axis_operator(x.data_ptr() + 4 * offset, so, sa, si, _lib.f4(chw), nch, n_outer, self.n_in,

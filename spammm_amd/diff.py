@@ -42,6 +42,7 @@ class Problem:
     max_iter: int = 1000
     f2conv: float = 1e-12
     check_before: bool = True
+    polish: int = 3             # FP64 Newton steps on each converged probe position (0: FIRE result as is)
     df_kind: str = "giessibl"
     amp: float = 1.0
     f0: float = 30000.0
@@ -78,12 +79,18 @@ def lj_cpar(R0, E0):
     return torch.stack([2.0 * E0d * r6, E0d * r6 * r6], dim=1).float().contiguous()
 
 
-def _forward_parts(atoms, cpar, prob: Problem):
-    import torch
-    grid = D.grid_project(atoms, cpar, _lib.PPM_MODEL_LJ, prob.spec, prob.Qs, prob.QZs, prob.npbc, prob.lvA,
-                          prob.lvB, cap=prob.cap)
+def _relax(grid, prob: Problem):
     FEs, PPs = D.relax_tilted(grid, prob.spec, prob.pts, prob.nz, prob.tip, prob.max_iter, prob.f2conv,
                               prob.check_before, scan_ny=int(prob.nxy[1]))
+    if prob.polish > 0:
+        D.relax_polish(grid, prob.spec, prob.pts, prob.nz, prob.tip, PPs, FEs, prob.polish)
+    return FEs, PPs
+
+
+def _forward_parts(atoms, cpar, prob: Problem):
+    grid = D.grid_project(atoms, cpar, _lib.PPM_MODEL_LJ, prob.spec, prob.Qs, prob.QZs, prob.npbc, prob.lvA,
+                          prob.lvB, cap=prob.cap)
+    FEs, PPs = _relax(grid, prob)
     op = DF.z_operator(prob.df_kind, prob.nz, prob.dz, atoms.device, prob.amp, prob.f0, prob.k_cant)
     df = op.apply(FEs, (prob.nz * 4, 4, 0), prob.n_scan, 1, offset=2)
     return grid, FEs, PPs, df.reshape(int(prob.nxy[0]), int(prob.nxy[1]), prob.nz)
@@ -121,8 +128,7 @@ def _make_function():
             atoms, cpar, grid, R0, E0, PPs = ctx.saved_tensors
             dev = atoms.device
             if prob.recompute:
-                _, PPs = D.relax_tilted(grid, prob.spec, prob.pts, prob.nz, prob.tip, prob.max_iter, prob.f2conv,
-                                        prob.check_before, scan_ny=int(prob.nxy[1]))
+                _, PPs = _relax(grid, prob)
             g = gdf.detach().float().contiguous().reshape(prob.n_scan, prob.nz)
             opT = DF.z_operator(prob.df_kind, prob.nz, prob.dz, dev, prob.amp, prob.f0, prob.k_cant, transpose=True)
             U = torch.zeros((prob.n_scan, prob.nz, 4), dtype=torch.float32, device=dev)

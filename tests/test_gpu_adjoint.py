@@ -72,10 +72,18 @@ def test_adjoint_matches_fp64_finite_differences():
     assert prob.last_status["bad"] == 0
     grads = {k: t[k].grad.cpu().numpy() for k in t}
     tgt = target.cpu().numpy()
+    d64 = fp64_df(w, prob, th)
+    dgpu = df.detach().cpu().numpy()
+    print(f"[adjoint] forward df GPU vs FP64: max {np.abs(dgpu - d64).max():.3e} "
+          f"(scale {np.abs(d64).max():.3e}), residual scale {np.abs(d64 - tgt).max():.3e}")
+
+    # The adjoint is the vector-Jacobian product v^T d(df)/d(theta) with v = dL/ddf of the GPU forward.
+    # Finite differences of phi(theta) = v . df_FP64(theta) with the same fixed v check that product
+    # directly; the FP32 forward's own df error (~1e-5 of df) does not enter.
+    v = dgpu - tgt
 
     def L64(thx):
-        d = fp64_df(w, prob, thx)
-        return 0.5 * ((d - tgt) ** 2).sum()
+        return float((v * fp64_df(w, prob, thx)).sum())
 
     na = th["q"].size
     picks = []
@@ -98,6 +106,31 @@ def test_adjoint_matches_fp64_finite_differences():
     for (k, idx), a, f, r in zip(picks, an, fd, rel):
         print(f"  {k}{idx}: adjoint {a:+.6e}  fd {f:+.6e}  rel {r:.2e}")
     assert rel.max() < 1e-3, f"worst relative error {rel.max():.3e}"
+
+
+def test_adjoint_descent_direction_on_loss():
+    """Along -grad, the loss of the GPU forward decreases at the rate |grad| (directional FD, eps=1e-3 A)."""
+    w, prob, th = setup_case(11)
+    rng = np.random.default_rng(2)
+    th_t = {k: v.copy() for k, v in th.items()}
+    th_t["xyz"] = th["xyz"] + rng.normal(0, 0.1, th["xyz"].shape)
+    with torch.no_grad():
+        target = gpu_df(prob, th_t)[0].double()
+    df, t = gpu_df(prob, th, requires_grad=True)
+    (0.5 * ((df.double() - target) ** 2).sum()).backward()
+    g = t["xyz"].grad.cpu().numpy()
+    d = -g / np.linalg.norm(g)
+    eps = 1e-3
+
+    def loss_at(x):
+        th2 = dict(th)
+        th2["xyz"] = x
+        with torch.no_grad():
+            return float(0.5 * ((gpu_df(prob, th2)[0].double() - target) ** 2).sum())
+    slope = (loss_at(th["xyz"] + eps * d) - loss_at(th["xyz"] - eps * d)) / (2 * eps)
+    print(f"[descent] directional slope {slope:.6e} vs -|g| {-np.linalg.norm(g):.6e}")
+    assert slope < 0
+    assert abs(slope + np.linalg.norm(g)) <= 2e-2 * np.linalg.norm(g)
 
 
 def test_adjoint_recompute_equals_stored():

@@ -202,7 +202,32 @@ def test_relax_residual_at_converged_states():
     ftip = (d - d0[:3]) * k[:3] + d * (k[3] * (r - d0[3]) / r)[:, None]
     res = np.linalg.norm(fe[:, :3] + ftip, axis=1).reshape(conv.shape)
     assert conv.mean() > 0.95
-    assert res[conv].max() < 1.0001e-4, f"residual {res[conv].max():.3e}"
+    # FP32 evaluation of |F|^2 < 1e-8 in the kernel vs FP64 re-evaluation: allow 1e-6 of rounding
+    assert res[conv].max() < 1e-4 + 1e-6, f"residual {res[conv].max():.3e}"
+
+
+def test_newton_polish_reaches_fp64_equilibrium():
+    """After FIRE (|F|<1e-6) the FP64 Newton polish puts probe positions on the FP64 equilibrium of the
+    same trilinear model to well below the FIRE tolerance (<= 1e-5 A)."""
+    w = W.small()
+    G, sp = grid_gpu(w)
+    d = dev()
+    pts = D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy)
+    tip = tip_of(w)
+    FEs, PPs = D.relax_tilted(G, sp, torch.as_tensor(pts, device=d), w.nz, tip, 4000, 1e-12, True)
+    pp_fire = PPs.cpu().numpy().copy()
+    D.relax_polish(G, sp, torch.as_tensor(pts, device=d), w.nz, tip, PPs, FEs, 3)
+    FEo, PPo = O.relax_tilted(G.cpu().numpy(), sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pts, w.nz, tip.tipA, tip.tipB,
+                              tip.tipC, tip.stiffness, tip.dpos0, tip.relax, tip.surfFF, 20000, 1e-24, True, "f64")
+    P = PPs.cpu().numpy()
+    conv = (P[..., 3] > 0) & (PPo[..., 3] > 0)
+    e_fire = np.abs(pp_fire[..., :3] - PPo[..., :3]).max(-1)[conv]
+    e_pol = np.abs(P[..., :3] - PPo[..., :3]).max(-1)[conv]
+    dfz = np.abs(FEs.cpu().numpy()[..., 2] - FEo[..., 2])[conv]
+    print(f"[polish] PP err FIRE max {e_fire.max():.2e} -> polished max {e_pol.max():.2e} "
+          f"median {np.median(e_pol):.2e}; Fz err max {dfz.max():.2e}")
+    assert e_pol.max() <= 1e-5
+    assert dfz.max() <= 1e-5
 
 
 def test_relax_strokes_and_raw_fe_bitexact():
