@@ -72,6 +72,46 @@ typedef struct PpmRelaxCtl {
                              (enables 8x8 pixel tiles per wavefront); 0: no raster */
 } PpmRelaxCtl;
 
+/* One molecule of a batch (an array of these lives in device memory).  The molecule's grid
+ * geometry is that of PpmGridSpec; only the z planes [klo, klo + nzw) are computed and stored
+ * (at grid_off float4s into the batch grid buffer, plane-major, x fastest).  Its scan is the
+ * ix-major raster start(ix,iy) = (scan_p0 + scan_da*ix) + scan_db*iy, ix < scan_nx,
+ * iy < scan_ny; per-pixel outputs start at pixel pix_off.  Work items item_off .. are the
+ * molecule's grid tiles followed by its relaxation blocks (ppm_batch_items computes them). */
+typedef struct PpmMolDesc {
+    int atom_off, na;
+    int nx, ny, nz;
+    int klo, nzw;
+    int npbc_a, npbc_b;
+    int pix_off, scan_nx, scan_ny;
+    int item_off, n_grid_items, n_relax_items, reserved;
+    long long grid_off;
+    float p0[4], dA[4], dB[4], dC[4];
+    float dinvA[4], dinvB[4], dinvC[4];
+    float lvA[4], lvB[4];
+    float scan_p0[4], scan_da[4], scan_db[4];
+} PpmMolDesc;
+
+/* Settings shared by every molecule of a batch. */
+typedef struct PpmBatchParams {
+    int model;          /* PPM_MODEL_LJ or PPM_MODEL_MORSE (cpar rows of 2 or 4 floats) */
+    int cap;            /* |E|>100 rescale of the grid */
+    int nz;             /* z-steps of every scan */
+    int nmol, nitems;
+    int vz;             /* grid voxels per thread along z: 4 or 8 */
+    int tile_w;         /* pixels along iy of a wavefront's pixel tile: 4, 8 or 16 (64/tile_w along ix) */
+    int mode;           /* PPM_BATCH_FUSED, PPM_BATCH_GRID_ONLY or PPM_BATCH_RELAX_ONLY */
+    int df_taps, df_nz_out;
+    int reserved[2];
+    float Qs[4], QZs[4];
+    PpmTip tip;
+    PpmRelaxCtl ctl;
+} PpmBatchParams;
+
+#define PPM_BATCH_FUSED       0   /* grid tiles and relaxation blocks from one work queue */
+#define PPM_BATCH_GRID_ONLY   1
+#define PPM_BATCH_RELAX_ONLY  2
+
 /* Library version (PPM_API_VERSION). */
 int ppm_version(void);
 
@@ -153,6 +193,31 @@ int ppm_relax_adjoint(const float* grid, const PpmGridSpec* spec,
 int ppm_relax_polish(const float* grid, const PpmGridSpec* spec,
                      const float* points, int n_scan, int nz, const PpmTip* tip,
                      int steps, float* PPs, float* FEs, void* stream);
+
+/* Bytes of device workspace (work queue, per-molecule counters and flags) of a batch of nmol. */
+size_t ppm_batch_workspace_size(int nmol);
+
+/* Batched forward (no reference counterpart: the reference runs one image at a time through
+ * make_forcefield / run_scan / compute_df, spammm/SPM/AFM.py:897-1010, :3048-3089).
+ * Runs grid projection, relaxed scan (relaxStrokesTilted semantics) and the df operator of
+ * every molecule described by `mols` (device array of params->nmol PpmMolDesc) in one
+ * persistent launch of `workers` workgroups.
+ *   atoms / cpar: concatenated rows of all molecules (PpmMolDesc.atom_off);
+ *   grids: stored z windows of all grids; FEs, PPs (PPs may be NULL): (n_pix_total, nz, 4);
+ *   df (may be NULL): (n_pix_total, df_nz_out), operator dfW (df_nz_out*df_taps), dfStart;
+ *   workspace: >= ppm_batch_workspace_size(nmol) bytes; after the call workspace int[1] is
+ *     the error word (bit 1: a wait hit its spin limit) and ints [4+nmol, 4+2*nmol) are
+ *     per-molecule flags (nonzero: a probe left the stored z window — recompute that molecule
+ *     with the whole grid);
+ *   reset: 1 = clear queue, counters and flags first; 2 = clear the queue head only (second
+ *     launch of a two-launch GRID_ONLY + RELAX_ONLY sequence); 0 = no clearing.
+ * The caller builds the descriptors (item_off, n_grid_items, n_relax_items, window, offsets)
+ * consistently with the buffers; the library cannot check device-resident descriptors. */
+int ppm_batch_forward(const PpmMolDesc* mols, const PpmBatchParams* params,
+                      const float* atoms, const float* cpar, float* grids,
+                      float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
+                      void* workspace, size_t workspace_bytes, int workers, int reset,
+                      void* stream);
 
 /* Bytes of device workspace ppm_grid_adjoint needs for na atoms on this grid (0 on bad input). */
 size_t ppm_grid_adjoint_workspace_size(int na, const PpmGridSpec* spec);

@@ -40,7 +40,8 @@ class PpmRelaxCtl(ctypes.Structure):
 EXPORTS = (
     "ppm_version", "ppm_last_error", "ppm_grid_project", "ppm_relax_strokes_tilted", "ppm_relax_strokes",
     "ppm_fe_in_strokes_tilted", "ppm_z_stencil", "ppm_axis_operator", "ppm_relax_adjoint",
-    "ppm_grid_adjoint_workspace_size", "ppm_grid_adjoint", "ppm_relax_polish",
+    "ppm_grid_adjoint_workspace_size", "ppm_grid_adjoint", "ppm_relax_polish", "ppm_batch_workspace_size",
+    "ppm_batch_forward",
 )
 
 _lib = None
@@ -84,6 +85,10 @@ def load():
     L.ppm_axis_operator.argtypes = [vp, ll, ll, ll, _F4, i, i, i, i, vp, vp, i, i, f, i, vp, vp]
     L.ppm_relax_adjoint.restype = i
     L.ppm_relax_adjoint.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), vp, vp, vp, vp, vp]
+    L.ppm_batch_workspace_size.restype = ctypes.c_size_t
+    L.ppm_batch_workspace_size.argtypes = [i]
+    L.ppm_batch_forward.restype = i
+    L.ppm_batch_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, i, i, vp]
     L.ppm_relax_polish.restype = i
     L.ppm_relax_polish.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), i, vp, vp, vp]
     L.ppm_grid_adjoint_workspace_size.restype = ctypes.c_size_t

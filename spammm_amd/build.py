@@ -25,9 +25,10 @@ ARCH = os.environ.get("PPM_OFFLOAD_ARCH", "gfx950")
 # (source, extra flags)
 SOURCES = [
     ("ppm_api.hip", []),
-    ("ppm_grid.hip", ["-ffp-contract=fast"]),
+    ("ppm_grid.hip", ["-ffp-contract=off"]),
     ("ppm_relax.hip", ["-ffp-contract=off"]),
     ("ppm_df.hip", ["-ffp-contract=fast"]),
+    ("ppm_batch.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -161,6 +161,35 @@ int ppm_relax_polish(const float* grid, const PpmGridSpec* spec, const float* po
     return launch_relax_polish(grid, *spec, points, n_scan, nz, *tip, steps, PPs, FEs, (hipStream_t)stream);
 }
 
+size_t ppm_batch_workspace_size(int nmol) { return nmol < 1 ? 0 : batch_workspace(nmol); }
+
+int ppm_batch_forward(const PpmMolDesc* mols, const PpmBatchParams* params, const float* atoms, const float* cpar,
+                      float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
+                      void* workspace, size_t workspace_bytes, int workers, int reset, void* stream) {
+    const char* fn = "ppm_batch_forward";
+    g_err[0] = 0;
+    if (!mols || !params || !atoms || !cpar || !grids || !FEs || !workspace) {
+        set_error("%s: NULL argument", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    const PpmBatchParams& P = *params;
+    if (P.model != PPM_MODEL_LJ && P.model != PPM_MODEL_MORSE) { set_error("%s: unknown model %d", fn, P.model); return PPM_ERR_INVALID_ARG; }
+    if (P.nmol < 1 || P.nitems < 1 || P.nz < 1) { set_error("%s: empty batch", fn); return PPM_ERR_INVALID_ARG; }
+    if (P.vz != 4 && P.vz != 8) { set_error("%s: vz=%d not in {4,8}", fn, P.vz); return PPM_ERR_INVALID_ARG; }
+    if (P.tile_w != 4 && P.tile_w != 8 && P.tile_w != 16) { set_error("%s: tile_w=%d not in {4,8,16}", fn, P.tile_w); return PPM_ERR_INVALID_ARG; }
+    if (P.mode < PPM_BATCH_FUSED || P.mode > PPM_BATCH_RELAX_ONLY) { set_error("%s: unknown mode %d", fn, P.mode); return PPM_ERR_INVALID_ARG; }
+    if (int r = check_ctl(fn, &P.ctl, 1)) return r;
+    if (df && (!dfW || !dfStart || P.df_taps < 1 || P.df_taps > P.nz || P.df_nz_out < 1)) {
+        set_error("%s: df requested with an invalid operator", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (workers < 1 || workers > (1 << 16)) { set_error("%s: workers=%d out of range", fn, workers); return PPM_ERR_INVALID_ARG; }
+    if (reset < 0 || reset > 2) { set_error("%s: reset=%d not in {0,1,2}", fn, reset); return PPM_ERR_INVALID_ARG; }
+    if (workspace_bytes < batch_workspace(P.nmol)) { set_error("%s: workspace too small", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_batch(mols, P, atoms, cpar, grids, FEs, PPs, df, dfW, dfStart, workspace, workers, reset,
+                        (hipStream_t)stream);
+}
+
 size_t ppm_grid_adjoint_workspace_size(int na, const PpmGridSpec* spec) {
     if (!spec || na < 1 || check_spec("ppm_grid_adjoint_workspace_size", spec) != PPM_OK) return 0;
     return grid_adjoint_workspace(na, *spec);

@@ -1,0 +1,231 @@
+// ppm_batch.hip — batched forward: many molecules (uneven atom counts, cell sizes and grid
+// shapes) through grid projection -> relaxed scan -> df in ONE persistent launch.
+//
+// Work items, in queue order, are for each molecule m: its grid tiles (256 columns x vz planes
+// of the stored z window), then its relaxation blocks (4 wavefronts x 64 pixels; each lane
+// relaxes one pixel through all z-steps and then applies the df operator to its own Fz column).
+// Persistent workgroups take items from an atomic counter.  A relaxation block of m waits (with
+// a bounded spin) until every grid tile of m has been published: the tile's stores are released
+// at agent scope before the per-molecule counter is incremented, and the waiting workgroup
+// acquires at agent scope before it reads the grid, so the data crosses XCD L2 caches correctly.
+// Items are taken in increasing order, so a waited-for tile has always been taken by a running
+// workgroup: the wait ends.  Every workgroup leaves the loop when the queue is exhausted.
+//
+// The grid and relaxation arithmetic is the device code of the plain kernels
+// (ppm_grid_core.hpp, ppm_relax_core.hpp), so each molecule's results equal the plain path's
+// on the same inputs, bit for bit, whenever no probe leaves the stored z window (a probe that
+// does sets the molecule's flag; the caller recomputes that molecule on the plain path).
+#include "ppm_grid_core.hpp"
+#include "ppm_relax_core.hpp"
+
+namespace ppm {
+namespace {
+
+using gc::GridArgs;
+using gc::Site;
+
+
+constexpr int kSpinLimit = 1 << 22;
+
+struct Queue {           // device workspace, zeroed by k_batch_reset
+    int head;            // next work item
+    int error;           // bit 1: a wait hit the spin limit
+    int pad[2];
+    // followed by: int done[nmol]; int flag[nmol];
+};
+
+__device__ __forceinline__ int* q_done(Queue* q) { return reinterpret_cast<int*>(q + 1); }
+__device__ __forceinline__ int* q_flag(Queue* q, int nmol) { return reinterpret_cast<int*>(q + 1) + nmol; }
+
+__global__ void k_batch_reset(Queue* q, int nmol, int head_only) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) { q->head = 0; if (!head_only) q->error = 0; }
+    if (!head_only && i < 2 * nmol) q_done(q)[i] = 0;
+}
+
+__device__ GridArgs mol_grid_args(const PpmMolDesc& d, const PpmBatchParams& P) {
+    GridArgs a{};
+    a.nx = d.nx; a.ny = d.ny; a.nz = d.nz; a.ncol = d.nx * d.ny;
+    for (int k = 0; k < 3; ++k) {
+        a.p0[k] = d.p0[k]; a.dA[k] = d.dA[k]; a.dB[k] = d.dB[k]; a.dC[k] = d.dC[k];
+        a.lvA[k] = d.lvA[k]; a.lvB[k] = d.lvB[k];
+    }
+    a.na = d.na;
+    a.npbc_a = d.npbc_a; a.npbc_b = d.npbc_b;
+    a.nimg_a = 2 * d.npbc_a + 1; a.nimg_b = 2 * d.npbc_b + 1;
+    a.nsites = d.na * a.nimg_a * a.nimg_b;
+    a.cap = P.cap;
+    a.q0 = 0.f;
+    a.noff = 0;
+    for (int k = 0; k < 4; ++k) {
+        if (P.Qs[k] == 0.f) continue;
+        if (P.QZs[k] == 0.f) {
+            a.q0 += P.Qs[k] * kCOULOMB;
+        } else {
+            a.qoff[a.noff] = P.Qs[k] * kCOULOMB;
+            a.zoff[a.noff] = P.QZs[k];
+            ++a.noff;
+        }
+    }
+    return a;
+}
+
+__device__ rc::TiltArgs tilt_args(const PpmBatchParams& P) {
+    rc::TiltArgs t{};
+    t.nz = P.nz; t.max_iter = P.ctl.max_iter; t.check_before = P.ctl.check_before; t.f2conv = P.ctl.f2conv;
+    for (int q = 0; q < 3; ++q) { t.A[q] = P.tip.tipA[q]; t.B[q] = P.tip.tipB[q]; t.C[q] = P.tip.tipC[q]; }
+    t.dtip = P.tip.tipC[3];
+    for (int q = 0; q < 4; ++q) { t.k[q] = P.tip.stiffness[q]; t.d0[q] = P.tip.dpos0[q]; }
+    t.dt0 = P.tip.relax[0]; t.damp0 = P.tip.relax[1]; t.surf = P.tip.surfFF[0];
+    return t;
+}
+
+template <int MODEL, bool HASQ0, int V>
+__device__ void do_grid_item(const PpmMolDesc& d, const PpmBatchParams& P, int local, const float4* atoms,
+                             const float* cpar, float4* grids, Site* sites) {
+    const GridArgs a = mol_grid_args(d, P);
+    const int ncolblk = (a.ncol + 255) / 256;
+    const int cb = local % ncolblk, zb = local / ncolblk;
+    const int cs = MODEL == PPM_MODEL_LJ ? 2 : 4;
+    gc::grid_tile<MODEL, HASQ0, V, true>(atoms + d.atom_off, cpar + (size_t)d.atom_off * cs, a, sites,
+                                         cb * 256 + threadIdx.x, d.klo + zb * V, d.klo + d.nzw, grids + d.grid_off,
+                                         d.klo);
+}
+
+__device__ void do_relax_item(const PpmMolDesc& d, const PpmBatchParams& P, int local, const float4* grids,
+                              float4* FEs, float4* PPs, float* df, const float* dfW, const int* dfStart,
+                              int* flag) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tw = P.tile_w, th = 64 / P.tile_w;
+    const int tiles_y = (d.scan_ny + tw - 1) / tw;
+    const int wt = local * 4 + wave;
+    const int tx = wt / tiles_y, ty = wt - tx * tiles_y;
+    const int ix = tx * th + lane / tw, iy = ty * tw + lane % tw;
+    if (ix >= d.scan_nx || iy >= d.scan_ny) return;
+    rc::GridDev g;
+    g.data = grids + d.grid_off;
+    g.nx = d.nx; g.ny = d.ny; g.nz = d.nz;
+    for (int q = 0; q < 4; ++q) { g.da[q] = d.dinvA[q]; g.db[q] = d.dinvB[q]; g.dc[q] = d.dinvC[q]; }
+    g.klo = d.klo; g.nzw = d.nzw; g.flag = flag;
+    const float fx = (float)ix, fy = (float)iy;
+    const f3 start = mk3(d.scan_p0[0] + d.scan_da[0] * fx + d.scan_db[0] * fy,
+                          d.scan_p0[1] + d.scan_da[1] * fx + d.scan_db[1] * fy,
+                          d.scan_p0[2] + d.scan_da[2] * fx + d.scan_db[2] * fy);
+    const size_t pix = (size_t)d.pix_off + (size_t)ix * d.scan_ny + iy;
+    const size_t o = pix * P.nz;
+    const rc::TiltArgs t = tilt_args(P);
+    rc::relax_tilted_lane(g, start, t, FEs + o, PPs ? PPs + o : nullptr);
+    if (df) {
+        // the lane's own Fz column, just written by this lane
+        for (int j = 0; j < P.df_nz_out; ++j) {
+            const int s0 = dfStart[j];
+            float acc = 0.f;
+            for (int k = 0; k < P.df_taps; ++k) {
+                const int iz = min(max(s0 + k, 0), P.nz - 1);
+                acc = fmaf(dfW[j * P.df_taps + k], FEs[o + iz].z, acc);
+            }
+            df[pix * P.df_nz_out + j] = acc;
+        }
+    }
+}
+
+template <int MODEL, bool HASQ0, int V>
+__global__ __launch_bounds__(256) void k_batch(const PpmMolDesc* __restrict__ mols, PpmBatchParams P,
+                                               const float4* __restrict__ atoms, const float* __restrict__ cpar,
+                                               float4* __restrict__ grids, float4* __restrict__ FEs,
+                                               float4* __restrict__ PPs, float* __restrict__ df,
+                                               const float* __restrict__ dfW, const int* __restrict__ dfStart,
+                                               Queue* q) {
+    __shared__ Site sites[gc::kTile];
+    __shared__ int s_item, s_mol;
+    int* done = q_done(q);
+    int* flags = q_flag(q, P.nmol);
+    while (true) {
+        if (threadIdx.x == 0) {
+            int item = atomicAdd(&q->head, 1);
+            int m = -1;
+            if (item < P.nitems) {
+                int lo = 0, hi = P.nmol - 1;            // last molecule with item_off <= item
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (mols[mid].item_off <= item) lo = mid; else hi = mid - 1;
+                }
+                m = lo;
+            }
+            s_item = item;
+            s_mol = m;
+        }
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(s_item), m = __builtin_amdgcn_readfirstlane(s_mol);
+        __syncthreads();
+        if (item >= P.nitems) break;
+        const PpmMolDesc& d = mols[m];
+        const int local = item - d.item_off;
+        if (local < d.n_grid_items) {
+            if (P.mode == PPM_BATCH_RELAX_ONLY) continue;
+            do_grid_item<MODEL, HASQ0, V>(d, P, local, atoms, cpar, grids, sites);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __syncthreads();
+            if (threadIdx.x == 0) __hip_atomic_fetch_add(&done[m], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (P.mode == PPM_BATCH_GRID_ONLY) continue;
+            if (threadIdx.x == 0) {
+                int spins = 0;
+                while (__hip_atomic_load(&done[m], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < d.n_grid_items) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (++spins > kSpinLimit) {
+                        atomicOr(&q->error, 1);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            do_relax_item(d, P, local - d.n_grid_items, grids, FEs, PPs, df, dfW, dfStart, &flags[m]);
+        }
+    }
+}
+
+template <int MODEL, bool HASQ0>
+void launch_v(int V, dim3 grd, hipStream_t st, const PpmMolDesc* mols, const PpmBatchParams& P, const float4* atoms,
+              const float* cpar, float4* grids, float4* FEs, float4* PPs, float* df, const float* dfW,
+              const int* dfStart, Queue* q) {
+    if (V == 8)
+        hipLaunchKernelGGL((k_batch<MODEL, HASQ0, 8>), grd, dim3(256), 0, st, mols, P, atoms, cpar, grids, FEs, PPs,
+                           df, dfW, dfStart, q);
+    else
+        hipLaunchKernelGGL((k_batch<MODEL, HASQ0, 4>), grd, dim3(256), 0, st, mols, P, atoms, cpar, grids, FEs, PPs,
+                           df, dfW, dfStart, q);
+}
+
+}  // namespace
+
+size_t batch_workspace(int nmol) { return sizeof(Queue) + 2 * (size_t)nmol * sizeof(int); }
+
+int launch_batch(const PpmMolDesc* mols, const PpmBatchParams& P, const float* atoms, const float* cpar,
+                 float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
+                 void* workspace, int workers, int reset, hipStream_t stream) {
+    Queue* q = reinterpret_cast<Queue*>(workspace);
+    if (reset) {
+        const int n = 2 * P.nmol + 1;
+        hipLaunchKernelGGL(k_batch_reset, dim3((n + 255) / 256), dim3(256), 0, stream, q, P.nmol, reset == 2);
+        if (int r = check_launch("batch_reset")) return r;
+    }
+    bool hasq0 = false;
+    for (int k = 0; k < 4; ++k) hasq0 |= (P.Qs[k] != 0.f && P.QZs[k] == 0.f);
+    const dim3 grd(workers);
+    const float4* at = reinterpret_cast<const float4*>(atoms);
+    float4* gr = reinterpret_cast<float4*>(grids);
+    float4* fe = reinterpret_cast<float4*>(FEs);
+    float4* pp = reinterpret_cast<float4*>(PPs);
+    if (P.model == PPM_MODEL_LJ) {
+        if (hasq0) launch_v<PPM_MODEL_LJ, true>(P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
+        else launch_v<PPM_MODEL_LJ, false>(P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
+    } else {
+        if (hasq0) launch_v<PPM_MODEL_MORSE, true>(P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
+        else launch_v<PPM_MODEL_MORSE, false>(P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
+    }
+    return check_launch("batch");
+}
+
+}  // namespace ppm

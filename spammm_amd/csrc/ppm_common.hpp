@@ -44,6 +44,10 @@ int launch_relax_adjoint(const float* grid, const PpmGridSpec& s, const float* p
                          hipStream_t stream);
 int launch_relax_polish(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
                         const PpmTip& tip, int steps, float* PPs, float* FEs, hipStream_t stream);
+size_t batch_workspace(int nmol);
+int launch_batch(const PpmMolDesc* mols, const PpmBatchParams& P, const float* atoms, const float* cpar,
+                 float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
+                 void* workspace, int workers, int reset, hipStream_t stream);
 size_t grid_adjoint_workspace(int na, const PpmGridSpec& g);
 int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],

@@ -1,0 +1,197 @@
+// ppm_relax_core.hpp — device code of the probe relaxation shared by the plain and the
+// batched kernels.  Translation units that include it are built with -ffp-contract=off: every
+// FP32 operation is evaluated exactly as written (see ppm_relax.hip for the semantics and
+// the reference kernels they follow).
+#pragma once
+#pragma clang fp contract(off)
+#include "ppm_common.hpp"
+
+namespace ppm {
+namespace rc {
+
+constexpr float kFTDEC = 0.5f, kFTINC = 1.1f, kFDAMP = 0.99f, kF2SAFE = 1e-8f;
+
+// A grid as the relaxation sees it.  Only the z planes [klo, klo + nzw) are stored (the
+// whole grid when klo = 0, nzw = nz); a sample that needs a plane outside them sets *flag and
+// reads clamped planes instead (its result is then not used: the caller recomputes with the
+// whole grid).
+struct GridDev {
+    const float4* __restrict__ data;
+    int nx, ny, nz;
+    float da[4], db[4], dc[4];
+    int klo, nzw;
+    int* flag;
+};
+
+struct Cell {
+    int i, j, k;       // cached cell origin (i0,j0,k0); -1 = empty
+    float4 c[8];       // corners 000,100,010,110,001,101,011,111
+};
+
+__device__ __forceinline__ void axis_repeat(float s, int n, int& i0, int& i1, float& a) {
+    const float u = (s - floorf(s)) * (float)n;
+    const float t = u - 0.5f;
+    float fl = floorf(t);
+    a = t - fl;
+    fl = fminf(fmaxf(fl, -1.0f), (float)n);   // keeps non-finite coordinates in range
+    int i = (int)fl;
+    int j = i + 1;
+    if (i < 0) i += n;
+    if (j > n - 1) j -= n;
+    i0 = min(max(i, 0), n - 1);
+    i1 = min(max(j, 0), n - 1);
+}
+
+__device__ __forceinline__ float4 lerp8(const Cell& C, float a, float b, float c) {
+    const float ma = 1.0f - a, mb = 1.0f - b, mc = 1.0f - c;
+    float4 o;
+#define PPM_TRI(q)                                                                                   \
+    {                                                                                               \
+        const float lo = (C.c[0].q * ma + C.c[1].q * a) * mb + (C.c[2].q * ma + C.c[3].q * a) * b;   \
+        const float hi = (C.c[4].q * ma + C.c[5].q * a) * mb + (C.c[6].q * ma + C.c[7].q * a) * b;   \
+        o.q = lo * mc + hi * c;                                                                      \
+    }
+    PPM_TRI(x) PPM_TRI(y) PPM_TRI(z) PPM_TRI(w)
+#undef PPM_TRI
+    return o;
+}
+
+__device__ __forceinline__ float4 sample(const GridDev& g, Cell& C, float px, float py, float pz) {
+    const float sa = px * g.da[0] + py * g.da[1] + pz * g.da[2] + g.da[3];
+    const float sb = px * g.db[0] + py * g.db[1] + pz * g.db[2] + g.db[3];
+    const float sc = px * g.dc[0] + py * g.dc[1] + pz * g.dc[2] + g.dc[3];
+    int i0, i1, j0, j1, k0, k1;
+    float a, b, c;
+    axis_repeat(sa, g.nx, i0, i1, a);
+    axis_repeat(sb, g.ny, j0, j1, b);
+    axis_repeat(sc, g.nz, k0, k1, c);
+    if (i0 != C.i || j0 != C.j || k0 != C.k) {
+        C.i = i0; C.j = j0; C.k = k0;
+        k0 -= g.klo;
+        k1 -= g.klo;
+        if ((unsigned)k0 >= (unsigned)g.nzw || (unsigned)k1 >= (unsigned)g.nzw) {
+            if (g.flag) atomicOr(g.flag, 1);
+            k0 = min(max(k0, 0), g.nzw - 1);
+            k1 = min(max(k1, 0), g.nzw - 1);
+        }
+        const size_t sx = (size_t)g.nx, sxy = (size_t)g.nx * g.ny;
+        const size_t r00 = (size_t)k0 * sxy + (size_t)j0 * sx, r10 = (size_t)k0 * sxy + (size_t)j1 * sx;
+        const size_t r01 = (size_t)k1 * sxy + (size_t)j0 * sx, r11 = (size_t)k1 * sxy + (size_t)j1 * sx;
+        C.c[0] = g.data[r00 + i0]; C.c[1] = g.data[r00 + i1];
+        C.c[2] = g.data[r10 + i0]; C.c[3] = g.data[r10 + i1];
+        C.c[4] = g.data[r01 + i0]; C.c[5] = g.data[r01 + i1];
+        C.c[6] = g.data[r11 + i0]; C.c[7] = g.data[r11 + i1];
+    }
+    return lerp8(C, a, b, c);
+}
+
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 rotMat(f3 v, f3 a, f3 b, f3 c) { return mk3(dot3(v, a), dot3(v, b), dot3(v, c)); }
+__device__ __forceinline__ f3 rotMatT(f3 v, f3 a, f3 b, f3 c) {
+    return mk3(a.x * v.x + b.x * v.y + c.x * v.z, a.y * v.x + b.y * v.y + c.y * v.z,
+               a.z * v.x + b.z * v.y + c.z * v.z);
+}
+__device__ __forceinline__ f3 tipForce(f3 d, const float* k, const float* d0) {
+    float r = sqrtf(dot3(d, d));
+    r = fmaxf(r, 1e-10f);
+    const float s = (k[3] * (r - d0[3])) / r;
+    return mk3((d.x - d0[0]) * k[0] + d.x * s, (d.y - d0[1]) * k[1] + d.y * s, (d.z - d0[2]) * k[2] + d.z * s);
+}
+// update_FIRE: returns the new velocity, updates dt and damp.
+__device__ __forceinline__ f3 fire(f3 f, f3 v, float& dt, float& damp, float dtmin, float dtmax, float damp0) {
+    const float ff = dot3(f, f), vv = dot3(v, v), vf = dot3(v, f);
+    if (vf < 0.0f) {
+        dt = fmaxf(dtmin, dt * kFTDEC);
+        damp = damp0;
+        return mk3(0.f, 0.f, 0.f);
+    }
+    const float cv = 1.0f - damp;
+    const float cf = damp * sqrtf(vv / (ff + kF2SAFE));
+    dt = fminf(dtmax, dt * kFTINC);
+    damp = damp * kFDAMP;
+    return mk3(v.x * cv + f.x * cf, v.y * cv + f.y * cf, v.z * cv + f.z * cf);
+}
+
+__device__ __forceinline__ bool finite4(float4 v) {
+    return isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w);
+}
+
+// lane -> scan point index; -1 when the lane has no point.
+__device__ __forceinline__ int lane_point(int n_scan, int scan_ny) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (scan_ny > 0) {
+        const int nxs = n_scan / scan_ny;
+        const int tiles_y = (scan_ny + 7) >> 3;
+        const int tx = wave / tiles_y, ty = wave - tx * tiles_y;
+        const int ix = tx * 8 + (lane >> 3), iy = ty * 8 + (lane & 7);
+        return (ix < nxs && iy < scan_ny) ? ix * scan_ny + iy : -1;
+    }
+    const int p = wave * 64 + lane;
+    return p < n_scan ? p : -1;
+}
+
+struct TiltArgs {
+    int n_scan, nz, max_iter, check_before, scan_ny;
+    float f2conv;
+    float A[3], B[3], C[3], dtip;
+    float k[4], d0[4];
+    float dt0, damp0, surf;
+};
+
+// relaxStrokesTilted (kernels/AFM.cl:478-562) for one scan point starting at tip anchor
+// `start`; writes FEs[iz] (tip-rotated sample FE) and PPs[iz] (position, +/-iterations).
+__device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, const TiltArgs& p,
+                                                  float4* __restrict__ FEs, float4* __restrict__ PPs) {
+    const f3 A = ld3(p.A), B = ld3(p.B), C = ld3(p.C);
+    const f3 dTip = mk3(C.x * p.dtip, C.y * p.dtip, C.z * p.dtip);
+    const f3 d0r = rotMatT(ld3(p.d0), A, B, C);
+    f3 tip = start;
+    f3 pos = mk3(tip.x + d0r.x, tip.y + d0r.y, tip.z + d0r.z);
+    float dt = p.dt0, damp = p.damp0;
+    const float dtmax = dt, dtmin = dtmax * 0.1f, damp0 = damp;
+    const f3 fsurf = mk3(C.x * p.surf, C.y * p.surf, C.z * p.surf);
+    Cell cell;
+    cell.i = -1; cell.j = -1; cell.k = -1;
+    f3 v = mk3(0.f, 0.f, 0.f);
+    int it = 0, iz = 0;
+    int o = 0;
+    while (true) {
+        const float4 fe = sample(g, cell, pos.x, pos.y, pos.z);
+        f3 f = mk3(fe.x, fe.y, fe.z);
+        const f3 dp = mk3(pos.x - tip.x, pos.y - tip.y, pos.z - tip.z);
+        const f3 ft = rotMatT(tipForce(rotMat(dp, A, B, C), p.k, p.d0), A, B, C);
+        f = mk3(f.x + ft.x, f.y + ft.y, f.z + ft.z);
+        f = mk3(f.x + fsurf.x, f.y + fsurf.y, f.z + fsurf.z);
+        const float ff = dot3(f, f);
+        bool conv = false, done = false;
+        if (p.check_before && ff < p.f2conv) {
+            conv = done = true;
+        } else {
+            v = fire(f, v, dt, damp, dtmin, dtmax, damp0);
+            v = mk3(v.x + f.x * dt, v.y + f.y * dt, v.z + f.z * dt);
+            pos = mk3(pos.x + v.x * dt, pos.y + v.y * dt, pos.z + v.z * dt);
+            if (!p.check_before && ff < p.f2conv) conv = done = true;
+            else if (it + 1 >= p.max_iter) done = true;
+        }
+        ++it;
+        if (done) {
+            const float4 fs = sample(g, cell, pos.x, pos.y, pos.z);
+            const f3 fr = rotMat(mk3(fs.x, fs.y, fs.z), A, B, C);
+            FEs[o] = make_float4(fr.x, fr.y, fr.z, fs.w);
+            if (PPs) {
+                const bool fin = finite4(fs) && isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z);
+                PPs[o] = make_float4(pos.x, pos.y, pos.z, (conv && fin) ? (float)it : -(float)it);
+            }
+            ++o;
+            tip = mk3(tip.x + dTip.x, tip.y + dTip.y, tip.z + dTip.z);
+            pos = mk3(pos.x + dTip.x, pos.y + dTip.y, pos.z + dTip.z);
+            v = mk3(0.f, 0.f, 0.f);
+            it = 0;
+            if (++iz >= p.nz) break;
+        }
+    }
+}
+
+}  // namespace rc
+}  // namespace ppm

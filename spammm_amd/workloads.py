@@ -234,6 +234,32 @@ def batch_molecule(i: int, base_seed=100) -> Workload:
                     nxy=(128, 128), nz=40)
 
 
+def batch(n_mol=512, base_seed=100):
+    """The batched configuration: ``n_mol`` molecules (see ``batch_molecule``)."""
+    return [batch_molecule(i, base_seed) for i in range(n_mol)]
+
+
+def small_batch_molecule(i: int, base_seed=500) -> Workload:
+    """A reduced batch member (coarse 0.25 A grid, small scan) that the oracle checks in seconds."""
+    rng = np.random.default_rng(base_seed + i)
+    na = int(rng.integers(6, 41))
+    Lxy = float(rng.uniform(8.0, 12.0))
+    en, pos = flake_geometry(na, rng)
+    q = rng.uniform(-0.3, 0.3, len(en))
+    q -= q.mean()
+    atoms = np.zeros((len(en), 4), np.float32)
+    atoms[:, :3] = pos
+    atoms[:, 3] = q
+    n = int(round(Lxy / 0.25))
+    L = np.array([n * 0.25, n * 0.25, 12.0], np.float32)
+    nxs, nys = int(rng.integers(9, 25)), int(rng.integers(7, 21))
+    return Workload(f"sbatch{i}", en, atoms, _lj_params(en), (n, n, 48),
+                    np.array([-L[0] / 2, -L[1] / 2, -3.0], np.float32), L, (1, 1),
+                    scan_p0=np.array([-L[0] / 2 + 0.3, -L[1] / 2 + 0.2, 10.5], np.float32),
+                    scan_da=np.array([(L[0] - 0.6) / nxs, 0., 0.], np.float32),
+                    scan_db=np.array([0., (L[1] - 0.4) / nys, 0.], np.float32), nxy=(nxs, nys), nz=14)
+
+
 def small(seed=7, n=(48, 40, 40), nxy=(12, 10), nz=12) -> Workload:
     """A reduced naphthalene case (coarser grid) that the FP64 oracle finishes in seconds."""
     w = naphthalene(seed)

@@ -97,3 +97,33 @@ def test_workloads_match_configuration():
     heavy = [e for e in f.enames if e != "H"]
     assert abs(sum(e in ("N", "O") for e in heavy) - 0.1 * len(heavy)) <= 1
     assert f.n == (300, 300, 240) and f.nxy == (256, 256) and f.nz == 60
+
+
+def test_batch_struct_layouts_match_header():
+    from spammm_amd import batch as B
+    assert ctypes.sizeof(B.PpmMolDesc) == 16 * 4 + 8 + 12 * 16
+    assert B.PpmMolDesc.grid_off.offset == 64
+    assert ctypes.sizeof(B.PpmBatchParams) == 12 * 4 + 2 * 16 + 7 * 16 + 16
+
+
+def test_batch_z_window_covers_probe_heights():
+    from spammm_amd import batch as B
+    w = W.batch_molecule(3)
+    sp = D.axis_aligned_spec(w.p0, w.L, w.n)
+    tip = D.Tip(stiffness=w.stiffness, dpos0=w.dpos0, tipC=np.array([0., 0., 1., w.dtip], np.float32))
+    klo, nzw = B.z_window(w, sp, tip, w.nz, 1.5)
+    z = sp.p0[2] + sp.dC[2] * np.array([klo, klo + nzw - 1])
+    probe_hi = w.scan_p0[2] - w.bond
+    probe_lo = probe_hi + (w.nz - 1) * w.dtip
+    assert z[0] <= probe_lo - 1.5 and z[1] >= probe_hi + 1.5 + sp.dC[2]
+    assert nzw < w.n[2] // 2
+    assert B.z_window(w, sp, tip, w.nz, 100.0) == (0, w.n[2])
+
+
+def test_batch_workload_matches_configuration():
+    ms = W.batch(16)
+    na = [m.atoms.shape[0] for m in ms]
+    assert min(na) >= 10 and max(na) <= 150
+    for m in ms:
+        assert 160 <= m.n[0] <= 320 and m.n[0] == m.n[1] and m.n[2] == 200
+        assert m.nxy == (128, 128) and m.nz == 40
