@@ -45,6 +45,7 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")
     ap.add_argument("--adam-steps", type=int, default=50)
+    ap.add_argument("--batch-mols", type=int, default=512, help="molecules of the batched leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (every box has one MI355X); "
@@ -167,11 +168,70 @@ def main():
 
     if not args.no_diff:
         out["differentiable"] = diff_leg(dev, args.adam_steps)
+    if args.batch_mols > 0:
+        out["batched"] = batch_leg(dev, args.batch_mols)
     if not args.no_check:
         out["accuracy"] = accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(w, sp, grid, pts_h, tip, states)
     print(json.dumps(out), flush=True)
+
+
+def batch_leg(dev, n_mol):
+    """Batched forward of n_mol molecules (U{10..150} atoms, square cells U(16,32) A x 20 A, 0.1 A grids,
+    128x128 pixels, 40 z-steps) in one persistent launch replayed from a hipGraph, against the loop of
+    plain per-image launches over the same molecules; outputs compared bit for bit."""
+    import torch
+    from spammm_amd import autotune, batch as B, workloads as WL
+    mols = WL.batch(n_mol)
+    t0 = time.perf_counter()
+    bf = B.BatchForward(mols, dev, want_pps=False)
+    tune_sub = B.BatchForward(mols[:min(32, n_mol)], dev, want_pps=False)
+    best, timings = autotune.tune(tune_sub, use_cache=False, reps=1)
+    del tune_sub
+    log(f"[bench] batch setup {time.perf_counter() - t0:.1f}s, autotuned {best.key()}")
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize(dev)
+            ts.append(a.elapsed_time(b))
+        return ts
+
+    bf.capture(best)
+    tb = timed(bf.graph.replay, 3)
+    err, flags = bf.status()
+    fe_plain = torch.empty_like(bf.FEs)
+    df_plain = torch.empty_like(bf.df)
+    tp = timed(lambda: bf.plain_loop(fe_plain, df_plain), 2)
+    fallback = np.flatnonzero(flags)
+    if fallback.size:
+        bf.recompute_plain(fallback)
+        torch.cuda.synchronize(dev)
+    same = bool(torch.equal(bf.df, df_plain) and torch.equal(bf.FEs, fe_plain))
+    states = bf.states
+    t_b, t_p = min(tb), min(tp)
+    res = {
+        "workload": f"{n_mol} molecules, U(10..150) atoms, cells U(16,32) A x 20 A, 0.1 A grids, 128x128 px x 40 z",
+        "batched_ms": t_b, "plain_loop_ms": t_p,
+        "batched_pixel_z_steps_per_s": states / (t_b * 1e-3),
+        "plain_pixel_z_steps_per_s": states / (t_p * 1e-3),
+        "speedup": t_p / t_b,
+        "launch": best.key(), "autotune_ms": timings,
+        "projected_pair_evals": bf.pair_evals,
+        "full_grid_pair_evals": int(sum(w.pair_evals for w in mols)),
+        "window_fallback_molecules": int(fallback.size), "wait_timeout": int(err),
+        "bitexact_vs_plain": same,
+    }
+    log(f"[bench] batched: {res}")
+    del bf, fe_plain, df_plain
+    torch.cuda.empty_cache()
+    return res
 
 
 def diff_leg(dev, adam_steps=50):
