@@ -177,8 +177,11 @@ int ppm_fe_in_strokes_tilted(const float* grid, const PpmGridSpec* spec,
  * ppm_relax_strokes_tilted) it solves J^T l = (dO/dq)^T u with the 3x3 Jacobian J of the
  * total force (FP64) and ADDS the resulting cotangent of every touched grid value into
  * grid_cot ((nz,ny,nx,4), caller-zeroed; float atomics).  lambdas (n_scan,nz,4): l and a
- * status word in .w (bit 1: state not converged, 2: non-finite, 4: singular J; states with
- * 2 or 4 add nothing).  points/tip must be those of the forward call. */
+ * status word in .w.  A state is used only if it passes the gates of the reference's
+ * direct-path adjoint (kernels/AFM.cl:1778, :1811-1840); status bits (0 = used): 1 residual
+ * |G(q*)| >= 5e-4, 2 non-finite, 4 singular or condition number >= 1e6, 8 unstable (smallest
+ * eigenvalue of the symmetric part of -J <= 1e-5).  A gated state adds nothing and has l = 0.
+ * points/tip must be those of the forward call. */
 int ppm_relax_adjoint(const float* grid, const PpmGridSpec* spec,
                       const float* points, int n_scan, int nz, const PpmTip* tip,
                       const float* PPs, const float* dL_dFEs,
@@ -193,6 +196,43 @@ int ppm_relax_adjoint(const float* grid, const PpmGridSpec* spec,
 int ppm_relax_polish(const float* grid, const PpmGridSpec* spec,
                      const float* points, int n_scan, int nz, const PpmTip* tip,
                      int steps, float* PPs, float* FEs, void* stream);
+
+/* Largest atom count of the gridless direct path (the reference's limit,
+ * spammm/SPM/AFM.py:2105-2106: nAtoms in [1,128], no fallback). */
+#define PPM_DIRECT_MAX_ATOMS 128
+
+/* Gridless direct Morse + point-charge relaxed scan: relaxStrokesTiltedMorseDirect
+ * (kernels/AFM.cl:1419-1511) as launched by AFMulator.run_scan_morse_direct
+ * (spammm/SPM/AFM.py:2063-2233).  atoms (na,4) = (x,y,z,Q); cMs (na,4) = (R0,E0,K<0,0);
+ * pair law uncapped, no periodic images; Qs/QZs = tip charges and their world-z offsets
+ * (scaled by COULOMB_CONST inside).  tip->tipC.w = dtip (must be < 0).  The |f|^2 < f2conv
+ * test runs before each update (ctl->check_before is ignored); ctl->max_iter plays the role of
+ * N_RELAX_STEP_MAX.  FEs (n_scan,nz,4): tip-rotated sample FE at the final probe position;
+ * PPs (n_scan,nz,4): (position, +iterations if converged else -iterations), may be NULL.
+ * 1 <= na <= PPM_DIRECT_MAX_ATOMS. */
+int ppm_relax_direct_morse(const float* atoms, const float* cMs, int na,
+                           const float* points, int n_scan, int nz, const PpmTip* tip,
+                           const float Qs[4], const float QZs[4], const PpmRelaxCtl* ctl,
+                           float* FEs, float* PPs, void* stream);
+
+/* Bytes of device workspace of ppm_direct_morse_vjp (per-block FP64 partial sums). */
+size_t ppm_direct_morse_vjp_workspace_size(int na, int n_scan, int nz);
+
+/* Implicit-equilibrium VJP of ppm_relax_direct_morse: morseDirectStateAdjoint,
+ * morseDirectParamPartials and reduceMorseDirectParamPartials (kernels/AFM.cl:1711-2029) as
+ * driven by AFMulator.vjp_scan_morse_direct (spammm/SPM/AFM.py:2241-2519), evaluated in FP64.
+ * PPs / dL_dFEs (n_scan,nz,4) as produced / consumed around the forward call (same atoms,
+ * points and tip).  Per state: lambdas (n_scan,nz,4) = (lambda, 0); diag (n_scan,nz,4) =
+ * (residual, lambda_min, condition estimate, status) with the reference's status codes
+ * 0 ok, 1 not converged or residual >= 5e-4, 2 non-finite, 3 unstable (lambda_min <= 1e-5),
+ * 4 singular or condition >= 1e6; a state with status != 0 contributes nothing.
+ * grad (na,6) double, theta order (x,y,z,R0,E0,Q), overwritten (no atomics: deterministic). */
+int ppm_direct_morse_vjp(const float* atoms, const float* cMs, int na,
+                         const float* points, int n_scan, int nz, const PpmTip* tip,
+                         const float Qs[4], const float QZs[4],
+                         const float* PPs, const float* dL_dFEs,
+                         float* lambdas, float* diag, void* workspace, size_t workspace_bytes,
+                         double* grad, void* stream);
 
 /* Bytes of device workspace (work queue, per-molecule counters and flags) of a batch of nmol. */
 size_t ppm_batch_workspace_size(int nmol);

@@ -34,6 +34,17 @@ own formula; the tests live in ``tests/test_oracle_*.py``):
   at tight convergence.
 * ``fe_in_strokes_tilted`` (kernels/AFM.cl:210-233): equals ``interp`` at the
   stroke positions, rotated into the tip frame.
+* ``relax_direct`` (relaxStrokesTiltedMorseDirect, kernels/AFM.cl:1419-1511):
+  residual |G(PP)| < 1e-4 at every converged state, recomputed independently
+  (tests/SPM/test_afm_morse.py:1048-1089); equals ``relax_tilted``'s loop with the
+  direct pair sum as sample (same code path); FP32 and FP64 modes agree at tight
+  convergence.
+* ``direct_vjp`` (kernels/AFM.cl:1711-2029): channel by channel against the
+  finite-difference component construction of the reference's CPU64 VJP oracle
+  (tests/SPM/test_afm_morse.py:1240-1309: J, dO/dq, dG/dtheta, dO/dtheta by central
+  differences of ``pair_fe_d``), and against re-relaxed finite differences of
+  ``relax_direct_d`` with a plateau over three step sizes
+  (tests/SPM/test_afm_morse.py:1523-1582); gates checked on constructed states.
 """
 from __future__ import annotations
 
@@ -88,6 +99,13 @@ def lib():
                                                _d, _d, _d, _d, c_int, _d]
         L.oracle_relax_tilted_f64g.argtypes = [_d, _i, _d, _d, _d, _d, c_int, c_int, _d, _d, _d, _d, _d, _d, _d,
                                                c_int, ctypes.c_double, c_int, _d, _d]
+        for nm in ("oracle_relax_direct_f64", "oracle_relax_direct_f32"):
+            getattr(L, nm).argtypes = [_f, _f, c_int, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, _f, _f, _f,
+                                       c_int, c_float, _f, _f]
+        L.oracle_relax_direct_f64d.argtypes = [_d, _d, c_int, _d, _d, _d, c_int, c_int, _d, _d, _d, _d, _d, _d, _d,
+                                               c_int, ctypes.c_double, _d, _d]
+        L.oracle_direct_vjp_f64.argtypes = [_d, _d, c_int, _d, _d, _d, c_int, c_int, _d, _d, _d, _d, _d, _d,
+                                            _d, _d, _d, _d, _d]
         _lib = L
     return _lib
 
@@ -280,3 +298,61 @@ def relax_tilted_d(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC, stif
                                    *[_p(t, _d) for t in ts], int(max_iter), float(f2conv), int(bool(check_before)),
                                    _p(FEs, _d), _p(PPs, _d))
     return FEs, PPs
+
+
+# ── gridless direct Morse path ────────────────────────────────────────────────
+
+def relax_direct(atoms, cMs, Qs, QZs, pts, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF, max_iter=128,
+                 f2conv=1e-8, precision="f64"):
+    """relaxStrokesTiltedMorseDirect: (FEs, PPs) float32 (n_scan,nz,4); cMs rows (R0,E0,K,0)."""
+    atoms = _cf(atoms).reshape(-1, 4)
+    cMs = _cf(cMs).reshape(-1, 4)
+    pts = _cf(pts).reshape(-1, 4)
+    ns = pts.shape[0]
+    FEs = np.zeros((ns, nz, 4), dtype=np.float32)
+    PPs = np.zeros((ns, nz, 4), dtype=np.float32)
+    q, qz = _vec(Qs, 4), _vec(QZs, 4)
+    vs = [_vec(x, 4) for x in (tipA, tipB, tipC, stiffness, dpos0, relax, surfFF)]
+    fn = lib().oracle_relax_direct_f64 if precision == "f64" else lib().oracle_relax_direct_f32
+    fn(_p(atoms, _f), _p(cMs, _f), atoms.shape[0], _p(q, _f), _p(qz, _f), _p(pts, _f), ns, int(nz),
+       *[_p(v, _f) for v in vs], int(max_iter), float(f2conv), _p(FEs, _f), _p(PPs, _f))
+    return FEs, PPs
+
+
+def relax_direct_d(atoms, cMs, Qs, QZs, pts, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF, max_iter=128,
+                   f2conv=1e-8):
+    """relax_direct in double end to end: (FEs, PPs) float64."""
+    atoms = _cd(atoms).reshape(-1, 4)
+    cMs = _cd(cMs).reshape(-1, 4)
+    pts = _cd(pts).reshape(-1, 4)
+    ns = pts.shape[0]
+    FEs = np.zeros((ns, nz, 4))
+    PPs = np.zeros((ns, nz, 4))
+    q, qz = _dvec(Qs, 4), _dvec(QZs, 4)
+    vs = [_dvec(x, 4) for x in (tipA, tipB, tipC, stiffness, dpos0, relax, surfFF)]
+    lib().oracle_relax_direct_f64d(_p(atoms, _d), _p(cMs, _d), atoms.shape[0], _p(q, _d), _p(qz, _d), _p(pts, _d),
+                                   ns, int(nz), *[_p(v, _d) for v in vs], int(max_iter), float(f2conv),
+                                   _p(FEs, _d), _p(PPs, _d))
+    return FEs, PPs
+
+
+def direct_vjp(atoms, cMs, Qs, QZs, pts, nz, tipA, tipB, tipC, stiffness, dpos0, surfFF, PPs, dL_dFEs):
+    """Implicit-equilibrium VJP of the direct path in double.
+
+    Returns (grad (na,6) in theta order x,y,z,R0,E0,Q; lambdas (n_state,4); diag (n_state,4) =
+    (residual, lambda_min, cond, status))."""
+    atoms = _cd(atoms).reshape(-1, 4)
+    cMs = _cd(cMs).reshape(-1, 4)
+    pts = _cd(pts).reshape(-1, 4)
+    ns = pts.shape[0]
+    PPs = _cd(PPs).reshape(ns * nz, 4)
+    U = _cd(dL_dFEs).reshape(ns * nz, 4)
+    q, qz = _dvec(Qs, 4), _dvec(QZs, 4)
+    vs = [_dvec(x, 4) for x in (tipA, tipB, tipC, stiffness, dpos0, surfFF)]
+    grad = np.zeros((atoms.shape[0], 6))
+    lam = np.zeros((ns * nz, 4))
+    diag = np.zeros((ns * nz, 4))
+    lib().oracle_direct_vjp_f64(_p(atoms, _d), _p(cMs, _d), atoms.shape[0], _p(q, _d), _p(qz, _d), _p(pts, _d), ns,
+                                int(nz), *[_p(v, _d) for v in vs], _p(PPs, _d), _p(U, _d), _p(lam, _d),
+                                _p(diag, _d), _p(grad, _d))
+    return grad, lam, diag

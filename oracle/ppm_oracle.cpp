@@ -18,7 +18,11 @@
 //   * tipForce (kernels/AFM.cl:53-58), update_FIRE (kernels/AFM.cl:121-157),
 //     rotMat/rotMatT (kernels/common.cl:44-45);
 //   * relaxStrokesTilted (kernels/AFM.cl:478-562), relaxStrokes
-//     (kernels/AFM.cl:345-394) and getFEinStrokesTilted (kernels/AFM.cl:210-233).
+//     (kernels/AFM.cl:345-394) and getFEinStrokesTilted (kernels/AFM.cl:210-233);
+//   * the gridless direct Morse path: relaxStrokesTiltedMorseDirect
+//     (kernels/AFM.cl:1419-1511, pair sum evalMorseCDirect_local :1391-1407) and its
+//     implicit-equilibrium adjoint morseDirectStateAdjoint / morseDirectParamPartials /
+//     reduceMorseDirectParamPartials (kernels/AFM.cl:1546-2029), restated in double.
 //
 // Each function exists in two precisions:
 //   *_f64 — every operation in double: the error oracle;
@@ -31,6 +35,7 @@
 #include <cstdint>
 #include <cstring>
 #include <algorithm>
+#include <vector>
 
 namespace {
 
@@ -141,10 +146,12 @@ template <class R> inline V3<R> update_FIRE(V3<R> f, V3<R> v, R& dt, R& damp, R 
 // convergence / -iters otherwise) in the convention of relaxStrokesTiltedMorseDirect
 // (kernels/AFM.cl:1494-1505).  check_before=1 moves the |f|^2 test before the update
 // as relaxStrokesTiltedMorseDirect does (kernels/AFM.cl:1478).
-template <class R, class GT, class P>
-void relax_tilted(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P* tipA, const P* tipB, const P* tipC,
-                  const P* stiffness, const P* dpos0, const P* relax, const P* surfFF, int max_iter, double f2conv_f,
-                  int check_before, P* FEs, P* PPs) {
+// `sample(pos)` returns the sample FE at a world point: the trilinear grid sample for
+// relaxStrokesTilted, the direct pair sum for relaxStrokesTiltedMorseDirect.
+template <class R, class SRC, class P>
+void relax_tilted_src(const SRC& sample, const P* pts, int n_scan, int nz, const P* tipA, const P* tipB,
+                      const P* tipC, const P* stiffness, const P* dpos0, const P* relax, const P* surfFF, int max_iter,
+                      double f2conv_f, int check_before, P* FEs, P* PPs) {
     const V3<R> A = ld3<R>(tipA), B = ld3<R>(tipB), C = ld3<R>(tipC);
     const R f2conv = (R)f2conv_f;
 #pragma omp parallel for schedule(dynamic, 8)
@@ -160,7 +167,7 @@ void relax_tilted(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P*
             int it = 0;
             bool conv = false;
             for (it = 0; it < max_iter; ++it) {
-                FE<R> fe = interp<R>(g, pos);
+                FE<R> fe = sample(pos);
                 V3<R> f = v3<R>(fe.x, fe.y, fe.z);
                 V3<R> dpos = sub(pos, tipPos);
                 V3<R> dpos_ = rotMat(dpos, A, B, C);
@@ -175,7 +182,7 @@ void relax_tilted(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P*
                 if (!check_before && ff < f2conv) { conv = true; break; }
             }
             int iters = conv ? it + 1 : max_iter;
-            FE<R> fe = interp<R>(g, pos);
+            FE<R> fe = sample(pos);
             V3<R> fr = rotMat(v3<R>(fe.x, fe.y, fe.z), A, B, C);
             const size_t o = 4 * ((size_t)ip * nz + iz);
             FEs[o + 0] = (P)fr.x; FEs[o + 1] = (P)fr.y; FEs[o + 2] = (P)fr.z; FEs[o + 3] = (P)fe.w;
@@ -189,6 +196,14 @@ void relax_tilted(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P*
             pos = add(pos, dTip);
         }
     }
+}
+
+template <class R, class GT, class P>
+void relax_tilted(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P* tipA, const P* tipB, const P* tipC,
+                  const P* stiffness, const P* dpos0, const P* relax, const P* surfFF, int max_iter, double f2conv_f,
+                  int check_before, P* FEs, P* PPs) {
+    relax_tilted_src<R>([&g](V3<R> p) { return interp<R>(g, p); }, pts, n_scan, nz, tipA, tipB, tipC, stiffness,
+                        dpos0, relax, surfFF, max_iter, f2conv_f, check_before, FEs, PPs);
 }
 
 // relaxStrokes   kernels/AFM.cl:345-394.  FE output is the sample taken at the start
@@ -322,6 +337,87 @@ template <class R, class P> inline void sample_fe(const AtomsT<P>& S, V3<R> p, R
                 pair_atom<R>(S.model, p, a, (R)at[3], cp, Qs, QZs, acc);
             }
     }
+}
+
+
+// ---------------------------------------------------------------------------------------
+// Gridless direct Morse path (aperiodic, uncapped): relaxStrokesTiltedMorseDirect
+// kernels/AFM.cl:1419-1511 — relaxStrokesTilted with the direct pair sum as the sample and
+// the |f|^2 test before the update (kernels/AFM.cl:1478).
+template <class R, class P>
+void relax_direct(const P* atoms, const P* cMs, int na, const P* Qs, const P* QZs, const P* pts, int n_scan, int nz,
+                  const P* tipA, const P* tipB, const P* tipC, const P* stiffness, const P* dpos0, const P* relax,
+                  const P* surfFF, int max_iter, double f2conv, P* FEs, P* PPs) {
+    AtomsT<P> S{1, na, atoms, cMs, 4, {}, {}, {0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < 4; ++k) { S.Qs[k] = (double)Qs[k]; S.QZs[k] = (double)QZs[k]; }
+    auto sample = [&S](V3<R> p) {
+        R acc[4];
+        sample_fe<R>(S, p, acc);
+        return FE<R>{acc[0], acc[1], acc[2], acc[3]};
+    };
+    relax_tilted_src<R>(sample, pts, n_scan, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF, max_iter, f2conv,
+                        1, FEs, PPs);
+}
+
+// Per-atom pair quantities of the direct Morse + Coulomb law at probe position q (double):
+// force F_i on the probe, energy E_i, energy Hessian H_i = d2E_i/dq2 and the parameter
+// derivatives used by the adjoint (kernels/AFM.cl:1546-1614 sample FE + Hessian,
+// :1873-1940 partials).
+struct PairD {
+    double F[3], E, H[3][3];
+    double dF_dR0[3], dF_dE0[3], dF_dQ[3];
+    double dE_dR0, dE_dE0, dE_dQ;
+};
+
+inline void pair_direct(const double q[3], const double a[4], const double cm[4], const double Qk[4],
+                        const double QZk[4], PairD& o) {
+    const double d[3] = {q[0] - a[0], q[1] - a[1], q[2] - a[2]};
+    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2] + kR2SAFE;
+    const double r = std::sqrt(r2);
+    const double R0 = cm[0], E0 = cm[1], K = cm[2];
+    const double s = std::exp(K * (r - R0));
+    const double Ep = 2.0 * K * E0 * s * (s - 1.0);             // dE/dr
+    const double Epp = 2.0 * K * K * E0 * s * (2.0 * s - 1.0);  // d2E/dr2
+    const double cout = (Epp - Ep / r) / r2, cdia = Ep / r;
+    o.E = E0 * s * (s - 2.0);
+    for (int i = 0; i < 3; ++i) {
+        o.F[i] = d[i] * (-Ep / r);
+        o.dF_dR0[i] = d[i] * (2.0 * K * K * E0 * s * (2.0 * s - 1.0) / r);
+        o.dF_dE0[i] = d[i] * (-2.0 * K * s * (s - 1.0) / r);
+        o.dF_dQ[i] = 0.0;
+        for (int j = 0; j < 3; ++j) o.H[i][j] = cout * d[i] * d[j] + (i == j ? cdia : 0.0);
+    }
+    o.dE_dR0 = -2.0 * K * E0 * s * (s - 1.0);
+    o.dE_dE0 = s * (s - 2.0);
+    o.dE_dQ = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        if (Qk[k] == 0.0) continue;
+        const double dk[3] = {q[0] - a[0], q[1] - a[1], q[2] + QZk[k] - a[2]};
+        const double rk2 = dk[0] * dk[0] + dk[1] * dk[1] + dk[2] * dk[2] + kR2SAFE;
+        const double rk = std::sqrt(rk2), rk3 = rk * rk2, rk5 = rk3 * rk2;
+        const double C = a[3] * Qk[k];
+        o.E += C / rk;
+        o.dE_dQ += Qk[k] / rk;
+        for (int i = 0; i < 3; ++i) {
+            o.F[i] += dk[i] * (C / rk3);
+            o.dF_dQ[i] += dk[i] * (Qk[k] / rk3);
+            for (int j = 0; j < 3; ++j) o.H[i][j] += 3.0 * C * dk[i] * dk[j] / rk5 - (i == j ? C / rk3 : 0.0);
+        }
+    }
+}
+
+// Smallest eigenvalue of a symmetric 3x3 matrix (Smith, Commun. ACM 4(4):168, 1961).
+inline double eigmin_sym3(const double S[3][3]) {
+    const double p1 = S[0][1] * S[0][1] + S[0][2] * S[0][2] + S[1][2] * S[1][2];
+    if (p1 == 0.0) return std::min(S[0][0], std::min(S[1][1], S[2][2]));
+    const double q = (S[0][0] + S[1][1] + S[2][2]) / 3.0;
+    const double a = S[0][0] - q, b = S[1][1] - q, c = S[2][2] - q;
+    const double p = std::sqrt((a * a + b * b + c * c + 2.0 * p1) / 6.0);
+    const double B00 = a / p, B11 = b / p, B22 = c / p, B01 = S[0][1] / p, B02 = S[0][2] / p, B12 = S[1][2] / p;
+    const double detB = B00 * (B11 * B22 - B12 * B12) - B01 * (B01 * B22 - B12 * B02) + B02 * (B01 * B12 - B11 * B02);
+    const double r = std::min(std::max(0.5 * detB, -1.0), 1.0);
+    const double phi = std::acos(r) / 3.0;
+    return q + 2.0 * p * std::cos(phi + 2.0 * M_PI / 3.0);
 }
 
 }  // namespace
@@ -520,6 +616,147 @@ void oracle_relax_tilted_f64g(const double* grid, const int* n, const double* di
     for (int q = 0; q < 4; ++q) { g.dinv[0][q] = dinvA[q]; g.dinv[1][q] = dinvB[q]; g.dinv[2][q] = dinvC[q]; }
     relax_tilted<double>(g, pts, n_scan, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF, max_iter, f2conv,
                          check_before, FEs, PPs);
+}
+
+// relaxStrokesTiltedMorseDirect  (kernels/AFM.cl:1419-1511); cMs rows (R0,E0,K,0).
+#define RELAX_DIRECT(NAME, R)                                                                                      \
+    void NAME(const float* atoms, const float* cMs, int na, const float* Qs, const float* QZs, const float* pts,    \
+              int n_scan, int nz, const float* tipA, const float* tipB, const float* tipC, const float* stiffness,  \
+              const float* dpos0, const float* relax, const float* surfFF, int max_iter, float f2conv, float* FEs,  \
+              float* PPs) {                                                                                         \
+        relax_direct<R>(atoms, cMs, na, Qs, QZs, pts, n_scan, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF, \
+                        max_iter, f2conv, FEs, PPs);                                                                \
+    }
+RELAX_DIRECT(oracle_relax_direct_f64, double)
+RELAX_DIRECT(oracle_relax_direct_f32, float)
+
+// The same in double end to end (double inputs and outputs), for finite differences.
+void oracle_relax_direct_f64d(const double* atoms, const double* cMs, int na, const double* Qs, const double* QZs,
+                              const double* pts, int n_scan, int nz, const double* tipA, const double* tipB,
+                              const double* tipC, const double* stiffness, const double* dpos0, const double* relax,
+                              const double* surfFF, int max_iter, double f2conv, double* FEs, double* PPs) {
+    relax_direct<double>(atoms, cMs, na, Qs, QZs, pts, n_scan, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF,
+                         max_iter, f2conv, FEs, PPs);
+}
+
+// Implicit-equilibrium VJP of the direct path in double (kernels/AFM.cl:1711-2029 semantics):
+// per state, gates in the reference order — non-finite PP/u (2), PP.w <= 0 (1),
+// residual |G| >= 5e-4 (1), non-finite Hessian (2), singular J (4), non-finite lambda (2),
+// smallest eigenvalue of -J <= 1e-5 (3), condition number >= 1e6 (4); J = -H + R^T (dg/dd) R,
+// b = -H R^T u - F u.w, J lambda = b; w = R^T u - lambda; per atom
+// dL/da = H_i w + F_i u.w, dL/dR0 = w.dF/dR0 + u.w dE/dR0, ... (theta order x,y,z,R0,E0,Q).
+// A gated state contributes nothing.  diag[s] = (residual, lambda_min, cond, status).
+void oracle_direct_vjp_f64(const double* atoms, const double* cMs, int na, const double* Qs, const double* QZs,
+                           const double* pts, int n_scan, int nz, const double* tipA, const double* tipB,
+                           const double* tipC, const double* stiffness, const double* dpos0, const double* surfFF,
+                           const double* PPs, const double* U, double* lambdas, double* diag, double* grad) {
+    double Qk[4], QZk[4];
+    for (int k = 0; k < 4; ++k) { Qk[k] = Qs[k] * kCOULOMB; QZk[k] = QZs[k]; }
+    const double Rm[3][3] = {{tipA[0], tipA[1], tipA[2]}, {tipB[0], tipB[1], tipB[2]}, {tipC[0], tipC[1], tipC[2]}};
+    for (int i = 0; i < 6 * na; ++i) grad[i] = 0.0;
+    std::vector<PairD> pr(na);
+    for (int ip = 0; ip < n_scan; ++ip)
+        for (int iz = 0; iz < nz; ++iz) {
+            const size_t st = (size_t)ip * nz + iz;
+            const double* pp = PPs + 4 * st;
+            const double* u = U + 4 * st;
+            double* lam = lambdas + 4 * st;
+            double* dg = diag + 4 * st;
+            lam[0] = lam[1] = lam[2] = lam[3] = 0.0;
+            dg[0] = dg[1] = dg[2] = 0.0;
+            bool fin = true;
+            for (int q = 0; q < 4; ++q) fin = fin && std::isfinite(pp[q]) && std::isfinite(u[q]);
+            if (!fin) { dg[3] = 2; continue; }
+            if (!(pp[3] > 0.0)) { dg[3] = 1; continue; }
+            const double q[3] = {pp[0], pp[1], pp[2]};
+            double tip[3];
+            for (int k = 0; k < 3; ++k) tip[k] = pts[4 * (size_t)ip + k] + tipC[k] * tipC[3] * (double)iz;
+            double F[3] = {0, 0, 0}, H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+            for (int j = 0; j < na; ++j) {
+                pair_direct(q, atoms + 4 * (size_t)j, cMs + 4 * (size_t)j, Qk, QZk, pr[j]);
+                for (int a = 0; a < 3; ++a) {
+                    F[a] += pr[j].F[a];
+                    for (int b = 0; b < 3; ++b) H[a][b] += pr[j].H[a][b];
+                }
+            }
+            // total force G and tip-spring Jacobian in tip coordinates
+            double dw[3] = {q[0] - tip[0], q[1] - tip[1], q[2] - tip[2]}, d[3];
+            for (int a = 0; a < 3; ++a) d[a] = Rm[a][0] * dw[0] + Rm[a][1] * dw[1] + Rm[a][2] * dw[2];
+            double r = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            r = std::max(r, 1e-10);
+            const double phi = stiffness[3] * (1.0 - dpos0[3] / r);
+            const double beta = stiffness[3] * dpos0[3] / (r * r * r);
+            double gt[3], Jt[3][3];
+            for (int a = 0; a < 3; ++a) {
+                gt[a] = (d[a] - dpos0[a]) * stiffness[a] + d[a] * (stiffness[3] * (r - dpos0[3]) / r);
+                for (int b = 0; b < 3; ++b) Jt[a][b] = beta * d[a] * d[b] + (a == b ? stiffness[a] + phi : 0.0);
+            }
+            double G[3], J[3][3];
+            for (int a = 0; a < 3; ++a) {
+                G[a] = F[a] + Rm[0][a] * gt[0] + Rm[1][a] * gt[1] + Rm[2][a] * gt[2] + tipC[a] * surfFF[0];
+                for (int b = 0; b < 3; ++b) {
+                    double acc = -H[a][b];
+                    for (int m = 0; m < 3; ++m)
+                        for (int n = 0; n < 3; ++n) acc += Rm[m][a] * Jt[m][n] * Rm[n][b];
+                    J[a][b] = acc;
+                }
+            }
+            const double res = std::sqrt(G[0] * G[0] + G[1] * G[1] + G[2] * G[2]);
+            dg[0] = res;
+            if (!(res < 5e-4)) { dg[3] = 1; continue; }
+            bool hfin = true;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) hfin = hfin && std::isfinite(H[a][b]);
+            if (!hfin) { dg[3] = 2; continue; }
+            double uw[3];
+            for (int a = 0; a < 3; ++a) uw[a] = Rm[0][a] * u[0] + Rm[1][a] * u[1] + Rm[2][a] * u[2];
+            double bv[3];
+            for (int a = 0; a < 3; ++a) bv[a] = -(H[a][0] * uw[0] + H[a][1] * uw[1] + H[a][2] * uw[2]) - F[a] * u[3];
+            // cofactors and determinant of J
+            double cof[3][3];
+            cof[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+            cof[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+            cof[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+            cof[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+            cof[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+            cof[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+            cof[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+            cof[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+            cof[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+            const double det = J[0][0] * cof[0][0] + J[0][1] * cof[0][1] + J[0][2] * cof[0][2];
+            const double scale = std::max(std::max(std::fabs(J[0][0]), std::fabs(J[1][1])),
+                                          std::max(std::fabs(J[2][2]), 1e-20));
+            if (std::fabs(det) < 1e-18 * scale * scale * scale) { dg[3] = 4; continue; }
+            double l[3];
+            for (int a = 0; a < 3; ++a) l[a] = (cof[0][a] * bv[0] + cof[1][a] * bv[1] + cof[2][a] * bv[2]) / det;
+            if (!(std::isfinite(l[0]) && std::isfinite(l[1]) && std::isfinite(l[2]))) { dg[3] = 2; continue; }
+            double S[3][3];
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) S[a][b] = -0.5 * (J[a][b] + J[b][a]);
+            const double lmin = eigmin_sym3(S);
+            dg[1] = lmin;
+            if (!(lmin > 1e-5)) { dg[3] = 3; continue; }
+            double jn = 0.0, an = 0.0;
+            for (int a = 0; a < 3; ++a) {
+                jn = std::max(jn, std::fabs(J[a][0]) + std::fabs(J[a][1]) + std::fabs(J[a][2]));
+                an = std::max(an, std::fabs(cof[0][a]) + std::fabs(cof[1][a]) + std::fabs(cof[2][a]));
+            }
+            const double cond = jn * an / std::max(std::fabs(det), 1e-300);
+            dg[2] = cond;
+            if (!(cond < 1e6)) { dg[3] = 4; continue; }
+            dg[3] = 0;
+            lam[0] = l[0]; lam[1] = l[1]; lam[2] = l[2];
+            const double w[3] = {uw[0] - l[0], uw[1] - l[1], uw[2] - l[2]};
+            for (int j = 0; j < na; ++j) {
+                const PairD& P = pr[j];
+                double* g = grad + 6 * (size_t)j;
+                for (int a = 0; a < 3; ++a)
+                    g[a] += P.H[a][0] * w[0] + P.H[a][1] * w[1] + P.H[a][2] * w[2] + P.F[a] * u[3];
+                g[3] += w[0] * P.dF_dR0[0] + w[1] * P.dF_dR0[1] + w[2] * P.dF_dR0[2] + u[3] * P.dE_dR0;
+                g[4] += w[0] * P.dF_dE0[0] + w[1] * P.dF_dE0[1] + w[2] * P.dF_dE0[2] + u[3] * P.dE_dE0;
+                g[5] += w[0] * P.dF_dQ[0] + w[1] * P.dF_dQ[1] + w[2] * P.dF_dQ[2] + u[3] * P.dE_dQ;
+            }
+        }
 }
 
 }  // extern "C"

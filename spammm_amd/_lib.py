@@ -17,6 +17,7 @@ PPM_ERR_HIP = 2
 PPM_ERR_UNSUPPORTED = 3
 PPM_MODEL_LJ = 0
 PPM_MODEL_MORSE = 1
+PPM_DIRECT_MAX_ATOMS = 128
 
 _F4 = ctypes.c_float * 4
 
@@ -41,7 +42,7 @@ EXPORTS = (
     "ppm_version", "ppm_last_error", "ppm_grid_project", "ppm_relax_strokes_tilted", "ppm_relax_strokes",
     "ppm_fe_in_strokes_tilted", "ppm_z_stencil", "ppm_axis_operator", "ppm_relax_adjoint",
     "ppm_grid_adjoint_workspace_size", "ppm_grid_adjoint", "ppm_relax_polish", "ppm_batch_workspace_size",
-    "ppm_batch_forward",
+    "ppm_batch_forward", "ppm_relax_direct_morse", "ppm_direct_morse_vjp_workspace_size", "ppm_direct_morse_vjp",
 )
 
 _lib = None
@@ -89,6 +90,13 @@ def load():
     L.ppm_batch_workspace_size.argtypes = [i]
     L.ppm_batch_forward.restype = i
     L.ppm_batch_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, i, i, vp]
+    L.ppm_relax_direct_morse.restype = i
+    L.ppm_relax_direct_morse.argtypes = [vp, vp, i, vp, i, i, P(PpmTip), _F4, _F4, P(PpmRelaxCtl), vp, vp, vp]
+    L.ppm_direct_morse_vjp_workspace_size.restype = ctypes.c_size_t
+    L.ppm_direct_morse_vjp_workspace_size.argtypes = [i, i, i]
+    L.ppm_direct_morse_vjp.restype = i
+    L.ppm_direct_morse_vjp.argtypes = [vp, vp, i, vp, i, i, P(PpmTip), _F4, _F4, vp, vp, vp, vp, vp, ctypes.c_size_t,
+                                       vp, vp]
     L.ppm_relax_polish.restype = i
     L.ppm_relax_polish.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), i, vp, vp, vp]
     L.ppm_grid_adjoint_workspace_size.restype = ctypes.c_size_t

@@ -343,6 +343,155 @@ class AFMulator:
         FEs = D.fe_in_strokes_tilted(self.FF, self.spec, pts_d, int(nz), tip, dTip, self.dpos0)
         return FEs.cpu().numpy().reshape(nx_s, ny_s, nz, 4), pts[:, :3].reshape(nx_s, ny_s, 3)
 
+    # ── gridless direct Morse path (spammm/SPM/AFM.py:2063-2519) ────────────
+
+    def _direct_inputs(self, fn):
+        assert self.use_morse, f"{fn} requires use_morse=True (call AFMulator(use_morse=True))"
+        assert self.atoms_arr is not None and self.cLJs_arr is not None, f"{fn}: call assign_params() first"
+        na = int(len(self.atoms_arr))
+        if not (1 <= na <= _lib.PPM_DIRECT_MAX_ATOMS):
+            raise ValueError(f"{fn}: nAtoms={na} out of range [1,{_lib.PPM_DIRECT_MAX_ATOMS}]; limit is "
+                             f"{_lib.PPM_DIRECT_MAX_ATOMS} (no silent grid fallback)")
+        assert self.atoms_arr.shape == (na, 4) and self.atoms_arr.dtype == np.float32, \
+            f"atoms_arr must be (nAtoms,4) float32, got {self.atoms_arr.shape} {self.atoms_arr.dtype}"
+        assert self.cLJs_arr.shape == (na, 4) and self.cLJs_arr.dtype == np.float32, \
+            f"cLJs_arr (Morse) must be (nAtoms,4) float32, got {self.cLJs_arr.shape} {self.cLJs_arr.dtype}"
+        return na
+
+    def run_scan_morse_direct(self, nxy=(50, 50), nz=60, dtip=-0.1, scan_p0=None, scan_da=None, scan_db=None, *,
+                              workgroup_size=64, bAlloc=True, return_pp=True):
+        """Direct Morse + Coulomb relaxed scan without a grid (relaxStrokesTiltedMorseDirect,
+        kernels/AFM.cl:1419-1511; spammm/SPM/AFM.py:2063-2233).  Returns FEs (nx,ny,nz,4) float32
+        (iz=0 = highest tip, tip-rotated sample FE, uncapped), points_xyz (nx,ny,3) and, if
+        return_pp, PPs (nx,ny,nz,4) = (position, +iterations / -iterations if not converged).
+        Raises RuntimeError on any non-converged or non-finite state, naming the first (ix,iy,iz).
+        ``workgroup_size`` is validated for interface parity; results do not depend on it."""
+        import torch
+        fn = "run_scan_morse_direct"
+        na = self._direct_inputs(fn)
+        if scan_p0 is None or scan_da is None or scan_db is None:
+            raise ValueError(f"{fn}: scan_p0, scan_da, scan_db are MANDATORY (no bbox auto-derive during fitting)")
+        scan_p0, scan_da, scan_db = (np.asarray(v, dtype=np.float32).reshape(3) for v in (scan_p0, scan_da, scan_db))
+        assert np.isfinite(scan_p0).all() and np.isfinite(scan_da).all() and np.isfinite(scan_db).all(), \
+            f"{fn}: scan_p0/da/db must be finite"
+        if not float(dtip) < 0.0:
+            raise ValueError(f"{fn}: dtip must be <0, got dtip={dtip}")
+        try:
+            nx_s, ny_s = int(nxy[0]), int(nxy[1])
+        except Exception as e:
+            raise ValueError(f"{fn}: nxy must be a 2-tuple of positive ints, got {nxy!r}") from e
+        nz = int(nz)
+        assert nx_s > 0 and ny_s > 0 and nz > 0, f"{fn}: nx,ny,nz must be positive, got {nxy} nz={nz}"
+        assert int(workgroup_size) > 0, f"{fn}: workgroup_size must be positive, got {workgroup_size}"
+        pts = D.scan_points(scan_p0, scan_da, scan_db, nx_s, ny_s)
+        dev = self.device
+        atoms = torch.as_tensor(self.atoms_arr, device=dev)
+        cMs = torch.as_tensor(self.cLJs_arr, device=dev)
+        FEs, PPs = D.relax_direct_morse(atoms, cMs, torch.as_tensor(pts, device=dev), nz, self._tip(dtip), self.tipQs,
+                                        self.tipQZs, self.max_iter, self.f2conv, scan_ny=ny_s)
+        FEs_h = FEs.cpu().numpy().reshape(nx_s, ny_s, nz, 4)
+        PPs_h = PPs.cpu().numpy().reshape(nx_s, ny_s, nz, 4)
+        points_xyz = pts[:, :3].reshape(nx_s, ny_s, 3)
+        bad = ~np.isfinite(PPs_h).all(axis=-1) | (PPs_h[..., 3] < 0.0)
+        if bad.any():
+            ix, iy, iz = np.unravel_index(int(np.argmax(bad)), bad.shape)
+            pp = PPs_h[ix, iy, iz]
+            reason = ("nonfinite" if not np.isfinite(pp).all() else f"non-converged (iters={int(abs(pp[3]))})")
+            raise RuntimeError(f"{fn}: PP telemetry failed at (ix={ix},iy={iy},iz={iz}): {reason}  "
+                               f"PP=({pp[0]:.6f},{pp[1]:.6f},{pp[2]:.6f}) PP.w={pp[3]:.1f}")
+        self._morse_direct_fwd_state = {
+            "nAtoms": na, "nx": nx_s, "ny": ny_s, "nz": nz, "dtip": float(dtip),
+            "points": points_xyz.copy(), "atoms": self.atoms_arr.tobytes(), "cLJs": self.cLJs_arr.tobytes(),
+            **{nm: getattr(self, nm).copy() for nm in ("tipA", "tipB", "stiffness", "dpos0", "tipQs", "tipQZs",
+                                                        "surfFF")},
+        }
+        if return_pp:
+            return FEs_h, points_xyz, PPs_h
+        return FEs_h, points_xyz
+
+    def vjp_scan_morse_direct(self, PPs, points_xyz, dtip, dL_dFEs, *, workgroup_size=64, bAlloc=True):
+        """Implicit-equilibrium VJP of run_scan_morse_direct (kernels/AFM.cl:1711-2029;
+        spammm/SPM/AFM.py:2241-2519): grad_theta (nAtoms,6) float64 in theta order
+        (x,y,z,R0,E0,Q) and diagnostics {residual_norms, lambda_mins, condition_estimates,
+        status_codes} (nx,ny,nz).  Raises ValueError on malformed input or non-converged PPs,
+        RuntimeError on a stale forward state or any failed adjoint state."""
+        import torch
+        fn = "vjp_scan_morse_direct"
+        na = self._direct_inputs(fn)
+        PPs, points_xyz, dL_dFEs = np.asarray(PPs), np.asarray(points_xyz), np.asarray(dL_dFEs)
+        if PPs.ndim != 4 or PPs.shape[3] != 4:
+            raise ValueError(f"{fn}: PPs must be (nx,ny,nz,4), got {PPs.shape}")
+        if PPs.dtype != np.float32:
+            raise ValueError(f"{fn}: PPs must be float32, got {PPs.dtype}")
+        if points_xyz.ndim != 3 or points_xyz.shape[2] != 3:
+            raise ValueError(f"{fn}: points_xyz must be (nx,ny,3), got {points_xyz.shape}")
+        if points_xyz.dtype != np.float32:
+            raise ValueError(f"{fn}: points_xyz must be float32, got {points_xyz.dtype}")
+        if dL_dFEs.shape != PPs.shape:
+            raise ValueError(f"{fn}: dL_dFEs shape {dL_dFEs.shape} != PPs shape {PPs.shape}")
+        if dL_dFEs.dtype != np.float32:
+            raise ValueError(f"{fn}: dL_dFEs must be float32, got {dL_dFEs.dtype}")
+        nx, ny, nz = PPs.shape[:3]
+        if points_xyz.shape[:2] != (nx, ny):
+            raise ValueError(f"{fn}: points_xyz ({points_xyz.shape[:2]}) != PPs ({nx},{ny})")
+        if not np.isfinite(PPs).all():
+            raise ValueError(f"{fn}: PPs contains nonfinite values")
+        if not np.isfinite(dL_dFEs).all():
+            raise ValueError(f"{fn}: dL_dFEs contains nonfinite values")
+        if (PPs[..., 3] <= 0.0).any():
+            ix, iy, iz = np.unravel_index(int(np.argmax(PPs[..., 3] <= 0.0)), (nx, ny, nz))
+            raise ValueError(f"{fn}: PPs[{ix},{iy},{iz}].w={PPs[ix, iy, iz, 3]:.1f} <= 0 (non-converged); "
+                             "VJP requires all converged states")
+        if not float(dtip) < 0.0:
+            raise ValueError(f"{fn}: dtip must be <0, got dtip={dtip}")
+        assert int(workgroup_size) > 0, f"{fn}: workgroup_size must be positive, got {workgroup_size}"
+        fwd = getattr(self, "_morse_direct_fwd_state", None)
+        if fwd is None:
+            raise RuntimeError(f"{fn}: no prior run_scan_morse_direct forward state recorded "
+                               "(call run_scan_morse_direct first); cannot validate VJP inputs")
+        bad = []
+        if fwd["nAtoms"] != na:
+            bad.append(f"nAtoms fwd={fwd['nAtoms']} vs vjp={na}")
+        if (fwd["nx"], fwd["ny"], fwd["nz"]) != (nx, ny, nz):
+            bad.append(f"scan shape fwd=({fwd['nx']},{fwd['ny']},{fwd['nz']}) vs vjp=({nx},{ny},{nz})")
+        if abs(fwd["dtip"] - float(dtip)) > 1e-12:
+            bad.append(f"dtip fwd={fwd['dtip']!r} vs vjp={float(dtip)!r}")
+        if fwd["points"].shape != points_xyz.shape or not np.array_equal(fwd["points"], points_xyz):
+            bad.append("points_xyz does not match the forward scan raster")
+        if self.atoms_arr.tobytes() != fwd["atoms"]:
+            bad.append("atoms_arr content changed since forward")
+        if self.cLJs_arr.tobytes() != fwd["cLJs"]:
+            bad.append("cLJs_arr content changed since forward")
+        for nm in ("tipA", "tipB", "stiffness", "dpos0", "tipQs", "tipQZs", "surfFF"):
+            if not np.array_equal(getattr(self, nm), fwd[nm]):
+                bad.append(f"{nm} changed since forward")
+        if bad:
+            raise RuntimeError(f"{fn}: stale or mismatched forward state:\n  - " + "\n  - ".join(bad))
+        dev = self.device
+        pts = np.zeros((nx * ny, 4), dtype=np.float32)
+        pts[:, :3] = points_xyz.reshape(-1, 3)
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=dev)  # noqa: E731
+        grad, lam, diag = D.direct_morse_vjp(t(self.atoms_arr), t(self.cLJs_arr), t(pts), nz, self._tip(dtip),
+                                             self.tipQs, self.tipQZs, t(PPs.reshape(-1, nz, 4)),
+                                             t(dL_dFEs.reshape(-1, nz, 4)))
+        diag_h = diag.cpu().numpy().reshape(nx, ny, nz, 4)
+        if not np.isfinite(diag_h).all():
+            ix, iy, iz = np.unravel_index(int(np.argmax(~np.isfinite(diag_h).all(axis=-1))), (nx, ny, nz))
+            raise RuntimeError(f"{fn}: adjoint_diag nonfinite at (ix={ix},iy={iy},iz={iz}): diag={diag_h[ix, iy, iz]}")
+        status = diag_h[..., 3].astype(np.int64)
+        if (status != 0).any():
+            ix, iy, iz = np.unravel_index(int(np.argmax(status != 0)), (nx, ny, nz))
+            d = diag_h[ix, iy, iz]
+            raise RuntimeError(f"{fn}: adjoint failed at (ix={ix},iy={iy},iz={iz}): status_code={int(d[3])} "
+                               f"residual_norm={d[0]:.6e} lambda_min={d[1]:.6e} condition_estimate={d[2]:.6e}")
+        g = grad.cpu().numpy()
+        if not np.isfinite(g).all():
+            raise RuntimeError(f"{fn}: grad_theta nonfinite at atom {int(np.argmax(~np.isfinite(g).all(-1)))}")
+        diagnostics = {"residual_norms": diag_h[..., 0].astype(np.float64),
+                       "lambda_mins": diag_h[..., 1].astype(np.float64),
+                       "condition_estimates": diag_h[..., 2].astype(np.float64), "status_codes": status}
+        return g, diagnostics
+
     @property
     def mol_z(self):
         if self.atoms_arr is None:

@@ -29,6 +29,7 @@ SOURCES = [
     ("ppm_relax.hip", ["-ffp-contract=off"]),
     ("ppm_df.hip", ["-ffp-contract=fast"]),
     ("ppm_batch.hip", ["-ffp-contract=off"]),
+    ("ppm_direct.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -161,6 +161,59 @@ int ppm_relax_polish(const float* grid, const PpmGridSpec* spec, const float* po
     return launch_relax_polish(grid, *spec, points, n_scan, nz, *tip, steps, PPs, FEs, (hipStream_t)stream);
 }
 
+static int check_direct(const char* fn, const float* atoms, const float* cMs, int na, const float* points,
+                        int n_scan, int nz, const PpmTip* tip, const float* Qs, const float* QZs) {
+    if (na < 1 || na > PPM_DIRECT_MAX_ATOMS) {
+        set_error("%s: nAtoms=%d out of range [1,%d]", fn, na, PPM_DIRECT_MAX_ATOMS);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (!atoms || !cMs || !points || !tip || !Qs || !QZs) { set_error("%s: NULL argument", fn); return PPM_ERR_INVALID_ARG; }
+    if (n_scan < 1 || nz < 1 || (long long)n_scan * nz > (1LL << 31)) {
+        set_error("%s: invalid n_scan=%d nz=%d", fn, n_scan, nz);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (!(tip->tipC[3] < 0.f)) { set_error("%s: dtip must be <0, got %g", fn, (double)tip->tipC[3]); return PPM_ERR_INVALID_ARG; }
+    if (!finite4(Qs, 4) || !finite4(QZs, 4)) { set_error("%s: non-finite tip charges", fn); return PPM_ERR_INVALID_ARG; }
+    return PPM_OK;
+}
+
+int ppm_relax_direct_morse(const float* atoms, const float* cMs, int na, const float* points, int n_scan, int nz,
+                           const PpmTip* tip, const float Qs[4], const float QZs[4], const PpmRelaxCtl* ctl,
+                           float* FEs, float* PPs, void* stream) {
+    const char* fn = "ppm_relax_direct_morse";
+    g_err[0] = 0;
+    if (int r = check_direct(fn, atoms, cMs, na, points, n_scan, nz, tip, Qs, QZs)) return r;
+    if (int r = check_ctl(fn, ctl, n_scan)) return r;
+    if (!FEs) { set_error("%s: NULL FEs", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_direct_relax(atoms, cMs, na, points, n_scan, nz, *tip, Qs, QZs, *ctl, FEs, PPs, (hipStream_t)stream);
+}
+
+size_t ppm_direct_morse_vjp_workspace_size(int na, int n_scan, int nz) {
+    if (na < 1 || na > PPM_DIRECT_MAX_ATOMS || n_scan < 1 || nz < 1) return 0;
+    return direct_vjp_workspace(na, n_scan, nz);
+}
+
+int ppm_direct_morse_vjp(const float* atoms, const float* cMs, int na, const float* points, int n_scan, int nz,
+                         const PpmTip* tip, const float Qs[4], const float QZs[4], const float* PPs,
+                         const float* dL_dFEs, float* lambdas, float* diag, void* workspace, size_t workspace_bytes,
+                         double* grad, void* stream) {
+    const char* fn = "ppm_direct_morse_vjp";
+    g_err[0] = 0;
+    if (int r = check_direct(fn, atoms, cMs, na, points, n_scan, nz, tip, Qs, QZs)) return r;
+    if (!PPs || !dL_dFEs || !lambdas || !diag || !workspace || !grad) {
+        set_error("%s: NULL argument", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    const size_t need = direct_vjp_workspace(na, n_scan, nz);
+    if (workspace_bytes < need) {
+        set_error("%s: workspace of %zu bytes < required %zu (ppm_direct_morse_vjp_workspace_size)", fn,
+                  workspace_bytes, need);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return launch_direct_vjp(atoms, cMs, na, points, n_scan, nz, *tip, Qs, QZs, PPs, dL_dFEs, lambdas, diag, workspace,
+                             grad, (hipStream_t)stream);
+}
+
 size_t ppm_batch_workspace_size(int nmol) { return nmol < 1 ? 0 : batch_workspace(nmol); }
 
 int ppm_batch_forward(const PpmMolDesc* mols, const PpmBatchParams* params, const float* atoms, const float* cpar,

@@ -56,6 +56,13 @@ int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na
 int launch_fe_in_strokes(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
                          const PpmTip& tip, const float dTip[4], const float dpos0[4], float* FEs,
                          hipStream_t stream);
+int launch_direct_relax(const float* atoms, const float* cms, int na, const float* pts, int n_scan, int nz,
+                        const PpmTip& tip, const float Qs[4], const float QZs[4], const PpmRelaxCtl& ctl, float* FEs,
+                        float* PPs, hipStream_t stream);
+size_t direct_vjp_workspace(int na, int n_scan, int nz);
+int launch_direct_vjp(const float* atoms, const float* cms, int na, const float* pts, int n_scan, int nz,
+                      const PpmTip& tip, const float Qs[4], const float QZs[4], const float* PPs, const float* dL_dFEs,
+                      float* lambdas, float* diag, void* workspace, double* grad, hipStream_t stream);
 int launch_axis_operator(const float* in, long long so, long long sa, long long si, const float chw[4], int nch,
                          int n_outer, int n_axis_in, int n_inner, const float* W, const int* start, int taps,
                          int n_axis_out, float alpha, int accumulate, float* out, hipStream_t stream);

@@ -31,6 +31,9 @@ namespace {
 
 using namespace rc;
 
+// State gates of the adjoint, the values of kernels/AFM.cl:1778, :1826, :1836.
+constexpr double kResidualGate = 5e-4, kCondGate = 1e6, kStableGate = 1e-5;
+
 // relaxStrokesTilted with PP telemetry.
 __global__ __launch_bounds__(64) void k_relax_tilted(GridDev g, const float4* __restrict__ pts, TiltArgs p,
                                                      float4* __restrict__ FEs, float4* __restrict__ PPs) {
@@ -277,13 +280,43 @@ __global__ __launch_bounds__(256) void k_relax_polish(GridDev g, const float4* _
     }
 }
 
+// Smallest eigenvalue of a symmetric 3x3 matrix: trigonometric closed form of
+// O. K. Smith, "Eigenvalues of a symmetric 3x3 matrix", Commun. ACM 4(4):168 (1961).
+__device__ __forceinline__ double eigmin_sym3(const double S[3][3]) {
+    const double p1 = S[0][1] * S[0][1] + S[0][2] * S[0][2] + S[1][2] * S[1][2];
+    if (p1 == 0.0) return fmin(S[0][0], fmin(S[1][1], S[2][2]));
+    const double q = (S[0][0] + S[1][1] + S[2][2]) / 3.0;
+    const double a = S[0][0] - q, b = S[1][1] - q, c = S[2][2] - q;
+    const double p = sqrt((a * a + b * b + c * c + 2.0 * p1) / 6.0);
+    const double ip = 1.0 / p;
+    const double B00 = a * ip, B11 = b * ip, B22 = c * ip;
+    const double B01 = S[0][1] * ip, B02 = S[0][2] * ip, B12 = S[1][2] * ip;
+    const double detB = B00 * (B11 * B22 - B12 * B12) - B01 * (B01 * B22 - B12 * B02) + B02 * (B01 * B12 - B11 * B02);
+    const double r = fmin(fmax(0.5 * detB, -1.0), 1.0);
+    const double phi = acos(r) / 3.0;
+    return q + 2.0 * p * cos(phi + 2.0943951023931957);      // + 2 pi / 3: the smallest root
+}
+
+// Infinity-norm condition number ||J|| ||J^{-1}|| with J^{-1} = cof^T / det.
+__device__ __forceinline__ double cond_inf(const double J[3][3], const double cof[3][3], double det) {
+    double jn = 0.0, an = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        jn = fmax(jn, fabs(J[i][0]) + fabs(J[i][1]) + fabs(J[i][2]));
+        an = fmax(an, fabs(cof[0][i]) + fabs(cof[1][i]) + fabs(cof[2][i]));
+    }
+    return jn * an / fmax(fabs(det), 1e-300);
+}
+
 // Implicit-function adjoint of one relaxed state.  The output O = (R F(q*), E(q*)); for an
 // upstream cotangent u = dL/dO:
 //   b      = (dO/dq)^T u = J_F^T R^T u_xyz + u_w grad E
 //   J^T l  = b                                  (3x3, FP64)
 //   dL/dG_c.xyz += w_c (R^T u_xyz - l),   dL/dG_c.w += w_c u_w     for the 8 corners c
-// (dL/dG = u . dO/dG - l^T dG/dG).  Status bits: 1 not converged, 2 non-finite, 4 singular J.
-// States with status 2 or 4 contribute nothing.
+// (dL/dG = u . dO/dG - l^T dG/dG).  A state is used only if it passes the gates of the
+// reference's direct-path adjoint (kernels/AFM.cl:1778, :1811-1840): residual |G(q*)| < 5e-4,
+// finite, regular J with condition number < 1e6, and a stable minimum (smallest eigenvalue of
+// the symmetric part of -J above 1e-5).  Status bits (0 = used): 1 residual, 2 non-finite,
+// 4 singular or ill-conditioned, 8 unstable.  A gated state contributes nothing.
 __device__ __forceinline__ void atomic_add4(float* p, float4 v) {
     atomicAdd(p + 0, v.x);
     atomicAdd(p + 1, v.y);
@@ -301,9 +334,10 @@ __global__ __launch_bounds__(256) void k_relax_adjoint(GridDev g, const float4* 
     const f3 tip = tip_anchor(pts, p, ip, iz);
     const float4 PP = PPs[s];
     const float4 u = U[s];
-    int status = PP.w > 0.f ? 0 : 1;
+    int status = 0;
     StateEval e;
     eval_state(g, p, tip, PP.x, PP.y, PP.z, e);
+    if (!(PP.w > 0.f) || !(norm3(e.G) < kResidualGate)) status |= 1;
     const double R[3][3] = {{p.A[0], p.A[1], p.A[2]}, {p.B[0], p.B[1], p.B[2]}, {p.C[0], p.C[1], p.C[2]}};
     double ru[3];
     for (int j = 0; j < 3; ++j) ru[j] = R[0][j] * u.x + R[1][j] * u.y + R[2][j] * u.z;
@@ -312,17 +346,22 @@ __global__ __launch_bounds__(256) void k_relax_adjoint(GridDev g, const float4* 
         bv[j] = e.JF[0][j] * ru[0] + e.JF[1][j] * ru[1] + e.JF[2][j] * ru[2] + (double)u.w * e.gE[j];
     double cof[3][3], det;
     double l[3] = {0.0, 0.0, 0.0};
-    if (!cofactors(e.J, cof, det)) {
+    if (!cofactors(e.J, cof, det) || !(cond_inf(e.J, cof, det) < kCondGate)) {
         status |= 4;
     } else {
         // J^T l = b:  (J^T)^{-1} = adj(J^T)/det = cof(J)/det
         for (int i = 0; i < 3; ++i) l[i] = (cof[i][0] * bv[0] + cof[i][1] * bv[1] + cof[i][2] * bv[2]) / det;
     }
+    double S[3][3];                      // symmetric part of the Hessian -J
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) S[i][j] = -0.5 * (e.J[i][j] + e.J[j][i]);
+    if (!(eigmin_sym3(S) > kStableGate)) status |= 8;
     if (!(isfinite(l[0]) && isfinite(l[1]) && isfinite(l[2]) && isfinite(PP.x) && isfinite(PP.y) &&
           isfinite(PP.z) && isfinite(bv[0]) && isfinite(bv[1]) && isfinite(bv[2])))
         status |= 2;
+    if (status) l[0] = l[1] = l[2] = 0.0;
     lam[s] = make_float4((float)l[0], (float)l[1], (float)l[2], (float)status);
-    if (status & 6) return;
+    if (status) return;
     const float gx = (float)(ru[0] - l[0]), gy = (float)(ru[1] - l[1]), gz = (float)(ru[2] - l[2]);
 #pragma unroll
     for (int m = 0; m < 8; ++m)

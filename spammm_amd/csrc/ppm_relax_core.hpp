@@ -141,8 +141,12 @@ struct TiltArgs {
 
 // relaxStrokesTilted (kernels/AFM.cl:478-562) for one scan point starting at tip anchor
 // `start`; writes FEs[iz] (tip-rotated sample FE) and PPs[iz] (position, +/-iterations).
-__device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, const TiltArgs& p,
-                                                  float4* __restrict__ FEs, float4* __restrict__ PPs) {
+// `sample(x, y, z)` is the sample FE at a world point (trilinear grid sample, or the direct
+// pair sum of relaxStrokesTiltedMorseDirect).  With the convergence test before the update
+// the final resample is at the position just sampled, so its value is reused.
+template <class SAMPLE>
+__device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, const TiltArgs& p,
+                                                      float4* __restrict__ FEs, float4* __restrict__ PPs) {
     const f3 A = ld3(p.A), B = ld3(p.B), C = ld3(p.C);
     const f3 dTip = mk3(C.x * p.dtip, C.y * p.dtip, C.z * p.dtip);
     const f3 d0r = rotMatT(ld3(p.d0), A, B, C);
@@ -151,13 +155,11 @@ __device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, co
     float dt = p.dt0, damp = p.damp0;
     const float dtmax = dt, dtmin = dtmax * 0.1f, damp0 = damp;
     const f3 fsurf = mk3(C.x * p.surf, C.y * p.surf, C.z * p.surf);
-    Cell cell;
-    cell.i = -1; cell.j = -1; cell.k = -1;
     f3 v = mk3(0.f, 0.f, 0.f);
     int it = 0, iz = 0;
     int o = 0;
     while (true) {
-        const float4 fe = sample(g, cell, pos.x, pos.y, pos.z);
+        const float4 fe = sample(pos.x, pos.y, pos.z);
         f3 f = mk3(fe.x, fe.y, fe.z);
         const f3 dp = mk3(pos.x - tip.x, pos.y - tip.y, pos.z - tip.z);
         const f3 ft = rotMatT(tipForce(rotMat(dp, A, B, C), p.k, p.d0), A, B, C);
@@ -176,7 +178,7 @@ __device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, co
         }
         ++it;
         if (done) {
-            const float4 fs = sample(g, cell, pos.x, pos.y, pos.z);
+            const float4 fs = (p.check_before && conv) ? fe : sample(pos.x, pos.y, pos.z);
             const f3 fr = rotMat(mk3(fs.x, fs.y, fs.z), A, B, C);
             FEs[o] = make_float4(fr.x, fr.y, fr.z, fs.w);
             if (PPs) {
@@ -191,6 +193,14 @@ __device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, co
             if (++iz >= p.nz) break;
         }
     }
+}
+
+__device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, const TiltArgs& p,
+                                                  float4* __restrict__ FEs, float4* __restrict__ PPs) {
+    Cell cell;
+    cell.i = -1; cell.j = -1; cell.k = -1;
+    auto smp = [&](float x, float y, float z) { return sample(g, cell, x, y, z); };
+    relax_tilted_lane_src(smp, start, p, FEs, PPs);
 }
 
 }  // namespace rc

@@ -286,6 +286,61 @@ def grid_adjoint(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, grid, grid_co
     return grad
 
 
+def relax_direct_morse(atoms, cMs, points, nz: int, tip: Tip, Qs, QZs, max_iter=128, f2conv=1e-8, scan_ny=0,
+                       FEs=None, PPs=None, want_pps=True):
+    """Gridless direct Morse relaxed scan (relaxStrokesTiltedMorseDirect): (FEs, PPs or None), (n_scan,nz,4)."""
+    import torch
+    _f32dev(atoms, "atoms", 4)
+    _f32dev(cMs, "cMs", 4)
+    _f32dev(points, "points", 4)
+    na, ns = int(atoms.shape[0]), int(points.shape[0])
+    if cMs.shape[0] != na:
+        raise ValueError("cMs must have one row per atom")
+    if FEs is None:
+        FEs = torch.empty((ns, nz, 4), dtype=torch.float32, device=atoms.device)
+    if PPs is None and want_pps:
+        PPs = torch.empty((ns, nz, 4), dtype=torch.float32, device=atoms.device)
+    for nm, t in (("FEs", FEs), ("PPs", PPs)):
+        if t is not None:
+            _f32dev(t, nm, 4)
+            if t.numel() != ns * nz * 4:
+                raise ValueError(f"{nm} has the wrong size")
+    L = _lib.load()
+    tp, ctl = tip.c(), relax_ctl(max_iter, f2conv, True, scan_ny)
+    rc = L.ppm_relax_direct_morse(atoms.data_ptr(), cMs.data_ptr(), na, points.data_ptr(), ns, int(nz),
+                                  ctypes.byref(tp), _lib.f4(Qs), _lib.f4(QZs), ctypes.byref(ctl), FEs.data_ptr(),
+                                  PPs.data_ptr() if PPs is not None else None, _stream(atoms))
+    _lib.check(rc)
+    return FEs, PPs
+
+
+def direct_morse_vjp(atoms, cMs, points, nz: int, tip: Tip, Qs, QZs, PPs, dL_dFEs):
+    """Implicit-equilibrium VJP of relax_direct_morse: (grad (na,6) float64 in theta order
+    (x,y,z,R0,E0,Q), lambdas (n_scan,nz,4), diag (n_scan,nz,4) = (residual, lambda_min, cond, status))."""
+    import torch
+    for nm, t in (("atoms", atoms), ("cMs", cMs), ("points", points), ("PPs", PPs), ("dL_dFEs", dL_dFEs)):
+        _f32dev(t, nm, 4)
+    na, ns = int(atoms.shape[0]), int(points.shape[0])
+    if PPs.numel() != ns * nz * 4 or dL_dFEs.numel() != ns * nz * 4:
+        raise ValueError("PPs / dL_dFEs must have n_scan*nz float4 entries")
+    L = _lib.load()
+    ws_bytes = int(L.ppm_direct_morse_vjp_workspace_size(na, ns, int(nz)))
+    if ws_bytes == 0:
+        raise ValueError(f"invalid direct VJP sizes (na={na}, n_scan={ns}, nz={nz})")
+    dev = atoms.device
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    lam = torch.empty((ns, nz, 4), dtype=torch.float32, device=dev)
+    diag = torch.empty((ns, nz, 4), dtype=torch.float32, device=dev)
+    grad = torch.empty((na, 6), dtype=torch.float64, device=dev)
+    tp = tip.c()
+    rc = L.ppm_direct_morse_vjp(atoms.data_ptr(), cMs.data_ptr(), na, points.data_ptr(), ns, int(nz),
+                                ctypes.byref(tp), _lib.f4(Qs), _lib.f4(QZs), PPs.data_ptr(), dL_dFEs.data_ptr(),
+                                lam.data_ptr(), diag.data_ptr(), ws.data_ptr(), ws_bytes, grad.data_ptr(),
+                                _stream(atoms))
+    _lib.check(rc)
+    return grad, lam, diag
+
+
 def scan_points(scan_p0, scan_da, scan_db, nx: int, ny: int) -> np.ndarray:
     """(nx*ny,4) float32 ix-major scan starts: p0 + da*ix + db*iy in float32, the arithmetic of the
     reference loop (spammm/SPM/AFM.py:977-980) and build_scan_xy_points_vectorized (:82-96)."""

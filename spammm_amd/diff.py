@@ -48,7 +48,7 @@ class Problem:
     f0: float = 30000.0
     k_cant: float = 1800.0
     recompute: bool = False     # backward re-runs the relaxation instead of keeping probe positions
-    strict: bool = True         # raise if any state is singular / non-finite in the adjoint
+    strict: bool = True         # raise if any state is non-finite in the adjoint (gated states are counted)
     last_status: dict = field(default_factory=dict)
 
     @property
@@ -136,10 +136,11 @@ def _make_function():
             U[..., 2] = u_z[..., 0]
             gG, lam = D.relax_adjoint(grid, prob.spec, prob.pts, prob.nz, prob.tip, PPs, U)
             st = lam[..., 3].to(torch.int32)
-            n_bad = int(((st & 6) != 0).sum())
-            prob.last_status = {"nonconverged": int(((st & 1) != 0).sum()), "bad": n_bad}
-            if prob.strict and n_bad:
-                raise RuntimeError(f"relax adjoint: {n_bad} states singular or non-finite")
+            prob.last_status = {"states": int(st.numel()), "gated": int((st != 0).sum()),
+                                "residual": int(((st & 1) != 0).sum()), "nonfinite": int(((st & 2) != 0).sum()),
+                                "singular": int(((st & 4) != 0).sum()), "unstable": int(((st & 8) != 0).sum())}
+            if prob.strict and prob.last_status["nonfinite"]:
+                raise RuntimeError(f"relax adjoint: {prob.last_status['nonfinite']} states non-finite")
             g6 = D.grid_adjoint(atoms, cpar, _lib.PPM_MODEL_LJ, prob.spec, prob.Qs, prob.QZs, grid, gG, prob.npbc,
                                 prob.lvA, prob.lvB, cap=prob.cap)
             R0d, E0d = R0.double(), E0.double()
