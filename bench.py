@@ -46,6 +46,7 @@ def main():
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")
     ap.add_argument("--adam-steps", type=int, default=50)
     ap.add_argument("--batch-mols", type=int, default=512, help="molecules of the batched leg (0: skip)")
+    ap.add_argument("--no-direct", action="store_true", help="skip the gridless direct Morse leg")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (every box has one MI355X); "
@@ -170,6 +171,8 @@ def main():
         out["differentiable"] = diff_leg(dev, args.adam_steps)
     if args.batch_mols > 0:
         out["batched"] = batch_leg(dev, args.batch_mols)
+    if not args.no_direct:
+        out["direct_morse"] = direct_leg(dev)
     if not args.no_check:
         out["accuracy"] = accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip)
     if not args.no_cpu:
@@ -232,6 +235,70 @@ def batch_leg(dev, n_mol):
     del bf, fe_plain, df_plain
     torch.cuda.empty_cache()
     return res
+
+
+def direct_leg(dev, reps=5):
+    """Gridless direct Morse path: 128-atom flake (the path's atom limit), 64x64 pixels x 60 z-steps
+    (the largest case of the reference's performance matrix for this path), CO-like tip; forward
+    relaxation and the FP64 implicit VJP timed with HIP events."""
+    import torch
+    from spammm_amd import device as D, params as P, workloads as WL
+    rng = np.random.default_rng(5)
+    na, nxy, nz, dtip = 128, (64, 64), 60, -0.1
+    en, pos = WL.flake_geometry(na, rng)
+    atoms = np.zeros((na, 4), np.float32)
+    atoms[:, :3] = pos
+    atoms[:, 3] = rng.uniform(-0.3, 0.3, na)
+    atoms[:, 3] -= atoms[:, 3].mean()
+    cMs = P.mix_morse(en, P.ELEMENTS_REFERENCE, 1.452, 0.0006808, -1.8).astype(np.float32)
+    lo, hi = pos[:, :2].min(0) - 1.0, pos[:, :2].max(0) + 1.0
+    step = (hi - lo) / np.array(nxy)
+    p0 = np.array([lo[0] + step[0] / 2, lo[1] + step[1] / 2, 13.0], np.float32)
+    pts = D.scan_points(p0, np.array([step[0], 0, 0], np.float32), np.array([0, step[1], 0], np.float32), *nxy)
+    kl, kr = P.stiffness_Nm_to_eVA2(0.25), P.stiffness_Nm_to_eVA2(20.0)
+    tip = D.Tip(stiffness=np.array([-kl, -kl, -kl, -kr], np.float32), tipC=np.array([0., 0., 1., dtip], np.float32))
+    Qs, QZs = np.array([-0.05, 0, 0, 0], np.float32), np.zeros(4, np.float32)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)  # noqa: E731
+    A, M, Pt = t(atoms), t(cMs), t(pts)
+    stream = torch.cuda.current_stream(dev)
+    ns = nxy[0] * nxy[1]
+    FEs = torch.empty((ns, nz, 4), dtype=torch.float32, device=dev)
+    PPs = torch.empty_like(FEs)
+    U = torch.zeros_like(FEs)
+    U[..., 2] = torch.randn((ns, nz), device=dev, generator=torch.Generator(device=dev).manual_seed(1))
+
+    def timed(fn):
+        ts = []
+        for r in range(reps + 1):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize(dev)
+            if r:
+                ts.append(a.elapsed_time(b))
+        return float(np.mean(ts))
+
+    t_f = timed(lambda: D.relax_direct_morse(A, M, Pt, nz, tip, Qs, QZs, 100, 1e-8, scan_ny=nxy[1], FEs=FEs,
+                                             PPs=PPs))
+    res = {}
+    t_b = timed(lambda: res.setdefault("v", D.direct_morse_vjp(A, M, Pt, nz, tip, Qs, QZs, PPs, U)))
+    pp = PPs.cpu().numpy()
+    evals = float(np.abs(pp[..., 3]).sum() + (pp[..., 3] < 0).sum())     # FE evaluations (+final for unconverged)
+    flop = evals * na * 54
+    tflops = flop / (t_f * 1e-3) / 1e12
+    diag = res["v"][2].cpu().numpy()
+    out = {
+        "workload": f"{na}-atom flake, Morse + 1 charge site, {nxy[0]}x{nxy[1]} pixels x {nz} z-steps, "
+                    "FIRE |F|<1e-4 <= 100 it",
+        "forward_ms": t_f, "vjp_ms": t_b, "vjp_over_forward": t_b / t_f,
+        "pixel_z_steps_per_s": ns * nz / (t_f * 1e-3),
+        "mean_fire_evals": evals / (ns * nz), "converged_fraction": float((pp[..., 3] > 0).mean()),
+        "forward_TFLOPs_algorithmic": tflops, "forward_frac_of_fp32_peak": tflops / PEAK_FP32_TFLOPS,
+        "vjp_states_used_fraction": float((diag[..., 3] == 0).mean()),
+    }
+    log(f"[bench] direct: {out}")
+    return out
 
 
 def diff_leg(dev, adam_steps=50):

@@ -25,7 +25,7 @@ using gc::GridArgs;
 using gc::Site;
 
 
-constexpr int kSpinLimit = 1 << 22;
+constexpr int kSpinLimit = 1 << 20;     // polls (~1 us each with the sleep): a bound of about a second
 
 struct Queue {           // device workspace, zeroed by k_batch_reset
     int head;            // next work item
@@ -57,13 +57,16 @@ __device__ GridArgs mol_grid_args(const PpmMolDesc& d, const PpmBatchParams& P) 
     a.cap = P.cap;
     a.q0 = 0.f;
     a.noff = 0;
+    // compaction with static indices only (a dynamically indexed private array lives in scratch)
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (P.Qs[k] == 0.f) continue;
         if (P.QZs[k] == 0.f) {
             a.q0 += P.Qs[k] * kCOULOMB;
         } else {
-            a.qoff[a.noff] = P.Qs[k] * kCOULOMB;
-            a.zoff[a.noff] = P.QZs[k];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (t == a.noff) { a.qoff[t] = P.Qs[k] * kCOULOMB; a.zoff[t] = P.QZs[k]; }
             ++a.noff;
         }
     }
@@ -140,8 +143,15 @@ __global__ __launch_bounds__(256) void k_batch(const PpmMolDesc* __restrict__ mo
     __shared__ int s_item, s_mol;
     int* done = q_done(q);
     int* flags = q_flag(q, P.nmol);
+    // Molecule whose grid tile this workgroup finished in the previous iteration (-1: none).
+    // Thread 0 publishes it at the top of the next iteration, in the same thread-0 section
+    // as the dequeue: a thread-0-only section at the END of the loop body would be merged by
+    // the compiler with the one at the top into a divergent inner loop whose other lanes
+    // re-read the stale item after the barrier and never leave it.
+    int signal_m = -1;
     while (true) {
         if (threadIdx.x == 0) {
+            if (signal_m >= 0) __hip_atomic_fetch_add(&done[signal_m], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             int item = atomicAdd(&q->head, 1);
             int m = -1;
             if (item < P.nitems) {
@@ -161,18 +171,20 @@ __global__ __launch_bounds__(256) void k_batch(const PpmMolDesc* __restrict__ mo
         if (item >= P.nitems) break;
         const PpmMolDesc& d = mols[m];
         const int local = item - d.item_off;
+        signal_m = -1;
         if (local < d.n_grid_items) {
-            if (P.mode == PPM_BATCH_RELAX_ONLY) continue;
-            do_grid_item<MODEL, HASQ0, V>(d, P, local, atoms, cpar, grids, sites);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __syncthreads();
-            if (threadIdx.x == 0) __hip_atomic_fetch_add(&done[m], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            if (P.mode == PPM_BATCH_GRID_ONLY) continue;
+            if (P.mode != PPM_BATCH_RELAX_ONLY) {
+                do_grid_item<MODEL, HASQ0, V>(d, P, local, atoms, cpar, grids, sites);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __syncthreads();            // every thread's tile stores precede the signal
+                signal_m = m;
+            }
+        } else if (P.mode != PPM_BATCH_GRID_ONLY) {
             if (threadIdx.x == 0) {
+                // relaxed polling (no cache invalidation per poll), one acquire after the wait
                 int spins = 0;
-                while (__hip_atomic_load(&done[m], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < d.n_grid_items) {
-                    __builtin_amdgcn_s_sleep(2);
+                while (__hip_atomic_load(&done[m], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < d.n_grid_items) {
+                    __builtin_amdgcn_s_sleep(8);
                     if (++spins > kSpinLimit) {
                         atomicOr(&q->error, 1);
                         break;

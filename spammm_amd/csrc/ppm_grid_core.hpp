@@ -127,7 +127,9 @@ __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, cons
                 fz[k] = fmaf(fr, dz, fz[k]);
             }
             // tip charge sites with a nonzero z offset (none for a single-charge probe)
-            for (int o = 0; o < a.noff; ++o) {
+#pragma unroll
+            for (int o = 0; o < 4; ++o) {
+                if (o >= a.noff) break;
                 const float qo = S.q * a.qoff[o];
                 const float zo = a.zoff[o];
 #pragma unroll

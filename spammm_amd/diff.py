@@ -14,6 +14,9 @@ re-running the deterministic relaxation (``recompute=True``).
 There is no reference counterpart for the grid path (the reference differentiates only its
 gridless direct-Morse path, spammm/SPM/AFM.py:2241-2519); the loss convention follows
 df_loss_seed (spammm/SPM/AFM.py:3125-3200).
+
+``direct_df_image`` is the same Function over the gridless direct Morse path:
+ppm_relax_direct_morse forward, ppm_direct_morse_vjp backward (kernels/AFM.cl:1419-2029).
 """
 from __future__ import annotations
 
@@ -157,3 +160,99 @@ def _make_function():
 def df_image(xyz, q, R0, E0, prob: Problem):
     """Differentiable df image (nx, ny, nz) on the GPU."""
     return DfImage.fn().apply(xyz, q, R0, E0, prob)
+
+
+# ── gridless direct Morse backend ────────────────────────────────────────────────
+
+@dataclass
+class DirectProblem:
+    """Non-differentiated part of the direct Morse forward (run_scan_morse_direct semantics)."""
+    pts: object                 # (n_scan,4) float32 device tensor
+    nxy: tuple
+    nz: int
+    tip: D.Tip
+    Qs: np.ndarray
+    QZs: np.ndarray
+    K: float = -1.8             # Morse exponent per atom row (cMs.z)
+    max_iter: int = 1000
+    f2conv: float = 1e-12
+    df_kind: str = "giessibl"
+    amp: float = 1.0
+    f0: float = 30000.0
+    k_cant: float = 1800.0
+    strict: bool = True         # raise if any adjoint state is non-finite (gated states are counted)
+    last_status: dict = field(default_factory=dict)
+
+    @property
+    def n_scan(self):
+        return int(self.nxy[0]) * int(self.nxy[1])
+
+    @property
+    def dz(self):
+        return float(self.tip.tipC[3])
+
+
+def _direct_inputs(xyz, q, R0, E0, prob: DirectProblem):
+    import torch
+    atoms = torch.cat([xyz.detach().float(), q.detach().float()[:, None]], dim=1).contiguous()
+    na = atoms.shape[0]
+    cMs = torch.zeros((na, 4), dtype=torch.float32, device=atoms.device)
+    cMs[:, 0] = R0.detach().float()
+    cMs[:, 1] = E0.detach().float()
+    cMs[:, 2] = float(prob.K)
+    return atoms, cMs
+
+
+def _make_direct_function():
+    import torch
+
+    class _DirectDf(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, xyz, q, R0, E0, prob):
+            atoms, cMs = _direct_inputs(xyz, q, R0, E0, prob)
+            FEs, PPs = D.relax_direct_morse(atoms, cMs, prob.pts, prob.nz, prob.tip, prob.Qs, prob.QZs, prob.max_iter,
+                                            prob.f2conv, scan_ny=int(prob.nxy[1]))
+            op = DF.z_operator(prob.df_kind, prob.nz, prob.dz, atoms.device, prob.amp, prob.f0, prob.k_cant)
+            df = op.apply(FEs, (prob.nz * 4, 4, 0), prob.n_scan, 1, offset=2)
+            ctx.prob = prob
+            ctx.dtypes = (xyz.dtype, q.dtype, R0.dtype, E0.dtype)
+            ctx.save_for_backward(atoms, cMs, PPs)
+            return df.reshape(int(prob.nxy[0]), int(prob.nxy[1]), prob.nz)
+
+        @staticmethod
+        def backward(ctx, gdf):
+            prob = ctx.prob
+            atoms, cMs, PPs = ctx.saved_tensors
+            dev = atoms.device
+            g = gdf.detach().float().contiguous().reshape(prob.n_scan, prob.nz)
+            opT = DF.z_operator(prob.df_kind, prob.nz, prob.dz, dev, prob.amp, prob.f0, prob.k_cant, transpose=True)
+            U = torch.zeros((prob.n_scan, prob.nz, 4), dtype=torch.float32, device=dev)
+            U[..., 2] = opT.apply(g, (prob.nz, 1, 0), prob.n_scan, 1)[..., 0]
+            grad, _, diag = D.direct_morse_vjp(atoms, cMs, prob.pts, prob.nz, prob.tip, prob.Qs, prob.QZs, PPs, U)
+            st = diag[..., 3].to(torch.int32)
+            prob.last_status = {"states": int(st.numel()), "gated": int((st != 0).sum()),
+                                **{name: int((st == c).sum()) for c, name in
+                                   ((1, "nonconverged"), (2, "nonfinite"), (3, "unstable"), (4, "singular"))}}
+            if prob.strict and prob.last_status["nonfinite"]:
+                raise RuntimeError(f"direct adjoint: {prob.last_status['nonfinite']} states non-finite")
+            tx, tq, tr, te = ctx.dtypes
+            return grad[:, :3].to(tx), grad[:, 5].to(tq), grad[:, 3].to(tr), grad[:, 4].to(te), None
+
+    return _DirectDf
+
+
+class DirectDfImage:
+    _fn = None
+
+    @classmethod
+    def fn(cls):
+        if cls._fn is None:
+            cls._fn = _make_direct_function()
+        return cls._fn
+
+
+def direct_df_image(xyz, q, R0, E0, prob: DirectProblem):
+    """Differentiable df image (nx, ny, nz) of the gridless direct Morse path on the GPU;
+    gradients w.r.t. atom positions, charges and Morse R0, E0 (the reference's theta,
+    spammm/SPM/AFM.py:2241-2519)."""
+    return DirectDfImage.fn().apply(xyz, q, R0, E0, prob)

@@ -256,3 +256,34 @@ def test_vjp_stale_forward_state_raises():
     afm.atoms_arr[0, 0] += 0.01
     with pytest.raises(RuntimeError, match="stale"):
         afm.vjp_scan_morse_direct(PPs, pts, -0.2, np.zeros_like(FEs))
+
+
+def test_direct_autograd_function_matches_facade_vjp():
+    """diff.direct_df_image backward equals vjp_scan_morse_direct fed the transposed df operator."""
+    from spammm_amd import df as DF, diff
+    afm, atoms, cMs = toy_afm(3, 19)
+    afm.f2conv, afm.max_iter = 1e-12, 4000
+    nxy, nz, dtip = (4, 3), 8, -0.1
+    p0, da, db = scan_geom(atoms, nxy)
+    FEs, pts, PPs = afm.run_scan_morse_direct(nxy=nxy, nz=nz, dtip=dtip, scan_p0=p0, scan_da=da, scan_db=db)
+    dev = torch.device("cuda:0")
+    prob = diff.DirectProblem(pts=torch.as_tensor(D.scan_points(p0, da, db, *nxy), device=dev), nxy=nxy, nz=nz,
+                              tip=afm._tip(dtip), Qs=afm.tipQs, QZs=afm.tipQZs, max_iter=4000, f2conv=1e-12)
+    a32 = afm.atoms_arr.astype(np.float64)
+    xyz = torch.tensor(a32[:, :3], device=dev, requires_grad=True)
+    q = torch.tensor(a32[:, 3], device=dev, requires_grad=True)
+    R0 = torch.tensor(afm.cLJs_arr[:, 0].astype(np.float64), device=dev, requires_grad=True)
+    E0 = torch.tensor(afm.cLJs_arr[:, 1].astype(np.float64), device=dev, requires_grad=True)
+    dfi = diff.direct_df_image(xyz, q, R0, E0, prob)
+    v = torch.randn(dfi.shape, device=dev, generator=torch.Generator(device=dev).manual_seed(3))
+    (dfi * v).sum().backward()
+    assert prob.last_status["gated"] == 0
+    op = DF.z_operator("giessibl", nz, dtip, dev, 1.0, 30000.0, 1800.0, transpose=True)
+    U = np.zeros((*nxy, nz, 4), np.float32)
+    U[..., 2] = op.apply(v.float().reshape(-1, nz).contiguous(), (nz, 1, 0), nxy[0] * nxy[1], 1)[..., 0] \
+        .cpu().numpy().reshape(*nxy, nz)
+    g, _ = afm.vjp_scan_morse_direct(PPs, pts, dtip, U)
+    np.testing.assert_allclose(xyz.grad.cpu().numpy(), g[:, :3], rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(R0.grad.cpu().numpy(), g[:, 3], rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(E0.grad.cpu().numpy(), g[:, 4], rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(q.grad.cpu().numpy(), g[:, 5], rtol=1e-6, atol=1e-12)
