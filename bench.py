@@ -237,6 +237,14 @@ def batch_leg(dev, n_mol):
     return res
 
 
+def _direct_group(n_scan, na):
+    """Lanes per scan point the direct relaxation launch uses (same rule as the library)."""
+    g = 1
+    while g < 64 and n_scan * g * 2 <= 4 * 1024 * 64 and g < na:
+        g *= 2
+    return g
+
+
 def direct_leg(dev, reps=5):
     """Gridless direct Morse path: 128-atom flake (the path's atom limit), 64x64 pixels x 60 z-steps
     (the largest case of the reference's performance matrix for this path), CO-like tip; forward
@@ -296,6 +304,7 @@ def direct_leg(dev, reps=5):
         "mean_fire_evals": evals / (ns * nz), "converged_fraction": float((pp[..., 3] > 0).mean()),
         "forward_TFLOPs_algorithmic": tflops, "forward_frac_of_fp32_peak": tflops / PEAK_FP32_TFLOPS,
         "vjp_states_used_fraction": float((diag[..., 3] == 0).mean()),
+        "lanes_per_point": _direct_group(ns, na),
     }
     log(f"[bench] direct: {out}")
     return out

@@ -4,10 +4,13 @@
 // Forward (relaxStrokesTiltedMorseDirect, kernels/AFM.cl:1419-1511): the relaxation of
 // relaxStrokesTilted with the trilinear grid sample replaced by the direct, uncapped pair sum
 // over all atoms (getMorse + getCoulombAFM per tip charge site, kernels/AFM.cl:1391-1407),
-// convergence tested before the update.  One lane per scan point; the atoms (<= 128) sit in
-// LDS and every lane of a wave reads the same atom at the same time (broadcast); lanes
-// advance through their z slices independently.  Built with -ffp-contract=off: the FP32
-// operations follow the reference formulas as written.
+// convergence tested before the update.  A scan point is relaxed by a group of G lanes
+// (G = 1..64, chosen so that a launch holds about 4 waves per SIMD: a 64x64 scan is only 64
+// waves at one lane per point): each lane sums a strided share of the atoms (held in LDS,
+// <= 128), a butterfly gives every lane of the group the total, and the group runs the FIRE
+// logic in lockstep; groups advance through their z slices independently.  Built with
+// -ffp-contract=off: the FP32 operations follow the reference formulas as written (with G = 1
+// the atoms are also summed in the reference's order).
 //
 // Adjoint (morseDirectStateAdjoint + morseDirectParamPartials + reduceMorseDirectParamPartials,
 // kernels/AFM.cl:1711-2029) in two launches:
@@ -33,34 +36,6 @@ struct DirectArgs {
     float Qs[4], QZs[4];     // non-zero tip charge sites only, Qs scaled by COULOMB_CONST
 };
 
-// Sum of the uncapped pair law over the LDS atoms at world point (x, y, z).
-__device__ __forceinline__ float4 direct_fe(const float4* __restrict__ A, const float4* __restrict__ M,
-                                            const DirectArgs& d, float x, float y, float z) {
-    float4 fe = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int j = 0; j < d.na; ++j) {
-        const float4 a = A[j];
-        const float4 m = M[j];
-        const float dx = x - a.x, dy = y - a.y, dz = z - a.z;
-        // getMorse (kernels/AFM.cl:747-757)
-        const float r = sqrtf(dx * dx + dy * dy + dz * dz + kR2SAFE);
-        const float ex = expf(m.z * (r - m.x));
-        const float E = m.y * ex * (ex - 2.0f);
-        const float fr = -m.y * ex * (ex - 1.0f) * 2.0f * m.z;
-        const float fq = fr / r;
-        fe.x += dx * fq; fe.y += dy * fq; fe.z += dz * fq; fe.w += E;
-        // getCoulombAFM at each charge site (kernels/AFM.cl:730-736)
-        for (int k = 0; k < d.nsite; ++k) {
-            const float kx = x - a.x, ky = y - a.y, kz = (z + d.QZs[k]) - a.z;
-            const float ir2 = 1.0f / (kx * kx + ky * ky + kz * kz + kR2SAFE);
-            const float Ec = a.w * sqrtf(ir2);
-            const float e2 = Ec * ir2;
-            fe.x += (kx * e2) * d.Qs[k]; fe.y += (ky * e2) * d.Qs[k]; fe.z += (kz * e2) * d.Qs[k];
-            fe.w += Ec * d.Qs[k];
-        }
-    }
-    return fe;
-}
-
 __device__ __forceinline__ void load_atoms(const float4* __restrict__ atoms, const float4* __restrict__ cms, int na,
                                            float4* A, float4* M) {
     for (int i = threadIdx.x; i < na; i += blockDim.x) {
@@ -70,17 +45,66 @@ __device__ __forceinline__ void load_atoms(const float4* __restrict__ atoms, con
     __syncthreads();
 }
 
-__global__ __launch_bounds__(64) void k_direct_relax(const float4* __restrict__ atoms, const float4* __restrict__ cms,
-                                                     DirectArgs d, const float4* __restrict__ pts, rc::TiltArgs p,
-                                                     float4* __restrict__ FEs, float4* __restrict__ PPs) {
+// Pair sum of one lane of a G-lane group: atoms sub, sub+G, sub+2G, ... then a butterfly over
+// the group, after which every lane of the group holds the same total (FP32 addition is
+// commutative, so both partners of each exchange compute the identical sum).
+template <int G>
+__device__ __forceinline__ float4 direct_fe_group(const float4* __restrict__ A, const float4* __restrict__ M,
+                                                  const DirectArgs& d, int sub, float x, float y, float z) {
+    float4 fe = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int j = sub; j < d.na; j += G) {
+        const float4 a = A[j];
+        const float4 m = M[j];
+        const float dx = x - a.x, dy = y - a.y, dz = z - a.z;
+        const float r = sqrtf(dx * dx + dy * dy + dz * dz + kR2SAFE);
+        const float ex = expf(m.z * (r - m.x));
+        const float E = m.y * ex * (ex - 2.0f);
+        const float fr = -m.y * ex * (ex - 1.0f) * 2.0f * m.z;
+        const float fq = fr / r;
+        fe.x += dx * fq; fe.y += dy * fq; fe.z += dz * fq; fe.w += E;
+        for (int k = 0; k < d.nsite; ++k) {
+            const float kx = x - a.x, ky = y - a.y, kz = (z + d.QZs[k]) - a.z;
+            const float ir2 = 1.0f / (kx * kx + ky * ky + kz * kz + kR2SAFE);
+            const float Ec = a.w * sqrtf(ir2);
+            const float e2 = Ec * ir2;
+            fe.x += (kx * e2) * d.Qs[k]; fe.y += (ky * e2) * d.Qs[k]; fe.z += (kz * e2) * d.Qs[k];
+            fe.w += Ec * d.Qs[k];
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        fe.x += __shfl_xor(fe.x, off, G);
+        fe.y += __shfl_xor(fe.y, off, G);
+        fe.z += __shfl_xor(fe.z, off, G);
+        fe.w += __shfl_xor(fe.w, off, G);
+    }
+    return fe;
+}
+
+// G lanes per scan point (G = 1: one lane per point, the atoms summed in order).  The group's
+// lanes share every branch (same data), so the shuffles only read active lanes.
+template <int G>
+__global__ __launch_bounds__(256) void k_direct_relax(const float4* __restrict__ atoms, const float4* __restrict__ cms,
+                                                      DirectArgs d, const float4* __restrict__ pts, rc::TiltArgs p,
+                                                      float4* __restrict__ FEs, float4* __restrict__ PPs) {
     __shared__ float4 A[kMaxAtoms], M[kMaxAtoms];
     load_atoms(atoms, cms, d.na, A, M);        // every lane takes part before any leaves
-    const int ip = rc::lane_point(p.n_scan, p.scan_ny);
-    if (ip < 0) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ip = t / G, sub = t - ip * G;
+    if (ip >= p.n_scan) return;                // whole groups leave together
     const float4 P = pts[ip];
-    auto smp = [&](float x, float y, float z) { return direct_fe(A, M, d, x, y, z); };
+    auto smp = [&](float x, float y, float z) { return direct_fe_group<G>(A, M, d, sub, x, y, z); };
     const size_t o = (size_t)ip * p.nz;
-    rc::relax_tilted_lane_src(smp, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr);
+    rc::relax_tilted_lane_src(smp, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, sub == 0);
+}
+
+// Lanes per scan point: enough groups for about 4 waves per SIMD (1024 SIMDs), at most one
+// lane per atom, a power of two in [1, 64].
+int direct_group(int n_scan, int na) {
+    const long long want = 4LL * 1024 * 64;
+    int g = 1;
+    while (g < 64 && (long long)n_scan * g * 2 <= want && g < na) g *= 2;
+    return g;
 }
 
 // ── adjoint ──────────────────────────────────────────────────────────────────────────
@@ -365,16 +389,22 @@ int launch_direct_relax(const float* atoms, const float* cms, int na, const floa
     p.dtip = tip.tipC[3];
     for (int q = 0; q < 4; ++q) { p.k[q] = tip.stiffness[q]; p.d0[q] = tip.dpos0[q]; }
     p.dt0 = tip.relax[0]; p.damp0 = tip.relax[1]; p.surf = tip.surfFF[0];
-    int waves;
-    if (ctl.scan_ny > 0) {
-        const int nxs = n_scan / ctl.scan_ny;
-        waves = ((nxs + 7) / 8) * ((ctl.scan_ny + 7) / 8);
-    } else {
-        waves = (n_scan + 63) / 64;
+    const int g = direct_group(n_scan, na);
+    const dim3 blocks((unsigned)(((long long)n_scan * g + 255) / 256));
+    const float4* A = reinterpret_cast<const float4*>(atoms);
+    const float4* M = reinterpret_cast<const float4*>(cms);
+    const float4* S = reinterpret_cast<const float4*>(pts);
+    float4* F = reinterpret_cast<float4*>(FEs);
+    float4* Q = reinterpret_cast<float4*>(PPs);
+    switch (g) {
+        case 1: hipLaunchKernelGGL(k_direct_relax<1>, blocks, dim3(256), 0, stream, A, M, d, S, p, F, Q); break;
+        case 2: hipLaunchKernelGGL(k_direct_relax<2>, blocks, dim3(256), 0, stream, A, M, d, S, p, F, Q); break;
+        case 4: hipLaunchKernelGGL(k_direct_relax<4>, blocks, dim3(256), 0, stream, A, M, d, S, p, F, Q); break;
+        case 8: hipLaunchKernelGGL(k_direct_relax<8>, blocks, dim3(256), 0, stream, A, M, d, S, p, F, Q); break;
+        case 16: hipLaunchKernelGGL(k_direct_relax<16>, blocks, dim3(256), 0, stream, A, M, d, S, p, F, Q); break;
+        case 32: hipLaunchKernelGGL(k_direct_relax<32>, blocks, dim3(256), 0, stream, A, M, d, S, p, F, Q); break;
+        default: hipLaunchKernelGGL(k_direct_relax<64>, blocks, dim3(256), 0, stream, A, M, d, S, p, F, Q); break;
     }
-    hipLaunchKernelGGL(k_direct_relax, dim3(waves), dim3(64), 0, stream, reinterpret_cast<const float4*>(atoms),
-                       reinterpret_cast<const float4*>(cms), d, reinterpret_cast<const float4*>(pts), p,
-                       reinterpret_cast<float4*>(FEs), reinterpret_cast<float4*>(PPs));
     return check_launch("direct_relax");
 }
 

@@ -144,9 +144,12 @@ struct TiltArgs {
 // `sample(x, y, z)` is the sample FE at a world point (trilinear grid sample, or the direct
 // pair sum of relaxStrokesTiltedMorseDirect).  With the convergence test before the update
 // the final resample is at the position just sampled, so its value is reused.
+// `writer` = false: the lane takes part in the samples (a lane group sharing one pixel) but
+// leaves the stores to the group's writer lane.
 template <class SAMPLE>
 __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, const TiltArgs& p,
-                                                      float4* __restrict__ FEs, float4* __restrict__ PPs) {
+                                                      float4* __restrict__ FEs, float4* __restrict__ PPs,
+                                                      bool writer = true) {
     const f3 A = ld3(p.A), B = ld3(p.B), C = ld3(p.C);
     const f3 dTip = mk3(C.x * p.dtip, C.y * p.dtip, C.z * p.dtip);
     const f3 d0r = rotMatT(ld3(p.d0), A, B, C);
@@ -180,8 +183,8 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
         if (done) {
             const float4 fs = (p.check_before && conv) ? fe : sample(pos.x, pos.y, pos.z);
             const f3 fr = rotMat(mk3(fs.x, fs.y, fs.z), A, B, C);
-            FEs[o] = make_float4(fr.x, fr.y, fr.z, fs.w);
-            if (PPs) {
+            if (writer) FEs[o] = make_float4(fr.x, fr.y, fr.z, fs.w);
+            if (PPs && writer) {
                 const bool fin = finite4(fs) && isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z);
                 PPs[o] = make_float4(pos.x, pos.y, pos.z, (conv && fin) ? (float)it : -(float)it);
             }
