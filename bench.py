@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--adam-steps", type=int, default=50)
     ap.add_argument("--batch-mols", type=int, default=512, help="molecules of the batched leg (0: skip)")
     ap.add_argument("--no-direct", action="store_true", help="skip the gridless direct Morse leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (scale of the HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (every box has one MI355X); "
@@ -92,6 +94,14 @@ def main():
             ev["relax"].append((e[1], e[2]))
             ev["df"].append((e[2], e[3]))
 
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("[bench] copy calibration: 3 x 1 GiB device copies")
     log(f"[bench] workload={w.name} grid={w.n} atoms={w.atoms.shape[0]} scan={w.nxy}x{nz} warmup={args.warmup}")
     for _ in range(args.warmup):
         step()
@@ -149,7 +159,7 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": grid_tflops / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": _pmc_traffic("k_grid_project", w.name),
             "work_unit": f"{pairs} pair evaluations x 52 FLOP per launch",
             "avg_launch_ms": t_grid,
         },
@@ -237,6 +247,19 @@ def batch_leg(dev, n_mol):
     return res
 
 
+def _pmc_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3 --pmc summary
+    (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same
+    counters' reading for a 1 GiB copy), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+        return rec[workload][kernel]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def _direct_group(n_scan, na):
     """Lanes per scan point the direct relaxation launch uses (same rule as the library)."""
     g = 1
@@ -310,33 +333,37 @@ def direct_leg(dev, reps=5):
     return out
 
 
-def diff_leg(dev, adam_steps=50):
-    """Differentiable df image of the flake molecule, 128x128 pixels x 40 z-steps: forward and adjoint
-    timed with HIP events (adjoint/forward ratio), then Adam on the atom positions towards a target
-    made from positions perturbed by N(0, 0.1 A)."""
+def _diff_case(dev, z0, adam_steps, lr):
+    """One height range of the differentiable flake case (see diff_leg)."""
     import torch
     from spammm_amd import diff, params as P, workloads as WL
     w = WL.flake()
     step = 30.0 / 128
     w.nxy, w.nz = (128, 128), 40
-    w.scan_p0 = np.array([-15. + step / 2, -15. + step / 2, 10.0], np.float32)
+    w.scan_p0 = np.array([-15. + step / 2, -15. + step / 2, z0], np.float32)
     w.scan_da = np.array([step, 0., 0.], np.float32)
     w.scan_db = np.array([0., step, 0.], np.float32)
     prob = diff.Problem.from_workload(w, dev, max_iter=1000, f2conv=1e-12, check_before=True, polish=3,
                                       strict=False)
     R0, E0 = P.mix_R0E0(w.enames, P.ELEMENTS_BENCH, 1.66, 0.0091)
     xyz0 = torch.tensor(w.atoms[:, :3], dtype=torch.float64, device=dev)
-    q = torch.tensor(w.atoms[:, 3], dtype=torch.float64, device=dev)
+    q0 = torch.tensor(w.atoms[:, 3], dtype=torch.float64, device=dev)
     R0 = torch.tensor(R0, device=dev)
     E0 = torch.tensor(E0, device=dev)
     rng = np.random.default_rng(3)
     xyz_t = xyz0 + torch.tensor(rng.normal(0, 0.1, w.atoms[:, :3].shape), device=dev)
     with torch.no_grad():
-        target = diff.df_image(xyz_t, q, R0, E0, prob).double()
+        target = diff.df_image(xyz_t, q0, R0, E0, prob).double()
+
+    def loss_at(xx, qq):
+        with torch.no_grad():
+            return float(0.5 * ((diff.df_image(xx, qq, R0, E0, prob).double() - target) ** 2).sum())
+
     stream = torch.cuda.current_stream(dev)
     t_f, t_b = [], []
     for rep in range(4):
         x = xyz0.clone().requires_grad_(True)
+        q = q0.clone().requires_grad_(True)
         e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         e0.record(stream)
         dfi = diff.df_image(x, q, R0, E0, prob)
@@ -349,42 +376,84 @@ def diff_leg(dev, adam_steps=50):
             t_f.append(e0.elapsed_time(e1))
             t_b.append(e1.elapsed_time(e2))
     tf, tb = float(np.mean(t_f)), float(np.mean(t_b))
-    # directional finite difference of the loss along -grad (GPU forward), eps = 1e-3 A
-    g = x.grad.detach()
-    dvec = -g / g.norm()
-
-    def loss_at(xx):
-        with torch.no_grad():
-            return float(0.5 * ((diff.df_image(xx, q, R0, E0, prob).double() - target) ** 2).sum())
-    eps = 1e-3
-    slope = (loss_at(xyz0 + eps * dvec) - loss_at(xyz0 - eps * dvec)) / (2 * eps)
-    # Adam on positions
+    gx, gq = x.grad.detach(), q.grad.detach()
+    status = dict(prob.last_status)
+    # central FD on 32 random parameters (seed 3): positions h = 1e-3 A, charges h = 1e-4 e
+    na = xyz0.shape[0]
+    sel = np.random.default_rng(3).choice(4 * na, 32, replace=False)
+    rows = []
+    for k in sel:
+        k = int(k)
+        if k < 3 * na:
+            i, c = divmod(k, 3)
+            h = 1e-3
+            xp, xm = xyz0.clone(), xyz0.clone()
+            xp[i, c] += h
+            xm[i, c] -= h
+            fd = (loss_at(xp, q0) - loss_at(xm, q0)) / (2 * h)
+            ad, name = float(gx[i, c]), f"xyz[{i},{c}]"
+        else:
+            i = k - 3 * na
+            h = 1e-4
+            qp, qm = q0.clone(), q0.clone()
+            qp[i] += h
+            qm[i] -= h
+            fd = (loss_at(xyz0, qp) - loss_at(xyz0, qm)) / (2 * h)
+            ad, name = float(gq[i]), f"q[{i}]"
+        rows.append((name, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30)))
+    rel = np.array([r[3] for r in rows])
+    # directional FD along -grad (all positions) at three step sizes
+    gn = float(gx.norm())
+    dvec = -gx / gn
+    dirfd = {}
+    for eps in (1e-2, 1e-3, 1e-4):
+        dirfd[f"{eps:g}"] = (loss_at(xyz0 + eps * dvec, q0) - loss_at(xyz0 - eps * dvec, q0)) / (2 * eps)
+    # Adam on positions (cosine-decayed learning rate)
     x = xyz0.clone().requires_grad_(True)
-    opt = torch.optim.Adam([x], lr=0.003)
-    hist = []
+    opt = torch.optim.Adam([x], lr=lr)
+    hist, rmsd = [], []
     t0 = time.perf_counter()
     for it in range(adam_steps):
+        for grp in opt.param_groups:
+            grp["lr"] = lr * 0.5 * (1.0 + np.cos(np.pi * it / max(adam_steps, 1)))
         opt.zero_grad()
-        loss = 0.5 * ((diff.df_image(x, q, R0, E0, prob).double() - target) ** 2).sum()
+        loss = 0.5 * ((diff.df_image(x, q0, R0, E0, prob).double() - target) ** 2).sum()
         loss.backward()
         opt.step()
         hist.append(float(loss))
+        rmsd.append(float(((x.detach() - xyz_t) ** 2).sum(1).mean().sqrt()))
     torch.cuda.synchronize()
     t_adam = time.perf_counter() - t0
     rmsd0 = float(((xyz0 - xyz_t) ** 2).sum(1).mean().sqrt())
-    rmsd1 = float(((x.detach() - xyz_t) ** 2).sum(1).mean().sqrt())
-    res = {
-        "workload": "flake 200 atoms, 300x300x240 grid, 128x128 pixels x 40 z, Giessibl df, L2 loss",
+    return {
+        "tip_z": [z0, z0 + (w.nz - 1) * float(w.dtip)], "probe_z_rest": [z0 - 4.0, z0 - 4.0 + (w.nz - 1) * float(w.dtip)],
         "forward_ms": tf, "adjoint_ms": tb, "adjoint_over_forward": tb / tf,
-        "grad_check_directional_fd": {"slope_fd": slope, "minus_grad_norm": -float(g.norm()),
-                                      "rel_err": abs(slope + float(g.norm())) / float(g.norm())},
         "pixel_z_steps_per_s_forward": w.n_scan * w.nz / (tf * 1e-3),
-        "adam_steps": adam_steps, "adam_wall_s": t_adam,
-        "loss_first": hist[0] if hist else None, "loss_last": hist[-1] if hist else None,
-        "loss_history_every5": hist[::5],
-        "position_rmsd_to_target_A": {"start": rmsd0, "end": rmsd1},
-        "adjoint_status": prob.last_status,
+        "adjoint_status": status,
+        "fd_32_params": {"h": "1e-3 A / 1e-4 e", "within_1e-3": int((rel <= 1e-3).sum()),
+                         "within_1e-2": int((rel <= 1e-2).sum()), "median_rel_err": float(np.median(rel)),
+                         "rows": [(n, round(a, 6), round(f, 6), float(f"{r:.3g}")) for n, a, f, r in rows]},
+        "directional_fd_minus_grad": {"adjoint": -gn, **{f"fd_eps_{k}": v for k, v in dirfd.items()}},
+        "adam": {"steps": adam_steps, "lr": lr, "schedule": "cosine", "wall_s": t_adam,
+                 "loss_first": hist[0] if hist else None, "loss_last": hist[-1] if hist else None,
+                 "loss_every5": hist[::5], "rmsd_start_A": rmsd0, "rmsd_end_A": rmsd[-1] if rmsd else None,
+                 "rmsd_every5": rmsd[::5]},
     }
+
+
+def diff_leg(dev, adam_steps=50, lr=3e-3):
+    """Differentiable df image of the flake molecule, 128x128 pixels x 40 z-steps, L2 loss to a target
+    made from positions perturbed by N(0, 0.1 A): forward and adjoint timed with HIP events
+    (adjoint/forward ratio), central FD on 32 random parameters (seed 3), directional FD along -grad
+    at three step sizes, and Adam on the positions.  Run at the configured tip heights (10 -> 6.1 A:
+    probe 6 -> 2.1 A, where probes flip between bistable positions and the loss is only piecewise
+    smooth) and 1 A higher (probe 7 -> 3.1 A, smooth)."""
+    res = {"workload": "flake 200 atoms, 300x300x240 grid, 128x128 pixels x 40 z, Giessibl df, L2 loss",
+           "configured_heights": _diff_case(dev, 10.0, adam_steps, lr),
+           "smooth_heights": _diff_case(dev, 11.0, adam_steps, lr)}
+    c = res["configured_heights"]
+    res["forward_ms"], res["adjoint_ms"], res["adjoint_over_forward"] = (c["forward_ms"], c["adjoint_ms"],
+                                                                       c["adjoint_over_forward"])
     log(f"[bench] differentiable: {res}")
     return res
 
