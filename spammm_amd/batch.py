@@ -57,16 +57,18 @@ class LaunchConfig:
 
 
 def z_window(w, sp, tip, nz, margin):
-    """Stored z planes [klo, klo+nzw) covering every probe rest height of the scan +- margin A
-    (rest height = tip anchor + tip-frame dpos0; one extra plane below and two above for the
-    trilinear neighbours).  The whole grid when the window would not be smaller."""
+    """Stored z planes [klo, klo+nzw) covering every probe rest height of the scan widened by
+    ``margin`` A (a number, or (below, above)); rest height = tip anchor + tip-frame dpos0; one
+    extra plane below and two above for the trilinear neighbours.  The whole grid when the
+    window would not be smaller."""
+    m_lo, m_hi = (float(margin), float(margin)) if np.isscalar(margin) else (float(margin[0]), float(margin[1]))
     A, B, C = (np.asarray(v, np.float64)[:3] for v in (tip.tipA, tip.tipB, tip.tipC))
     d0 = np.asarray(tip.dpos0, np.float64)[:3]
     d0r = A * d0[0] + B * d0[1] + C * d0[2]
     dTip = C * float(tip.tipC[3])
     z0 = float(w.scan_p0[2]) + d0r[2]
     z1 = z0 + (nz - 1) * dTip[2]
-    zlo, zhi = min(z0, z1) - margin, max(z0, z1) + margin
+    zlo, zhi = min(z0, z1) - m_lo, max(z0, z1) + m_hi
     dz = float(sp.dC[2])
     nzg = int(sp.n[2])
     if zhi < zlo:

@@ -127,3 +127,19 @@ def test_batch_workload_matches_configuration():
     for m in ms:
         assert 160 <= m.n[0] <= 320 and m.n[0] == m.n[1] and m.n[2] == 200
         assert m.nxy == (128, 128) and m.nz == 40
+
+
+def test_batch_z_window_asymmetric_margins():
+    from spammm_amd import batch as B
+    w = W.batch_molecule(5)
+    sp = D.axis_aligned_spec(w.p0, w.L, w.n)
+    tip = D.Tip(stiffness=w.stiffness, dpos0=w.dpos0, tipC=np.array([0., 0., 1., w.dtip], np.float32))
+    klo_s, nzw_s = B.z_window(w, sp, tip, w.nz, 1.0)
+    klo_a, nzw_a = B.z_window(w, sp, tip, w.nz, (1.0, 1.0))
+    assert (klo_s, nzw_s) == (klo_a, nzw_a)
+    klo, nzw = B.z_window(w, sp, tip, w.nz, (0.2, 0.6))
+    z = sp.p0[2] + sp.dC[2] * np.array([klo, klo + nzw - 1])
+    probe_hi = w.scan_p0[2] - w.bond
+    probe_lo = probe_hi + (w.nz - 1) * w.dtip
+    assert z[0] <= probe_lo - 0.2 and z[1] >= probe_hi + 0.6 + sp.dC[2]
+    assert nzw < nzw_s
