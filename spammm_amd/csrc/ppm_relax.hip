@@ -34,14 +34,39 @@ using namespace rc;
 // State gates of the adjoint, the values of kernels/AFM.cl:1778, :1826, :1836.
 constexpr double kResidualGate = 5e-4, kCondGate = 1e6, kStableGate = 1e-5;
 
-// relaxStrokesTilted with PP telemetry.
+// relaxStrokesTilted with PP telemetry; G lanes per scan point (G = 1, 2, 4): with G > 1 each
+// lane interpolates 4/G channels and the group shares the FIRE logic, results identical to G = 1.
+template <int G>
 __global__ __launch_bounds__(64) void k_relax_tilted(GridDev g, const float4* __restrict__ pts, TiltArgs p,
                                                      float4* __restrict__ FEs, float4* __restrict__ PPs) {
-    const int ip = lane_point(p.n_scan, p.scan_ny);
-    if (ip < 0) return;
+    int sub;
+    const int ip = group_point<G>(p.n_scan, p.scan_ny, sub);
+    if (ip < 0) return;                       // whole groups leave together
     const float4 P = pts[ip];
     const size_t o = (size_t)ip * p.nz;
-    relax_tilted_lane(g, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr);
+    if constexpr (G == 1) {
+        relax_tilted_lane(g, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr);
+    } else {
+        GroupCell<G> cell;
+        cell.i = -1; cell.j = -1; cell.k = -1;
+        auto smp = [&](float x, float y, float z) { return sample_group<G>(g, cell, sub, x, y, z); };
+        relax_tilted_lane_src(smp, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, sub == 0);
+    }
+}
+
+// Lanes per scan point of k_relax_tilted: enough waves for about 4 per SIMD (1024 SIMDs).
+int relax_group(int n_scan) {
+    int g = 1;
+    while (g < 4 && (long long)n_scan * g * 2 <= 4LL * 1024 * 64) g *= 2;
+    return g;
+}
+
+int group_waves(int n_scan, int scan_ny, int g) {
+    if (scan_ny > 0) {
+        const int nxs = n_scan / scan_ny, tx = 8 / g;
+        return ((nxs + tx - 1) / tx) * ((scan_ny + 7) / 8);
+    }
+    return (int)(((long long)n_scan * g + 63) / 64);
 }
 
 struct StrokeArgs {
@@ -398,10 +423,15 @@ int launch_relax_tilted(const float* grid, const PpmGridSpec& s, const float* pt
     p.dtip = tip.tipC[3];
     for (int q = 0; q < 4; ++q) { p.k[q] = tip.stiffness[q]; p.d0[q] = tip.dpos0[q]; }
     p.dt0 = tip.relax[0]; p.damp0 = tip.relax[1]; p.surf = tip.surfFF[0];
-    const int waves = n_waves(n_scan, ctl.scan_ny);
-    hipLaunchKernelGGL(k_relax_tilted, dim3(waves), dim3(64), 0, stream, make_grid(grid, s),
-                       reinterpret_cast<const float4*>(pts), p, reinterpret_cast<float4*>(FEs),
-                       reinterpret_cast<float4*>(PPs));
+    const int G = relax_group(n_scan);
+    const dim3 blocks((unsigned)group_waves(n_scan, ctl.scan_ny, G));
+    const GridDev gd = make_grid(grid, s);
+    const float4* P = reinterpret_cast<const float4*>(pts);
+    float4* F = reinterpret_cast<float4*>(FEs);
+    float4* Q = reinterpret_cast<float4*>(PPs);
+    if (G == 4) hipLaunchKernelGGL(k_relax_tilted<4>, blocks, dim3(64), 0, stream, gd, P, p, F, Q);
+    else if (G == 2) hipLaunchKernelGGL(k_relax_tilted<2>, blocks, dim3(64), 0, stream, gd, P, p, F, Q);
+    else hipLaunchKernelGGL(k_relax_tilted<1>, blocks, dim3(64), 0, stream, gd, P, p, F, Q);
     return check_launch("relax_strokes_tilted");
 }
 

@@ -85,6 +85,63 @@ __device__ __forceinline__ float4 sample(const GridDev& g, Cell& C, float px, fl
     return lerp8(C, a, b, c);
 }
 
+// The same sample shared by a group of G lanes (G = 2 or 4) that relax one scan point: lane
+// `sub` caches and interpolates only channels [sub*4/G, (sub+1)*4/G) of the 8 corners, with
+// the per-channel formula of lerp8 (so every channel is bit-identical to sample()), and the
+// group assembles the float4 with shuffles.  The group's lanes hold the same position, so
+// they take the same branches and every shuffle reads an active lane.
+template <int G>
+struct GroupCell {
+    int i, j, k;
+    float c[4 / G][8];
+};
+
+template <int G>
+__device__ __forceinline__ float4 sample_group(const GridDev& g, GroupCell<G>& C, int sub, float px, float py,
+                                               float pz) {
+    constexpr int NC = 4 / G;
+    const float sa = px * g.da[0] + py * g.da[1] + pz * g.da[2] + g.da[3];
+    const float sb = px * g.db[0] + py * g.db[1] + pz * g.db[2] + g.db[3];
+    const float sc = px * g.dc[0] + py * g.dc[1] + pz * g.dc[2] + g.dc[3];
+    int i0, i1, j0, j1, k0, k1;
+    float a, b, c;
+    axis_repeat(sa, g.nx, i0, i1, a);
+    axis_repeat(sb, g.ny, j0, j1, b);
+    axis_repeat(sc, g.nz, k0, k1, c);
+    if (i0 != C.i || j0 != C.j || k0 != C.k) {
+        C.i = i0; C.j = j0; C.k = k0;
+        k0 -= g.klo;
+        k1 -= g.klo;
+        if ((unsigned)k0 >= (unsigned)g.nzw || (unsigned)k1 >= (unsigned)g.nzw) {
+            if (g.flag && sub == 0) atomicOr(g.flag, 1);
+            k0 = min(max(k0, 0), g.nzw - 1);
+            k1 = min(max(k1, 0), g.nzw - 1);
+        }
+        const size_t sx = (size_t)g.nx, sxy = (size_t)g.nx * g.ny;
+        const size_t r00 = (size_t)k0 * sxy + (size_t)j0 * sx, r10 = (size_t)k0 * sxy + (size_t)j1 * sx;
+        const size_t r01 = (size_t)k1 * sxy + (size_t)j0 * sx, r11 = (size_t)k1 * sxy + (size_t)j1 * sx;
+        const size_t idx[8] = {r00 + i0, r00 + i1, r10 + i0, r10 + i1, r01 + i0, r01 + i1, r11 + i0, r11 + i1};
+        const float* base = reinterpret_cast<const float*>(g.data);
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) C.c[q][m] = base[4 * idx[m] + sub * NC + q];
+    }
+    const float ma = 1.0f - a, mb = 1.0f - b, mc = 1.0f - c;
+    float v[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        const float* f = C.c[q];
+        const float lo = (f[0] * ma + f[1] * a) * mb + (f[2] * ma + f[3] * a) * b;
+        const float hi = (f[4] * ma + f[5] * a) * mb + (f[6] * ma + f[7] * a) * b;
+        v[q] = lo * mc + hi * c;
+    }
+    float o[4];
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) o[ch] = __shfl(v[ch % NC], ch / NC, G);
+    return make_float4(o[0], o[1], o[2], o[3]);
+}
+
 __device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ f3 rotMat(f3 v, f3 a, f3 b, f3 c) { return mk3(dot3(v, a), dot3(v, b), dot3(v, c)); }
 __device__ __forceinline__ f3 rotMatT(f3 v, f3 a, f3 b, f3 c) {
@@ -128,6 +185,26 @@ __device__ __forceinline__ int lane_point(int n_scan, int scan_ny) {
         return (ix < nxs && iy < scan_ny) ? ix * scan_ny + iy : -1;
     }
     const int p = wave * 64 + lane;
+    return p < n_scan ? p : -1;
+}
+
+// Scan point of this lane when G lanes share a point (point index, or -1), and the lane's
+// index `sub` in its group.  Raster scans map a wave to a tile of (8/G) x 8 points.
+template <int G>
+__device__ __forceinline__ int group_point(int n_scan, int scan_ny, int& sub) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    sub = lane % G;
+    const int q = lane / G;
+    constexpr int TY = 8, TX = 8 / G;
+    if (scan_ny > 0) {
+        const int nxs = n_scan / scan_ny;
+        const int tiles_y = (scan_ny + TY - 1) / TY;
+        const int tx = wave / tiles_y, ty = wave - tx * tiles_y;
+        const int ix = tx * TX + q / TY, iy = ty * TY + q % TY;
+        return (ix < nxs && iy < scan_ny) ? ix * scan_ny + iy : -1;
+    }
+    const int p = wave * (64 / G) + q;
     return p < n_scan ? p : -1;
 }
 
