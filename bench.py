@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--adam-steps", type=int, default=50)
     ap.add_argument("--batch-mols", type=int, default=512, help="molecules of the batched leg (0: skip)")
     ap.add_argument("--no-direct", action="store_true", help="skip the gridless direct Morse leg")
+    ap.add_argument("--z-margin", default=None,
+                    help="z window margins of the batched leg in A, 'below,above' (default: the library's)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scale of the HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -180,7 +182,7 @@ def main():
     if not args.no_diff:
         out["differentiable"] = diff_leg(dev, args.adam_steps)
     if args.batch_mols > 0:
-        out["batched"] = batch_leg(dev, args.batch_mols)
+        out["batched"] = batch_leg(dev, args.batch_mols, args.z_margin)
     if not args.no_direct:
         out["direct_morse"] = direct_leg(dev)
     if not args.no_check:
@@ -190,7 +192,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def batch_leg(dev, n_mol):
+def batch_leg(dev, n_mol, z_margin=None):
     """Batched forward of n_mol molecules (U{10..150} atoms, square cells U(16,32) A x 20 A, 0.1 A grids,
     128x128 pixels, 40 z-steps) in one persistent launch replayed from a hipGraph, against the loop of
     plain per-image launches over the same molecules; outputs compared bit for bit."""
@@ -198,8 +200,9 @@ def batch_leg(dev, n_mol):
     from spammm_amd import autotune, batch as B, workloads as WL
     mols = WL.batch(n_mol)
     t0 = time.perf_counter()
-    bf = B.BatchForward(mols, dev, want_pps=False)
-    tune_sub = B.BatchForward(mols[:min(32, n_mol)], dev, want_pps=False)
+    kw = {} if z_margin is None else {"z_margin": tuple(float(v) for v in str(z_margin).split(","))}
+    bf = B.BatchForward(mols, dev, want_pps=False, **kw)
+    tune_sub = B.BatchForward(mols[:min(32, n_mol)], dev, want_pps=False, **kw)
     best, timings = autotune.tune(tune_sub, use_cache=False, reps=1)
     del tune_sub
     log(f"[bench] batch setup {time.perf_counter() - t0:.1f}s, autotuned {best.key()}")
@@ -236,7 +239,7 @@ def batch_leg(dev, n_mol):
         "plain_pixel_z_steps_per_s": states / (t_p * 1e-3),
         "speedup": t_p / t_b,
         "launch": best.key(), "autotune_ms": timings,
-        "projected_pair_evals": bf.pair_evals,
+        "projected_pair_evals": bf.pair_evals, "z_margin_A": list(bf.z_margin),
         "full_grid_pair_evals": int(sum(w.pair_evals for w in mols)),
         "window_fallback_molecules": int(fallback.size), "wait_timeout": int(err),
         "bitexact_vs_plain": same,

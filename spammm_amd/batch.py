@@ -101,6 +101,7 @@ class BatchForward:
             if not (np.array_equal(w.Qs, w0.Qs) and np.array_equal(w.QZs, w0.QZs)):
                 raise ValueError("every molecule of a batch must use the same probe charges")
         self.molecules = mols
+        self.z_margin = (float(z_margin), float(z_margin)) if np.isscalar(z_margin) else tuple(map(float, z_margin))
         self.model = int(model)
         self.cap = bool(cap)
         self.tip = tip if tip is not None else D.Tip(stiffness=w0.stiffness, dpos0=w0.dpos0,
