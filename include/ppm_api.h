@@ -136,6 +136,17 @@ int ppm_grid_project(int model, const float* atoms, const float* cpar, int na,
                      int npbc_a, int npbc_b, const float lvA[3], const float lvB[3],
                      int cap, float* grid, void* stream);
 
+/* ppm_grid_project over a base: grid = cap(base + pair sum of `atoms`), the |E|>100 rescale
+ * (if cap) applied to the total.  base: a raw (cap = 0) grid of other atoms on the same spec,
+ * e.g. a substrate computed once and shared by many molecule placements; it must not be the
+ * same buffer as grid.  No reference counterpart (the reference projects one atom list per
+ * image, kernels/AFM.cl:916-1056); with base = a grid of atoms S, the result equals
+ * ppm_grid_project of S + atoms up to FP32 summation order. */
+int ppm_grid_project_add(int model, const float* atoms, const float* cpar, int na,
+                         const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
+                         int npbc_a, int npbc_b, const float lvA[3], const float lvB[3], int cap,
+                         const float* base, float* grid, void* stream);
+
 /* Relaxed scan with a tilted tip frame.
  * Replaces relaxStrokesTilted (kernels/AFM.cl:478-562) as launched by
  * AFMulator.run_scan (spammm/SPM/AFM.py:950-1010).
@@ -274,6 +285,14 @@ int ppm_grid_adjoint(int model, const float* atoms, const float* cpar, int na,
                      int npbc_a, int npbc_b, const float lvA[3], const float lvB[3], int cap,
                      const float* grid, const float* grid_cot, void* workspace,
                      size_t workspace_bytes, double* grad, void* stream);
+
+/* ppm_grid_adjoint for a grid made by ppm_grid_project_add: gradients of `atoms` only; `base`
+ * (the same base grid) enters the chain rule of the |E|>100 rescale of capped voxels. */
+int ppm_grid_adjoint_add(int model, const float* atoms, const float* cpar, int na,
+                         const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
+                         int npbc_a, int npbc_b, const float lvA[3], const float lvB[3], int cap,
+                         const float* base, const float* grid, const float* grid_cot,
+                         void* workspace, size_t workspace_bytes, double* grad, void* stream);
 
 /* Linear operator along z, one output per (pixel, z_out):
  *   out[p*nz_out + j] = sum_{t<taps} W[j*taps + t] * in[(p*nz_in + start[j] + t)*in_stride + in_offset]

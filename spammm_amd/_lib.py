@@ -43,6 +43,7 @@ EXPORTS = (
     "ppm_fe_in_strokes_tilted", "ppm_z_stencil", "ppm_axis_operator", "ppm_relax_adjoint",
     "ppm_grid_adjoint_workspace_size", "ppm_grid_adjoint", "ppm_relax_polish", "ppm_batch_workspace_size",
     "ppm_batch_forward", "ppm_relax_direct_morse", "ppm_direct_morse_vjp_workspace_size", "ppm_direct_morse_vjp",
+    "ppm_grid_project_add", "ppm_grid_adjoint_add",
 )
 
 _lib = None
@@ -72,6 +73,12 @@ def load():
     L.ppm_grid_project.restype = i
     L.ppm_grid_project.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
                                    P(ctypes.c_float * 3), i, vp, vp]
+    L.ppm_grid_project_add.restype = i
+    L.ppm_grid_project_add.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
+                                       P(ctypes.c_float * 3), i, vp, vp, vp]
+    L.ppm_grid_adjoint_add.restype = i
+    L.ppm_grid_adjoint_add.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
+                                       P(ctypes.c_float * 3), i, vp, vp, vp, vp, ctypes.c_size_t, vp, vp]
     L.ppm_relax_strokes_tilted.restype = i
     L.ppm_relax_strokes_tilted.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), P(PpmRelaxCtl), vp, vp, vp]
     L.ppm_relax_strokes.restype = i

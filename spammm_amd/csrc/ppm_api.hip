@@ -84,10 +84,10 @@ int ppm_version(void) { return PPM_API_VERSION; }
 
 const char* ppm_last_error(void) { return g_err; }
 
-int ppm_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
-                     const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
-                     const float lvB[3], int cap, float* grid, void* stream) {
-    const char* fn = "ppm_grid_project";
+static int grid_project_impl(const char* fn, int model, const float* atoms, const float* cpar, int na,
+                             const PpmGridSpec* spec, const float Qs[4], const float QZs[4], int npbc_a, int npbc_b,
+                             const float lvA[3], const float lvB[3], int cap, const float* base, float* grid,
+                             void* stream) {
     g_err[0] = 0;
     if (int r = check_spec(fn, spec)) return r;
     if (model != PPM_MODEL_LJ && model != PPM_MODEL_MORSE) { set_error("%s: unknown model %d", fn, model); return PPM_ERR_INVALID_ARG; }
@@ -100,8 +100,28 @@ int ppm_grid_project(int model, const float* atoms, const float* cpar, int na, c
     if ((npbc_a > 0 && !lvA) || (npbc_b > 0 && !lvB)) { set_error("%s: images requested without lattice vectors", fn); return PPM_ERR_INVALID_ARG; }
     if ((long long)na * (2 * npbc_a + 1) * (2 * npbc_b + 1) > (1LL << 28)) { set_error("%s: too many image sites", fn); return PPM_ERR_INVALID_ARG; }
     if (!finite4(Qs, 4) || !finite4(QZs, 4)) { set_error("%s: non-finite tip charges", fn); return PPM_ERR_INVALID_ARG; }
-    return launch_grid_project(model, atoms, cpar, na, *spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, grid,
+    return launch_grid_project(model, atoms, cpar, na, *spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, base, grid,
                                (hipStream_t)stream);
+}
+
+int ppm_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
+                     const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                     const float lvB[3], int cap, float* grid, void* stream) {
+    return grid_project_impl("ppm_grid_project", model, atoms, cpar, na, spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap,
+                             nullptr, grid, stream);
+}
+
+int ppm_grid_project_add(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
+                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                         const float lvB[3], int cap, const float* base, float* grid, void* stream) {
+    const char* fn = "ppm_grid_project_add";
+    if (!base || base == grid) {
+        g_err[0] = 0;
+        set_error("%s: base must be a separate grid buffer", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return grid_project_impl(fn, model, atoms, cpar, na, spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, base, grid,
+                             stream);
 }
 
 int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
@@ -248,11 +268,11 @@ size_t ppm_grid_adjoint_workspace_size(int na, const PpmGridSpec* spec) {
     return grid_adjoint_workspace(na, *spec);
 }
 
-int ppm_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
-                     const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
-                     const float lvB[3], int cap, const float* grid, const float* grid_cot, void* workspace,
-                     size_t workspace_bytes, double* grad, void* stream) {
-    const char* fn = "ppm_grid_adjoint";
+static int grid_adjoint_impl(const char* fn, int model, const float* atoms, const float* cpar, int na,
+                             const PpmGridSpec* spec, const float Qs[4], const float QZs[4], int npbc_a, int npbc_b,
+                             const float lvA[3], const float lvB[3], int cap, const float* grid,
+                             const float* grid_cot, const float* base, void* workspace, size_t workspace_bytes,
+                             double* grad, void* stream) {
     g_err[0] = 0;
     if (int r = check_spec(fn, spec)) return r;
     if (model != PPM_MODEL_LJ && model != PPM_MODEL_MORSE) { set_error("%s: unknown model %d", fn, model); return PPM_ERR_INVALID_ARG; }
@@ -269,7 +289,29 @@ int ppm_grid_adjoint(int model, const float* atoms, const float* cpar, int na, c
         return PPM_ERR_INVALID_ARG;
     }
     return launch_grid_adjoint(model, atoms, cpar, na, *spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, grid, grid_cot,
-                               workspace, grad, (hipStream_t)stream);
+                               base, workspace, grad, (hipStream_t)stream);
+}
+
+int ppm_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
+                     const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                     const float lvB[3], int cap, const float* grid, const float* grid_cot, void* workspace,
+                     size_t workspace_bytes, double* grad, void* stream) {
+    return grid_adjoint_impl("ppm_grid_adjoint", model, atoms, cpar, na, spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap,
+                             grid, grid_cot, nullptr, workspace, workspace_bytes, grad, stream);
+}
+
+int ppm_grid_adjoint_add(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
+                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                         const float lvB[3], int cap, const float* base, const float* grid, const float* grid_cot,
+                         void* workspace, size_t workspace_bytes, double* grad, void* stream) {
+    const char* fn = "ppm_grid_adjoint_add";
+    if (!base) {
+        g_err[0] = 0;
+        set_error("%s: NULL base", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return grid_adjoint_impl(fn, model, atoms, cpar, na, spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, grid, grid_cot,
+                             base, workspace, workspace_bytes, grad, stream);
 }
 
 int ppm_z_stencil(const float* in, int in_stride, int in_offset, int n_pix, int nz_in, const float* W,

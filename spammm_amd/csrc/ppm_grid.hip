@@ -27,10 +27,10 @@ using namespace gc;
 template <int MODEL, bool HASQ0, int V, bool DCZ>
 __global__ __launch_bounds__(256) void k_grid_project(const float4* __restrict__ atoms,
                                                       const float* __restrict__ cpar, GridArgs a,
-                                                      float4* __restrict__ out) {
+                                                      float4* __restrict__ out, const float4* base) {
     __shared__ Site sites[kTile];
     grid_tile<MODEL, HASQ0, V, DCZ>(atoms, cpar, a, sites, blockIdx.x * 256 + threadIdx.x, blockIdx.y * V, a.nz,
-                                    out, 0);
+                                    out, 0, base);
 }
 
 // ── grid -> atom adjoint ─────────────────────────────────────────────────────────────
@@ -134,8 +134,8 @@ __device__ __forceinline__ void pair_adjoint(float dx, float dy, float dz, float
 template <int MODEL, int V, bool DCZ>
 __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__ atoms, const float* __restrict__ cpar,
                                                       GridArgs a, const float4* __restrict__ grid,
-                                                      const float4* __restrict__ gcot, float* __restrict__ partials,
-                                                      int* __restrict__ active) {
+                                                      const float4* __restrict__ gcot, const float4* __restrict__ base,
+                                                      float* __restrict__ partials, int* __restrict__ active) {
     __shared__ float red[4][6];
     const int bid = blockIdx.y * gridDim.x + blockIdx.x;
     const int col = blockIdx.x * 256 + threadIdx.x;
@@ -201,6 +201,15 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
                     }
                 }
         }
+        if (base) {            // the raw sum includes the base grid's atoms
+#pragma unroll
+            for (int k = 0; k < V; ++k) {
+                if (colok && ic0 + k < a.nz) {
+                    const float4 b = base[(size_t)(ic0 + k) * a.ncol + col];
+                    fe[k].x += b.x; fe[k].y += b.y; fe[k].z += b.z; fe[k].w += b.w;
+                }
+            }
+        }
 #pragma unroll
         for (int k = 0; k < V; ++k) {
             if (fabsf(fe[k].w) > 100.f) {
@@ -264,20 +273,21 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const float* __restrict
 }
 
 template <int MODEL, bool HASQ0>
-int dispatch(const float4* atoms, const float* cpar, const GridArgs& a, float4* out, hipStream_t st) {
+int dispatch(const float4* atoms, const float* cpar, const GridArgs& a, float4* out, const float4* base,
+             hipStream_t st) {
     const bool dcz = (a.dC[0] == 0.f && a.dC[1] == 0.f);
     dim3 blk(256);
     if (dcz) {
         if (a.nz >= 8) {
             dim3 grd((a.ncol + 255) / 256, (a.nz + 7) / 8);
-            hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 8, true>), grd, blk, 0, st, atoms, cpar, a, out);
+            hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 8, true>), grd, blk, 0, st, atoms, cpar, a, out, base);
         } else {
             dim3 grd((a.ncol + 255) / 256, (a.nz + 3) / 4);
-            hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 4, true>), grd, blk, 0, st, atoms, cpar, a, out);
+            hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 4, true>), grd, blk, 0, st, atoms, cpar, a, out, base);
         }
     } else {
         dim3 grd((a.ncol + 255) / 256, (a.nz + 3) / 4);
-        hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 4, false>), grd, blk, 0, st, atoms, cpar, a, out);
+        hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 4, false>), grd, blk, 0, st, atoms, cpar, a, out, base);
     }
     return check_launch("grid_project");
 }
@@ -321,16 +331,17 @@ static int adjoint_V(const PpmGridSpec& g) { return (g.dC[0] == 0.f && g.dC[1] =
 
 int launch_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
-                        const float lvB[3], int cap, float* grid, hipStream_t stream) {
+                        const float lvB[3], int cap, const float* base, float* grid, hipStream_t stream) {
     const GridArgs a = make_args(na, g, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap);
     const float4* at = reinterpret_cast<const float4*>(atoms);
     float4* o = reinterpret_cast<float4*>(grid);
+    const float4* b = reinterpret_cast<const float4*>(base);
     const bool hasq0 = a.q0 != 0.f;
     if (model == PPM_MODEL_LJ)
-        return hasq0 ? dispatch<PPM_MODEL_LJ, true>(at, cpar, a, o, stream)
-                     : dispatch<PPM_MODEL_LJ, false>(at, cpar, a, o, stream);
-    return hasq0 ? dispatch<PPM_MODEL_MORSE, true>(at, cpar, a, o, stream)
-                 : dispatch<PPM_MODEL_MORSE, false>(at, cpar, a, o, stream);
+        return hasq0 ? dispatch<PPM_MODEL_LJ, true>(at, cpar, a, o, b, stream)
+                     : dispatch<PPM_MODEL_LJ, false>(at, cpar, a, o, b, stream);
+    return hasq0 ? dispatch<PPM_MODEL_MORSE, true>(at, cpar, a, o, b, stream)
+                 : dispatch<PPM_MODEL_MORSE, false>(at, cpar, a, o, b, stream);
 }
 
 size_t grid_adjoint_workspace(int na, const PpmGridSpec& g) {
@@ -341,8 +352,8 @@ size_t grid_adjoint_workspace(int na, const PpmGridSpec& g) {
 
 int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
-                        const float lvB[3], int cap, const float* grid, const float* grid_cot, void* workspace,
-                        double* grad, hipStream_t stream) {
+                        const float lvB[3], int cap, const float* grid, const float* grid_cot, const float* base,
+                        void* workspace, double* grad, hipStream_t stream) {
     const GridArgs a = make_args(na, g, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap);
     const int V = adjoint_V(g);
     const dim3 blocks = grid_blocks(g, V);
@@ -352,20 +363,21 @@ int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na
     const float4* at = reinterpret_cast<const float4*>(atoms);
     const float4* G = reinterpret_cast<const float4*>(grid);
     const float4* gc = reinterpret_cast<const float4*>(grid_cot);
+    const float4* B = reinterpret_cast<const float4*>(base);
     if (model == PPM_MODEL_LJ) {
         if (V == 8)
             hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_LJ, 8, true>), blocks, dim3(256), 0, stream, at, cpar, a, G,
-                               gc, partials, active);
+                               gc, B, partials, active);
         else
             hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_LJ, 4, false>), blocks, dim3(256), 0, stream, at, cpar, a,
-                               G, gc, partials, active);
+                               G, gc, B, partials, active);
     } else {
         if (V == 8)
             hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_MORSE, 8, true>), blocks, dim3(256), 0, stream, at, cpar, a,
-                               G, gc, partials, active);
+                               G, gc, B, partials, active);
         else
             hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_MORSE, 4, false>), blocks, dim3(256), 0, stream, at, cpar,
-                               a, G, gc, partials, active);
+                               a, G, gc, B, partials, active);
     }
     if (int r = check_launch("grid_adjoint")) return r;
     hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)(na * 6)), dim3(256), 0, stream, partials, active, (int)nb,

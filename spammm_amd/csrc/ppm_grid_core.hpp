@@ -57,12 +57,14 @@ __device__ __forceinline__ void load_site(const float4* __restrict__ atoms, cons
 // One workgroup tile: 256 columns starting at column `col - threadIdx.x`, V planes from ic0.
 // Voxels with ic < ic_end are stored at out[(ic - out_klo)*ncol + col].  Every thread of the
 // workgroup must call it (it synchronizes on the shared site tile).
+// `base` (optional, never the same buffer as `out`): a raw (uncapped) grid of other atoms added
+// to every voxel before the |E|>100 rescale.
 // V voxels per thread along c (z); DCZ: dC has no x,y component (every grid set-up of
 // spammm/SPM/AFM.py:772-893 has dC = (0,0,L/nz)), so all V voxels share x,y.
 template <int MODEL, bool HASQ0, int V, bool DCZ>
 __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, const float* __restrict__ cpar,
                                           const GridArgs& a, Site* sites, int col, int ic0, int ic_end,
-                                          float4* __restrict__ out, int out_klo) {
+                                          float4* __restrict__ out, int out_klo, const float4* base = nullptr) {
     const bool colok = col < a.ncol;
     const int cc = colok ? col : a.ncol - 1;
     const int ia = cc % a.nx, ib = cc / a.nx;
@@ -153,6 +155,10 @@ __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, cons
         const int ic = ic0 + k;
         if (ic >= ic_end) break;
         float4 v = make_float4(fx[k], fy[k], fz[k], fe[k]);
+        if (base) {            // raw sum of other atoms (e.g. a substrate), same layout as out
+            const float4 b = base[(size_t)(ic - out_klo) * a.ncol + col];
+            v.x += b.x; v.y += b.y; v.z += b.z; v.w += b.w;
+        }
         if (a.cap) {
             const float renorm = 100.0f / fabsf(v.w);
             if (renorm < 1.0f) { v.x *= renorm; v.y *= renorm; v.z *= renorm; v.w *= renorm; }
