@@ -9,9 +9,11 @@ and applies the df operator to its own Fz column.  The launch can be captured in
 workgroups per CU, fused or two-launch schedule) is chosen by ``spammm_amd.autotune``.
 
 Only the z planes a probe can reach are projected: the planes covering the probe rest heights
-of the scan (tip anchor + dpos0) widened by ``z_margin`` A.  A probe that nevertheless samples
-outside them flags its molecule, and ``forward`` recomputes flagged molecules on the plain
-per-image path with the whole grid, so every output equals the plain path's.
+of the scan (tip anchor + dpos0) widened by ``z_margin`` A below and above (default 0.3, 0.5:
+over 48 molecules of the batched configuration the probes reached at most 0.34 A above the
+highest and 0.03 A below the lowest rest height).  A probe that nevertheless samples outside
+them flags its molecule, and ``forward`` recomputes flagged molecules on the plain per-image
+path with the whole grid, so every output equals the plain path's.
 """
 from __future__ import annotations
 
@@ -86,7 +88,7 @@ class BatchForward:
 
     def __init__(self, molecules, device=None, model=_lib.PPM_MODEL_LJ, cpars=None, tip=None, max_iter=100,
                  f2conv=1e-8, check_before=False, df_kind="giessibl", amp=1.0, f0=30000.0, k_cant=1800.0,
-                 z_margin=1.5, cap=True, want_pps=True):
+                 z_margin=(0.3, 0.5), cap=True, want_pps=True):
         import torch
         self.device = D.require_gpu(device)
         mols = list(molecules)

@@ -123,8 +123,9 @@ def relax_ctl(max_iter=128, f2conv=1e-8, check_before=False, scan_ny=0) -> _lib.
 
 
 def grid_project(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 0), lvA=(0, 0, 0), lvB=(0, 0, 0),
-                 cap=True, out=None):
-    """Force-field grid (nz,ny,nx,4) on the device (evalLJC_QZs_toImg / evalMorseC_QZs_toImg)."""
+                 cap=True, out=None, base=None):
+    """Force-field grid (nz,ny,nx,4) on the device (evalLJC_QZs_toImg / evalMorseC_QZs_toImg).
+    ``base``: a raw (cap=False) grid of other atoms on the same spec, added before the cap."""
     import torch
     _f32dev(atoms, "atoms", 4)
     _f32dev(cpar, "cpar", 2 if model == _lib.PPM_MODEL_LJ else 4)
@@ -141,9 +142,15 @@ def grid_project(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 0), 
     lva = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvA, float).reshape(3)])
     lvb = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvB, float).reshape(3)])
     s = spec.c()
-    rc = L.ppm_grid_project(int(model), atoms.data_ptr(), cpar.data_ptr(), int(atoms.shape[0]), ctypes.byref(s),
-                            _lib.f4(Qs), _lib.f4(QZs), int(npbc[0]), int(npbc[1]), ctypes.byref(lva),
-                            ctypes.byref(lvb), int(bool(cap)), out.data_ptr(), _stream(atoms))
+    args = (int(model), atoms.data_ptr(), cpar.data_ptr(), int(atoms.shape[0]), ctypes.byref(s), _lib.f4(Qs),
+            _lib.f4(QZs), int(npbc[0]), int(npbc[1]), ctypes.byref(lva), ctypes.byref(lvb), int(bool(cap)))
+    if base is None:
+        rc = L.ppm_grid_project(*args, out.data_ptr(), _stream(atoms))
+    else:
+        _f32dev(base, "base", 4)
+        if base.numel() != out.numel():
+            raise ValueError("base has the wrong size")
+        rc = L.ppm_grid_project_add(*args, base.data_ptr(), out.data_ptr(), _stream(atoms))
     _lib.check(rc)
     return out
 
@@ -260,8 +267,9 @@ def relax_polish(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, FEs=None,
 
 
 def grid_adjoint(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, grid, grid_cot, npbc=(0, 0), lvA=(0, 0, 0),
-                 lvB=(0, 0, 0), cap=True, workspace=None):
-    """dL/d(x,y,z,q,p0,p1) per atom (na,6) float64 from dL/dgrid (p0,p1 = c6,c12 for LJ, R0,E0 for Morse)."""
+                 lvB=(0, 0, 0), cap=True, workspace=None, base=None):
+    """dL/d(x,y,z,q,p0,p1) per atom (na,6) float64 from dL/dgrid (p0,p1 = c6,c12 for LJ, R0,E0 for Morse).
+    ``base``: the base grid the forward was projected over (grid_project(..., base=))."""
     import torch
     _f32dev(atoms, "atoms", 4)
     _f32dev(cpar, "cpar", 2 if model == _lib.PPM_MODEL_LJ else 4)
@@ -278,10 +286,17 @@ def grid_adjoint(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, grid, grid_co
     grad = torch.empty((na, 6), dtype=torch.float64, device=atoms.device)
     lva = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvA, float).reshape(3)])
     lvb = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvB, float).reshape(3)])
-    rc = L.ppm_grid_adjoint(int(model), atoms.data_ptr(), cpar.data_ptr(), na, ctypes.byref(s), _lib.f4(Qs),
-                            _lib.f4(QZs), int(npbc[0]), int(npbc[1]), ctypes.byref(lva), ctypes.byref(lvb),
-                            int(bool(cap)), grid.data_ptr(), grid_cot.data_ptr(), workspace.data_ptr(),
-                            workspace.numel(), grad.data_ptr(), _stream(atoms))
+    head = (int(model), atoms.data_ptr(), cpar.data_ptr(), na, ctypes.byref(s), _lib.f4(Qs), _lib.f4(QZs),
+            int(npbc[0]), int(npbc[1]), ctypes.byref(lva), ctypes.byref(lvb), int(bool(cap)))
+    tail = (grid.data_ptr(), grid_cot.data_ptr(), workspace.data_ptr(), workspace.numel(), grad.data_ptr(),
+            _stream(atoms))
+    if base is None:
+        rc = L.ppm_grid_adjoint(*head, *tail)
+    else:
+        _f32dev(base, "base", 4)
+        if base.numel() != grid.numel():
+            raise ValueError("base has the wrong size")
+        rc = L.ppm_grid_adjoint_add(*head, base.data_ptr(), *tail)
     _lib.check(rc)
     return grad
 

@@ -44,14 +44,18 @@ ELEMENTS_REFERENCE = {
     "C": ElementType("C", 6, 1.9255, 0.00455323095),
     "N": ElementType("N", 7, 1.8300, 0.00299212319),
     "O": ElementType("O", 8, 1.7500, 0.00260184625),
+    "Na": ElementType("Na", 11, 1.4915, 0.00130092313),     # data/ElementTypes.dat:14
+    "Cl": ElementType("Cl", 17, 1.9735, 0.00984365166),     # data/ElementTypes.dat:21
 }
 ELEMENTS_BENCH = {
     "H": ElementType("H", 1, 1.487, 0.00068),
     "C": ElementType("C", 6, 1.908, 0.0037),
     "N": ElementType("N", 7, 1.8300, 0.00299212319),
     "O": ElementType("O", 8, 1.7500, 0.00260184625),
+    "Na": ElementType("Na", 11, 1.4915, 0.00130092313),
+    "Cl": ElementType("Cl", 17, 1.9735, 0.00984365166),
 }
-Z_TO_NAME = {1: "H", 6: "C", 7: "N", 8: "O"}
+Z_TO_NAME = {1: "H", 6: "C", 7: "N", 8: "O", 11: "Na", 17: "Cl"}
 
 
 def read_element_types(path: str) -> dict:

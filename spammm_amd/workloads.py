@@ -13,6 +13,10 @@
   256x256 over the cell, 60 z-steps of 0.1 A.
 * ``batch`` — molecules with U{10..150} atoms, square cells U(16,32) A x 20 A,
   0.1 A grids, 128x128 pixels, 40 z-steps.
+* ``substrate_scene`` — a 60-atom flat molecule 3 A above a 4-layer rock-salt slab
+  (8x8 cells, a = 5.64 A, +-0.8 e alternating, Na/Cl parameters, periodic in x,y) on a
+  0.05 A grid (902x902x300), 64 placements (shift U(+-2 A), rotation U(0, 2 pi) about z),
+  192x192 pixels x 60 z-steps.
 """
 from __future__ import annotations
 
@@ -274,3 +278,64 @@ def small(seed=7, n=(48, 40, 40), nxy=(12, 10), nz=12) -> Workload:
     w.nxy = nxy
     w.nz = nz
     return w
+
+
+def rocksalt_slab(cells=8, a=5.64, layers=4, q=0.8):
+    """Rock-salt slab: `layers` (001) layers of (2 cells)^2 ions at a/2 spacing, top layer at
+    z = 0, charges +q / -q alternating in x, y and z (Na+ / Cl-).  Returns (enames, atoms)."""
+    m = 2 * cells
+    en, rows = [], []
+    for l in range(layers):
+        for i in range(m):
+            for j in range(m):
+                pos = (i + j + l) % 2 == 0
+                en.append("Na" if pos else "Cl")
+                rows.append((i * a / 2, j * a / 2, -l * a / 2, q if pos else -q))
+    return en, np.array(rows, np.float32)
+
+
+def substrate_scene(seed=4, n_mol=60, n_poses=64, cells=8, step=0.05, nz_grid=300, nxy=(192, 192), nz=60,
+                    scan_size=20.0, tip_top=16.0):
+    """The substrate configuration (see the module docstring); returns a dict of host arrays."""
+    rng = np.random.default_rng(seed)
+    a = 5.64
+    sen, sub = rocksalt_slab(cells, a)
+    Lxy = cells * a
+    men, pos = flake_geometry(n_mol, rng)
+    mol = np.zeros((n_mol, 4), np.float32)
+    mol[:, 0] = pos[:, 0] + Lxy / 2
+    mol[:, 1] = pos[:, 1] + Lxy / 2
+    mol[:, 2] = pos[:, 2] + 3.0
+    q = rng.uniform(-0.3, 0.3, n_mol)
+    mol[:, 3] = q - q.mean()
+    poses = np.zeros((n_poses, 4))
+    poses[:, :2] = rng.uniform(-2.0, 2.0, (n_poses, 2))
+    poses[:, 3] = rng.uniform(0.0, 2 * np.pi, n_poses)
+    n_xy = int(round(Lxy / step))
+    L = np.array([Lxy, Lxy, nz_grid * step], np.float32)
+    p0 = np.array([0.0, 0.0, -2.5], np.float32)
+    sstep = scan_size / nxy[0]
+    scan_p0 = np.array([Lxy / 2 - scan_size / 2 + sstep / 2, Lxy / 2 - scan_size / 2 + sstep / 2, tip_top],
+                       np.float32)
+    return {
+        "sub_enames": sen, "sub_atoms": sub, "sub_cLJ": mix_lj(sen, ELEMENTS_BENCH, PROBE_R0, PROBE_E0),
+        "mol_enames": men, "mol_atoms": mol, "mol_cLJ": mix_lj(men, ELEMENTS_BENCH, PROBE_R0, PROBE_E0),
+        "poses": poses, "n": (n_xy, n_xy, nz_grid), "p0": p0, "L": L,
+        "lvA": np.array([Lxy, 0, 0], np.float32), "lvB": np.array([0, Lxy, 0], np.float32), "npbc": (1, 1),
+        "scan_p0": scan_p0, "scan_da": np.array([sstep, 0, 0], np.float32),
+        "scan_db": np.array([0, sstep, 0], np.float32), "nxy": tuple(nxy), "nz": nz, "dtip": -0.1,
+        "Qs": np.array([PROBE_Q, 0, 0, 0], np.float32), "QZs": np.zeros(4, np.float32),
+        "k_lat_Nm": 0.25, "k_rad_Nm": 20.0, "bond": 4.0,
+    }
+
+
+def small_substrate_scene(seed=9):
+    """A reduced substrate configuration that the FP64 oracle finishes in seconds: 1x1-cell slab
+    of 2 layers, a 6-atom molecule, 0.2 A grid, 10x10 pixels x 8 z-steps."""
+    d = substrate_scene(seed=seed, n_mol=6, n_poses=4, cells=2, step=0.25, nz_grid=44, nxy=(10, 10), nz=8,
+                        scan_size=4.0, tip_top=11.0)
+    en, sub = rocksalt_slab(2, 5.64, layers=2)
+    d["sub_enames"], d["sub_atoms"] = en, sub
+    d["sub_cLJ"] = mix_lj(en, ELEMENTS_BENCH, PROBE_R0, PROBE_E0)
+    d["poses"][:, :2] *= 0.25
+    return d

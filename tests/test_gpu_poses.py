@@ -1,0 +1,120 @@
+"""Molecule placements over a substrate: grid over a base and the pose adjoint.
+
+Bars: the grid projected over a raw substrate base equals the FP64 oracle grid of the
+combined atom list per voxel within |d| <= 1e-5 ||(F,E)|| + 1e-6 (the FP32 grid bar of
+tests/test_gpu_parity.py); the pose/charge adjoint equals central finite differences of the
+FP64 oracle forward (phi = v . df with fixed v) within 1e-3 relative, denominator floored at
+1e-2 of the largest |FD| (the bar of tests/test_gpu_adjoint.py).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import oracle as O            # noqa: E402
+from spammm_amd import device as D        # noqa: E402
+from spammm_amd import df as DF           # noqa: E402
+from spammm_amd import poses as PS        # noqa: E402
+from spammm_amd import workloads as W     # noqa: E402
+
+
+def make_scene(sc):
+    dev = D.require_gpu()
+    spec = D.axis_aligned_spec(sc["p0"], sc["L"], sc["n"])
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)  # noqa: E731
+    kl = sc["k_lat_Nm"] / 16.02176634
+    kr = sc["k_rad_Nm"] / 16.02176634
+    tip = D.Tip(stiffness=np.array([-kl, -kl, -kl, -kr], np.float32),
+                dpos0=np.array([0, 0, -sc["bond"], sc["bond"]], np.float32),
+                tipC=np.array([0., 0., 1., sc["dtip"]], np.float32))
+    pts = D.scan_points(sc["scan_p0"], sc["scan_da"], sc["scan_db"], *sc["nxy"])
+    scene = PS.Scene(spec=spec, sub_atoms=t(sc["sub_atoms"]), sub_cpar=t(sc["sub_cLJ"]),
+                     mol_xyz0=sc["mol_atoms"][:, :3].astype(np.float64), mol_cpar=t(sc["mol_cLJ"]), pts=t(pts),
+                     nxy=sc["nxy"], nz=sc["nz"], tip=tip, Qs=sc["Qs"], QZs=sc["QZs"], npbc=sc["npbc"],
+                     lvA=sc["lvA"], lvB=sc["lvB"], max_iter=3000, f2conv=1e-12)
+    return scene.prepare(), pts
+
+
+def combined(sc, pose, q):
+    x = PS.place(sc["mol_atoms"][:, :3].astype(np.float64), sc["mol_atoms"][:, :3].astype(np.float64).mean(0),
+                 np.asarray(pose, np.float64))
+    mol = np.concatenate([x, np.asarray(q, np.float64)[:, None]], 1)
+    atoms = np.concatenate([sc["sub_atoms"].astype(np.float64), mol])
+    cpar = np.concatenate([sc["sub_cLJ"], sc["mol_cLJ"]]).astype(np.float64)
+    return atoms, cpar
+
+
+def test_grid_over_base_equals_combined_grid():
+    sc = W.small_substrate_scene()
+    scene, _ = make_scene(sc)
+    pose = sc["poses"][1]
+    q = sc["mol_atoms"][:, 3].astype(np.float64)
+    x = PS.place(scene.mol_xyz0, scene.center, pose)
+    atoms = torch.as_tensor(np.concatenate([x, q[:, None]], 1).astype(np.float32), device=scene.pts.device)
+    G = D.grid_project(atoms, scene.mol_cpar, 0, scene.spec, sc["Qs"], sc["QZs"], sc["npbc"], sc["lvA"], sc["lvB"],
+                       cap=True, base=scene.base).cpu().numpy().astype(np.float64)
+    ca, cp = combined(sc, pose, q)
+    sp = scene.spec
+    Go = O.grid_project_d(0, ca.astype(np.float32).astype(np.float64), cp.astype(np.float32).astype(np.float64),
+                          sp.n, sp.p0, sp.dA, sp.dB, sp.dC, sc["Qs"], sc["QZs"], sc["npbc"], sc["lvA"], sc["lvB"])
+    mag = np.linalg.norm(Go, axis=-1, keepdims=True)
+    err = np.abs(G - Go)
+    print(f"[poses] grid over base: max |d| {err.max():.3e}, worst relative {(err / (mag + 1e-30)).max():.3e}")
+    assert (err <= 1e-5 * mag + 1e-6).all()
+
+
+def fp64_df(sc, scene, pose, q):
+    ca, cp = combined(sc, pose, q)
+    sp = scene.spec
+    G = O.grid_project_d(0, ca, cp, sp.n, sp.p0, sp.dA, sp.dB, sp.dC, sc["Qs"], sc["QZs"], sc["npbc"], sc["lvA"],
+                         sc["lvB"])
+    tip = scene.tip
+    pts = scene.pts.cpu().numpy()
+    FEs, PPs = O.relax_tilted_d(G, sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pts, scene.nz, tip.tipA, tip.tipB, tip.tipC,
+                                tip.stiffness, tip.dpos0, tip.relax, tip.surfFF, max_iter=40000, f2conv=1e-24,
+                                check_before=True)
+    assert (PPs[..., 3] > 0).all(), "FP64 reference relaxation did not converge"
+    M = DF.giessibl_matrix(scene.nz, scene.dz, scene.amp, scene.f0, scene.k_cant)
+    return (FEs[..., 2] @ M.T).reshape(scene.nxy[0], scene.nxy[1], scene.nz)
+
+
+def test_pose_adjoint_matches_fp64_finite_differences():
+    sc = W.small_substrate_scene()
+    scene, _ = make_scene(sc)
+    dev = scene.pts.device
+    pose0 = sc["poses"][2].astype(np.float64)
+    q0 = sc["mol_atoms"][:, 3].astype(np.float64)
+    pose = torch.tensor(pose0, device=dev, requires_grad=True)
+    q = torch.tensor(q0, device=dev, requires_grad=True)
+    df = PS.pose_df(pose, q, scene)
+    v = np.random.default_rng(4).standard_normal(df.shape)
+    (df.double() * torch.as_tensor(v, device=dev)).sum().backward()
+    assert scene.last_status["gated"] == 0, scene.last_status
+    gp, gq = pose.grad.cpu().numpy(), q.grad.cpu().numpy()
+
+    def phi(p, qq):
+        return float((v * fp64_df(sc, scene, p, qq)).sum())
+
+    fd, an, names = [], [], []
+    for k, h in ((0, 1e-3), (1, 1e-3), (2, 1e-3), (3, 1e-4)):
+        pp, pm = pose0.copy(), pose0.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd.append((phi(pp, q0) - phi(pm, q0)) / (2 * h))
+        an.append(gp[k])
+        names.append(("sx", "sy", "sz", "theta")[k])
+    for i in range(3):
+        qp, qm = q0.copy(), q0.copy()
+        qp[i] += 1e-4
+        qm[i] -= 1e-4
+        fd.append((phi(pose0, qp) - phi(pose0, qm)) / 2e-4)
+        an.append(gq[i])
+        names.append(f"q{i}")
+    fd, an = np.array(fd), np.array(an)
+    floor = 1e-2 * np.abs(fd).max()
+    rel = np.abs(an - fd) / np.maximum(np.abs(fd), floor)
+    for n, a, f, r in zip(names, an, fd, rel):
+        print(f"[poses] {n}: adjoint {a:+.6e} fd {f:+.6e} rel {r:.2e}")
+    assert rel.max() < 1e-3

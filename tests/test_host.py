@@ -143,3 +143,24 @@ def test_batch_z_window_asymmetric_margins():
     probe_lo = probe_hi + (w.nz - 1) * w.dtip
     assert z[0] <= probe_lo - 0.2 and z[1] >= probe_hi + 0.6 + sp.dC[2]
     assert nzw < nzw_s
+
+
+def test_rocksalt_slab_and_placements():
+    en, sub = W.rocksalt_slab()
+    assert sub.shape == (1024, 4) and abs(sub[:, 3].sum()) < 1e-4
+    assert set(en) == {"Na", "Cl"} and abs(np.abs(sub[:, 3]) - 0.8).max() < 1e-6
+    top = sub[sub[:, 2] == 0]
+    d = np.linalg.norm(top[:, None, :2] - top[None, :, :2], axis=-1)
+    nn = np.isclose(d, 5.64 / 2, atol=1e-5)
+    assert (top[:, 3][:, None] * top[:, 3][None, :])[nn].max() < 0      # nearest neighbours alternate
+    sc = W.substrate_scene()
+    assert sc["n"] == (902, 902, 300) and sc["mol_atoms"].shape == (60, 4) and sc["poses"].shape == (64, 4)
+    assert abs(sc["mol_atoms"][:, 2].mean() - 3.0) < 0.1
+    from spammm_amd import poses as PS
+    x0 = sc["mol_atoms"][:, :3].astype(np.float64)
+    c = x0.mean(0)
+    x = PS.place(x0, c, sc["poses"][5])
+    d0 = np.linalg.norm(x0[:, None] - x0[None], axis=-1)
+    d1 = np.linalg.norm(x[:, None] - x[None], axis=-1)
+    np.testing.assert_allclose(d0, d1, atol=1e-12)                    # rigid
+    np.testing.assert_allclose(x.mean(0) - c, [*sc["poses"][5][:2], 0.0], atol=1e-12)
