@@ -47,6 +47,7 @@ def main():
     ap.add_argument("--adam-steps", type=int, default=50)
     ap.add_argument("--batch-mols", type=int, default=512, help="molecules of the batched leg (0: skip)")
     ap.add_argument("--no-direct", action="store_true", help="skip the gridless direct Morse leg")
+    ap.add_argument("--poses", type=int, default=64, help="placements of the substrate leg (0: skip)")
     ap.add_argument("--z-margin", default=None,
                     help="z window margins of the batched leg in A, 'below,above' (default: the library's)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -185,6 +186,8 @@ def main():
         out["batched"] = batch_leg(dev, args.batch_mols, args.z_margin)
     if not args.no_direct:
         out["direct_morse"] = direct_leg(dev)
+    if args.poses > 0:
+        out["substrate_poses"] = substrate_leg(dev, args.poses)
     if not args.no_check:
         out["accuracy"] = accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip)
     if not args.no_cpu:
@@ -248,6 +251,92 @@ def batch_leg(dev, n_mol, z_margin=None):
     del bf, fe_plain, df_plain
     torch.cuda.empty_cache()
     return res
+
+
+def substrate_leg(dev, n_poses=64):
+    """Placements of a 60-atom molecule 3 A above a 4-layer rock-salt slab (1024 ions, periodic
+    x,y, 902x902x300 grid at 0.05 A, 192x192 pixels x 60 z-steps): the slab's raw grid once, then
+    per placement the forward (grid over the slab base, relaxation, df) and the adjoint of an L2
+    loss w.r.t. the placement (shift, rotation) and the molecule's charges, recomputing the grid
+    and the probe positions in the backward (one shared grid buffer)."""
+    import torch
+    from spammm_amd import poses as PS, workloads as WL
+    sc = WL.substrate_scene(n_poses=n_poses)
+    spec = D_axis(sc)
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)  # noqa: E731
+    from spammm_amd import device as D, params as PP
+    kl, kr = PP.stiffness_Nm_to_eVA2(sc["k_lat_Nm"]), PP.stiffness_Nm_to_eVA2(sc["k_rad_Nm"])
+    tip = D.Tip(stiffness=np.array([-kl, -kl, -kl, -kr], np.float32),
+                dpos0=np.array([0, 0, -sc["bond"], sc["bond"]], np.float32),
+                tipC=np.array([0., 0., 1., sc["dtip"]], np.float32))
+    pts = D.scan_points(sc["scan_p0"], sc["scan_da"], sc["scan_db"], *sc["nxy"])
+    scene = PS.Scene(spec=spec, sub_atoms=t(sc["sub_atoms"]), sub_cpar=t(sc["sub_cLJ"]),
+                     mol_xyz0=sc["mol_atoms"][:, :3].astype(np.float64), mol_cpar=t(sc["mol_cLJ"]), pts=t(pts),
+                     nxy=sc["nxy"], nz=sc["nz"], tip=tip, Qs=sc["Qs"], QZs=sc["QZs"], npbc=sc["npbc"],
+                     lvA=sc["lvA"], lvB=sc["lvB"], max_iter=1000, f2conv=1e-12, strict=False)
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    scene.prepare()
+    e1.record(stream)
+    torch.cuda.synchronize(dev)
+    t_base = e0.elapsed_time(e1)
+    log(f"[bench] substrate base grid {t_base:.1f} ms")
+    q0 = torch.tensor(sc["mol_atoms"][:, 3], dtype=torch.float64, device=dev)
+    dpose = torch.tensor([0.1, -0.1, 0.0, 0.02], dtype=torch.float64, device=dev)
+    t_f, t_b, gnorm, gated = [], [], [], 0
+    checks = []
+    for k in range(n_poses):
+        pose_k = torch.tensor(sc["poses"][k], dtype=torch.float64, device=dev)
+        with torch.no_grad():
+            target = PS.pose_df(pose_k + dpose, q0, scene).double()
+        pose = pose_k.clone().requires_grad_(True)
+        q = q0.clone().requires_grad_(True)
+        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a.record(stream)
+        dfi = PS.pose_df(pose, q, scene)
+        loss = 0.5 * ((dfi.double() - target) ** 2).sum()
+        b.record(stream)
+        loss.backward()
+        c.record(stream)
+        torch.cuda.synchronize(dev)
+        t_f.append(a.elapsed_time(b))
+        t_b.append(b.elapsed_time(c))
+        gated += scene.last_status.get("gated", 0)
+        gnorm.append(float(pose.grad.norm()))
+        if k < 2:       # directional finite difference along -grad(pose) on two placements
+            gp = pose.grad.detach()
+            d = -gp / gp.norm()
+            row = {"pose": k, "adjoint": -float(gp.norm())}
+            for eps in (1e-3, 1e-4):
+                with torch.no_grad():
+                    lp = float(0.5 * ((PS.pose_df(pose_k + eps * d, q0, scene).double() - target) ** 2).sum())
+                    lm = float(0.5 * ((PS.pose_df(pose_k - eps * d, q0, scene).double() - target) ** 2).sum())
+                row[f"fd_eps_{eps:g}"] = (lp - lm) / (2 * eps)
+            checks.append(row)
+        if (k + 1) % 16 == 0:
+            log(f"[bench] substrate placements {k + 1}/{n_poses}")
+    states = scene.n_scan * scene.nz
+    res = {
+        "workload": f"60-atom molecule x {n_poses} placements over a 1024-ion rock-salt slab, "
+                    f"{sc['n'][0]}x{sc['n'][1]}x{sc['n'][2]} grid (0.05 A), {sc['nxy'][0]}x{sc['nxy'][1]} px x {sc['nz']} z",
+        "base_grid_ms": t_base, "forward_ms_mean": float(np.mean(t_f)), "adjoint_ms_mean": float(np.mean(t_b)),
+        "adjoint_over_forward": float(np.mean(t_b) / np.mean(t_f)),
+        "pixel_z_steps_per_s_forward": states / (np.mean(t_f) * 1e-3),
+        "placements_per_s_forward_plus_adjoint": 1e3 / (np.mean(t_f) + np.mean(t_b)),
+        "gated_states_total": int(gated), "pose_grad_norm_median": float(np.median(gnorm)),
+        "directional_fd": checks,
+        "grid_bytes": int(scene.base.numel() * 4), "peak_mem_GB": torch.cuda.max_memory_allocated(dev) / 1e9,
+    }
+    log(f"[bench] substrate: {res}")
+    del scene
+    torch.cuda.empty_cache()
+    return res
+
+
+def D_axis(sc):
+    from spammm_amd import device as D
+    return D.axis_aligned_spec(sc["p0"], sc["L"], sc["n"])
 
 
 def _pmc_traffic(kernel, workload):

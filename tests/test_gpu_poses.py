@@ -1,7 +1,7 @@
 """Molecule placements over a substrate: grid over a base and the pose adjoint.
 
 Bars: the grid projected over a raw substrate base equals the FP64 oracle grid of the
-combined atom list per voxel within |d| <= 1e-5 ||(F,E)|| + 1e-6 (the FP32 grid bar of
+combined atom list per voxel within |d| <= 1e-5 ||(F,E)|| + 4e-6 sum|terms| + 1e-6 (the FP32 grid bar of
 tests/test_gpu_parity.py); the pose/charge adjoint equals central finite differences of the
 FP64 oracle forward (phi = v . df with fixed v) within 1e-3 relative, denominator floored at
 1e-2 of the largest |FD| (the bar of tests/test_gpu_adjoint.py).
@@ -59,10 +59,45 @@ def test_grid_over_base_equals_combined_grid():
     sp = scene.spec
     Go = O.grid_project_d(0, ca.astype(np.float32).astype(np.float64), cp.astype(np.float32).astype(np.float64),
                           sp.n, sp.p0, sp.dA, sp.dB, sp.dC, sc["Qs"], sc["QZs"], sc["npbc"], sc["lvA"], sc["lvB"])
+    # The ionic slab's terms cancel to a small net field above it, so the FP32 sum's rounding is
+    # bounded by the sum of the terms' magnitudes, not by the net value: |d| <= 1e-5 |net| +
+    # 4e-6 sum|terms| + 1e-6 (a missing or doubled atom term, ~1/300 of the sum, is far above it).
+    S = abs_term_sums(ca.astype(np.float32).astype(np.float64), cp.astype(np.float32).astype(np.float64), sp, sc)
     mag = np.linalg.norm(Go, axis=-1, keepdims=True)
     err = np.abs(G - Go)
-    print(f"[poses] grid over base: max |d| {err.max():.3e}, worst relative {(err / (mag + 1e-30)).max():.3e}")
-    assert (err <= 1e-5 * mag + 1e-6).all()
+    bound = 1e-5 * mag + 4e-6 * S + 1e-6
+    print(f"[poses] grid over base: max |d| {err.max():.3e}, worst |d|/bound {(err / bound).max():.3e}")
+    assert (err <= bound).all()
+
+
+def abs_term_sums(atoms, cpar, sp, sc):
+    """Per voxel, the sum over atoms and images of |F_term| and |E_term| (FP64, NumPy), capped
+    voxels scaled like the grid (100/|E|) so that the bound follows the stored values."""
+    nx, ny, nz = sp.n
+    ia, ib, ic = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    P = (np.asarray(sp.p0, np.float64)[None, :] + np.outer(ia.ravel(), sp.dA) + np.outer(ib.ravel(), sp.dB)
+         + np.outer(ic.ravel(), sp.dC))
+    SF = np.zeros(len(P))
+    SE = np.zeros(len(P))
+    net = np.zeros(len(P))
+    qs = float(sc["Qs"][0]) * 14.3996448915
+    for kx in range(-sc["npbc"][0], sc["npbc"][0] + 1):
+        for ky in range(-sc["npbc"][1], sc["npbc"][1] + 1):
+            sh = kx * np.asarray(sc["lvA"], np.float64) + ky * np.asarray(sc["lvB"], np.float64)
+            for (x, y, z, q), (c6, c12) in zip(atoms, cpar):
+                d = P - (np.array([x, y, z]) + sh)
+                r2 = (d * d).sum(1) + 1e-4
+                ir2 = 1.0 / r2
+                ir6 = ir2 ** 3
+                e_lj = (c12 * ir6 - c6) * ir6
+                f_lj = np.abs((12 * c12 * ir6 - 6 * c6) * ir6 * ir2) * np.sqrt(r2)
+                e_c = q * qs * np.sqrt(ir2)
+                SF += f_lj + np.abs(e_c) * ir2 * np.sqrt(r2)
+                SE += np.abs(e_lj) + np.abs(e_c)
+                net += e_lj + e_c
+    scale = np.minimum(1.0, 100.0 / np.maximum(np.abs(net), 1e-300))
+    S = (SF + SE) * scale
+    return S.reshape(nx, ny, nz).transpose(2, 1, 0)[..., None]
 
 
 def fp64_df(sc, scene, pose, q):
