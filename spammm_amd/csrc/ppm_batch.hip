@@ -83,14 +83,14 @@ __device__ rc::TiltArgs tilt_args(const PpmBatchParams& P) {
     return t;
 }
 
-template <int MODEL, bool HASQ0, int V>
+template <int MODEL, bool HASQ0, int V, bool OFFQ>
 __device__ void do_grid_item(const PpmMolDesc& d, const PpmBatchParams& P, int local, const float4* atoms,
                              const float* cpar, float4* grids, Site* sites) {
     const GridArgs a = mol_grid_args(d, P);
     const int ncolblk = (a.ncol + 255) / 256;
     const int cb = local % ncolblk, zb = local / ncolblk;
     const int cs = MODEL == PPM_MODEL_LJ ? 2 : 4;
-    gc::grid_tile<MODEL, HASQ0, V, true>(atoms + d.atom_off, cpar + (size_t)d.atom_off * cs, a, sites,
+    gc::grid_tile<MODEL, HASQ0, V, true, OFFQ ? gc::kOffStatic : gc::kOffNone>(atoms + d.atom_off, cpar + (size_t)d.atom_off * cs, a, sites,
                                          cb * 256 + threadIdx.x, d.klo + zb * V, d.klo + d.nzw, grids + d.grid_off,
                                          d.klo);
 }
@@ -132,7 +132,8 @@ __device__ void do_relax_item(const PpmMolDesc& d, const PpmBatchParams& P, int 
     }
 }
 
-template <int MODEL, bool HASQ0, int V>
+// OFFQ: the tip has charge sites at a nonzero z offset (compiled out when it has none).
+template <int MODEL, bool HASQ0, int V, bool OFFQ>
 __global__ __launch_bounds__(256) void k_batch(const PpmMolDesc* __restrict__ mols, PpmBatchParams P,
                                                const float4* __restrict__ atoms, const float* __restrict__ cpar,
                                                float4* __restrict__ grids, float4* __restrict__ FEs,
@@ -174,7 +175,7 @@ __global__ __launch_bounds__(256) void k_batch(const PpmMolDesc* __restrict__ mo
         signal_m = -1;
         if (local < d.n_grid_items) {
             if (P.mode != PPM_BATCH_RELAX_ONLY) {
-                do_grid_item<MODEL, HASQ0, V>(d, P, local, atoms, cpar, grids, sites);
+                do_grid_item<MODEL, HASQ0, V, OFFQ>(d, P, local, atoms, cpar, grids, sites);
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                 __syncthreads();            // every thread's tile stores precede the signal
                 signal_m = m;
@@ -198,16 +199,29 @@ __global__ __launch_bounds__(256) void k_batch(const PpmMolDesc* __restrict__ mo
     }
 }
 
-template <int MODEL, bool HASQ0>
+template <int MODEL, bool HASQ0, bool OFFQ>
 void launch_v(int V, dim3 grd, hipStream_t st, const PpmMolDesc* mols, const PpmBatchParams& P, const float4* atoms,
               const float* cpar, float4* grids, float4* FEs, float4* PPs, float* df, const float* dfW,
               const int* dfStart, Queue* q) {
     if (V == 8)
-        hipLaunchKernelGGL((k_batch<MODEL, HASQ0, 8>), grd, dim3(256), 0, st, mols, P, atoms, cpar, grids, FEs, PPs,
-                           df, dfW, dfStart, q);
+        hipLaunchKernelGGL((k_batch<MODEL, HASQ0, 8, OFFQ>), grd, dim3(256), 0, st, mols, P, atoms, cpar, grids, FEs,
+                           PPs, df, dfW, dfStart, q);
     else
-        hipLaunchKernelGGL((k_batch<MODEL, HASQ0, 4>), grd, dim3(256), 0, st, mols, P, atoms, cpar, grids, FEs, PPs,
-                           df, dfW, dfStart, q);
+        hipLaunchKernelGGL((k_batch<MODEL, HASQ0, 4, OFFQ>), grd, dim3(256), 0, st, mols, P, atoms, cpar, grids, FEs,
+                           PPs, df, dfW, dfStart, q);
+}
+
+template <int MODEL>
+void launch_q(bool hasq0, bool offq, int V, dim3 grd, hipStream_t st, const PpmMolDesc* mols,
+              const PpmBatchParams& P, const float4* atoms, const float* cpar, float4* grids, float4* FEs,
+              float4* PPs, float* df, const float* dfW, const int* dfStart, Queue* q) {
+    if (hasq0) {
+        if (offq) launch_v<MODEL, true, true>(V, grd, st, mols, P, atoms, cpar, grids, FEs, PPs, df, dfW, dfStart, q);
+        else launch_v<MODEL, true, false>(V, grd, st, mols, P, atoms, cpar, grids, FEs, PPs, df, dfW, dfStart, q);
+    } else {
+        if (offq) launch_v<MODEL, false, true>(V, grd, st, mols, P, atoms, cpar, grids, FEs, PPs, df, dfW, dfStart, q);
+        else launch_v<MODEL, false, false>(V, grd, st, mols, P, atoms, cpar, grids, FEs, PPs, df, dfW, dfStart, q);
+    }
 }
 
 }  // namespace
@@ -223,20 +237,20 @@ int launch_batch(const PpmMolDesc* mols, const PpmBatchParams& P, const float* a
         hipLaunchKernelGGL(k_batch_reset, dim3((n + 255) / 256), dim3(256), 0, stream, q, P.nmol, reset == 2);
         if (int r = check_launch("batch_reset")) return r;
     }
-    bool hasq0 = false;
-    for (int k = 0; k < 4; ++k) hasq0 |= (P.Qs[k] != 0.f && P.QZs[k] == 0.f);
+    bool hasq0 = false, offq = false;
+    for (int k = 0; k < 4; ++k) {
+        hasq0 |= (P.Qs[k] != 0.f && P.QZs[k] == 0.f);
+        offq |= (P.Qs[k] != 0.f && P.QZs[k] != 0.f);
+    }
     const dim3 grd(workers);
     const float4* at = reinterpret_cast<const float4*>(atoms);
     float4* gr = reinterpret_cast<float4*>(grids);
     float4* fe = reinterpret_cast<float4*>(FEs);
     float4* pp = reinterpret_cast<float4*>(PPs);
-    if (P.model == PPM_MODEL_LJ) {
-        if (hasq0) launch_v<PPM_MODEL_LJ, true>(P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
-        else launch_v<PPM_MODEL_LJ, false>(P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
-    } else {
-        if (hasq0) launch_v<PPM_MODEL_MORSE, true>(P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
-        else launch_v<PPM_MODEL_MORSE, false>(P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
-    }
+    if (P.model == PPM_MODEL_LJ)
+        launch_q<PPM_MODEL_LJ>(hasq0, offq, P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
+    else
+        launch_q<PPM_MODEL_MORSE>(hasq0, offq, P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
     return check_launch("batch");
 }
 

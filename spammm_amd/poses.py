@@ -3,8 +3,9 @@ placement (shift, rotation about z) and the molecule's charges.
 
 The substrate's raw force field (no |E|>100 rescale) is projected once; every placement adds
 the molecule's pair sum to it and applies the rescale to the total (ppm_grid_project_add), then
-relaxes the scan and applies the df operator.  The backward recomputes the placement's grid
-and probe positions instead of storing them (one grid buffer is shared by all placements),
+relaxes the scan and applies the df operator.  The backward recomputes the probe positions
+and, unless the shared grid buffer still holds this placement's grid (no other placement's
+forward ran in between), the grid (one grid buffer is shared by all placements),
 runs the implicit-function relax adjoint, the grid->atom adjoint of the molecule's atoms over
 the substrate base (ppm_grid_adjoint_add) and the rigid-motion chain rule:
     x_i = R(theta) (x0_i - c) + c + s
@@ -51,6 +52,7 @@ class Scene:
     base: object = None         # raw substrate grid, filled by prepare()
     grid: object = None         # placement grid buffer
     grid_cot: object = None     # adjoint buffer
+    grid_serial: int = 0        # forward count; identifies the placement whose grid `grid` holds
 
     @property
     def n_scan(self):
@@ -94,9 +96,11 @@ def place(xyz0, center, pose):
     return lib.stack([x, y, z], 1)
 
 
-def _forward(scene: Scene, atoms):
-    D.grid_project(atoms, scene.mol_cpar, _lib.PPM_MODEL_LJ, scene.spec, scene.Qs, scene.QZs, scene.npbc,
-                   scene.lvA, scene.lvB, cap=True, out=scene.grid, base=scene.base)
+def _forward(scene: Scene, atoms, grid_valid=False):
+    if not grid_valid:
+        D.grid_project(atoms, scene.mol_cpar, _lib.PPM_MODEL_LJ, scene.spec, scene.Qs, scene.QZs, scene.npbc,
+                       scene.lvA, scene.lvB, cap=True, out=scene.grid, base=scene.base)
+        scene.grid_serial += 1
     FEs, PPs = D.relax_tilted(scene.grid, scene.spec, scene.pts, scene.nz, scene.tip, scene.max_iter, scene.f2conv,
                               True, scan_ny=int(scene.nxy[1]))
     if scene.polish > 0:
@@ -114,6 +118,7 @@ def _make_function():
             xyz = place(xyz0, torch.as_tensor(scene.center, device=pose.device), pose.detach())
             atoms = torch.cat([xyz.float(), q.detach().float()[:, None]], 1).contiguous()
             FEs, _ = _forward(scene, atoms)
+            ctx.grid_serial = scene.grid_serial
             op = DF.z_operator(scene.df_kind, scene.nz, scene.dz, pose.device, scene.amp, scene.f0, scene.k_cant)
             df = op.apply(FEs, (scene.nz * 4, 4, 0), scene.n_scan, 1, offset=2)
             ctx.scene = scene
@@ -129,7 +134,8 @@ def _make_function():
             ctr = torch.as_tensor(scene.center, device=dev)
             xyz = place(xyz0, ctr, pose)
             atoms = torch.cat([xyz.float(), q.float()[:, None]], 1).contiguous()
-            _, PPs = _forward(scene, atoms)                     # recompute grid and probe positions
+            # recompute the probe positions, and the grid if another placement has overwritten it
+            _, PPs = _forward(scene, atoms, grid_valid=scene.grid_serial == ctx.grid_serial)
             g = gdf.detach().float().contiguous().reshape(scene.n_scan, scene.nz)
             opT = DF.z_operator(scene.df_kind, scene.nz, scene.dz, dev, scene.amp, scene.f0, scene.k_cant,
                                 transpose=True)

@@ -153,3 +153,30 @@ def test_pose_adjoint_matches_fp64_finite_differences():
     for n, a, f, r in zip(names, an, fd, rel):
         print(f"[poses] {n}: adjoint {a:+.6e} fd {f:+.6e} rel {r:.2e}")
     assert rel.max() < 1e-3
+
+
+def test_backward_after_another_placement_recomputes_the_grid():
+    """The shared grid buffer is overwritten by a later placement's forward; the backward of the
+    earlier placement must then rebuild its own grid and give the same gradients (to the
+    rounding of the relax adjoint's atomic scatter; the other placement's grid would be far off)."""
+    sc = W.small_substrate_scene()
+    scene, _ = make_scene(sc)
+    dev = scene.pts.device
+    q0 = torch.tensor(sc["mol_atoms"][:, 3], dtype=torch.float64, device=dev)
+    v = torch.as_tensor(np.random.default_rng(6).standard_normal((*sc["nxy"], sc["nz"])), device=dev)
+
+    def grads(interleave):
+        pose = torch.tensor(sc["poses"][1], dtype=torch.float64, device=dev, requires_grad=True)
+        q = q0.clone().requires_grad_(True)
+        df = PS.pose_df(pose, q, scene)
+        if interleave:
+            with torch.no_grad():
+                PS.pose_df(torch.tensor(sc["poses"][3], dtype=torch.float64, device=dev), q0, scene)
+        (df.double() * v).sum().backward()
+        return pose.grad.cpu().numpy(), q.grad.cpu().numpy()
+
+    gp_a, gq_a = grads(False)
+    gp_b, gq_b = grads(True)
+    print(f"[poses] pose grad {gp_a} (interleaved {gp_b})")
+    for a, b in ((gp_a, gp_b), (gq_a, gq_b)):
+        assert np.allclose(a, b, rtol=1e-5, atol=1e-6 * np.abs(a).max())
