@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_batch(const PpmMolDesc* __restrict__ mo
                                                float4* __restrict__ PPs, float* __restrict__ df,
                                                const float* __restrict__ dfW, const int* __restrict__ dfStart,
                                                Queue* q) {
-    __shared__ Site sites[gc::kTile];
+    __shared__ Site sites[gc::kTile + 1];
     __shared__ int s_item, s_mol;
     int* done = q_done(q);
     int* flags = q_flag(q, P.nmol);

@@ -79,7 +79,8 @@ __device__ __forceinline__ void offset_site(const Site& S, float qf, float zo, c
 
 // One workgroup tile: 256 columns starting at column `col - threadIdx.x`, V planes from ic0.
 // Voxels with ic < ic_end are stored at out[(ic - out_klo)*ncol + col].  Every thread of the
-// workgroup must call it (it synchronizes on the shared site tile).
+// workgroup must call it (it synchronizes on the shared site tile); `sites` holds kTile + 1
+// entries.
 // `base` (optional, never the same buffer as `out`): a raw (uncapped) grid of other atoms added
 // to every voxel before the |E|>100 rescale.
 // V voxels per thread along c (z); DCZ: dC has no x,y component (every grid set-up of
@@ -111,14 +112,18 @@ __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, cons
 #pragma unroll
     for (int k = 0; k < V; ++k) { fx[k] = 0.f; fy[k] = 0.f; fz[k] = 0.f; fe[k] = 0.f; }
 
-    for (int base = 0; base < a.nsites; base += kTile) {
+    for (int t0 = 0; t0 < a.nsites; t0 += kTile) {
         __syncthreads();
-        const int s = base + threadIdx.x;
+        const int s = t0 + threadIdx.x;
         if (s < a.nsites) load_site<MODEL>(atoms, cpar, a, s, sites[threadIdx.x]);
         __syncthreads();
-        const int cnt = min(kTile, a.nsites - base);
+        const int cnt = min(kTile, a.nsites - t0);
+        // the next site is read one iteration ahead (sites[] has kTile + 1 entries; the read
+        // of entry cnt is never used), so the LDS latency overlaps the current site's math
+        Site Sn = sites[0];
         for (int j = 0; j < cnt; ++j) {
-            const Site S = sites[j];
+            const Site S = Sn;
+            Sn = sites[j + 1];
             const float qq0 = S.q * a.q0;
             float dxs[DCZ ? 1 : V], dys[DCZ ? 1 : V], dxy2[DCZ ? 1 : V];
 #pragma unroll
@@ -136,7 +141,13 @@ __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, cons
                 const float ir = rsq(r2);
                 const float ir2 = ir * ir;
                 float fr;
-                if (MODEL == PPM_MODEL_LJ) {
+                if (MODEL == PPM_MODEL_LJ && HASQ0) {
+                    // LJ and the charge term share the final 1/r^2: 9 VALU instead of 10
+                    const float ir6 = ir2 * ir2 * ir2;
+                    const float ec = qq0 * ir;
+                    fe[k] = fmaf(fmaf(S.c1, ir6, -S.c0), ir6, fe[k] + ec);  // E += ec + (c12 ir6 - c6) ir6
+                    fr = fmaf(fmaf(S.c3, ir6, -S.c2), ir6, ec) * ir2;       // fr = ((c3 ir6 - c2) ir6 + ec) / r^2
+                } else if (MODEL == PPM_MODEL_LJ) {
                     const float ir6 = ir2 * ir2 * ir2;
                     fe[k] = fmaf(fmaf(S.c1, ir6, -S.c0), ir6, fe[k]);      // E = (c12 ir6 - c6) ir6
                     fr = fmaf(S.c3, ir6, -S.c2) * ir6 * ir2;
@@ -147,7 +158,7 @@ __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, cons
                     fe[k] = fmaf(ex - 2.0f, Ae, fe[k]);                      // E = E0 ex (ex - 2)
                     fr = S.c3 * (Ae * (ex - 1.0f)) * ir;
                 }
-                if (HASQ0) {
+                if (HASQ0 && MODEL != PPM_MODEL_LJ) {
                     const float ec = qq0 * ir;
                     fr = fmaf(ec, ir2, fr);
                     fe[k] += ec;
