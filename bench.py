@@ -173,8 +173,11 @@ def main():
             "relax_pixel_z_steps_per_s": states / (t_relax * 1e-3),
             "relax_mean_fire_iters": mean_iter,
             "relax_converged_fraction": conv_frac,
+            # algorithmic bytes: 8 float4 corners per force evaluation + FE/PP write; the corners are
+            # served mostly from registers and L2/MALL, so this is not HBM traffic (relax_hbm_bytes)
             "relax_algorithmic_GBps": relax_gbs,
-            "relax_frac_of_hbm_peak": relax_gbs / PEAK_HBM_GBS,
+            "relax_algorithmic_frac_of_hbm_peak": relax_gbs / PEAK_HBM_GBS,
+            "relax_hbm_bytes_per_launch": _pmc_traffic("k_relax_tilted", w.name),
             "df_ms": t_df,
             "df_GBps": df_gbs,
         },
@@ -559,7 +562,17 @@ def accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip):
     Go = O.grid_voxels(0, w.atoms, w.cLJ, w.n, sp.p0, sp.dA, sp.dB, sp.dC, w.Qs, w.QZs, idx, w.npbc, w.lvA, w.lvB)
     Gg = G.reshape(-1, 4)[idx].astype(np.float64)
     mag = np.linalg.norm(Go, axis=1)
-    grid_rel = float((np.abs(Gg - Go).max(1) / np.maximum(mag, 1e-30)).max())
+    gerr = np.abs(Gg - Go).max(1)
+    grid_rel = gerr / np.maximum(mag, 1e-30)
+    # the FP32 CPU oracle at the same voxels: the error any FP32 sum of these terms makes
+    nx, ny = w.n[0], w.n[1]
+    G32 = np.empty((idx.size, 4))
+    for k, v in enumerate(idx):
+        pv = (np.asarray(sp.p0, np.float64) + (v % nx) * np.asarray(sp.dA, np.float64)
+              + ((v // nx) % ny) * np.asarray(sp.dB, np.float64) + (v // (nx * ny)) * np.asarray(sp.dC, np.float64))
+        G32[k] = O.grid_project(0, w.atoms, w.cLJ, (1, 1, 1), pv.astype(np.float32), sp.dA, sp.dB, sp.dC, w.Qs,
+                                w.QZs, w.npbc, w.lvA, w.lvB, cap=True, precision="f32").reshape(4)
+    gerr32 = np.abs(G32 - Go).max(1)
     sel = rng.choice(w.n_scan, 64, replace=False)
     F = FEs.cpu().numpy()[sel]
     P = PPs.cpu().numpy()[sel]
@@ -572,13 +585,27 @@ def accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip):
     z = w.scan_p0[2] + w.dtip * np.arange(w.nz)
     dfo = R.giessibl_df_piecewise(F[..., 2].astype(np.float64), z, w.amp, w.f0, w.k_cant)
     dfg = df.cpu().numpy().reshape(w.n_scan, w.nz)[sel]
+    ddf = np.abs(dfg - dfo)
+    rms = lambda x: float(np.sqrt(np.mean(np.square(x))))  # noqa: E731
     res = {
-        "grid_max_rel_err_vs_fp64": grid_rel,
+        # per voxel: max channel error relative to |(F,E)|; the parity bar of the tests is
+        # |d| <= 1e-5 |(F,E)| + 1e-6.  High above the flake the net field (~1e-2) is a sum of
+        # ~1800 Coulomb terms of ~1e-2 each, so any FP32 sum errs by a few 1e-6 there: the FP32
+        # oracle's own error at the same voxels is reported beside the GPU's
+        "grid_rel_err_vs_fp64_max": float(grid_rel.max()), "grid_rel_err_vs_fp64_rms": rms(grid_rel),
+        "grid_abs_err_vs_fp64_max": float(gerr.max()),
+        "grid_err_over_parity_bar_max": float((gerr / (1e-5 * mag + 1e-6)).max()),
+        "grid_fp32_oracle_err_over_parity_bar_max": float((gerr32 / (1e-5 * mag + 1e-6)).max()),
+        "grid_fp32_oracle_abs_err_vs_fp64_max": float(gerr32.max()),
         "relax_bitexact_vs_fp32_oracle": bool(np.array_equal(F, F32) and np.array_equal(P, P32)),
-        "pp_err_vs_fp64_max_A": float(dpp.max()), "pp_err_vs_fp64_median_A": float(np.median(dpp)),
-        "fz_err_vs_fp64_max": float(dfz.max()), "fz_err_vs_fp64_median": float(np.median(dfz)),
-        "df_max_rel_err_vs_fp64": float(np.abs(dfg - dfo).max() / max(np.abs(dfo).max(), 1e-30)),
-        "sample": "1000 grid voxels, 64 pixels x all z",
+        "pp_err_vs_fp64_max_A": float(dpp.max()), "pp_err_vs_fp64_rms_A": rms(dpp),
+        "pp_err_vs_fp64_median_A": float(np.median(dpp)),
+        "fz_err_vs_fp64_max": float(dfz.max()), "fz_err_vs_fp64_rms": rms(dfz),
+        "fz_err_vs_fp64_median": float(np.median(dfz)),
+        "df_err_vs_fp64_max": float(ddf.max()), "df_err_vs_fp64_rms": rms(ddf),
+        "df_max_rel_err_vs_fp64": float(ddf.max() / max(np.abs(dfo).max(), 1e-30)),
+        "sample": "1000 grid voxels, 64 pixels x all z; relaxation errors at the reference's stopping "
+                  "rule (|F| < 1e-4 eV/A, <= 100 iterations), FP64 oracle relaxed on the same grid",
     }
     log(f"[bench] accuracy leg {time.perf_counter() - t0:.1f}s: {res}")
     return res
