@@ -497,6 +497,8 @@ def _diff_case(dev, z0, adam_steps, lr):
             ad, name = float(gq[i]), f"q[{i}]"
         rows.append((name, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30)))
     rel = np.array([r[3] for r in rows])
+    fd64 = _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, gx, gq, (1e-3, 1e-4))
+    fd64_small_h = _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, gx, gq, (1e-4, 1e-5))
     # directional FD along -grad (all positions) at three step sizes
     gn = float(gx.norm())
     dvec = -gx / gn
@@ -526,12 +528,92 @@ def _diff_case(dev, z0, adam_steps, lr):
         "fd_32_params": {"h": "1e-3 A / 1e-4 e", "within_1e-3": int((rel <= 1e-3).sum()),
                          "within_1e-2": int((rel <= 1e-2).sum()), "median_rel_err": float(np.median(rel)),
                          "rows": [(n, round(a, 6), round(f, 6), float(f"{r:.3g}")) for n, a, f, r in rows]},
+        "fd_32_params_fp64_forward": fd64,
+        "fd_32_params_fp64_forward_h_1e-4": fd64_small_h,
         "directional_fd_minus_grad": {"adjoint": -gn, **{f"fd_eps_{k}": v for k, v in dirfd.items()}},
         "adam": {"steps": adam_steps, "lr": lr, "schedule": "constant", "wall_s": t_adam,
                  "loss_first": hist[0] if hist else None, "loss_last": hist[-1] if hist else None,
                  "loss_every5": hist[::5], "rmsd_start_A": rmsd0, "rmsd_end_A": rmsd[-1] if rmsd else None,
                  "rmsd_every5": rmsd[::5]},
     }
+
+
+def _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, gx, gq, steps=(1e-3, 1e-4)):
+    """Central differences of the same loss with the forward in FP64 on the device (FP64 grid
+    projection and FP64 relaxation to |F| < 1e-12 eV/A, FP64 Giessibl df), on the same 32
+    parameters as the FP32 check, steps (position A, charge e): the reference-precision
+    comparison of the adjoint."""
+    import torch
+    from spammm_amd import device as D, df as DF
+    dev = xyz0.device
+    spec, nz = prob.spec, prob.nz
+    pts64 = prob.pts.double().contiguous()
+    M = torch.as_tensor(DF.giessibl_matrix(nz, prob.dz, prob.amp, prob.f0, prob.k_cant), device=dev)
+    r6 = R0.double() ** 6
+    cpar = torch.stack([2.0 * E0.double() * r6, E0.double() * r6 * r6], 1).contiguous()
+    tgt = target.reshape(prob.n_scan, nz)
+    buf = None
+    t0 = time.perf_counter()
+
+    def df64(xx, qq):
+        nonlocal buf
+        atoms = torch.cat([xx, qq[:, None]], 1).contiguous()
+        buf = D.grid_project_f64(atoms, cpar, 0, spec, prob.Qs, prob.QZs, prob.npbc, prob.lvA, prob.lvB,
+                                 cap=prob.cap, out=buf)
+        FEs, _ = D.relax_tilted_f64(buf, spec, pts64, nz, prob.tip, max_iter=40000, f2conv=1e-24,
+                                    check_before=True, want_pps=False)
+        return FEs[..., 2] @ M.T
+
+    def loss64(xx, qq):
+        return float(0.5 * ((df64(xx, qq) - tgt) ** 2).sum())
+
+    # the adjoint of the FP32 forward with the FP64 forward's residual as its upstream: separates
+    # the accuracy of the adjoint's derivatives from the FP32 forward's own error in df - target
+    from spammm_amd import diff
+    res64 = (df64(xyz0, q0) - tgt).reshape(int(prob.nxy[0]), int(prob.nxy[1]), nz)
+    xr, qr = xyz0.clone().requires_grad_(True), q0.clone().requires_grad_(True)
+    diff.df_image(xr, qr, R0, E0, prob).backward(res64.float())
+    gx_r, gq_r = xr.grad.detach(), qr.grad.detach()
+
+    na = xyz0.shape[0]
+    rows = []
+    for k in sel:
+        k = int(k)
+        if k < 3 * na:
+            i, c = divmod(k, 3)
+            h = steps[0]
+            xp, xm = xyz0.clone(), xyz0.clone()
+            xp[i, c] += h
+            xm[i, c] -= h
+            fd = (loss64(xp, q0) - loss64(xm, q0)) / (2 * h)
+            ad, ar, name = float(gx[i, c]), float(gx_r[i, c]), f"xyz[{i},{c}]"
+        else:
+            i = k - 3 * na
+            h = steps[1]
+            qp, qm = q0.clone(), q0.clone()
+            qp[i] += h
+            qm[i] -= h
+            fd = (loss64(xyz0, qp) - loss64(xyz0, qm)) / (2 * h)
+            ad, ar, name = float(gq[i]), float(gq_r[i]), f"q[{i}]"
+        rows.append((name, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30), ar, abs(ar - fd) / max(abs(fd), 1e-30)))
+    rel = np.array([r[3] for r in rows])
+    rel_r = np.array([r[5] for r in rows])
+    floor = 1e-2 * max(abs(r[2]) for r in rows)
+    rel_floor = np.array([abs(r[1] - r[2]) / max(abs(r[2]), floor) for r in rows])
+    torch.cuda.synchronize()
+    return {"h": f"{steps[0]:g} A / {steps[1]:g} e",
+            "forward": "FP64 grid + FP64 relaxation (|F| < 1e-12) + FP64 df, on the device",
+            "within_1e-3": int((rel <= 1e-3).sum()), "within_1e-2": int((rel <= 1e-2).sum()),
+            "median_rel_err": float(np.median(rel)), "max_rel_err": float(rel.max()),
+            "within_1e-3_floor_1pct_of_max": int((rel_floor <= 1e-3).sum()),
+            "adjoint_with_fp64_residual": {"within_1e-3": int((rel_r <= 1e-3).sum()),
+                                           "within_1e-2": int((rel_r <= 1e-2).sum()),
+                                           "median_rel_err": float(np.median(rel_r)),
+                                           "max_rel_err": float(rel_r.max())},
+            "wall_s": time.perf_counter() - t0,
+            "rows (name, adjoint, fd, rel, adjoint_fp64_residual, rel)":
+                [(n, round(a, 6), round(f, 6), float(f"{r:.3g}"), round(ar, 6), float(f"{rr:.3g}"))
+                 for n, a, f, r, ar, rr in rows]}
 
 
 def diff_leg(dev, adam_steps=50):

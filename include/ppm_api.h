@@ -147,6 +147,28 @@ int ppm_grid_project_add(int model, const float* atoms, const float* cpar, int n
                          int npbc_a, int npbc_b, const float lvA[3], const float lvB[3], int cap,
                          const float* base, float* grid, void* stream);
 
+/* ppm_grid_project in FP64: double atoms (na,4), cpar ((na,2) LJ c6,c12 / (na,4) Morse R0,E0,K,0),
+ * tip charges, lattice vectors and output grid (nz,ny,nx,4); voxel positions from the spec in
+ * double.  Same pair laws, images and rescale as ppm_grid_project (evalLJC_QZs_toImg /
+ * evalMorseC_QZs_toImg, kernels/AFM.cl:916-1056, images as kernels/gridFF.cl:938-996) at the
+ * precision of the reference's CPU64 test oracle (tests/SPM/test_afm_morse.py:410-624); for
+ * finite-difference gradient checks and reference-quality fields, not a hot path. */
+int ppm_grid_project_f64(int model, const double* atoms, const double* cpar, int na,
+                         const PpmGridSpec* spec, const double Qs[4], const double QZs[4],
+                         int npbc_a, int npbc_b, const double lvA[3], const double lvB[3],
+                         int cap, double* grid, void* stream);
+
+/* ppm_relax_strokes_tilted in FP64: double grid (nz,ny,nx,4) (e.g. from ppm_grid_project_f64),
+ * double points (n_scan,4), double FEs / PPs (n_scan,nz,4) (PPs may be NULL); same semantics
+ * as ppm_relax_strokes_tilted (relaxStrokesTilted, kernels/AFM.cl:478-562) with every operation
+ * in FP64, at the precision of the reference's CPU64 test oracle
+ * (tests/SPM/test_afm_morse.py:410-624).  Not a hot path: the reference-precision forward for
+ * finite-difference gradient checks. */
+int ppm_relax_strokes_tilted_f64(const double* grid, const PpmGridSpec* spec,
+                                 const double* points, int n_scan, int nz,
+                                 const PpmTip* tip, const PpmRelaxCtl* ctl,
+                                 double* FEs, double* PPs, void* stream);
+
 /* Relaxed scan with a tilted tip frame.
  * Replaces relaxStrokesTilted (kernels/AFM.cl:478-562) as launched by
  * AFMulator.run_scan (spammm/SPM/AFM.py:950-1010).

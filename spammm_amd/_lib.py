@@ -43,7 +43,8 @@ EXPORTS = (
     "ppm_fe_in_strokes_tilted", "ppm_z_stencil", "ppm_axis_operator", "ppm_relax_adjoint",
     "ppm_grid_adjoint_workspace_size", "ppm_grid_adjoint", "ppm_relax_polish", "ppm_batch_workspace_size",
     "ppm_batch_forward", "ppm_relax_direct_morse", "ppm_direct_morse_vjp_workspace_size", "ppm_direct_morse_vjp",
-    "ppm_grid_project_add", "ppm_grid_adjoint_add",
+    "ppm_grid_project_add", "ppm_grid_adjoint_add", "ppm_grid_project_f64",
+    "ppm_relax_strokes_tilted_f64",
 )
 
 _lib = None
@@ -76,6 +77,11 @@ def load():
     L.ppm_grid_project_add.restype = i
     L.ppm_grid_project_add.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
                                        P(ctypes.c_float * 3), i, vp, vp, vp]
+    D4, D3 = ctypes.c_double * 4, ctypes.c_double * 3
+    L.ppm_grid_project_f64.restype = i
+    L.ppm_grid_project_f64.argtypes = [i, vp, vp, i, P(PpmGridSpec), D4, D4, i, i, P(D3), P(D3), i, vp, vp]
+    L.ppm_relax_strokes_tilted_f64.restype = i
+    L.ppm_relax_strokes_tilted_f64.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), P(PpmRelaxCtl), vp, vp, vp]
     L.ppm_grid_adjoint_add.restype = i
     L.ppm_grid_adjoint_add.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
                                        P(ctypes.c_float * 3), i, vp, vp, vp, vp, ctypes.c_size_t, vp, vp]

@@ -4,8 +4,8 @@
 
 Each source is compiled to an object under spammm_amd/_build/ and the objects are
 linked into spammm_amd/libppm_hip.so.  The relaxation kernels are compiled with
--ffp-contract=off (their FP32 operation order is part of their contract); the
-grid projection with FMA contraction.
+-ffp-contract=off (their FP32 operation order is part of their contract), and so is
+the FP64 reference forward (the operation order of the FP64 CPU oracle).
 """
 from __future__ import annotations
 
@@ -30,6 +30,7 @@ SOURCES = [
     ("ppm_df.hip", ["-ffp-contract=fast"]),
     ("ppm_batch.hip", ["-ffp-contract=off"]),
     ("ppm_direct.hip", ["-ffp-contract=off"]),
+    ("ppm_f64.hip", ["-ffp-contract=off"]),
 ]
 
 

@@ -124,6 +124,43 @@ int ppm_grid_project_add(int model, const float* atoms, const float* cpar, int n
                              stream);
 }
 
+int ppm_grid_project_f64(int model, const double* atoms, const double* cpar, int na, const PpmGridSpec* spec,
+                         const double Qs[4], const double QZs[4], int npbc_a, int npbc_b, const double lvA[3],
+                         const double lvB[3], int cap, double* grid, void* stream) {
+    const char* fn = "ppm_grid_project_f64";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (model != PPM_MODEL_LJ && model != PPM_MODEL_MORSE) { set_error("%s: unknown model %d", fn, model); return PPM_ERR_INVALID_ARG; }
+    if (na < 1 || na > (1 << 20)) { set_error("%s: invalid atom count %d", fn, na); return PPM_ERR_INVALID_ARG; }
+    if (!atoms || !cpar || !grid || !Qs || !QZs) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    if (npbc_a < 0 || npbc_b < 0 || npbc_a > 64 || npbc_b > 64) {
+        set_error("%s: image range (%d,%d) out of [0,64]", fn, npbc_a, npbc_b);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if ((npbc_a > 0 && !lvA) || (npbc_b > 0 && !lvB)) { set_error("%s: images requested without lattice vectors", fn); return PPM_ERR_INVALID_ARG; }
+    if ((long long)na * (2 * npbc_a + 1) * (2 * npbc_b + 1) > (1LL << 28)) { set_error("%s: too many image sites", fn); return PPM_ERR_INVALID_ARG; }
+    for (int k = 0; k < 4; ++k)
+        if (!std::isfinite(Qs[k]) || !std::isfinite(QZs[k])) { set_error("%s: non-finite tip charges", fn); return PPM_ERR_INVALID_ARG; }
+    if (spec->nz > 65535) { set_error("%s: nz=%d above 65535", fn, spec->nz); return PPM_ERR_INVALID_ARG; }
+    return launch_grid_project_f64(model, atoms, cpar, na, *spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, grid,
+                                   (hipStream_t)stream);
+}
+
+int ppm_relax_strokes_tilted_f64(const double* grid, const PpmGridSpec* spec, const double* points, int n_scan,
+                                 int nz, const PpmTip* tip, const PpmRelaxCtl* ctl, double* FEs, double* PPs,
+                                 void* stream) {
+    const char* fn = "ppm_relax_strokes_tilted_f64";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (!grid || !points || !FEs || !tip) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    if (n_scan < 1 || nz < 1 || (long long)n_scan * nz > (1LL << 31) / 4) {
+        set_error("%s: invalid scan size %d x %d", fn, n_scan, nz);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (int r = check_ctl(fn, ctl, n_scan)) return r;
+    return launch_relax_tilted_f64(grid, *spec, points, n_scan, nz, *tip, *ctl, FEs, PPs, (hipStream_t)stream);
+}
+
 int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
                              const PpmTip* tip, const PpmRelaxCtl* ctl, float* FEs, float* PPs, void* stream) {
     const char* fn = "ppm_relax_strokes_tilted";

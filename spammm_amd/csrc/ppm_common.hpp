@@ -12,6 +12,8 @@ namespace ppm {
 // Physical constants of kernels/common.cl:28-29.
 constexpr float kR2SAFE = 1e-4f;
 constexpr float kCOULOMB = 14.3996448915f;
+constexpr double kR2SAFE_D = 1e-4;            // the same constants for the FP64 projection
+constexpr double kCOULOMB_D = 14.3996448915;
 
 // Algorithmic FLOP per pair evaluation (grid projection roofline unit), counted on
 // the reference pair laws with add/sub/mul = 1, FMA = 2, rcp/sqrt/rsqrt/exp = 1:
@@ -44,6 +46,12 @@ int launch_relax_adjoint(const float* grid, const PpmGridSpec& s, const float* p
                          hipStream_t stream);
 int launch_relax_polish(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
                         const PpmTip& tip, int steps, float* PPs, float* FEs, hipStream_t stream);
+int launch_grid_project_f64(int model, const double* atoms, const double* cpar, int na, const PpmGridSpec& g,
+                            const double Qs[4], const double QZs[4], int npbc_a, int npbc_b, const double lvA[3],
+                            const double lvB[3], int cap, double* grid, hipStream_t stream);
+int launch_relax_tilted_f64(const double* grid, const PpmGridSpec& g, const double* pts, int n_scan, int nz,
+                            const PpmTip& tip, const PpmRelaxCtl& ctl, double* FEs, double* PPs,
+                            hipStream_t stream);
 size_t batch_workspace(int nmol);
 int launch_batch(const PpmMolDesc* mols, const PpmBatchParams& P, const float* atoms, const float* cpar,
                  float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,

@@ -45,7 +45,8 @@ __global__ __launch_bounds__(256) void k_grid_project(const float4* __restrict__
 // the rescale) are differentiated through the rescale: g_raw = s g - s (g.fe_raw)/E_raw e_w,
 // s = 100/|E_raw|, with fe_raw recomputed for the workgroup.
 // Workgroups whose voxels all have zero cotangent exit at once (flag 0); the others write
-// one partial sum per atom, reduced over workgroups by k_reduce_partials in FP64.
+// one FP64 partial sum per atom (the V voxels of one site image summed in FP32, the rest in
+// FP64), reduced over workgroups by k_reduce_partials in FP64.
 template <int MODEL>
 __device__ __forceinline__ void pair_fe_raw(float dx, float dy, float dz, float dxy2, float q, float p0, float p1,
                                             float K, const GridArgs& a, float4& fe) {
@@ -136,8 +137,8 @@ template <int MODEL, int V, bool DCZ>
 __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__ atoms, const float* __restrict__ cpar,
                                                       GridArgs a, const float4* __restrict__ grid,
                                                       const float4* __restrict__ gcot, const float4* __restrict__ base,
-                                                      float* __restrict__ partials, int* __restrict__ active) {
-    __shared__ float red[4][6];
+                                                      double* __restrict__ partials, int* __restrict__ active) {
+    __shared__ double red[4][6];
     const int bid = blockIdx.y * gridDim.x + blockIdx.x;
     const int col = blockIdx.x * 256 + threadIdx.x;
     const bool colok = col < a.ncol;
@@ -226,23 +227,30 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
     for (int j = 0; j < a.na; ++j) {
         const float4 at = atoms[j];
         const float p0 = cpar[j * cs], p1 = cpar[j * cs + 1], K = MODEL == PPM_MODEL_LJ ? 0.f : cpar[j * cs + 2];
-        float acc[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        // The V voxels of one image are summed in FP32, everything above in FP64: the grid
+        // cotangent alternates in sign along z (it is a Δf adjoint), so contractions with the
+        // long-range charge field cancel by ~1e3 and an FP32 sum over the workgroup would lose
+        // ~1e-3 of the charge gradient.
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int iy = -a.npbc_b; iy <= a.npbc_b; ++iy)
             for (int ix = -a.npbc_a; ix <= a.npbc_a; ++ix) {
                 const float sx = at.x + ix * a.lvA[0] + iy * a.lvB[0];
                 const float sy = at.y + ix * a.lvA[1] + iy * a.lvB[1];
                 const float sz = at.z + ix * a.lvA[2] + iy * a.lvB[2];
+                float t[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int k = 0; k < V; ++k) {
                     const int kk = DCZ ? 0 : k;
                     const float dx = px[kk] - sx, dy = py[kk] - sy;
                     pair_adjoint<MODEL>(dx, dy, pz[k] - sz, fmaf(dx, dx, fmaf(dy, dy, kR2SAFE)), at.w, p0, p1, K,
-                                        a, g[k], acc);
+                                        a, g[k], t);
                 }
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[c] += (double)t[c];
             }
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            float v = acc[c];
+            double v = acc[c];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
             if (lane == 0) red[wave][c] = v;
@@ -256,14 +264,14 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
 }
 
 // grad[j*6 + c] = sum over active workgroups b of partials[(b*na + j)*6 + c], in FP64.
-__global__ __launch_bounds__(256) void k_reduce_partials(const float* __restrict__ partials,
+__global__ __launch_bounds__(256) void k_reduce_partials(const double* __restrict__ partials,
                                                          const int* __restrict__ active, int nblocks, int na,
                                                          double* __restrict__ grad) {
     __shared__ double red[256];
     const int j = blockIdx.x / 6, c = blockIdx.x - (blockIdx.x / 6) * 6;
     double acc = 0.0;
     for (int b = threadIdx.x; b < nblocks; b += 256)
-        if (active[b]) acc += (double)partials[((size_t)b * na + j) * 6 + c];
+        if (active[b]) acc += partials[((size_t)b * na + j) * 6 + c];
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
@@ -351,7 +359,7 @@ int launch_grid_project(int model, const float* atoms, const float* cpar, int na
 size_t grid_adjoint_workspace(int na, const PpmGridSpec& g) {
     const dim3 b = grid_blocks(g, adjoint_V(g));
     const size_t nb = (size_t)b.x * b.y;
-    return nb * (size_t)na * 6 * sizeof(float) + nb * sizeof(int) + 256;
+    return nb * (size_t)na * 6 * sizeof(double) + nb * sizeof(int) + 256;
 }
 
 int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
@@ -362,8 +370,8 @@ int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na
     const int V = adjoint_V(g);
     const dim3 blocks = grid_blocks(g, V);
     const size_t nb = (size_t)blocks.x * blocks.y;
-    float* partials = reinterpret_cast<float*>(workspace);
-    int* active = reinterpret_cast<int*>(reinterpret_cast<char*>(workspace) + nb * (size_t)na * 6 * sizeof(float));
+    double* partials = reinterpret_cast<double*>(workspace);
+    int* active = reinterpret_cast<int*>(reinterpret_cast<char*>(workspace) + nb * (size_t)na * 6 * sizeof(double));
     const float4* at = reinterpret_cast<const float4*>(atoms);
     const float4* G = reinterpret_cast<const float4*>(grid);
     const float4* gc = reinterpret_cast<const float4*>(grid_cot);

@@ -155,6 +155,43 @@ def grid_project(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 0), 
     return out
 
 
+def _f64dev(t, name, last):
+    import torch
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous float64 device tensor")
+    if t.dim() < 1 or t.shape[-1] != last:
+        raise ValueError(f"{name} must have a last dimension of {last}")
+
+
+def grid_project_f64(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 0), lvA=(0, 0, 0), lvB=(0, 0, 0),
+                     cap=True, out=None):
+    """The force-field grid (nz,ny,nx,4) in FP64 from FP64 atoms / parameters (reference precision,
+    e.g. for finite-difference gradient checks); same laws as `grid_project`."""
+    import torch
+    _f64dev(atoms, "atoms", 4)
+    _f64dev(cpar, "cpar", 2 if model == _lib.PPM_MODEL_LJ else 4)
+    if atoms.shape[0] != cpar.shape[0]:
+        raise ValueError("atoms and cpar must have the same number of rows")
+    nx, ny, nz = (int(v) for v in spec.n)
+    if out is None:
+        out = torch.empty((nz, ny, nx, 4), dtype=torch.float64, device=atoms.device)
+    else:
+        _f64dev(out, "out", 4)
+        if out.numel() != 4 * nx * ny * nz:
+            raise ValueError("out has the wrong size")
+    L = _lib.load()
+    D3, D4 = ctypes.c_double * 3, ctypes.c_double * 4
+    d4 = lambda v: D4(*([float(x) for x in list(np.asarray(v, float).ravel())[:4]] + [0.0] * 4)[:4])  # noqa: E731
+    lva = D3(*[float(x) for x in np.asarray(lvA, float).reshape(3)])
+    lvb = D3(*[float(x) for x in np.asarray(lvB, float).reshape(3)])
+    s = spec.c()
+    rc = L.ppm_grid_project_f64(int(model), atoms.data_ptr(), cpar.data_ptr(), int(atoms.shape[0]), ctypes.byref(s),
+                                d4(Qs), d4(QZs), int(npbc[0]), int(npbc[1]), ctypes.byref(lva), ctypes.byref(lvb),
+                                int(bool(cap)), out.data_ptr(), _stream(atoms))
+    _lib.check(rc)
+    return out
+
+
 def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, f2conv=1e-8, check_before=False,
                  scan_ny=0, FEs=None, PPs=None, want_pps=True):
     """relaxStrokesTilted on the device: returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None)."""
@@ -178,6 +215,27 @@ def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, 
     rc = L.ppm_relax_strokes_tilted(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz),
                                     ctypes.byref(tp), ctypes.byref(ctl), FEs.data_ptr(),
                                     PPs.data_ptr() if PPs is not None else None, _stream(grid))
+    _lib.check(rc)
+    return FEs, PPs
+
+
+def relax_tilted_f64(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, f2conv=1e-8,
+                     check_before=False, want_pps=True):
+    """`relax_tilted` in FP64 on an FP64 grid and FP64 points (reference precision): returns
+    (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None) as float64 device tensors."""
+    import torch
+    _f64dev(grid, "grid", 4)
+    _f64dev(points, "points", 4)
+    if grid.numel() != 4 * spec.nvox:
+        raise ValueError("grid size does not match spec")
+    ns = int(points.shape[0])
+    FEs = torch.empty((ns, nz, 4), dtype=torch.float64, device=grid.device)
+    PPs = torch.empty((ns, nz, 4), dtype=torch.float64, device=grid.device) if want_pps else None
+    L = _lib.load()
+    s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, 0)
+    rc = L.ppm_relax_strokes_tilted_f64(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz),
+                                        ctypes.byref(tp), ctypes.byref(ctl), FEs.data_ptr(),
+                                        PPs.data_ptr() if PPs is not None else None, _stream(grid))
     _lib.check(rc)
     return FEs, PPs
 
