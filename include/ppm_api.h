@@ -169,6 +169,56 @@ int ppm_relax_strokes_tilted_f64(const double* grid, const PpmGridSpec* spec,
                                  const PpmTip* tip, const PpmRelaxCtl* ctl,
                                  double* FEs, double* PPs, void* stream);
 
+/* ---- FDBM force-field producer (spammm/SPM/AFM.py:3304-3566, :4204-4253; kernels/AFM.cl:1151-1303) ----
+ * Scalar fields are (nx,ny,nz) C-order float32 (z fastest, the reference's host layout);
+ * grid spacing `step` is isotropic.  FFTs are FP32 hipFFT real<->complex transforms. */
+enum {
+    PPM_FFT_CORRELATE = 0,          /* out = scale * IFFT(FFT(a) . conj(FFT(b)))            */
+    PPM_FFT_CONVOLVE = 1,           /* out = scale * IFFT(FFT(a) . FFT(b))                  */
+    PPM_FFT_POISSON = 2,            /* out = scale * IFFT(FFT(a) / k^2), k = 0 -> 0; b unused */
+    PPM_FFT_POISSON_CONVOLVE = 3    /* out = scale * IFFT(FFT(a) . FFT(b) / k^2), k = 0 -> 0 */
+};
+
+/* Bytes of device workspace of ppm_fdbm_fft_combine: two half spectra (nx*ny*(nz/2+1) complex). */
+size_t ppm_fdbm_workspace_size(int nx, int ny, int nz);
+
+/* The FFT products of the FDBM stage (IFFT normalised by 1/(nx ny nz) as NumPy's ifftn):
+ * Pauli overlap = CORRELATE(rho, tip_tot) with scale dV (compute_pauli_overlap,
+ * spammm/SPM/AFM.py:4092-4103, corr_conj :3469-3478); Poisson potential = POISSON(rho) with
+ * scale 4 pi COULOMB_CONST (fft_poisson :3580-3587, poisson :3458-3467); fused electrostatics =
+ * POISSON_CONVOLVE(rho_diff, flip(tip_del)) with scale 4 pi COULOMB_CONST dV
+ * (es_fused_from_rho_cl :3543-3565, fdbm_mul_poisson_tip_c64 kernels/AFM.cl:1287-1303);
+ * electrostatic convolution = CONVOLVE(V, flip(tip_del)) with scale dV
+ * (compute_es_conv_field :4153-4175).  a, b, out: device (nx,ny,nz); out may not alias a or b.
+ * hipFFT plans are created on the first call per shape and device and kept (plan creation
+ * allocates hipFFT's own work area). */
+int ppm_fdbm_fft_combine(int mode, int nx, int ny, int nz, float step, const float* a, const float* b,
+                         float scale, float* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* field[i] = A * max(field[i], clip)^beta over n values (fdbm_scale_pauli_pow_f32,
+ * kernels/AFM.cl:1258-1267; scale_pauli_field spammm/SPM/AFM.py:4106-4128). */
+int ppm_fdbm_pauli_scale(float* field, long long n, float A, float beta, float clip, void* stream);
+
+/* dst (shape) = roll(pad(src (src_shape) at offset), -roll): dst[i] = padded[(i + roll) mod n]
+ * (fdbm_pad_roll_f32, kernels/AFM.cl:1202-1221). */
+int ppm_fdbm_pad_roll(const float* src, const int src_shape[3], float* dst, const int shape[3],
+                      const int offset[3], const int roll[3], void* stream);
+
+/* dst[ix,iy,iz] = src[nx-1-ix, ny-1-iy, nz-1-iz] (fdbm_flip3_f32, kernels/AFM.cl:1223-1233). */
+int ppm_fdbm_flip3(const float* src, float* dst, const int shape[3], void* stream);
+
+/* London dispersion on the grid p = origin + step*(ix,iy,iz): out (+)= -sum_a w_a / (|p - r_a|^2 + RA^2)^3
+ * with atoms (na,4) = (x,y,z, w = sqrt(C6_a C6_tip)) (evalDispersion_toImg + getLondon,
+ * kernels/AFM.cl:1066-1113, :804-814; compute_dispersion_to_img_cl spammm/SPM/AFM.py:431-462). */
+int ppm_fdbm_dispersion(const float* atoms, int na, const float origin[3], float step, const int shape[3],
+                        float RA, int accumulate, float* out, void* stream);
+
+/* E = e1 + e2 + e3 (any may be NULL); grid (nz,ny,nx,4) = (-dE/dx, -dE/dy, -dE/dz, E) by periodic
+ * central differences (fdbm_compose_E_to_img + gradient_central_diff, kernels/AFM.cl:1269-1285,
+ * :1151-1196; compose_E_and_gradient_fast_cl spammm/SPM/AFM.py:464-483). */
+int ppm_fdbm_compose_gradient(const float* e1, const float* e2, const float* e3, const int shape[3],
+                              float step, float* grid, void* stream);
+
 /* Relaxed scan with a tilted tip frame.
  * Replaces relaxStrokesTilted (kernels/AFM.cl:478-562) as launched by
  * AFMulator.run_scan (spammm/SPM/AFM.py:950-1010).

@@ -18,6 +18,10 @@ PPM_ERR_UNSUPPORTED = 3
 PPM_MODEL_LJ = 0
 PPM_MODEL_MORSE = 1
 PPM_DIRECT_MAX_ATOMS = 128
+PPM_FFT_CORRELATE = 0
+PPM_FFT_CONVOLVE = 1
+PPM_FFT_POISSON = 2
+PPM_FFT_POISSON_CONVOLVE = 3
 
 _F4 = ctypes.c_float * 4
 
@@ -44,7 +48,8 @@ EXPORTS = (
     "ppm_grid_adjoint_workspace_size", "ppm_grid_adjoint", "ppm_relax_polish", "ppm_batch_workspace_size",
     "ppm_batch_forward", "ppm_relax_direct_morse", "ppm_direct_morse_vjp_workspace_size", "ppm_direct_morse_vjp",
     "ppm_grid_project_add", "ppm_grid_adjoint_add", "ppm_grid_project_f64",
-    "ppm_relax_strokes_tilted_f64",
+    "ppm_relax_strokes_tilted_f64", "ppm_fdbm_workspace_size", "ppm_fdbm_fft_combine", "ppm_fdbm_pauli_scale",
+    "ppm_fdbm_pad_roll", "ppm_fdbm_flip3", "ppm_fdbm_dispersion", "ppm_fdbm_compose_gradient",
 )
 
 _lib = None
@@ -82,6 +87,22 @@ def load():
     L.ppm_grid_project_f64.argtypes = [i, vp, vp, i, P(PpmGridSpec), D4, D4, i, i, P(D3), P(D3), i, vp, vp]
     L.ppm_relax_strokes_tilted_f64.restype = i
     L.ppm_relax_strokes_tilted_f64.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), P(PpmRelaxCtl), vp, vp, vp]
+    I3 = ctypes.c_int * 3
+    F3 = ctypes.c_float * 3
+    L.ppm_fdbm_workspace_size.restype = ctypes.c_size_t
+    L.ppm_fdbm_workspace_size.argtypes = [i, i, i]
+    L.ppm_fdbm_fft_combine.restype = i
+    L.ppm_fdbm_fft_combine.argtypes = [i, i, i, i, f, vp, vp, f, vp, vp, ctypes.c_size_t, vp]
+    L.ppm_fdbm_pauli_scale.restype = i
+    L.ppm_fdbm_pauli_scale.argtypes = [vp, ctypes.c_longlong, f, f, f, vp]
+    L.ppm_fdbm_pad_roll.restype = i
+    L.ppm_fdbm_pad_roll.argtypes = [vp, P(I3), vp, P(I3), P(I3), P(I3), vp]
+    L.ppm_fdbm_flip3.restype = i
+    L.ppm_fdbm_flip3.argtypes = [vp, vp, P(I3), vp]
+    L.ppm_fdbm_dispersion.restype = i
+    L.ppm_fdbm_dispersion.argtypes = [vp, i, P(F3), f, P(I3), f, i, vp, vp]
+    L.ppm_fdbm_compose_gradient.restype = i
+    L.ppm_fdbm_compose_gradient.argtypes = [vp, vp, vp, P(I3), f, vp, vp]
     L.ppm_grid_adjoint_add.restype = i
     L.ppm_grid_adjoint_add.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
                                        P(ctypes.c_float * 3), i, vp, vp, vp, vp, ctypes.c_size_t, vp, vp]

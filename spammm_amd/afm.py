@@ -512,6 +512,28 @@ class AFMulator:
         self.fdbm_spec = D.axis_aligned_spec(origin, L, (nx, ny, nz))
         self.fdbm_origin, self.fdbm_step, self.fdbm_shape = origin, float(step), (nx, ny, nz)
 
+    def install_fdbm_grid(self, grid, origin, step, shape):
+        """Use a device force grid (nz,ny,nx,4) (e.g. from fdbm.stage3_fdbm_fields_fast) as the FDBM
+        grid without a host round trip (setup_fdbm_grid_from_img, spammm/SPM/AFM.py:413-429)."""
+        nx, ny, nz = (int(v) for v in shape[:3])
+        D._f32dev(grid, "grid", 4)
+        if grid.numel() != 4 * nx * ny * nz:
+            raise ValueError("grid does not match shape")
+        L = np.array([nx * step, ny * step, nz * step], dtype=np.float32)
+        origin = np.asarray(origin, dtype=np.float32)
+        self.img_FF_fdbm = grid
+        self.fdbm_spec = D.axis_aligned_spec(origin, L, (nx, ny, nz))
+        self.fdbm_origin, self.fdbm_step, self.fdbm_shape = origin, float(step), (nx, ny, nz)
+
+    def compute_gradient_cl(self, E_field, step, bAlloc=True):
+        """(nx,ny,nz,4) = (-grad E, E) of a host (nx,ny,nz) energy by periodic central differences
+        on the GPU (spammm/SPM/AFM.py:2740-2790, gradient_central_diff kernels/AFM.cl:1151-1196)."""
+        import torch
+        from . import fdbm
+        E = torch.as_tensor(np.ascontiguousarray(E_field, dtype=np.float32), device=self.device)
+        grid = fdbm.compose_gradient((E,), float(step))
+        return np.ascontiguousarray(grid.cpu().numpy().transpose(2, 1, 0, 3))
+
     def scan_fdbm(self, scan_xs, scan_ys, probe_heights, mol_z=0.0, ppm_mode=True, use_fire=True, K_LAT=0.03,
                   K_RAD=20.0, bond_length=3.0, stiffness=None, dpos0=None, relax_pars=None, osc_dir=(0., 0., 1.),
                   base_pos=(0., 0., 0.), reverse=False):

@@ -31,6 +31,7 @@ SOURCES = [
     ("ppm_batch.hip", ["-ffp-contract=off"]),
     ("ppm_direct.hip", ["-ffp-contract=off"]),
     ("ppm_f64.hip", ["-ffp-contract=off"]),
+    ("ppm_fdbm.hip", []),
 ]
 
 
@@ -67,7 +68,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lhipfft"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -161,6 +161,80 @@ int ppm_relax_strokes_tilted_f64(const double* grid, const PpmGridSpec* spec, co
     return launch_relax_tilted_f64(grid, *spec, points, n_scan, nz, *tip, *ctl, FEs, PPs, (hipStream_t)stream);
 }
 
+static int check_shape(const char* fn, const int* n) {
+    if (!n) { set_error("%s: shape is NULL", fn); return PPM_ERR_INVALID_ARG; }
+    for (int k = 0; k < 3; ++k)
+        if (n[k] < 1 || n[k] > 16384) { set_error("%s: shape[%d]=%d out of [1,16384]", fn, k, n[k]); return PPM_ERR_INVALID_ARG; }
+    if ((long long)n[0] * n[1] * n[2] > (1LL << 31) - 1) { set_error("%s: grid too large", fn); return PPM_ERR_INVALID_ARG; }
+    return PPM_OK;
+}
+
+size_t ppm_fdbm_workspace_size(int nx, int ny, int nz) {
+    const int n[3] = {nx, ny, nz};
+    if (check_shape("ppm_fdbm_workspace_size", n)) return 0;
+    return 2 * fdbm_spectrum_bytes(nx, ny, nz);
+}
+
+int ppm_fdbm_fft_combine(int mode, int nx, int ny, int nz, float step, const float* a, const float* b, float scale,
+                         float* out, void* workspace, size_t workspace_bytes, void* stream) {
+    const char* fn = "ppm_fdbm_fft_combine";
+    g_err[0] = 0;
+    const int n[3] = {nx, ny, nz};
+    if (int r = check_shape(fn, n)) return r;
+    if (mode < PPM_FFT_CORRELATE || mode > PPM_FFT_POISSON_CONVOLVE) { set_error("%s: unknown mode %d", fn, mode); return PPM_ERR_INVALID_ARG; }
+    if (!a || !out || !workspace || (mode != PPM_FFT_POISSON && !b)) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    if (out == a || out == b) { set_error("%s: out must not alias an input", fn); return PPM_ERR_INVALID_ARG; }
+    if (!(step > 0.f) || !std::isfinite(step) || !std::isfinite(scale)) { set_error("%s: invalid step or scale", fn); return PPM_ERR_INVALID_ARG; }
+    if (workspace_bytes < 2 * fdbm_spectrum_bytes(nx, ny, nz)) { set_error("%s: workspace too small", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_fft_combine(mode, nx, ny, nz, step, a, b, scale, out, workspace, (hipStream_t)stream);
+}
+
+int ppm_fdbm_pauli_scale(float* field, long long n, float A, float beta, float clip, void* stream) {
+    const char* fn = "ppm_fdbm_pauli_scale";
+    g_err[0] = 0;
+    if (!field || n < 1) { set_error("%s: empty or NULL field", fn); return PPM_ERR_INVALID_ARG; }
+    if (!std::isfinite(A) || !std::isfinite(beta) || !(clip > 0.f)) { set_error("%s: invalid A, beta or clip", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_pauli_scale(field, n, A, beta, clip, (hipStream_t)stream);
+}
+
+int ppm_fdbm_pad_roll(const float* src, const int src_shape[3], float* dst, const int shape[3], const int offset[3],
+                      const int roll[3], void* stream) {
+    const char* fn = "ppm_fdbm_pad_roll";
+    g_err[0] = 0;
+    if (int r = check_shape(fn, src_shape)) return r;
+    if (int r = check_shape(fn, shape)) return r;
+    if (!src || !dst || !offset || !roll || src == dst) { set_error("%s: NULL or aliased pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_pad_roll(src, src_shape, dst, shape, offset, roll, (hipStream_t)stream);
+}
+
+int ppm_fdbm_flip3(const float* src, float* dst, const int shape[3], void* stream) {
+    const char* fn = "ppm_fdbm_flip3";
+    g_err[0] = 0;
+    if (int r = check_shape(fn, shape)) return r;
+    if (!src || !dst || src == dst) { set_error("%s: NULL or aliased pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_flip3(src, dst, shape, (hipStream_t)stream);
+}
+
+int ppm_fdbm_dispersion(const float* atoms, int na, const float origin[3], float step, const int shape[3], float RA,
+                        int accumulate, float* out, void* stream) {
+    const char* fn = "ppm_fdbm_dispersion";
+    g_err[0] = 0;
+    if (int r = check_shape(fn, shape)) return r;
+    if (!atoms || !origin || !out || na < 1) { set_error("%s: NULL pointer or no atoms", fn); return PPM_ERR_INVALID_ARG; }
+    if (!(step > 0.f) || !std::isfinite(RA) || !finite4(origin, 3)) { set_error("%s: invalid step, RA or origin", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_dispersion(atoms, na, origin, step, shape, RA * RA, accumulate, out, (hipStream_t)stream);
+}
+
+int ppm_fdbm_compose_gradient(const float* e1, const float* e2, const float* e3, const int shape[3], float step,
+                              float* grid, void* stream) {
+    const char* fn = "ppm_fdbm_compose_gradient";
+    g_err[0] = 0;
+    if (int r = check_shape(fn, shape)) return r;
+    if (!grid || (!e1 && !e2 && !e3)) { set_error("%s: no output or no input field", fn); return PPM_ERR_INVALID_ARG; }
+    if (!(step > 0.f)) { set_error("%s: invalid step", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_compose_gradient(e1, e2, e3, shape, step, grid, (hipStream_t)stream);
+}
+
 int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
                              const PpmTip* tip, const PpmRelaxCtl* ctl, float* FEs, float* PPs, void* stream) {
     const char* fn = "ppm_relax_strokes_tilted";

@@ -52,6 +52,17 @@ int launch_grid_project_f64(int model, const double* atoms, const double* cpar, 
 int launch_relax_tilted_f64(const double* grid, const PpmGridSpec& g, const double* pts, int n_scan, int nz,
                             const PpmTip& tip, const PpmRelaxCtl& ctl, double* FEs, double* PPs,
                             hipStream_t stream);
+size_t fdbm_spectrum_bytes(int nx, int ny, int nz);
+int launch_fft_combine(int mode, int nx, int ny, int nz, float step, const float* a, const float* b, float scale,
+                       float* out, void* workspace, hipStream_t stream);
+int launch_pauli_scale(float* field, long long n, float A, float beta, float clip, hipStream_t stream);
+int launch_pad_roll(const float* src, const int s[3], float* dst, const int n[3], const int o[3], const int r[3],
+                    hipStream_t stream);
+int launch_flip3(const float* src, float* dst, const int n[3], hipStream_t stream);
+int launch_dispersion(const float* atoms, int na, const float origin[3], float step, const int n[3], float ra2,
+                      int accumulate, float* out, hipStream_t stream);
+int launch_compose_gradient(const float* e1, const float* e2, const float* e3, const int n[3], float step,
+                            float* grid, hipStream_t stream);
 size_t batch_workspace(int nmol);
 int launch_batch(const PpmMolDesc* mols, const PpmBatchParams& P, const float* atoms, const float* cpar,
                  float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
