@@ -48,6 +48,7 @@ def main():
     ap.add_argument("--batch-mols", type=int, default=512, help="molecules of the batched leg (0: skip)")
     ap.add_argument("--no-direct", action="store_true", help="skip the gridless direct Morse leg")
     ap.add_argument("--poses", type=int, default=64, help="placements of the substrate leg (0: skip)")
+    ap.add_argument("--no-fdbm", action="store_true", help="skip the FDBM force-field producer leg")
     ap.add_argument("--z-margin", default=None,
                     help="z window margins of the batched leg in A, 'below,above' (default: the library's)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -191,6 +192,8 @@ def main():
         out["direct_morse"] = direct_leg(dev)
     if args.poses > 0:
         out["substrate_poses"] = substrate_leg(dev, args.poses)
+    if not args.no_fdbm:
+        out["fdbm"] = fdbm_leg(dev)
     if not args.no_check:
         out["accuracy"] = accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip)
     if not args.no_cpu:
@@ -335,6 +338,99 @@ def substrate_leg(dev, n_poses=64):
     del scene
     torch.cuda.empty_cache()
     return res
+
+
+def fdbm_leg(dev, reps=5):
+    """FDBM product path on a PTCDA-sized grid (200x160x128 at 0.1 A): synthetic Gaussian densities of
+    a 38-atom planar molecule (sample density, neutral charge density) and CO-like tip densities
+    (total, dipolar charge) on a 24^3 box.  Times stage 3 with the inputs resident on the device
+    (Pauli correlation + scaling, fused Poisson/electrostatic convolution, dispersion, composed
+    force grid), the host-API call (uploads, F_total download included), and the forward and
+    backward relaxed scans (scan_fdbm) of 100x80 pixels x 21 heights with shared_postprocess."""
+    import torch
+    from spammm_amd import _lib, afm, fdbm, postprocess as PP, workloads as WL
+    rng = np.random.default_rng(8)
+    en, pos = WL.flake_geometry(38, rng)
+    Z = np.array([{"H": 1, "C": 6, "N": 7, "O": 8}[e] for e in en])
+    q = rng.uniform(-0.2, 0.2, len(Z))
+    q -= q.mean()
+    shape, st = (200, 160, 128), 0.1
+    origin = np.array([pos[:, 0].mean() - 10.0, pos[:, 1].mean() - 8.0, -6.4])
+
+    def gauss(centers, sigma, weights, shp, org):
+        ax = [torch.arange(shp[a], dtype=torch.float64, device=dev) * st + float(org[a]) for a in range(3)]
+        X, Y, Zg = torch.meshgrid(*ax, indexing="ij")
+        out = torch.zeros(shp, dtype=torch.float64, device=dev)
+        for c, w in zip(centers, weights):
+            r2 = (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Zg - c[2]) ** 2
+            out += float(w) * torch.exp(-r2 / (2 * sigma * sigma)) / (2 * np.pi * sigma * sigma) ** 1.5
+        return out.float().contiguous()
+
+    rho = gauss(pos, 0.55, Z, shape, origin)
+    rdiff = gauss(pos, 0.45, q, shape, origin) - gauss(pos, 0.65, q, shape, origin)
+    tshape = (24, 24, 24)
+    t0 = -0.5 * st * (np.array(tshape) - 1)
+    tip_tot = gauss([[0, 0, 0]], 0.6, [6.0], tshape, t0).cpu().numpy()
+    tip_del = (gauss([[0, 0, 0.2]], 0.5, [0.4], tshape, t0) - gauss([[0, 0, -0.2]], 0.5, [0.4], tshape, t0)).cpu().numpy()
+    A, beta = 18.0, 0.9
+    stream = torch.cuda.current_stream(dev)
+    # device-resident stage 3
+    tt = fdbm._place_tip(tip_tot, shape, False)
+    td = fdbm._place_tip(tip_del, shape, False)
+    atoms_w = torch.as_tensor(fdbm._c6_atoms(pos, Z, None, 30.0, 1.0), device=dev)
+    bufs = {}
+
+    def stage3_device():
+        bufs["p"] = fdbm.pauli_scale_(fdbm.fft_combine(_lib.PPM_FFT_CORRELATE, rho, tt, st, st ** 3,
+                                                       out=bufs.get("p")), A, beta)
+        bufs["e"] = fdbm.fft_combine(_lib.PPM_FFT_POISSON_CONVOLVE, rdiff, fdbm.flip3(td), st,
+                                     fdbm.POISSON_SCALE * st ** 3, out=bufs.get("e"))
+        bufs["v"] = fdbm.dispersion(atoms_w, origin, st, shape, out=bufs.get("v"))
+        bufs["g"] = fdbm.compose_gradient((bufs["p"], bufs["e"], bufs["v"]), st, grid=bufs.get("g"))
+
+    stage3_device()
+    torch.cuda.synchronize(dev)
+    ts = []
+    for _ in range(reps):
+        a_, b_ = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a_.record(stream)
+        stage3_device()
+        b_.record(stream)
+        torch.cuda.synchronize(dev)
+        ts.append(a_.elapsed_time(b_))
+    # host-API call (the reference's signature: host arrays in, F_total out)
+    am = afm.AFMulator(use_morse=False)
+    rho_h, rdiff_h = rho.cpu().numpy(), rdiff.cpu().numpy()
+    t0w = time.perf_counter()
+    fdbm.stage3_fdbm_fields_fast(am, rho_h, rdiff_h, tip_tot, tip_del, origin, st, shape, pos, Z, A, beta,
+                                 download_fields=False)
+    torch.cuda.synchronize(dev)
+    t_api = time.perf_counter() - t0w
+    h_df, h_Fz, h_scan = PP.afm_df_height_stacks(3.7, 4.7, 0.1, amp=1.0)
+    xs = (pos[:, 0].mean() - 5.0 + 0.1 * np.arange(100)).astype(np.float32)
+    ys = (pos[:, 1].mean() - 4.0 + 0.1 * np.arange(80)).astype(np.float32)
+    kw = dict(mol_z=float(pos[:, 2].max()), K_LAT=afm.stiffness_Nm_to_eVA2(0.5), K_RAD=20.0, bond_length=3.0)
+    t0s = time.perf_counter()
+    FEf, disp = am.scan_fdbm(xs, ys, h_scan, **kw)
+    FEb, _ = am.scan_fdbm(xs, ys, h_scan, reverse=True, **kw)
+    t_scan = time.perf_counter() - t0s
+    spec = PP.ScanSpec(scan_xs=xs, scan_ys=ys, h_df=h_df, h_Fz=h_Fz, h_scan=h_scan, amplitude=1.0,
+                       osc_dir=(0., 0., 1.), K_LAT=kw["K_LAT"], K_RAD=20.0, bond_length=3.0)
+    t0p = time.perf_counter()
+    res = PP.shared_postprocess(FEf, spec, FEs_bwd=FEb, tip_disp=disp, backend_name="fdbm", fft_path="GPU")
+    t_post = time.perf_counter() - t0p
+    nvox = int(np.prod(shape))
+    out = {
+        "workload": f"FDBM: 38-atom planar molecule, synthetic Gaussian densities, {shape[0]}x{shape[1]}x{shape[2]} "
+                    f"grid at 0.1 A, tip 24^3, scan {len(xs)}x{len(ys)} px x {len(h_scan)} heights fwd + bwd",
+        "stage3_device_ms": float(np.median(ts)), "stage3_host_api_s": t_api,
+        "grid_points_per_s_stage3": nvox / (np.median(ts) * 1e-3),
+        "scan_fwd_bwd_s": t_scan, "postprocess_s": t_post,
+        "df_range": [float(res.df.min()), float(res.df.max())], "E_diss_max": float(res.E_diss.max()),
+        "fft_count_per_stage3": 5,
+    }
+    log(f"[bench] fdbm: {out}")
+    return out
 
 
 def D_axis(sc):

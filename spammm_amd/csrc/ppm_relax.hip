@@ -77,25 +77,24 @@ struct StrokeArgs {
     float dt0, damp0;
 };
 
-// relaxStrokes with optional displacement and PP outputs.
-__global__ __launch_bounds__(64) void k_relax_strokes(GridDev g, const float4* __restrict__ pts, StrokeArgs p,
-                                                      float4* __restrict__ FEs, float4* __restrict__ disps,
-                                                      float4* __restrict__ PPs) {
-    const int ip = lane_point(p.n_scan, p.scan_ny);
-    if (ip < 0) return;
+// relaxStrokes with optional displacement and PP outputs, for one scan point; `smp(x,y,z)` is the
+// trilinear sample (one lane's cell cache, or a lane group's channel-split sample), `writer` the
+// lane that stores (all lanes of a group hold the same state).
+template <class SAMPLE>
+__device__ __forceinline__ void strokes_lane(SAMPLE& smp, int ip, const float4* __restrict__ pts,
+                                             const StrokeArgs& p, float4* __restrict__ FEs,
+                                             float4* __restrict__ disps, float4* __restrict__ PPs, bool writer) {
     const f3 dTip = ld3(p.dTip);
     const float4 P = pts[ip];
     f3 tip = mk3(P.x, P.y, P.z);
     f3 pos = mk3(tip.x + p.d0[0], tip.y + p.d0[1], tip.z + p.d0[2]);
     float dt = p.dt0, damp = p.damp0;
     const float dtmax = dt, dtmin = dtmax * 0.1f, damp0 = damp;
-    Cell cell;
-    cell.i = -1; cell.j = -1; cell.k = -1;
     f3 v = mk3(0.f, 0.f, 0.f);
     int it = 0, iz = 0;
     size_t o = (size_t)ip * p.nz;
     while (true) {
-        const float4 fe = sample(g, cell, pos.x, pos.y, pos.z);
+        const float4 fe = smp(pos.x, pos.y, pos.z);
         const f3 ft = tipForce(mk3(pos.x - tip.x, pos.y - tip.y, pos.z - tip.z), p.k, p.d0);
         const f3 f = mk3(fe.x + ft.x, fe.y + ft.y, fe.z + ft.z);
         const float ff = dot3(f, f);
@@ -105,13 +104,15 @@ __global__ __launch_bounds__(64) void k_relax_strokes(GridDev g, const float4* _
         const bool conv = ff < p.f2conv;
         ++it;
         if (conv || it >= p.max_iter) {
-            FEs[o] = fe;
-            if (disps)
-                disps[o] = make_float4(pos.x - (tip.x + p.d0[0]), pos.y - (tip.y + p.d0[1]),
-                                       pos.z - (tip.z + p.d0[2]), 0.f);
-            if (PPs) {
-                const bool fin = finite4(fe) && isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z);
-                PPs[o] = make_float4(pos.x, pos.y, pos.z, (conv && fin) ? (float)it : -(float)it);
+            if (writer) {
+                FEs[o] = fe;
+                if (disps)
+                    disps[o] = make_float4(pos.x - (tip.x + p.d0[0]), pos.y - (tip.y + p.d0[1]),
+                                           pos.z - (tip.z + p.d0[2]), 0.f);
+                if (PPs) {
+                    const bool fin = finite4(fe) && isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z);
+                    PPs[o] = make_float4(pos.x, pos.y, pos.z, (conv && fin) ? (float)it : -(float)it);
+                }
             }
             ++o;
             tip = mk3(tip.x + dTip.x, tip.y + dTip.y, tip.z + dTip.z);
@@ -120,6 +121,27 @@ __global__ __launch_bounds__(64) void k_relax_strokes(GridDev g, const float4* _
             it = 0;
             if (++iz >= p.nz) break;
         }
+    }
+}
+
+// relaxStrokes; G lanes per scan point as k_relax_tilted (results identical to G = 1).
+template <int G>
+__global__ __launch_bounds__(64) void k_relax_strokes(GridDev g, const float4* __restrict__ pts, StrokeArgs p,
+                                                      float4* __restrict__ FEs, float4* __restrict__ disps,
+                                                      float4* __restrict__ PPs) {
+    int sub;
+    const int ip = group_point<G>(p.n_scan, p.scan_ny, sub);
+    if (ip < 0) return;                       // whole groups leave together
+    if constexpr (G == 1) {
+        Cell cell;
+        cell.i = -1; cell.j = -1; cell.k = -1;
+        auto smp = [&](float x, float y, float z) { return sample(g, cell, x, y, z); };
+        strokes_lane(smp, ip, pts, p, FEs, disps, PPs, true);
+    } else {
+        GroupCell<G> cell;
+        cell.i = -1; cell.j = -1; cell.k = -1;
+        auto smp = [&](float x, float y, float z) { return sample_group<G>(g, cell, sub, x, y, z); };
+        strokes_lane(smp, ip, pts, p, FEs, disps, PPs, sub == 0);
     }
 }
 
@@ -443,10 +465,15 @@ int launch_relax_strokes(const float* grid, const PpmGridSpec& s, const float* p
     for (int q = 0; q < 3; ++q) p.dTip[q] = dTip[q];
     for (int q = 0; q < 4; ++q) { p.k[q] = stiffness[q]; p.d0[q] = dpos0[q]; }
     p.dt0 = relax[0]; p.damp0 = relax[1];
-    const int waves = n_waves(n_scan, ctl.scan_ny);
-    hipLaunchKernelGGL(k_relax_strokes, dim3(waves), dim3(64), 0, stream, make_grid(grid, s),
-                       reinterpret_cast<const float4*>(pts), p, reinterpret_cast<float4*>(FEs),
-                       reinterpret_cast<float4*>(disps), reinterpret_cast<float4*>(PPs));
+    const int G = relax_group(n_scan);
+    const dim3 blocks((unsigned)group_waves(n_scan, ctl.scan_ny, G));
+    const GridDev gd = make_grid(grid, s);
+    const float4* P = reinterpret_cast<const float4*>(pts);
+    float4 *F = reinterpret_cast<float4*>(FEs), *Dp = reinterpret_cast<float4*>(disps),
+           *Q = reinterpret_cast<float4*>(PPs);
+    if (G == 4) hipLaunchKernelGGL(k_relax_strokes<4>, blocks, dim3(64), 0, stream, gd, P, p, F, Dp, Q);
+    else if (G == 2) hipLaunchKernelGGL(k_relax_strokes<2>, blocks, dim3(64), 0, stream, gd, P, p, F, Dp, Q);
+    else hipLaunchKernelGGL(k_relax_strokes<1>, blocks, dim3(64), 0, stream, gd, P, p, F, Dp, Q);
     return check_launch("relax_strokes");
 }
 
