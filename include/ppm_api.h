@@ -219,6 +219,62 @@ int ppm_fdbm_dispersion(const float* atoms, int na, const float origin[3], float
 int ppm_fdbm_compose_gradient(const float* e1, const float* e2, const float* e3, const int shape[3],
                               float step, float* grid, void* stream);
 
+/* ---- 2D Ewald electrostatics of periodic slabs (spammm/surfaces/Ewald2D.py,
+ * spammm/surfaces/SurfaceEwald.py, kernels/surface.cl:1036-1381) ----
+ *   phi(rho,z) = C [ -(2 pi/A) sum_i q_i |z - z_i|
+ *                    + sum_{G!=0} sum_i (2 pi/(A|G|)) q_i Re e^{iG.(rho-rho_i)} e^{-|G||z-z_i|} ],
+ * G = h b1 + k b2 with |h|,|k| <= n_harm (make_reciprocal_2d / generate_G_vectors,
+ * Ewald2D.py:39-84), C = 14.3996448915 eV A / e^2.  Lattice vectors a, b are host double[2].
+ * Ions are (n,4) float (x,y,z,q) in device memory, grouped by layer: layer l holds
+ * ions[layer_off[l] .. layer_off[l+1]) and all of them share one z (layer_off: device int[n_layers+1]). */
+enum {
+    PPM_EWALD_FULL = 0,      /* eval_potential_full (kernels/surface.cl:1172-1246), any z             */
+    PPM_EWALD_VACUUM = 1     /* eval_potential_vacuum (kernels/surface.cl:1103-1157), z above the ions */
+};
+
+/* Number of G vectors: (2 n_harm + 1)^2 - 1. */
+int ppm_ewald_num_g(int n_harm);
+
+/* Bytes of the prepared-system buffer of ppm_ewald_prepare (1 <= n_layers, 1 <= n_harm <= 64). */
+size_t ppm_ewald_system_bytes(int n_layers, int n_harm);
+
+/* Prepares a slab (SurfaceEwaldCL.prepare_system, SurfaceEwald.py:165-234, whose
+ * compute_ewald_coefficients kernel is kernels/surface.cl:1036-1086): per-layer structure factors
+ * W_l(G) = (2 pi/(A|G|)) sum_{i in l} q_i e^{-iG.rho_i} in FP64, layer heights and charges, and the
+ * ordered list of G kept for evaluation: those with max_l |W_l(G)| > drop_tol * max |W| (drop_tol 0
+ * keeps every G with a nonzero structure factor).  system: device, ppm_ewald_system_bytes. */
+int ppm_ewald_prepare(const float* ions, const int* layer_off, int n_layers, const double a[2],
+                      const double b[2], int n_harm, double drop_tol, void* system, size_t system_bytes,
+                      void* stream);
+
+/* phi (np floats, eV/e) at points (np,4) = (x,y,z,-) (SurfaceEwaldCL.eval_full / eval_vacuum,
+ * SurfaceEwald.py:236-347), FP64 accumulation; mode PPM_EWALD_FULL or PPM_EWALD_VACUUM. */
+int ppm_ewald_eval(int mode, const float* points, int np, const void* system, int n_layers, int n_harm,
+                   const double a[2], const double b[2], float* phi, void* stream);
+
+/* Direct sum over the images n a + m b: square = 0 takes n^2 + m^2 <= n_rep^2
+ * (SurfaceEwaldCL.eval_brute, SurfaceEwald.py:349-405; eval_potential_brute
+ * kernels/surface.cl:1260-1305), square = 1 every |n|,|m| <= n_rep (eval_potential_brute,
+ * Ewald2D.py:176-204); FP64 accumulation. */
+int ppm_ewald_brute(const float* points, int np, const float* ions, int n_ions, const double a[2],
+                    const double b[2], int n_rep, int square, float* phi, void* stream);
+
+/* Finite-cluster Coulomb sum without images (SurfaceEwaldCL.eval_cluster, SurfaceEwald.py:407-449;
+ * eval_potential_cluster kernels/surface.cl:1321-1381); FP64 accumulation. */
+int ppm_ewald_cluster(const float* points, int np, const float* ions, int n_ions, float* phi, void* stream);
+
+/* Bytes of device workspace of ppm_ewald_tip_grid. */
+size_t ppm_ewald_grid_workspace_size(int n_harm, int nz);
+
+/* The slab's electrostatics as a probe force-field grid: grid (nz,ny,nx,4) (+)= (F, E) with
+ * E = sum_k Qs_k phi(r + QZs_k z) and F = -grad E (analytic), at r = p0 + ia dA + ib dB + ic dC.
+ * Requires dA, dB in the xy plane and dC along z.  Same voxel layout and tip-charge convention
+ * as ppm_grid_project (the Coulomb sites of evalLJC_QZs_toImg, kernels/AFM.cl:916-991), with the
+ * slab's periodic sum done by the Ewald series instead of a finite image sum; no energy cap. */
+int ppm_ewald_tip_grid(const void* system, int n_layers, int n_harm, const double a[2], const double b[2],
+                       const PpmGridSpec* spec, const float Qs[4], const float QZs[4], int accumulate,
+                       void* workspace, size_t workspace_bytes, float* grid, void* stream);
+
 /* Relaxed scan with a tilted tip frame.
  * Replaces relaxStrokesTilted (kernels/AFM.cl:478-562) as launched by
  * AFMulator.run_scan (spammm/SPM/AFM.py:950-1010).

@@ -78,16 +78,27 @@ class Slab:
                   + np.real((-self.Gn[None, :, None] * sg[:, None, :] * t).sum((1, 2))))
         return -COULOMB_CONST * np.stack([dphidx, dphidy, dphidz], 1)
 
-    def phi_brute(self, P, n_rep):
-        """Direct sum over images inside growing circles |n a + m b| <= n_rep |a|."""
+    def phi_brute(self, P, n_rep, square=False):
+        """Direct sum over the images n a + m b with n^2 + m^2 <= n_rep^2 (eval_potential_brute,
+        kernels/surface.cl:1260-1305), or every |n|,|m| <= n_rep if square (eval_potential_brute,
+        spammm/surfaces/Ewald2D.py:176-204)."""
         P = np.atleast_2d(P)
         R = [(n * self.a + m * self.b) for n in range(-n_rep, n_rep + 1) for m in range(-n_rep, n_rep + 1)
-             if n * n + m * m <= n_rep * n_rep]
+             if square or n * n + m * m <= n_rep * n_rep]
         out = np.zeros(len(P))
         for Rv in R:
             d = P[:, None, :] - (self.ions[None, :, :3] + np.array([Rv[0], Rv[1], 0.0]))
             out += (self.ions[None, :, 3] / np.sqrt((d * d).sum(2))).sum(1)
         return COULOMB_CONST * out
+
+
+def phi_cluster(P, ions):
+    """Coulomb potential of a finite set of ions (n,4), no images (eval_potential_cluster,
+    kernels/surface.cl:1321-1381)."""
+    P = np.atleast_2d(np.asarray(P, float))
+    ions = np.asarray(ions, float)
+    d = P[:, None, :3] - ions[None, :, :3]
+    return COULOMB_CONST * (ions[None, :, 3] / np.sqrt((d * d).sum(2))).sum(1)
 
 
 def tip_grid(slab, spec_p0, step, shape, Qs, QZs):

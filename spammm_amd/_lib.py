@@ -22,6 +22,8 @@ PPM_FFT_CORRELATE = 0
 PPM_FFT_CONVOLVE = 1
 PPM_FFT_POISSON = 2
 PPM_FFT_POISSON_CONVOLVE = 3
+PPM_EWALD_FULL = 0
+PPM_EWALD_VACUUM = 1
 
 _F4 = ctypes.c_float * 4
 
@@ -50,6 +52,8 @@ EXPORTS = (
     "ppm_grid_project_add", "ppm_grid_adjoint_add", "ppm_grid_project_f64",
     "ppm_relax_strokes_tilted_f64", "ppm_fdbm_workspace_size", "ppm_fdbm_fft_combine", "ppm_fdbm_pauli_scale",
     "ppm_fdbm_pad_roll", "ppm_fdbm_flip3", "ppm_fdbm_dispersion", "ppm_fdbm_compose_gradient",
+    "ppm_ewald_num_g", "ppm_ewald_system_bytes", "ppm_ewald_prepare", "ppm_ewald_eval", "ppm_ewald_brute",
+    "ppm_ewald_cluster", "ppm_ewald_grid_workspace_size", "ppm_ewald_tip_grid",
 )
 
 _lib = None
@@ -103,6 +107,24 @@ def load():
     L.ppm_fdbm_dispersion.argtypes = [vp, i, P(F3), f, P(I3), f, i, vp, vp]
     L.ppm_fdbm_compose_gradient.restype = i
     L.ppm_fdbm_compose_gradient.argtypes = [vp, vp, vp, P(I3), f, vp, vp]
+    D2 = ctypes.c_double * 2
+    sz = ctypes.c_size_t
+    L.ppm_ewald_num_g.restype = i
+    L.ppm_ewald_num_g.argtypes = [i]
+    L.ppm_ewald_system_bytes.restype = sz
+    L.ppm_ewald_system_bytes.argtypes = [i, i]
+    L.ppm_ewald_prepare.restype = i
+    L.ppm_ewald_prepare.argtypes = [vp, vp, i, P(D2), P(D2), i, ctypes.c_double, vp, sz, vp]
+    L.ppm_ewald_eval.restype = i
+    L.ppm_ewald_eval.argtypes = [i, vp, i, vp, i, i, P(D2), P(D2), vp, vp]
+    L.ppm_ewald_brute.restype = i
+    L.ppm_ewald_brute.argtypes = [vp, i, vp, i, P(D2), P(D2), i, i, vp, vp]
+    L.ppm_ewald_cluster.restype = i
+    L.ppm_ewald_cluster.argtypes = [vp, i, vp, i, vp, vp]
+    L.ppm_ewald_grid_workspace_size.restype = sz
+    L.ppm_ewald_grid_workspace_size.argtypes = [i, i]
+    L.ppm_ewald_tip_grid.restype = i
+    L.ppm_ewald_tip_grid.argtypes = [vp, i, i, P(D2), P(D2), P(PpmGridSpec), _F4, _F4, i, vp, sz, vp, vp]
     L.ppm_grid_adjoint_add.restype = i
     L.ppm_grid_adjoint_add.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
                                        P(ctypes.c_float * 3), i, vp, vp, vp, vp, ctypes.c_size_t, vp, vp]

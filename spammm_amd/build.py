@@ -32,6 +32,7 @@ SOURCES = [
     ("ppm_direct.hip", ["-ffp-contract=off"]),
     ("ppm_f64.hip", ["-ffp-contract=off"]),
     ("ppm_fdbm.hip", []),
+    ("ppm_ewald.hip", []),
 ]
 
 

@@ -235,6 +235,95 @@ int ppm_fdbm_compose_gradient(const float* e1, const float* e2, const float* e3,
     return launch_compose_gradient(e1, e2, e3, shape, step, grid, (hipStream_t)stream);
 }
 
+static int check_lattice(const char* fn, const double a[2], const double b[2]) {
+    if (!a || !b) { set_error("%s: lattice vectors are NULL", fn); return PPM_ERR_INVALID_ARG; }
+    for (int k = 0; k < 2; ++k)
+        if (!std::isfinite(a[k]) || !std::isfinite(b[k])) { set_error("%s: non-finite lattice vector", fn); return PPM_ERR_INVALID_ARG; }
+    if (!(std::fabs(a[0] * b[1] - a[1] * b[0]) > 1e-12)) { set_error("%s: degenerate lattice (|a x b| = 0)", fn); return PPM_ERR_INVALID_ARG; }
+    return PPM_OK;
+}
+
+static int check_ewald_sys(const char* fn, int n_layers, int n_harm) {
+    if (n_layers < 1 || n_layers > 65536) { set_error("%s: n_layers=%d out of [1,65536]", fn, n_layers); return PPM_ERR_INVALID_ARG; }
+    if (n_harm < 1 || n_harm > 64) { set_error("%s: n_harm=%d out of [1,64]", fn, n_harm); return PPM_ERR_INVALID_ARG; }
+    if ((long long)n_layers * ewald_num_g(n_harm) > (1LL << 31) - 1) { set_error("%s: n_layers * n_g too large", fn); return PPM_ERR_INVALID_ARG; }
+    return PPM_OK;
+}
+
+int ppm_ewald_num_g(int n_harm) { return n_harm < 0 ? 0 : ewald_num_g(n_harm); }
+
+size_t ppm_ewald_system_bytes(int n_layers, int n_harm) {
+    if (check_ewald_sys("ppm_ewald_system_bytes", n_layers, n_harm)) return 0;
+    return ewald_system_bytes(n_layers, n_harm);
+}
+
+int ppm_ewald_prepare(const float* ions, const int* layer_off, int n_layers, const double a[2], const double b[2],
+                      int n_harm, double drop_tol, void* system, size_t system_bytes, void* stream) {
+    const char* fn = "ppm_ewald_prepare";
+    g_err[0] = 0;
+    if (int r = check_ewald_sys(fn, n_layers, n_harm)) return r;
+    if (int r = check_lattice(fn, a, b)) return r;
+    if (!ions || !layer_off || !system) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    if (!(drop_tol >= 0.0) || !(drop_tol < 1.0)) { set_error("%s: drop_tol=%g out of [0,1)", fn, drop_tol); return PPM_ERR_INVALID_ARG; }
+    if (system_bytes < ewald_system_bytes(n_layers, n_harm)) { set_error("%s: system buffer too small", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_ewald_prepare(ions, layer_off, n_layers, a, b, n_harm, drop_tol, system, (hipStream_t)stream);
+}
+
+int ppm_ewald_eval(int mode, const float* points, int np, const void* system, int n_layers, int n_harm,
+                   const double a[2], const double b[2], float* phi, void* stream) {
+    const char* fn = "ppm_ewald_eval";
+    g_err[0] = 0;
+    if (mode != PPM_EWALD_FULL && mode != PPM_EWALD_VACUUM) { set_error("%s: unknown mode %d", fn, mode); return PPM_ERR_INVALID_ARG; }
+    if (int r = check_ewald_sys(fn, n_layers, n_harm)) return r;
+    if (int r = check_lattice(fn, a, b)) return r;
+    if (np < 0) { set_error("%s: np=%d < 0", fn, np); return PPM_ERR_INVALID_ARG; }
+    if (np > 0 && (!points || !system || !phi)) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_ewald_points(mode, points, np, system, n_layers, n_harm, a, b, phi, (hipStream_t)stream);
+}
+
+int ppm_ewald_brute(const float* points, int np, const float* ions, int n_ions, const double a[2], const double b[2],
+                    int n_rep, int square, float* phi, void* stream) {
+    const char* fn = "ppm_ewald_brute";
+    g_err[0] = 0;
+    if (int r = check_lattice(fn, a, b)) return r;
+    if (np < 0 || n_ions < 0 || n_rep < 0 || n_rep > 4096) { set_error("%s: invalid np=%d n_ions=%d n_rep=%d", fn, np, n_ions, n_rep); return PPM_ERR_INVALID_ARG; }
+    if (np > 0 && (!points || !phi || (n_ions > 0 && !ions))) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_ewald_brute(points, np, ions, n_ions, a, b, n_rep, square != 0, phi, (hipStream_t)stream);
+}
+
+int ppm_ewald_cluster(const float* points, int np, const float* ions, int n_ions, float* phi, void* stream) {
+    const char* fn = "ppm_ewald_cluster";
+    g_err[0] = 0;
+    if (np < 0 || n_ions < 0) { set_error("%s: invalid np=%d n_ions=%d", fn, np, n_ions); return PPM_ERR_INVALID_ARG; }
+    if (np > 0 && (!points || !phi || (n_ions > 0 && !ions))) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_ewald_cluster(points, np, ions, n_ions, phi, (hipStream_t)stream);
+}
+
+size_t ppm_ewald_grid_workspace_size(int n_harm, int nz) {
+    if (n_harm < 1 || n_harm > 64 || nz < 1) return 0;
+    return ewald_grid_workspace(n_harm, nz);
+}
+
+int ppm_ewald_tip_grid(const void* system, int n_layers, int n_harm, const double a[2], const double b[2],
+                       const PpmGridSpec* spec, const float Qs[4], const float QZs[4], int accumulate, void* workspace,
+                       size_t workspace_bytes, float* grid, void* stream) {
+    const char* fn = "ppm_ewald_tip_grid";
+    g_err[0] = 0;
+    if (int r = check_ewald_sys(fn, n_layers, n_harm)) return r;
+    if (int r = check_lattice(fn, a, b)) return r;
+    if (int r = check_spec(fn, spec)) return r;
+    if (!system || !Qs || !QZs || !workspace || !grid) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    if (!finite4(Qs, 4) || !finite4(QZs, 4)) { set_error("%s: non-finite Qs/QZs", fn); return PPM_ERR_INVALID_ARG; }
+    if (spec->dA[2] != 0.f || spec->dB[2] != 0.f || spec->dC[0] != 0.f || spec->dC[1] != 0.f) {
+        set_error("%s: dA, dB must lie in the xy plane and dC along z", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (spec->nz > 65535 * 8) { set_error("%s: nz=%d too large", fn, spec->nz); return PPM_ERR_INVALID_ARG; }
+    if (workspace_bytes < ewald_grid_workspace(n_harm, spec->nz)) { set_error("%s: workspace too small", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_ewald_tip_grid(system, n_layers, n_harm, a, b, *spec, Qs, QZs, accumulate, workspace, grid,
+                                 (hipStream_t)stream);
+}
+
 int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
                              const PpmTip* tip, const PpmRelaxCtl* ctl, float* FEs, float* PPs, void* stream) {
     const char* fn = "ppm_relax_strokes_tilted";

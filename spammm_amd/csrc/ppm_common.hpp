@@ -63,6 +63,19 @@ int launch_dispersion(const float* atoms, int na, const float origin[3], float s
                       int accumulate, float* out, hipStream_t stream);
 int launch_compose_gradient(const float* e1, const float* e2, const float* e3, const int n[3], float step,
                             float* grid, hipStream_t stream);
+int ewald_num_g(int n_harm);
+size_t ewald_system_bytes(int n_layers, int n_harm);
+size_t ewald_grid_workspace(int n_harm, int nz);
+int launch_ewald_prepare(const float* ions, const int* layer_off, int n_layers, const double a[2], const double b[2],
+                         int n_harm, double drop_tol, void* system, hipStream_t stream);
+int launch_ewald_points(int mode, const float* pts, int np, const void* system, int n_layers, int n_harm,
+                        const double a[2], const double b[2], float* phi, hipStream_t stream);
+int launch_ewald_brute(const float* pts, int np, const float* ions, int ni, const double a[2], const double b[2],
+                       int n_rep, bool square, float* phi, hipStream_t stream);
+int launch_ewald_cluster(const float* pts, int np, const float* ions, int ni, float* phi, hipStream_t stream);
+int launch_ewald_tip_grid(const void* system, int n_layers, int n_harm, const double a[2], const double b[2],
+                          const PpmGridSpec& gs, const float Qs[4], const float QZs[4], int accumulate,
+                          void* workspace, float* grid, hipStream_t stream);
 size_t batch_workspace(int nmol);
 int launch_batch(const PpmMolDesc* mols, const PpmBatchParams& P, const float* atoms, const float* cpar,
                  float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,

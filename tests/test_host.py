@@ -164,3 +164,26 @@ def test_rocksalt_slab_and_placements():
     d1 = np.linalg.norm(x[:, None] - x[None], axis=-1)
     np.testing.assert_allclose(d0, d1, atol=1e-12)                    # rigid
     np.testing.assert_allclose(x.mean(0) - c, [*sc["poses"][5][:2], 0.0], atol=1e-12)
+
+
+def test_ewald_sizes_and_validation_on_host():
+    """ppm_ewald_* sizes and argument checks (no device work)."""
+    L = _lib.load()
+    assert L.ppm_ewald_num_g(3) == 48 and L.ppm_ewald_num_g(32) == 65 * 65 - 1
+    assert L.ppm_ewald_system_bytes(4, 3) >= 4 * 48 * 16 + 48 * (4 + 16)
+    assert L.ppm_ewald_system_bytes(4, 0) == 0 and b"n_harm" in L.ppm_last_error()
+    assert L.ppm_ewald_grid_workspace_size(8, 10) >= 10 * 288 * 16 + 10 * 8
+    D2 = ctypes.c_double * 2
+    a, b, deg = D2(4.0, 0.0), D2(0.0, 4.0), D2(8.0, 0.0)
+    rc = L.ppm_ewald_prepare(1, 1, 2, ctypes.byref(a), ctypes.byref(deg), 3, 0.0, 1, 1 << 20, None)
+    assert rc == _lib.PPM_ERR_INVALID_ARG and b"degenerate" in L.ppm_last_error()
+    rc = L.ppm_ewald_prepare(1, 1, 2, ctypes.byref(a), ctypes.byref(b), 3, 0.0, 1, 16, None)
+    assert rc == _lib.PPM_ERR_INVALID_ARG and b"too small" in L.ppm_last_error()
+    rc = L.ppm_ewald_eval(7, 1, 1, 1, 2, 3, ctypes.byref(a), ctypes.byref(b), 1, None)
+    assert rc == _lib.PPM_ERR_INVALID_ARG and b"mode" in L.ppm_last_error()
+    s = _lib.PpmGridSpec()
+    s.nx, s.ny, s.nz = 4, 4, 4
+    s.dA, s.dB, s.dC = _lib.f4((1, 0, 0)), _lib.f4((0, 1, 0)), _lib.f4((0.2, 0, 1))
+    rc = L.ppm_ewald_tip_grid(1, 2, 3, ctypes.byref(a), ctypes.byref(b), ctypes.byref(s), _lib.f4((1,)),
+                              _lib.f4((0,)), 0, 1, 1 << 20, 1, None)
+    assert rc == _lib.PPM_ERR_INVALID_ARG and b"dC along z" in L.ppm_last_error()

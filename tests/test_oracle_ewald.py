@@ -48,3 +48,76 @@ def test_harmonic_periodic_and_field():
         e = np.zeros(3); e[ax] = 1e-5
         fd[:, ax] = -(s.phi_full(P_ABOVE + e) - s.phi_full(P_ABOVE - e)) / 2e-5
     assert np.allclose(s.field_full(P_ABOVE), fd, atol=1e-7 * np.abs(fd).max() + 1e-9)
+
+
+# ---- the reference's own Ewald2D acceptance tests (tests/test_surface.py:104-158), on the oracle
+# and the reference's NaCl fixtures ----
+
+from tests import ewald_cases as EC
+
+
+def _fixture_slab(name, n_harm):
+    rx, ry, rz, q, a, b = EC.load_substrate(name)
+    return E.Slab(np.stack([rx, ry, rz, q], 1), a, b, n_harm), (rx, ry, rz, q, a, b)
+
+
+def test_fixture_cells_are_neutral():
+    for name in ("NaCl_1x1_L3.xyz", "NaCl_1x1_L2_checker.xyz", "NaCl_8x8_L3.xyz"):
+        q = EC.load_substrate(name)[3]
+        assert abs(q.sum()) < 1e-6, name
+
+
+def test_vacuum_decay_and_xy_symmetry():
+    s, _ = _fixture_slab("NaCl_1x1_L3.xyz", 4)
+    z = np.linspace(5.0, 20.0, 50)
+    phi = s.phi_full(np.stack([np.full_like(z, 0.05), np.full_like(z, 0.05), z], 1))
+    assert np.isfinite(phi).all() and abs(phi[0]) > abs(phi[-1])
+    xv = np.linspace(0, 4, 20)
+    X, Y = np.meshgrid(xv, xv)
+    top = s.ions[:, 2].max()
+    phi = s.phi_vacuum(np.stack([X.ravel(), Y.ravel(), np.full(X.size, 3.0 + top)], 1)).reshape(X.shape)
+    assert np.allclose(phi, phi.T, atol=1e-6)
+
+
+def test_full_form_tracks_the_square_image_sum_and_it_converges():
+    s, _ = _fixture_slab("NaCl_1x1_L3.xyz", 4)
+    z = np.linspace(1.0, 5.0, 30)
+    P = np.stack([np.full_like(z, 0.05), np.full_like(z, 0.05), z], 1)
+    ew = s.phi_full(P)
+    br = s.phi_brute(P, 20, square=True)
+    assert EC.correlation(ew, br) > 0.999
+    assert EC.rmse(ew - ew.mean(), br - br.mean()) < 1e-3
+    p3 = np.array([[0.05, 0.05, 3.0]])
+    b20, b40, b60 = (s.phi_brute(p3, n, square=True)[0] for n in (20, 40, 60))
+    assert abs(b60 - b40) < abs(b40 - b20)
+
+
+def test_full_form_equals_an_evjen_cluster_up_to_a_constant():
+    """The finite Evjen cluster (K=20, 3362 ions) reproduces the periodic potential up to a
+    constant offset; the reference documents ~5e-9 eV RMSE after the offset at z = 2 A.  With
+    n_harm = 4 the series truncation leaves 1.9e-6 eV; n_harm = 8 brings it to 1.1e-8 eV."""
+    s, (rx, ry, rz, q, a, b) = _fixture_slab("NaCl_1x1_L2_checker.xyz", 8)
+    cl = EC.evjen_cluster(rx, ry, rz, q, a, b, K=20)
+    top = rz.max()
+    x = np.linspace(-2.0, 2.0, 40, endpoint=False)
+    P = np.stack([x, np.zeros_like(x), np.full_like(x, top + 2.0)], 1)
+    ew = s.phi_full(P)
+    c = E.phi_cluster(P, cl)
+    d = ew - c
+    assert np.abs(d - d.mean()).max() < 3e-8
+    assert np.ptp(ew) > 0.5                    # the lateral signal is resolved, not a flat zero
+
+
+def test_group_layers_orders_by_height_and_keeps_ions():
+    from spammm_amd.ewald import group_layers
+    en, ions = WL.rocksalt_slab(cells=2, layers=3)
+    rng = np.random.default_rng(0)
+    shuffled = ions[rng.permutation(len(ions))]
+    srt, off = group_layers(shuffled)
+    assert off[0] == 0 and off[-1] == len(ions) and len(off) == 4
+    for l in range(3):
+        zl = srt[off[l]:off[l + 1], 2]
+        assert (zl == zl[0]).all()
+    assert (np.diff(srt[off[:-1], 2]) > 0).all()
+    key = lambda a: np.lexsort(a.T[::-1])
+    assert np.array_equal(srt[key(srt)], shuffled[key(shuffled)])
