@@ -49,6 +49,7 @@ def main():
     ap.add_argument("--no-direct", action="store_true", help="skip the gridless direct Morse leg")
     ap.add_argument("--poses", type=int, default=64, help="placements of the substrate leg (0: skip)")
     ap.add_argument("--no-fdbm", action="store_true", help="skip the FDBM force-field producer leg")
+    ap.add_argument("--no-ewald", action="store_true", help="skip the 2D Ewald substrate electrostatics leg")
     ap.add_argument("--z-margin", default=None,
                     help="z window margins of the batched leg in A, 'below,above' (default: the library's)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -194,6 +195,8 @@ def main():
         out["substrate_poses"] = substrate_leg(dev, args.poses)
     if not args.no_fdbm:
         out["fdbm"] = fdbm_leg(dev)
+    if not args.no_ewald:
+        out["ewald"] = ewald_leg(dev)
     if not args.no_check:
         out["accuracy"] = accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip)
     if not args.no_cpu:
@@ -336,6 +339,80 @@ def substrate_leg(dev, n_poses=64):
     }
     log(f"[bench] substrate: {res}")
     del scene
+    torch.cuda.empty_cache()
+    return res
+
+
+def ewald_leg(dev, n_harm=40, reps=3):
+    """Periodic electrostatics of the substrate configuration's 1024-ion rock-salt slab (8x8 cells,
+    4 layers, 45.12 A cell) on its 902x902x300 grid at 0.05 A: the 2D Ewald probe grid (prepare =
+    layer structure factors + G compaction; tip grid = plane tables + grid kernel) against the
+    direct Coulomb sum over 3x3 cells that the substrate base otherwise uses (the grid-projection
+    kernel with zero LJ parameters), and their difference in the probe region."""
+    import torch
+    from spammm_amd import _lib, device as D, workloads as WL
+    from spammm_amd.ewald import SurfaceEwald
+    sc = WL.substrate_scene(n_poses=1)
+    spec = D_axis(sc)
+    nx, ny, nz = spec.n
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn, n):
+        ts = []
+        for _ in range(n):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            torch.cuda.synchronize(dev)
+            ts.append(e0.elapsed_time(e1))
+        return float(np.min(ts)), float(np.mean(ts))
+
+    # drop_tol 1e-6: the slab's FP32 coordinates break its 5.64 A periodicity at the 1e-7 level, which
+    # leaves ~900 G of the 45 A supercell with structure factors of 2.7e-7 of the largest; those are
+    # dropped (36 G kept).  drop_tol 1e-12 keeps them; both are timed and compared.
+    ew_all = SurfaceEwald(dev).prepare_system(sc["sub_atoms"], sc["lvA"][:2], sc["lvB"][:2], n_harm, 1e-12)
+    ew = SurfaceEwald(dev)
+    t_prep = timed(lambda: ew.prepare_system(sc["sub_atoms"], sc["lvA"][:2], sc["lvB"][:2], n_harm, 1e-6), 2)[0]
+    kept = ew.n_kept()
+    g = torch.empty((nz, ny, nx, 4), dtype=torch.float32, device=dev)
+    ew_all.tip_grid(spec, sc["Qs"], sc["QZs"], out=g)
+    t_all = timed(lambda: ew_all.tip_grid(spec, sc["Qs"], sc["QZs"], out=g), 1)[0]
+    g_all = g.clone()
+    ew.tip_grid(spec, sc["Qs"], sc["QZs"], out=g)
+    t_grid, t_grid_mean = timed(lambda: ew.tip_grid(spec, sc["Qs"], sc["QZs"], out=g), reps)
+    sub = torch.as_tensor(sc["sub_atoms"], device=dev)
+    zc = torch.zeros((sub.shape[0], 2), dtype=torch.float32, device=dev)
+    gd = torch.empty_like(g)
+    run_direct = lambda: D.grid_project(sub, zc, _lib.PPM_MODEL_LJ, spec, sc["Qs"], sc["QZs"], sc["npbc"],  # noqa: E731
+                                        sc["lvA"], sc["lvB"], cap=False, out=gd)
+    run_direct()
+    t_direct = timed(run_direct, 2)[0]
+    dz = float(spec.dC[2])
+    res = {"workload": f"1024-ion rock-salt slab (8x8 cells, 4 layers), {nx}x{ny}x{nz} grid (0.05 A), "
+                       f"n_harm {n_harm}",
+           "n_g": _lib.load().ppm_ewald_num_g(n_harm), "kept_g": kept, "prepare_ms": t_prep,
+           "tip_grid_ms": t_grid, "tip_grid_ms_mean": t_grid_mean, "direct_3x3_coulomb_ms": t_direct,
+           "kept_g_drop_1e-12": ew_all.n_kept(), "tip_grid_ms_drop_1e-12": t_all}
+    nvox = nx * ny * nz
+    res["tip_grid_GBps_written"] = nvox * 16 / (t_grid * 1e-3) / 1e9
+    res["tip_grid_fp32_TFLOPs"] = nvox * kept * 15 / (t_grid * 1e-3) / 1e12
+    for h in (3.0, 5.0):
+        iz0 = int(np.ceil((h - float(spec.p0[2])) / dz))
+        dF = (g[iz0:, :, :, :3] - gd[iz0:, :, :, :3]).abs().amax().item()
+        Fm = g[iz0:, :, :, :3].abs().amax().item()
+        res[f"direct_vs_ewald_max_dF_above_{h:g}A"] = dF
+        res[f"drop_1e-6_vs_1e-12_max_dF_above_{h:g}A"] = (g[iz0:, :, :, :3] - g_all[iz0:, :, :, :3]).abs().amax().item()
+        res[f"ewald_max_F_above_{h:g}A"] = Fm
+    # series truncation: the potential's change from n_harm to n_harm + 16 at probe heights 2..5.5 A
+    # above the top layer, and at 1 A (near the surface the series needs more harmonics)
+    hi = SurfaceEwald(dev).prepare_system(sc["sub_atoms"], sc["lvA"][:2], sc["lvB"][:2], n_harm + 16, 1e-6)
+    for zlo in (1.0, 2.0):
+        P = np.array([[22.56 + 0.3 * i, 22.56 - 0.2 * i, zlo + 0.5 * i] for i in range(8)], np.float32)
+        res[f"phi_change_n_harm_plus_16_z_{zlo:g}_to_{zlo + 3.5:g}A"] = float(np.abs(
+            hi.eval_full(P[:, 0], P[:, 1], P[:, 2]) - ew.eval_full(P[:, 0], P[:, 1], P[:, 2])).max())
+    log(f"[bench] ewald: {res}")
+    del g, gd, g_all
     torch.cuda.empty_cache()
     return res
 

@@ -54,6 +54,8 @@ class Slab:
 
     def phi_full(self, P):
         P = np.atleast_2d(P)
+        if len(P) > 1024:
+            return np.concatenate([self.phi_full(P[i:i + 1024]) for i in range(0, len(P), 1024)])
         x, y, z, q = self.ions.T
         dz = np.abs(P[:, 2:3] - z[None, :])                                      # (np, ni)
         phi0 = -(2 * np.pi / self.area) * (q[None, :] * dz).sum(1)
@@ -65,6 +67,8 @@ class Slab:
     def field_full(self, P):
         """(E_x, E_y, E_z) = -grad phi, analytic."""
         P = np.atleast_2d(P)
+        if len(P) > 1024:
+            return np.concatenate([self.field_full(P[i:i + 1024]) for i in range(0, len(P), 1024)])
         x, y, z, q = self.ions.T
         d = P[:, 2:3] - z[None, :]
         sg = np.sign(d)

@@ -27,7 +27,7 @@ namespace {
 
 constexpr double kPi = 3.141592653589793;
 constexpr int kGChunk = 64;
-constexpr int kVz = 8;           // z planes per thread of the grid kernel
+constexpr int kVz = 16;          // z planes per thread of the grid kernel
 constexpr int kScanThreads = 1024;
 
 struct EwGeom {

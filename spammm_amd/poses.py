@@ -1,7 +1,9 @@
 """Rigid placements of a molecule over a substrate: df images and their adjoint w.r.t. the
 placement (shift, rotation about z) and the molecule's charges.
 
-The substrate's raw force field (no |E|>100 rescale) is projected once; every placement adds
+The substrate's raw force field (no |E|>100 rescale) is projected once (its Coulomb part either
+as the direct sum over npbc x,y images or, sub_coulomb = "ewald", as the 2D Ewald series of
+the periodic slab, spammm_amd.ewald); every placement adds
 the molecule's pair sum to it and applies the rescale to the total (ppm_grid_project_add), then
 relaxes the scan and applies the df operator.  The backward recomputes the probe positions
 and, unless the shared grid buffer still holds this placement's grid (no other placement's
@@ -53,6 +55,9 @@ class Scene:
     grid: object = None         # placement grid buffer
     grid_cot: object = None     # adjoint buffer
     grid_serial: int = 0        # forward count; identifies the placement whose grid `grid` holds
+    sub_coulomb: str = "direct"  # "direct": npbc image sum; "ewald": 2D Ewald series over lvA, lvB
+    ewald_n_harm: int = 40
+    ewald_drop_tol: float = 1e-6  # G with |W| below this fraction of the largest: FP32 position noise
 
     @property
     def n_scan(self):
@@ -69,8 +74,20 @@ class Scene:
     def prepare(self):
         """Project the substrate once (raw, no rescale) and allocate the shared buffers."""
         import torch
-        self.base = D.grid_project(self.sub_atoms, self.sub_cpar, _lib.PPM_MODEL_LJ, self.spec, self.Qs, self.QZs,
-                                   self.npbc, self.lvA, self.lvB, cap=False)
+        if self.sub_coulomb == "direct":
+            self.base = D.grid_project(self.sub_atoms, self.sub_cpar, _lib.PPM_MODEL_LJ, self.spec, self.Qs,
+                                       self.QZs, self.npbc, self.lvA, self.lvB, cap=False)
+        elif self.sub_coulomb == "ewald":
+            from .ewald import SurfaceEwald
+            self.base = D.grid_project(self.sub_atoms, self.sub_cpar, _lib.PPM_MODEL_LJ, self.spec,
+                                       np.zeros(4, np.float32), np.zeros(4, np.float32), self.npbc, self.lvA,
+                                       self.lvB, cap=False)
+            ew = SurfaceEwald(self.sub_atoms.device)
+            ew.prepare_system(self.sub_atoms.cpu().numpy(), self.lvA[:2], self.lvB[:2], self.ewald_n_harm,
+                              self.ewald_drop_tol)
+            ew.tip_grid(self.spec, self.Qs, self.QZs, out=self.base, accumulate=True)
+        else:
+            raise ValueError(f"sub_coulomb must be 'direct' or 'ewald', got {self.sub_coulomb!r}")
         self.grid = torch.empty_like(self.base)
         self.grid_cot = torch.empty_like(self.base)
         return self

@@ -20,7 +20,7 @@ from spammm_amd import poses as PS        # noqa: E402
 from spammm_amd import workloads as W     # noqa: E402
 
 
-def make_scene(sc):
+def make_scene(sc, **kw):
     dev = D.require_gpu()
     spec = D.axis_aligned_spec(sc["p0"], sc["L"], sc["n"])
     t = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float32), device=dev)  # noqa: E731
@@ -33,7 +33,7 @@ def make_scene(sc):
     scene = PS.Scene(spec=spec, sub_atoms=t(sc["sub_atoms"]), sub_cpar=t(sc["sub_cLJ"]),
                      mol_xyz0=sc["mol_atoms"][:, :3].astype(np.float64), mol_cpar=t(sc["mol_cLJ"]), pts=t(pts),
                      nxy=sc["nxy"], nz=sc["nz"], tip=tip, Qs=sc["Qs"], QZs=sc["QZs"], npbc=sc["npbc"],
-                     lvA=sc["lvA"], lvB=sc["lvB"], max_iter=3000, f2conv=1e-12)
+                     lvA=sc["lvA"], lvB=sc["lvB"], max_iter=3000, f2conv=1e-12, **kw)
     return scene.prepare(), pts
 
 
@@ -70,9 +70,9 @@ def test_grid_over_base_equals_combined_grid():
     assert (err <= bound).all()
 
 
-def abs_term_sums(atoms, cpar, sp, sc):
+def abs_term_sums(atoms, cpar, sp, sc, cap=True):
     """Per voxel, the sum over atoms and images of |F_term| and |E_term| (FP64, NumPy), capped
-    voxels scaled like the grid (100/|E|) so that the bound follows the stored values."""
+    voxels scaled like the grid (100/|E|, if cap) so that the bound follows the stored values."""
     nx, ny, nz = sp.n
     ia, ib, ic = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     P = (np.asarray(sp.p0, np.float64)[None, :] + np.outer(ia.ravel(), sp.dA) + np.outer(ib.ravel(), sp.dB)
@@ -95,7 +95,7 @@ def abs_term_sums(atoms, cpar, sp, sc):
                 SF += f_lj + np.abs(e_c) * ir2 * np.sqrt(r2)
                 SE += np.abs(e_lj) + np.abs(e_c)
                 net += e_lj + e_c
-    scale = np.minimum(1.0, 100.0 / np.maximum(np.abs(net), 1e-300))
+    scale = np.minimum(1.0, 100.0 / np.maximum(np.abs(net), 1e-300)) if cap else 1.0
     S = (SF + SE) * scale
     return S.reshape(nx, ny, nz).transpose(2, 1, 0)[..., None]
 
@@ -180,3 +180,38 @@ def test_backward_after_another_placement_recomputes_the_grid():
     print(f"[poses] pose grad {gp_a} (interleaved {gp_b})")
     for a, b in ((gp_a, gp_b), (gq_a, gq_b)):
         assert np.allclose(a, b, rtol=1e-5, atol=1e-6 * np.abs(a).max())
+
+
+def test_ewald_substrate_base_equals_lj_plus_ewald_oracle():
+    """sub_coulomb = "ewald": the raw base is the slab's LJ image sum (no tip charges) plus the
+    2D Ewald electrostatics of the tip charges; checked on 3000 voxels against the FP64 LJ oracle
+    plus the FP64 Ewald oracle (oracle/ewald_ref.py) within the grid bar of this module plus 2e-5
+    of the largest Ewald channel (the Ewald grid's FP32 bar, tests/test_gpu_ewald.py)."""
+    from oracle import ewald_ref as EW
+    sc = W.small_substrate_scene()
+    scene, _ = make_scene(sc, sub_coulomb="ewald", ewald_n_harm=8)
+    base = scene.base.cpu().numpy().astype(np.float64)
+    sp = scene.spec
+    sub = sc["sub_atoms"].astype(np.float64)
+    cp = sc["sub_cLJ"].astype(np.float64)
+    lj = O.grid_project_d(0, sub, cp, sp.n, sp.p0, sp.dA, sp.dB, sp.dC, np.zeros(4), np.zeros(4), sc["npbc"],
+                          sc["lvA"], sc["lvB"], cap=False)
+    slab = EW.Slab(sub, sc["lvA"][:2], sc["lvB"][:2], 8)
+    nx, ny, nz = sp.n
+    rng = np.random.default_rng(5)
+    idx = rng.choice(nx * ny * nz, 3000, replace=False)
+    ic, ib, ia = np.unravel_index(idx, (nz, ny, nx))
+    P = (np.asarray(sp.p0, np.float64)[None, :] + np.outer(ia, sp.dA) + np.outer(ib, sp.dB) + np.outer(ic, sp.dC))
+    ew = np.zeros((len(P), 4))
+    for qq, dz in zip(sc["Qs"], sc["QZs"]):
+        if qq != 0:
+            ew[:, 3] += qq * slab.phi_full(P + [0, 0, dz])
+            ew[:, :3] += qq * slab.field_full(P + [0, 0, dz])
+    want = lj[ic, ib, ia] + ew
+    got = base[ic, ib, ia]
+    S = abs_term_sums(sub, cp, sp, dict(sc, Qs=np.zeros(4), QZs=np.zeros(4)), cap=False)[ic, ib, ia]
+    bound = 1e-5 * np.linalg.norm(lj[ic, ib, ia], axis=-1, keepdims=True) + 4e-6 * S + 1e-6 + \
+        2e-5 * np.abs(ew).max(0)
+    err = np.abs(got - want)
+    print(f"[poses] ewald base: max |d| {err.max():.3e}, worst |d|/bound {(err / bound).max():.3e}")
+    assert (err <= bound).all()
