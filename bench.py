@@ -267,7 +267,9 @@ def substrate_leg(dev, n_poses=64):
     x,y, 902x902x300 grid at 0.05 A, 192x192 pixels x 60 z-steps): the slab's raw grid once, then
     per placement the forward (grid over the slab base, relaxation, df) and the adjoint of an L2
     loss w.r.t. the placement (shift, rotation) and the molecule's charges, recomputing the grid
-    and the probe positions in the backward (one shared grid buffer)."""
+    and the probe positions in the backward (one shared grid buffer).  Every grid lives on the
+    scan window (the sub-box of the 902x902x300 grid the probes can reach); a placement whose
+    probes leave it is recomputed on the full grid (counted)."""
     import torch
     from spammm_amd import poses as PS, workloads as WL
     sc = WL.substrate_scene(n_poses=n_poses)
@@ -330,11 +332,14 @@ def substrate_leg(dev, n_poses=64):
         "workload": f"60-atom molecule x {n_poses} placements over a 1024-ion rock-salt slab, "
                     f"{sc['n'][0]}x{sc['n'][1]}x{sc['n'][2]} grid (0.05 A), {sc['nxy'][0]}x{sc['nxy'][1]} px x {sc['nz']} z",
         "base_grid_ms": t_base, "forward_ms_mean": float(np.mean(t_f)), "adjoint_ms_mean": float(np.mean(t_b)),
+        "forward_ms_median": float(np.median(t_f)), "adjoint_ms_median": float(np.median(t_b)),
+        "first_placement_forward_ms": float(t_f[0]),
         "adjoint_over_forward": float(np.mean(t_b) / np.mean(t_f)),
         "pixel_z_steps_per_s_forward": states / (np.mean(t_f) * 1e-3),
         "placements_per_s_forward_plus_adjoint": 1e3 / (np.mean(t_f) + np.mean(t_b)),
         "gated_states_total": int(gated), "pose_grad_norm_median": float(np.median(gnorm)),
         "directional_fd": checks,
+        "scan_window": list(scene.spec_run.n), "window_fallbacks": int(scene.window_fallbacks),
         "grid_bytes": int(scene.base.numel() * 4), "peak_mem_GB": torch.cuda.max_memory_allocated(dev) / 1e9,
     }
     log(f"[bench] substrate: {res}")

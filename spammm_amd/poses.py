@@ -1,7 +1,11 @@
 """Rigid placements of a molecule over a substrate: df images and their adjoint w.r.t. the
 placement (shift, rotation about z) and the molecule's charges.
 
-The substrate's raw force field (no |E|>100 rescale) is projected once (its Coulomb part either
+Every grid of a placement lives on the scan window: the sub-box of the grid (same voxels, same
+spacing) that holds every probe position the scan can reach (the scan's tip positions moved by
+the rest offset, plus lateral and vertical margins).  A placement whose probes leave the window
+(checked after each relaxation) is recomputed on the full grid.  The substrate's raw force
+field (no |E|>100 rescale) is projected once (its Coulomb part either
 as the direct sum over npbc x,y images or, sub_coulomb = "ewald", as the 2D Ewald series of
 the periodic slab, spammm_amd.ewald); every placement adds
 the molecule's pair sum to it and applies the rescale to the total (ppm_grid_project_add), then
@@ -58,6 +62,11 @@ class Scene:
     sub_coulomb: str = "direct"  # "direct": npbc image sum; "ewald": 2D Ewald series over lvA, lvB
     ewald_n_harm: int = 40
     ewald_drop_tol: float = 1e-6  # G with |W| below this fraction of the largest: FP32 position noise
+    window_margin: tuple = (2.0, 1.0)  # lateral, vertical margins [A] of the scan window; None: full grid
+    spec_run: object = None     # the grid the placements run on (the scan window or `spec`)
+    window_fallbacks: int = 0   # placements recomputed on the full grid
+    window_faces: tuple = None  # (lo, hi) bool[3]: window faces that are faces of the full grid
+    full: object = None         # (base, grid, grid_cot, serial) of the full grid, built on first fallback
 
     @property
     def n_scan(self):
@@ -72,25 +81,103 @@ class Scene:
         return self.mol_xyz0.mean(axis=0)
 
     def prepare(self):
-        """Project the substrate once (raw, no rescale) and allocate the shared buffers."""
+        """Project the substrate once (raw, no rescale) on the grid the placements run on and
+        allocate the shared buffers."""
         import torch
-        if self.sub_coulomb == "direct":
-            self.base = D.grid_project(self.sub_atoms, self.sub_cpar, _lib.PPM_MODEL_LJ, self.spec, self.Qs,
-                                       self.QZs, self.npbc, self.lvA, self.lvB, cap=False)
-        elif self.sub_coulomb == "ewald":
-            from .ewald import SurfaceEwald
-            self.base = D.grid_project(self.sub_atoms, self.sub_cpar, _lib.PPM_MODEL_LJ, self.spec,
-                                       np.zeros(4, np.float32), np.zeros(4, np.float32), self.npbc, self.lvA,
-                                       self.lvB, cap=False)
-            ew = SurfaceEwald(self.sub_atoms.device)
-            ew.prepare_system(self.sub_atoms.cpu().numpy(), self.lvA[:2], self.lvB[:2], self.ewald_n_harm,
-                              self.ewald_drop_tol)
-            ew.tip_grid(self.spec, self.Qs, self.QZs, out=self.base, accumulate=True)
-        else:
+        if self.sub_coulomb not in ("direct", "ewald"):
             raise ValueError(f"sub_coulomb must be 'direct' or 'ewald', got {self.sub_coulomb!r}")
+        self.spec_run = self.spec
+        if self.window_margin is not None:
+            w = scan_window(self.spec, self.pts.cpu().numpy(), self.nz, self.tip, *self.window_margin)
+            if w is not None:
+                self.spec_run, lo, hi = w
+                self.window_faces = (lo, hi)
+        self.base = self._substrate(self.spec_run)
         self.grid = torch.empty_like(self.base)
         self.grid_cot = torch.empty_like(self.base)
         return self
+
+    @property
+    def windowed(self):
+        return self.spec_run is not self.spec
+
+    def _substrate(self, spec):
+        if self.sub_coulomb == "direct":
+            return D.grid_project(self.sub_atoms, self.sub_cpar, _lib.PPM_MODEL_LJ, spec, self.Qs, self.QZs,
+                                  self.npbc, self.lvA, self.lvB, cap=False)
+        from .ewald import SurfaceEwald
+        base = D.grid_project(self.sub_atoms, self.sub_cpar, _lib.PPM_MODEL_LJ, spec, np.zeros(4, np.float32),
+                              np.zeros(4, np.float32), self.npbc, self.lvA, self.lvB, cap=False)
+        ew = SurfaceEwald(self.sub_atoms.device)
+        ew.prepare_system(self.sub_atoms.cpu().numpy(), self.lvA[:2], self.lvB[:2], self.ewald_n_harm,
+                          self.ewald_drop_tol)
+        return ew.tip_grid(spec, self.Qs, self.QZs, out=base, accumulate=True)
+
+    def buffers(self, full=False):
+        """(spec, base, grid, grid_cot) of the scan window (or of the full grid)."""
+        import torch
+        if not full or not self.windowed:
+            return self.spec_run, self.base, self.grid, self.grid_cot
+        if self.full is None:
+            b = self._substrate(self.spec)
+            self.full = [b, torch.empty_like(b), torch.empty_like(b), 0]
+        return self.spec, self.full[0], self.full[1], self.full[2]
+
+    def inside(self, PPs, guard=0.5) -> bool:
+        """True if every probe position of PPs samples the window exactly as it samples the full
+        grid: at least `guard` A inside each window face that cuts the full grid (so the FIRE
+        steps between stored positions stay on the window's voxels), and without wrapping at a
+        face the window shares with the full grid."""
+        if not self.windowed:
+            return True
+        sp = self.spec_run
+        d = np.array([sp.dA[0], sp.dB[1], sp.dC[2]], np.float64)
+        shared_lo, shared_hi = self.window_faces
+        g_lo = np.where(shared_lo, 0.0, np.ceil(guard / d))
+        g_hi = np.where(shared_hi, 0.0, np.ceil(guard / d))
+        import torch
+        mn, mx = torch.aminmax(PPs.reshape(-1, 4), dim=0)
+        lo = mn[:3].cpu().numpy().astype(np.float64)
+        hi = mx[:3].cpu().numpy().astype(np.float64)
+        ulo = (lo - sp.p0.astype(np.float64)) / d
+        uhi = (hi - sp.p0.astype(np.float64)) / d
+        return bool(np.isfinite(ulo).all() and np.isfinite(uhi).all() and (ulo >= g_lo).all()
+                    and (uhi < np.asarray(sp.n, np.float64) - 1 - g_hi).all())
+
+
+def scan_window(spec: D.GridSpec, pts, nz: int, tip: D.Tip, margin_xy: float, margin_z: float):
+    """The sub-box of an axis-aligned grid that holds every probe rest position of the scan
+    (tip positions pts + k dTip, k < nz, plus the rest offset dpos0 in the tip frame) widened by
+    the margins, clamped to the grid's faces: (GridSpec on the same voxels, lo, hi) with lo / hi
+    bool[3] marking the window faces that are faces of the grid.  None if the grid is not
+    axis-aligned or the window is the whole grid."""
+    dA, dB, dC = (np.asarray(v, np.float64)[:3] for v in (spec.dA, spec.dB, spec.dC))
+    if dA[1] or dA[2] or dB[0] or dB[2] or dC[0] or dC[1]:
+        return None
+    d = np.array([dA[0], dB[1], dC[2]])
+    R = np.stack([np.asarray(tip.tipA, np.float64)[:3], np.asarray(tip.tipB, np.float64)[:3],
+                  np.asarray(tip.tipC, np.float64)[:3]])
+    rest = R.T @ np.asarray(tip.dpos0, np.float64)[:3]
+    dtip = np.asarray(tip.tipC, np.float64)[:3] * float(tip.tipC[3])
+    P = np.asarray(pts, np.float64)[:, :3] + rest
+    ends = np.concatenate([P, P + (nz - 1) * dtip])
+    m = np.array([margin_xy, margin_xy, margin_z])
+    lo, hi = ends.min(0) - m, ends.max(0) + m
+    p0 = np.asarray(spec.p0, np.float64)[:3]
+    n = np.asarray(spec.n, np.int64)
+    i0 = np.maximum(np.floor((lo - p0) / d).astype(np.int64), 0)
+    i1 = np.minimum(np.ceil((hi - p0) / d).astype(np.int64) + 1, n - 1)
+    shared_lo, shared_hi = i0 == 0, i1 == n - 1
+    if (shared_lo & shared_hi).all() or (i1 <= i0).any():
+        return None
+    nw = i1 - i0 + 1
+    p0w = (p0 + i0 * d).astype(np.float32)
+    L = nw * d
+    dinvA, dinvB, dinvC = D.dinv_axis_aligned(L, p0w.astype(np.float64), tuple(int(v) for v in nw))
+    w = D.GridSpec(tuple(int(v) for v in nw), p0w, np.asarray(spec.dA, np.float32).copy(),
+                   np.asarray(spec.dB, np.float32).copy(), np.asarray(spec.dC, np.float32).copy(),
+                   dinvA, dinvB, dinvC)
+    return w, shared_lo, shared_hi
 
 
 def place(xyz0, center, pose):
@@ -113,15 +200,23 @@ def place(xyz0, center, pose):
     return lib.stack([x, y, z], 1)
 
 
-def _forward(scene: Scene, atoms, grid_valid=False):
+def _serial(scene: Scene, full: bool):
+    return scene.full[3] if (full and scene.windowed) else scene.grid_serial
+
+
+def _forward(scene: Scene, atoms, grid_valid=False, full=False):
+    spec, base, grid, _ = scene.buffers(full)
     if not grid_valid:
-        D.grid_project(atoms, scene.mol_cpar, _lib.PPM_MODEL_LJ, scene.spec, scene.Qs, scene.QZs, scene.npbc,
-                       scene.lvA, scene.lvB, cap=True, out=scene.grid, base=scene.base)
-        scene.grid_serial += 1
-    FEs, PPs = D.relax_tilted(scene.grid, scene.spec, scene.pts, scene.nz, scene.tip, scene.max_iter, scene.f2conv,
+        D.grid_project(atoms, scene.mol_cpar, _lib.PPM_MODEL_LJ, spec, scene.Qs, scene.QZs, scene.npbc,
+                       scene.lvA, scene.lvB, cap=True, out=grid, base=base)
+        if full and scene.windowed:
+            scene.full[3] += 1
+        else:
+            scene.grid_serial += 1
+    FEs, PPs = D.relax_tilted(grid, spec, scene.pts, scene.nz, scene.tip, scene.max_iter, scene.f2conv,
                               True, scan_ny=int(scene.nxy[1]))
     if scene.polish > 0:
-        D.relax_polish(scene.grid, scene.spec, scene.pts, scene.nz, scene.tip, PPs, FEs, scene.polish)
+        D.relax_polish(grid, spec, scene.pts, scene.nz, scene.tip, PPs, FEs, scene.polish)
     return FEs, PPs
 
 
@@ -134,8 +229,14 @@ def _make_function():
             xyz0 = torch.as_tensor(scene.mol_xyz0, dtype=torch.float64, device=pose.device)
             xyz = place(xyz0, torch.as_tensor(scene.center, device=pose.device), pose.detach())
             atoms = torch.cat([xyz.float(), q.detach().float()[:, None]], 1).contiguous()
-            FEs, _ = _forward(scene, atoms)
-            ctx.grid_serial = scene.grid_serial
+            FEs, PPs = _forward(scene, atoms)
+            full = False
+            if not scene.inside(PPs):
+                scene.window_fallbacks += 1
+                full = True
+                FEs, _ = _forward(scene, atoms, full=True)
+            ctx.full = full
+            ctx.grid_serial = _serial(scene, full)
             op = DF.z_operator(scene.df_kind, scene.nz, scene.dz, pose.device, scene.amp, scene.f0, scene.k_cant)
             df = op.apply(FEs, (scene.nz * 4, 4, 0), scene.n_scan, 1, offset=2)
             ctx.scene = scene
@@ -152,23 +253,23 @@ def _make_function():
             xyz = place(xyz0, ctr, pose)
             atoms = torch.cat([xyz.float(), q.float()[:, None]], 1).contiguous()
             # recompute the probe positions, and the grid if another placement has overwritten it
-            _, PPs = _forward(scene, atoms, grid_valid=scene.grid_serial == ctx.grid_serial)
+            full = ctx.full
+            spec, base, grid, grid_cot = scene.buffers(full)
+            _, PPs = _forward(scene, atoms, grid_valid=_serial(scene, full) == ctx.grid_serial, full=full)
             g = gdf.detach().float().contiguous().reshape(scene.n_scan, scene.nz)
             opT = DF.z_operator(scene.df_kind, scene.nz, scene.dz, dev, scene.amp, scene.f0, scene.k_cant,
                                 transpose=True)
             U = torch.zeros((scene.n_scan, scene.nz, 4), dtype=torch.float32, device=dev)
             U[..., 2] = opT.apply(g, (scene.nz, 1, 0), scene.n_scan, 1)[..., 0]
-            scene.grid_cot.zero_()
-            _, lam = D.relax_adjoint(scene.grid, scene.spec, scene.pts, scene.nz, scene.tip, PPs, U,
-                                     grid_cot=scene.grid_cot)
+            grid_cot.zero_()
+            _, lam = D.relax_adjoint(grid, spec, scene.pts, scene.nz, scene.tip, PPs, U, grid_cot=grid_cot)
             st = lam[..., 3].to(torch.int32)
             scene.last_status = {"states": int(st.numel()), "gated": int((st != 0).sum()),
                                  "nonfinite": int(((st & 2) != 0).sum())}
             if scene.strict and scene.last_status["nonfinite"]:
                 raise RuntimeError(f"pose adjoint: {scene.last_status['nonfinite']} states non-finite")
-            g6 = D.grid_adjoint(atoms, scene.mol_cpar, _lib.PPM_MODEL_LJ, scene.spec, scene.Qs, scene.QZs,
-                                scene.grid, scene.grid_cot, scene.npbc, scene.lvA, scene.lvB, cap=True,
-                                base=scene.base)
+            g6 = D.grid_adjoint(atoms, scene.mol_cpar, _lib.PPM_MODEL_LJ, spec, scene.Qs, scene.QZs,
+                                grid, grid_cot, scene.npbc, scene.lvA, scene.lvB, cap=True, base=base)
             gx = g6[:, :3]
             th = pose[3]
             dx = xyz0[:, 0] - ctr[0]

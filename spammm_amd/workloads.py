@@ -330,9 +330,9 @@ def substrate_scene(seed=4, n_mol=60, n_poses=64, cells=8, step=0.05, nz_grid=30
 
 
 def small_substrate_scene(seed=9):
-    """A reduced substrate configuration that the FP64 oracle finishes in seconds: 1x1-cell slab
-    of 2 layers, a 6-atom molecule, 0.2 A grid, 10x10 pixels x 8 z-steps."""
-    d = substrate_scene(seed=seed, n_mol=6, n_poses=4, cells=2, step=0.25, nz_grid=44, nxy=(10, 10), nz=8,
+    """A reduced substrate configuration that the FP64 oracle finishes in seconds: 2x2-cell slab
+    of 2 layers, a 6-atom molecule, 0.25 A grid (45x45x52), 10x10 pixels x 8 z-steps."""
+    d = substrate_scene(seed=seed, n_mol=6, n_poses=4, cells=2, step=0.25, nz_grid=52, nxy=(10, 10), nz=8,
                         scan_size=4.0, tip_top=11.0)
     en, sub = rocksalt_slab(2, 5.64, layers=2)
     d["sub_enames"], d["sub_atoms"] = en, sub

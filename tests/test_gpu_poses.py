@@ -53,10 +53,11 @@ def test_grid_over_base_equals_combined_grid():
     q = sc["mol_atoms"][:, 3].astype(np.float64)
     x = PS.place(scene.mol_xyz0, scene.center, pose)
     atoms = torch.as_tensor(np.concatenate([x, q[:, None]], 1).astype(np.float32), device=scene.pts.device)
-    G = D.grid_project(atoms, scene.mol_cpar, 0, scene.spec, sc["Qs"], sc["QZs"], sc["npbc"], sc["lvA"], sc["lvB"],
-                       cap=True, base=scene.base).cpu().numpy().astype(np.float64)
+    assert scene.windowed
+    G = D.grid_project(atoms, scene.mol_cpar, 0, scene.spec_run, sc["Qs"], sc["QZs"], sc["npbc"], sc["lvA"],
+                       sc["lvB"], cap=True, base=scene.base).cpu().numpy().astype(np.float64)
     ca, cp = combined(sc, pose, q)
-    sp = scene.spec
+    sp = scene.spec_run
     Go = O.grid_project_d(0, ca.astype(np.float32).astype(np.float64), cp.astype(np.float32).astype(np.float64),
                           sp.n, sp.p0, sp.dA, sp.dB, sp.dC, sc["Qs"], sc["QZs"], sc["npbc"], sc["lvA"], sc["lvB"])
     # The ionic slab's terms cancel to a small net field above it, so the FP32 sum's rounding is
@@ -191,7 +192,7 @@ def test_ewald_substrate_base_equals_lj_plus_ewald_oracle():
     sc = W.small_substrate_scene()
     scene, _ = make_scene(sc, sub_coulomb="ewald", ewald_n_harm=8)
     base = scene.base.cpu().numpy().astype(np.float64)
-    sp = scene.spec
+    sp = scene.spec_run
     sub = sc["sub_atoms"].astype(np.float64)
     cp = sc["sub_cLJ"].astype(np.float64)
     lj = O.grid_project_d(0, sub, cp, sp.n, sp.p0, sp.dA, sp.dB, sp.dC, np.zeros(4), np.zeros(4), sc["npbc"],
@@ -199,7 +200,7 @@ def test_ewald_substrate_base_equals_lj_plus_ewald_oracle():
     slab = EW.Slab(sub, sc["lvA"][:2], sc["lvB"][:2], 8)
     nx, ny, nz = sp.n
     rng = np.random.default_rng(5)
-    idx = rng.choice(nx * ny * nz, 3000, replace=False)
+    idx = rng.choice(nx * ny * nz, min(3000, nx * ny * nz), replace=False)
     ic, ib, ia = np.unravel_index(idx, (nz, ny, nx))
     P = (np.asarray(sp.p0, np.float64)[None, :] + np.outer(ia, sp.dA) + np.outer(ib, sp.dB) + np.outer(ic, sp.dC))
     ew = np.zeros((len(P), 4))
@@ -215,3 +216,38 @@ def test_ewald_substrate_base_equals_lj_plus_ewald_oracle():
     err = np.abs(got - want)
     print(f"[poses] ewald base: max |d| {err.max():.3e}, worst |d|/bound {(err / bound).max():.3e}")
     assert (err <= bound).all()
+
+
+def _df_and_grads(scene, sc, k, v):
+    dev = scene.pts.device
+    pose = torch.tensor(sc["poses"][k], dtype=torch.float64, device=dev, requires_grad=True)
+    q = torch.tensor(sc["mol_atoms"][:, 3], dtype=torch.float64, device=dev, requires_grad=True)
+    df = PS.pose_df(pose, q, scene)
+    (df.double() * v).sum().backward()
+    return df.detach().cpu().numpy(), pose.grad.cpu().numpy(), q.grad.cpu().numpy()
+
+
+def test_scan_window_equals_full_grid_and_falls_back():
+    """Placements on the scan window give the df and gradients of the full grid (the window's
+    voxels are the full grid's; only the FP32 rounding of the window origin differs); with zero
+    margins the probes come within the guard of the window's faces and the placement is
+    recomputed on the full grid, with the full grid's results."""
+    sc = W.small_substrate_scene()
+    win, _ = make_scene(sc)
+    full, _ = make_scene(sc, window_margin=None)
+    tight, _ = make_scene(sc, window_margin=(0.0, 0.0))
+    assert win.windowed and not full.windowed and tight.windowed
+    assert win.base.numel() < full.base.numel() // 3
+    v = torch.as_tensor(np.random.default_rng(8).standard_normal((*sc["nxy"], sc["nz"])), device=win.pts.device)
+    for k in (0, 2):
+        a = _df_and_grads(win, sc, k, v)
+        b = _df_and_grads(full, sc, k, v)
+        c = _df_and_grads(tight, sc, k, v)
+        print(f"[poses] window vs full, placement {k}: df {np.abs(a[0] - b[0]).max():.3e} "
+              f"(max {np.abs(b[0]).max():.3e}), pose grad {a[1]} vs {b[1]}")
+        for x, y in zip(a, b):
+            assert np.allclose(x, y, rtol=1e-4, atol=1e-5 * np.abs(y).max())
+        assert np.array_equal(c[0], b[0])
+        for x, y in zip(c[1:], b[1:]):
+            assert np.allclose(x, y, rtol=1e-5, atol=1e-6 * np.abs(y).max())
+    assert win.window_fallbacks == 0 and tight.window_fallbacks == 2
