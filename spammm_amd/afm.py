@@ -525,6 +525,12 @@ class AFMulator:
         self.fdbm_spec = D.axis_aligned_spec(origin, L, (nx, ny, nz))
         self.fdbm_origin, self.fdbm_step, self.fdbm_shape = origin, float(step), (nx, ny, nz)
 
+    def setup_fdbm_grid_from_img(self, img_F, shape, origin, step):
+        """Device-side copy of a force grid (nz,ny,nx,4) (e.g. the make_forcefield grid) into the FDBM
+        grid for scan_fdbm (setup_fdbm_grid_from_img, spammm/SPM/AFM.py:413-429)."""
+        D._f32dev(img_F, "img_F", 4)
+        self.install_fdbm_grid(img_F.clone(), origin, step, shape)
+
     def compute_gradient_cl(self, E_field, step, bAlloc=True):
         """(nx,ny,nz,4) = (-grad E, E) of a host (nx,ny,nz) energy by periodic central differences
         on the GPU (spammm/SPM/AFM.py:2740-2790, gradient_central_diff kernels/AFM.cl:1151-1196)."""
