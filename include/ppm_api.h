@@ -238,6 +238,10 @@ int ppm_ewald_num_g(int n_harm);
 /* Bytes of the prepared-system buffer of ppm_ewald_prepare (1 <= n_layers, 1 <= n_harm <= 64). */
 size_t ppm_ewald_system_bytes(int n_layers, int n_harm);
 
+/* Byte offset, inside the prepared-system buffer, of the int count of G kept for evaluation
+ * (written by ppm_ewald_prepare; read it with a device-to-host copy). */
+size_t ppm_ewald_kept_count_offset(int n_layers, int n_harm);
+
 /* Prepares a slab (SurfaceEwaldCL.prepare_system, SurfaceEwald.py:165-234, whose
  * compute_ewald_coefficients kernel is kernels/surface.cl:1036-1086): per-layer structure factors
  * W_l(G) = (2 pi/(A|G|)) sum_{i in l} q_i e^{-iG.rho_i} in FP64, layer heights and charges, and the

@@ -53,7 +53,7 @@ EXPORTS = (
     "ppm_relax_strokes_tilted_f64", "ppm_fdbm_workspace_size", "ppm_fdbm_fft_combine", "ppm_fdbm_pauli_scale",
     "ppm_fdbm_pad_roll", "ppm_fdbm_flip3", "ppm_fdbm_dispersion", "ppm_fdbm_compose_gradient",
     "ppm_ewald_num_g", "ppm_ewald_system_bytes", "ppm_ewald_prepare", "ppm_ewald_eval", "ppm_ewald_brute",
-    "ppm_ewald_cluster", "ppm_ewald_grid_workspace_size", "ppm_ewald_tip_grid",
+    "ppm_ewald_cluster", "ppm_ewald_grid_workspace_size", "ppm_ewald_tip_grid", "ppm_ewald_kept_count_offset",
 )
 
 _lib = None
@@ -113,6 +113,8 @@ def load():
     L.ppm_ewald_num_g.argtypes = [i]
     L.ppm_ewald_system_bytes.restype = sz
     L.ppm_ewald_system_bytes.argtypes = [i, i]
+    L.ppm_ewald_kept_count_offset.restype = sz
+    L.ppm_ewald_kept_count_offset.argtypes = [i, i]
     L.ppm_ewald_prepare.restype = i
     L.ppm_ewald_prepare.argtypes = [vp, vp, i, P(D2), P(D2), i, ctypes.c_double, vp, sz, vp]
     L.ppm_ewald_eval.restype = i

@@ -257,6 +257,11 @@ size_t ppm_ewald_system_bytes(int n_layers, int n_harm) {
     return ewald_system_bytes(n_layers, n_harm);
 }
 
+size_t ppm_ewald_kept_count_offset(int n_layers, int n_harm) {
+    if (check_ewald_sys("ppm_ewald_kept_count_offset", n_layers, n_harm)) return 0;
+    return ewald_kept_count_offset(n_layers, n_harm);
+}
+
 int ppm_ewald_prepare(const float* ions, const int* layer_off, int n_layers, const double a[2], const double b[2],
                       int n_harm, double drop_tol, void* system, size_t system_bytes, void* stream) {
     const char* fn = "ppm_ewald_prepare";

@@ -65,6 +65,7 @@ int launch_compose_gradient(const float* e1, const float* e2, const float* e3, c
                             float* grid, hipStream_t stream);
 int ewald_num_g(int n_harm);
 size_t ewald_system_bytes(int n_layers, int n_harm);
+size_t ewald_kept_count_offset(int n_layers, int n_harm);
 size_t ewald_grid_workspace(int n_harm, int nz);
 int launch_ewald_prepare(const float* ions, const int* layer_off, int n_layers, const double a[2], const double b[2],
                          int n_harm, double drop_tol, void* system, hipStream_t stream);

@@ -412,6 +412,11 @@ int ewald_num_g(int n_harm) { return (2 * n_harm + 1) * (2 * n_harm + 1) - 1; }
 
 size_t ewald_system_bytes(int n_layers, int n_harm) { return sys_bytes(n_layers, ewald_num_g(n_harm)); }
 
+size_t ewald_kept_count_offset(int n_layers, int n_harm) {
+    const SysView v = sys_view(nullptr, n_layers, ewald_num_g(n_harm));
+    return (size_t)(reinterpret_cast<const char*>(v.cnt) - static_cast<const char*>(nullptr));
+}
+
 size_t ewald_grid_workspace(int n_harm, int nz) {
     return align256((size_t)nz * ewald_num_g(n_harm) * sizeof(float4)) + align256((size_t)nz * sizeof(float2));
 }

@@ -10,8 +10,8 @@ Coulomb part of a substrate force field.
 
 Differences from the reference, all in accuracy: structure factors and every point evaluation
 accumulate in FP64 (the reference's kernels run in FP32, kernels/surface.cl:71); ions sharing
-a height are summed into one layer term per G; G whose structure factor vanishes in every layer
-(drop_tol, default 0 = exactly zero in FP64 up to rounding) are skipped.
+a height are summed into one layer term per G; G whose structure factor is below drop_tol of the
+largest in every layer (default 1e-12: those that vanish up to FP64 rounding) are skipped.
 """
 from __future__ import annotations
 
@@ -145,9 +145,7 @@ class SurfaceEwald:
         """Number of G kept for evaluation (reads the count from the device)."""
         self._need()
         import torch
-        L = _lib.load()
-        ng = L.ppm_ewald_num_g(self.n_harm)
-        off = (self.n_layers * ng * 16 + 255) // 256 * 256 + 2 * ((self.n_layers * 8 + 255) // 256 * 256)
+        off = int(_lib.load().ppm_ewald_kept_count_offset(self.n_layers, self.n_harm))
         return int(self.system[off:off + 4].view(torch.int32).item())
 
     def eval_points(self, P, mode=_lib.PPM_EWALD_FULL):
