@@ -9,10 +9,11 @@ field (no |E|>100 rescale) is projected once (its Coulomb part either
 as the direct sum over npbc x,y images or, sub_coulomb = "ewald", as the 2D Ewald series of
 the periodic slab, spammm_amd.ewald); every placement adds
 the molecule's pair sum to it and applies the rescale to the total (ppm_grid_project_add), then
-relaxes the scan and applies the df operator.  The backward recomputes the probe positions
+relaxes the scan and applies the df operator.  The backward takes the probe positions the
+forward kept (16 B per state; with keep_states = False it recomputes the relaxation instead)
 and, unless the shared grid buffer still holds this placement's grid (no other placement's
-forward ran in between), the grid (one grid buffer is shared by all placements),
-runs the implicit-function relax adjoint, the grid->atom adjoint of the molecule's atoms over
+forward ran in between), recomputes the grid (one grid buffer is shared by all placements);
+relaxation trajectories are never stored.  It then runs the implicit-function relax adjoint, the grid->atom adjoint of the molecule's atoms over
 the substrate base (ppm_grid_adjoint_add) and the rigid-motion chain rule:
     x_i = R(theta) (x0_i - c) + c + s
     dL/ds = sum_i dL/dx_i,   dL/dtheta = sum_i dL/dx_i . R'(theta) (x0_i - c).
@@ -67,6 +68,7 @@ class Scene:
     window_fallbacks: int = 0   # placements recomputed on the full grid
     window_faces: tuple = None  # (lo, hi) bool[3]: window faces that are faces of the full grid
     full: object = None         # (base, grid, grid_cot, serial) of the full grid, built on first fallback
+    keep_states: bool = True    # the backward reuses the forward's probe positions (else recomputes them)
 
     @property
     def n_scan(self):
@@ -204,15 +206,20 @@ def _serial(scene: Scene, full: bool):
     return scene.full[3] if (full and scene.windowed) else scene.grid_serial
 
 
+def _project(scene: Scene, atoms, full=False):
+    spec, base, grid, _ = scene.buffers(full)
+    D.grid_project(atoms, scene.mol_cpar, _lib.PPM_MODEL_LJ, spec, scene.Qs, scene.QZs, scene.npbc,
+                   scene.lvA, scene.lvB, cap=True, out=grid, base=base)
+    if full and scene.windowed:
+        scene.full[3] += 1
+    else:
+        scene.grid_serial += 1
+
+
 def _forward(scene: Scene, atoms, grid_valid=False, full=False):
     spec, base, grid, _ = scene.buffers(full)
     if not grid_valid:
-        D.grid_project(atoms, scene.mol_cpar, _lib.PPM_MODEL_LJ, spec, scene.Qs, scene.QZs, scene.npbc,
-                       scene.lvA, scene.lvB, cap=True, out=grid, base=base)
-        if full and scene.windowed:
-            scene.full[3] += 1
-        else:
-            scene.grid_serial += 1
+        _project(scene, atoms, full)
     FEs, PPs = D.relax_tilted(grid, spec, scene.pts, scene.nz, scene.tip, scene.max_iter, scene.f2conv,
                               True, scan_ny=int(scene.nxy[1]))
     if scene.polish > 0:
@@ -234,8 +241,9 @@ def _make_function():
             if not scene.inside(PPs):
                 scene.window_fallbacks += 1
                 full = True
-                FEs, _ = _forward(scene, atoms, full=True)
+                FEs, PPs = _forward(scene, atoms, full=True)
             ctx.full = full
+            ctx.PPs = PPs if scene.keep_states else None
             ctx.grid_serial = _serial(scene, full)
             op = DF.z_operator(scene.df_kind, scene.nz, scene.dz, pose.device, scene.amp, scene.f0, scene.k_cant)
             df = op.apply(FEs, (scene.nz * 4, 4, 0), scene.n_scan, 1, offset=2)
@@ -252,10 +260,17 @@ def _make_function():
             ctr = torch.as_tensor(scene.center, device=dev)
             xyz = place(xyz0, ctr, pose)
             atoms = torch.cat([xyz.float(), q.float()[:, None]], 1).contiguous()
-            # recompute the probe positions, and the grid if another placement has overwritten it
+            # the grid if another placement has overwritten it; the probe positions unless kept
             full = ctx.full
             spec, base, grid, grid_cot = scene.buffers(full)
-            _, PPs = _forward(scene, atoms, grid_valid=_serial(scene, full) == ctx.grid_serial, full=full)
+            grid_valid = _serial(scene, full) == ctx.grid_serial
+            if ctx.PPs is not None:
+                PPs = ctx.PPs
+                ctx.PPs = None
+                if not grid_valid:
+                    _project(scene, atoms, full)
+            else:
+                _, PPs = _forward(scene, atoms, grid_valid=grid_valid, full=full)
             g = gdf.detach().float().contiguous().reshape(scene.n_scan, scene.nz)
             opT = DF.z_operator(scene.df_kind, scene.nz, scene.dz, dev, scene.amp, scene.f0, scene.k_cant,
                                 transpose=True)

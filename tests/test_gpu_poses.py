@@ -251,3 +251,28 @@ def test_scan_window_equals_full_grid_and_falls_back():
         for x, y in zip(c[1:], b[1:]):
             assert np.allclose(x, y, rtol=1e-5, atol=1e-6 * np.abs(y).max())
     assert win.window_fallbacks == 0 and tight.window_fallbacks == 2
+
+
+def test_kept_states_equal_recomputed_states():
+    """The backward on the forward's kept probe positions gives the gradients of the backward
+    that recomputes the relaxation (same positions bit for bit; the relax adjoint's atomic
+    scatter order is the only difference), also after another placement overwrote the grid."""
+    sc = W.small_substrate_scene()
+    kept, _ = make_scene(sc)
+    rec, _ = make_scene(sc, keep_states=False)
+    v = torch.as_tensor(np.random.default_rng(9).standard_normal((*sc["nxy"], sc["nz"])), device=kept.pts.device)
+    for k in (1, 3):
+        a = _df_and_grads(kept, sc, k, v)
+        b = _df_and_grads(rec, sc, k, v)
+        np.testing.assert_array_equal(a[0], b[0])
+        for x, y in zip(a[1:], b[1:]):
+            assert np.allclose(x, y, rtol=1e-5, atol=1e-6 * np.abs(y).max())
+    dev = kept.pts.device
+    pose = torch.tensor(sc["poses"][0], dtype=torch.float64, device=dev, requires_grad=True)
+    q = torch.tensor(sc["mol_atoms"][:, 3], dtype=torch.float64, device=dev, requires_grad=True)
+    df = PS.pose_df(pose, q, kept)
+    with torch.no_grad():
+        PS.pose_df(torch.tensor(sc["poses"][2], dtype=torch.float64, device=dev), q.detach(), kept)
+    (df.double() * v).sum().backward()
+    ref = _df_and_grads(rec, sc, 0, v)
+    assert np.allclose(pose.grad.cpu().numpy(), ref[1], rtol=1e-5, atol=1e-6 * np.abs(ref[1]).max())
