@@ -76,20 +76,26 @@ __device__ __forceinline__ void pair_fe_raw(float dx, float dy, float dz, float 
     }
 }
 
-// contributions of one pair to the 6 per-atom accumulators (x,y,z,q,p0,p1)
-template <int MODEL>
+// Contributions of one pair to the 6 per-atom accumulators: t[0..2] += d(g.(F,E))/d(voxel - atom)
+// (the caller subtracts them: the atom moves opposite to the voxel offset), t[3] += d/dq,
+// t[4], t[5] += d/d(c6, c12) (LJ) or d/d(R0, E0) (Morse).  OFFS: the tip has charge sites at
+// nonzero z offsets.
+template <int MODEL, bool OFFS>
 __device__ __forceinline__ void pair_adjoint(float dx, float dy, float dz, float dxy2, float q, float p0, float p1,
-                                             float K, const GridArgs& a, float4 g, float acc[6]) {
+                                             float K, const GridArgs& a, float4 g, float t[6]) {
+#pragma clang fp contract(fast)
     const float s = fmaf(dz, dz, dxy2);
     const float ir = rsq(s), ir2 = ir * ir;
-    const float gd = g.x * dx + g.y * dy + g.z * dz;
-    float fr, dfr, dp0, dp1;
+    const float gd = fmaf(g.z, dz, fmaf(g.y, dy, g.x * dx));
+    const float gdi = gd * ir2;
+    float fr, dfr;
     if (MODEL == PPM_MODEL_LJ) {
         const float ir6 = ir2 * ir2 * ir2, ir8 = ir6 * ir2;
-        fr = (12.f * p1 * ir6 - 6.f * p0) * ir8;
-        dfr = (-84.f * p1 * ir6 + 24.f * p0) * ir8 * ir2;
-        dp0 = -ir6 * (6.f * gd * ir2 + g.w);              // d/dc6
-        dp1 = ir6 * ir6 * (12.f * gd * ir2 + g.w);        // d/dc12
+        const float t12 = p1 * ir6;
+        fr = fmaf(12.f, t12, -6.f * p0) * ir8;
+        dfr = fmaf(-84.f, t12, 24.f * p0) * (ir8 * ir2);
+        t[4] = fmaf(-ir6, fmaf(6.f, gdi, g.w), t[4]);                 // d/dc6
+        t[5] = fmaf(ir6 * ir6, fmaf(12.f, gdi, g.w), t[5]);           // d/dc12
     } else {
         const float r = s * ir;
         const float ex = __expf(K * (r - p0));
@@ -100,44 +106,43 @@ __device__ __forceinline__ void pair_adjoint(float dx, float dy, float dz, float
         dfr = dfr_dr * 0.5f * ir;
         const float dE_dR0 = -2.f * K * A * (ex - 1.f);
         const float dfr_dR0 = 2.f * K * K * A * (2.f * ex - 1.f) * ir;
-        dp0 = gd * dfr_dR0 + g.w * dE_dR0;                                   // d/dR0
-        dp1 = gd * (-2.f * K * ex * (ex - 1.f) * ir) + g.w * ex * (ex - 2.f); // d/dE0
+        t[4] += gd * dfr_dR0 + g.w * dE_dR0;                                   // d/dR0
+        t[5] += gd * (-2.f * K * ex * (ex - 1.f) * ir) + g.w * ex * (ex - 2.f); // d/dE0
     }
-    float cq = 0.f;
     if (a.q0 != 0.f) {
         const float ec = q * a.q0 * ir;          // Coulomb at z offset 0 shares d with the pair law
         const float frc = ec * ir2;
         fr += frc;
-        dfr += -1.5f * frc * ir2;
-        cq += a.q0 * ir * (gd * ir2 + g.w);
+        dfr = fmaf(-1.5f * frc, ir2, dfr);
+        t[3] = fmaf(a.q0 * ir, gdi + g.w, t[3]);
     }
-    const float coef = 2.f * gd * dfr - g.w * fr;
-    acc[0] -= fr * g.x + coef * dx;
-    acc[1] -= fr * g.y + coef * dy;
-    acc[2] -= fr * g.z + coef * dz;
-    acc[4] += dp0;
-    acc[5] += dp1;
-    for (int o = 0; o < a.noff; ++o) {
-        const float dzo = dz + a.zoff[o];
-        const float so = fmaf(dzo, dzo, dxy2);
-        const float iro = rsq(so), iro2 = iro * iro;
-        const float gdo = g.x * dx + g.y * dy + g.z * dzo;
-        const float eo = q * a.qoff[o] * iro;
-        const float fro = eo * iro2;
-        const float co = 2.f * gdo * (-1.5f * fro * iro2) - g.w * fro;
-        acc[0] -= fro * g.x + co * dx;
-        acc[1] -= fro * g.y + co * dy;
-        acc[2] -= fro * g.z + co * dzo;
-        cq += a.qoff[o] * iro * (gdo * iro2 + g.w);
+    const float coef = fmaf(2.f * gd, dfr, -g.w * fr);
+    t[0] = fmaf(fr, g.x, fmaf(coef, dx, t[0]));
+    t[1] = fmaf(fr, g.y, fmaf(coef, dy, t[1]));
+    t[2] = fmaf(fr, g.z, fmaf(coef, dz, t[2]));
+    if (OFFS) {
+        for (int o = 0; o < a.noff; ++o) {
+            const float dzo = dz + a.zoff[o];
+            const float so = fmaf(dzo, dzo, dxy2);
+            const float iro = rsq(so), iro2 = iro * iro;
+            const float gdo = fmaf(g.z, dzo, fmaf(g.y, dy, g.x * dx));
+            const float eo = q * a.qoff[o] * iro;
+            const float fro = eo * iro2;
+            const float co = 2.f * gdo * (-1.5f * fro * iro2) - g.w * fro;
+            t[0] = fmaf(fro, g.x, fmaf(co, dx, t[0]));
+            t[1] = fmaf(fro, g.y, fmaf(co, dy, t[1]));
+            t[2] = fmaf(fro, g.z, fmaf(co, dzo, t[2]));
+            t[3] = fmaf(a.qoff[o] * iro, fmaf(gdo, iro2, g.w), t[3]);
+        }
     }
-    acc[3] += cq;
 }
 
-template <int MODEL, int V, bool DCZ>
+template <int MODEL, int V, bool DCZ, bool OFFS>
 __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__ atoms, const float* __restrict__ cpar,
                                                       GridArgs a, const float4* __restrict__ grid,
                                                       const float4* __restrict__ gcot, const float4* __restrict__ base,
                                                       double* __restrict__ partials, int* __restrict__ active) {
+#pragma clang fp contract(fast)
     __shared__ double red[4][6];
     const int bid = blockIdx.y * gridDim.x + blockIdx.x;
     const int col = blockIdx.x * 256 + threadIdx.x;
@@ -242,11 +247,13 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
                 for (int k = 0; k < V; ++k) {
                     const int kk = DCZ ? 0 : k;
                     const float dx = px[kk] - sx, dy = py[kk] - sy;
-                    pair_adjoint<MODEL>(dx, dy, pz[k] - sz, fmaf(dx, dx, fmaf(dy, dy, kR2SAFE)), at.w, p0, p1, K,
-                                        a, g[k], t);
+                    pair_adjoint<MODEL, OFFS>(dx, dy, pz[k] - sz, fmaf(dx, dx, fmaf(dy, dy, kR2SAFE)), at.w, p0,
+                                              p1, K, a, g[k], t);
                 }
 #pragma unroll
-                for (int c = 0; c < 6; ++c) acc[c] += (double)t[c];
+                for (int c = 0; c < 3; ++c) acc[c] -= (double)t[c];
+#pragma unroll
+                for (int c = 3; c < 6; ++c) acc[c] += (double)t[c];
             }
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
@@ -376,21 +383,18 @@ int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na
     const float4* G = reinterpret_cast<const float4*>(grid);
     const float4* gc = reinterpret_cast<const float4*>(grid_cot);
     const float4* B = reinterpret_cast<const float4*>(base);
+    const bool offs = a.noff > 0;
+#define PPM_ADJ(M, VV, DCZ, OF)                                                                                  \
+    hipLaunchKernelGGL((k_grid_adjoint<M, VV, DCZ, OF>), blocks, dim3(256), 0, stream, at, cpar, a, G, gc, B, \
+                       partials, active)
     if (model == PPM_MODEL_LJ) {
-        if (V == 8)
-            hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_LJ, 8, true>), blocks, dim3(256), 0, stream, at, cpar, a, G,
-                               gc, B, partials, active);
-        else
-            hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_LJ, 4, false>), blocks, dim3(256), 0, stream, at, cpar, a,
-                               G, gc, B, partials, active);
+        if (V == 8) { if (offs) PPM_ADJ(PPM_MODEL_LJ, 8, true, true); else PPM_ADJ(PPM_MODEL_LJ, 8, true, false); }
+        else { if (offs) PPM_ADJ(PPM_MODEL_LJ, 4, false, true); else PPM_ADJ(PPM_MODEL_LJ, 4, false, false); }
     } else {
-        if (V == 8)
-            hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_MORSE, 8, true>), blocks, dim3(256), 0, stream, at, cpar, a,
-                               G, gc, B, partials, active);
-        else
-            hipLaunchKernelGGL((k_grid_adjoint<PPM_MODEL_MORSE, 4, false>), blocks, dim3(256), 0, stream, at, cpar,
-                               a, G, gc, B, partials, active);
+        if (V == 8) { if (offs) PPM_ADJ(PPM_MODEL_MORSE, 8, true, true); else PPM_ADJ(PPM_MODEL_MORSE, 8, true, false); }
+        else { if (offs) PPM_ADJ(PPM_MODEL_MORSE, 4, false, true); else PPM_ADJ(PPM_MODEL_MORSE, 4, false, false); }
     }
+#undef PPM_ADJ
     if (int r = check_launch("grid_adjoint")) return r;
     hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)(na * 6)), dim3(256), 0, stream, partials, active, (int)nb,
                        na, grad);
