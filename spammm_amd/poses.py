@@ -125,18 +125,19 @@ class Scene:
             self.full = [b, torch.empty_like(b), torch.empty_like(b), 0]
         return self.spec, self.full[0], self.full[1], self.full[2]
 
-    def inside(self, PPs, guard=0.5) -> bool:
+    def inside(self, PPs, guard=0.5, guard_shared=2) -> bool:
         """True if every probe position of PPs samples the window exactly as it samples the full
         grid: at least `guard` A inside each window face that cuts the full grid (so the FIRE
-        steps between stored positions stay on the window's voxels), and without wrapping at a
-        face the window shares with the full grid."""
+        steps between stored positions stay on the window's voxels), and `guard_shared` voxels
+        inside a face the window shares with the full grid (where both would wrap, each to its
+        own opposite face)."""
         if not self.windowed:
             return True
         sp = self.spec_run
         d = np.array([sp.dA[0], sp.dB[1], sp.dC[2]], np.float64)
         shared_lo, shared_hi = self.window_faces
-        g_lo = np.where(shared_lo, 0.0, np.ceil(guard / d))
-        g_hi = np.where(shared_hi, 0.0, np.ceil(guard / d))
+        g_lo = np.where(shared_lo, float(guard_shared), np.ceil(guard / d))
+        g_hi = np.where(shared_hi, float(guard_shared), np.ceil(guard / d))
         import torch
         mn, mx = torch.aminmax(PPs.reshape(-1, 4), dim=0)
         lo = mn[:3].cpu().numpy().astype(np.float64)
