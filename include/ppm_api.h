@@ -252,9 +252,11 @@ int ppm_ewald_prepare(const float* ions, const int* layer_off, int n_layers, con
                       void* stream);
 
 /* phi (np floats, eV/e) at points (np,4) = (x,y,z,-) (SurfaceEwaldCL.eval_full / eval_vacuum,
- * SurfaceEwald.py:236-347), FP64 accumulation; mode PPM_EWALD_FULL or PPM_EWALD_VACUUM. */
-int ppm_ewald_eval(int mode, const float* points, int np, const void* system, int n_layers, int n_harm,
-                   const double a[2], const double b[2], float* phi, void* stream);
+ * SurfaceEwald.py:236-347), FP64 accumulation; mode PPM_EWALD_FULL or PPM_EWALD_VACUUM.  system:
+ * prepared by ppm_ewald_prepare with the same n_layers, n_harm and lattice (system_bytes is checked
+ * against ppm_ewald_system_bytes, so mismatched sizes are rejected instead of read past). */
+int ppm_ewald_eval(int mode, const float* points, int np, const void* system, size_t system_bytes, int n_layers,
+                   int n_harm, const double a[2], const double b[2], float* phi, void* stream);
 
 /* Direct sum over the images n a + m b: square = 0 takes n^2 + m^2 <= n_rep^2
  * (SurfaceEwaldCL.eval_brute, SurfaceEwald.py:349-405; eval_potential_brute
@@ -275,9 +277,9 @@ size_t ppm_ewald_grid_workspace_size(int n_harm, int nz);
  * Requires dA, dB in the xy plane and dC along z.  Same voxel layout and tip-charge convention
  * as ppm_grid_project (the Coulomb sites of evalLJC_QZs_toImg, kernels/AFM.cl:916-991), with the
  * slab's periodic sum done by the Ewald series instead of a finite image sum; no energy cap. */
-int ppm_ewald_tip_grid(const void* system, int n_layers, int n_harm, const double a[2], const double b[2],
-                       const PpmGridSpec* spec, const float Qs[4], const float QZs[4], int accumulate,
-                       void* workspace, size_t workspace_bytes, float* grid, void* stream);
+int ppm_ewald_tip_grid(const void* system, size_t system_bytes, int n_layers, int n_harm, const double a[2],
+                       const double b[2], const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
+                       int accumulate, void* workspace, size_t workspace_bytes, float* grid, void* stream);
 
 /* Relaxed scan with a tilted tip frame.
  * Replaces relaxStrokesTilted (kernels/AFM.cl:478-562) as launched by

@@ -118,7 +118,7 @@ def load():
     L.ppm_ewald_prepare.restype = i
     L.ppm_ewald_prepare.argtypes = [vp, vp, i, P(D2), P(D2), i, ctypes.c_double, vp, sz, vp]
     L.ppm_ewald_eval.restype = i
-    L.ppm_ewald_eval.argtypes = [i, vp, i, vp, i, i, P(D2), P(D2), vp, vp]
+    L.ppm_ewald_eval.argtypes = [i, vp, i, vp, sz, i, i, P(D2), P(D2), vp, vp]
     L.ppm_ewald_brute.restype = i
     L.ppm_ewald_brute.argtypes = [vp, i, vp, i, P(D2), P(D2), i, i, vp, vp]
     L.ppm_ewald_cluster.restype = i
@@ -126,7 +126,7 @@ def load():
     L.ppm_ewald_grid_workspace_size.restype = sz
     L.ppm_ewald_grid_workspace_size.argtypes = [i, i]
     L.ppm_ewald_tip_grid.restype = i
-    L.ppm_ewald_tip_grid.argtypes = [vp, i, i, P(D2), P(D2), P(PpmGridSpec), _F4, _F4, i, vp, sz, vp, vp]
+    L.ppm_ewald_tip_grid.argtypes = [vp, sz, i, i, P(D2), P(D2), P(PpmGridSpec), _F4, _F4, i, vp, sz, vp, vp]
     L.ppm_grid_adjoint_add.restype = i
     L.ppm_grid_adjoint_add.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
                                        P(ctypes.c_float * 3), i, vp, vp, vp, vp, ctypes.c_size_t, vp, vp]

@@ -274,8 +274,8 @@ int ppm_ewald_prepare(const float* ions, const int* layer_off, int n_layers, con
     return launch_ewald_prepare(ions, layer_off, n_layers, a, b, n_harm, drop_tol, system, (hipStream_t)stream);
 }
 
-int ppm_ewald_eval(int mode, const float* points, int np, const void* system, int n_layers, int n_harm,
-                   const double a[2], const double b[2], float* phi, void* stream) {
+int ppm_ewald_eval(int mode, const float* points, int np, const void* system, size_t system_bytes, int n_layers,
+                   int n_harm, const double a[2], const double b[2], float* phi, void* stream) {
     const char* fn = "ppm_ewald_eval";
     g_err[0] = 0;
     if (mode != PPM_EWALD_FULL && mode != PPM_EWALD_VACUUM) { set_error("%s: unknown mode %d", fn, mode); return PPM_ERR_INVALID_ARG; }
@@ -283,6 +283,7 @@ int ppm_ewald_eval(int mode, const float* points, int np, const void* system, in
     if (int r = check_lattice(fn, a, b)) return r;
     if (np < 0) { set_error("%s: np=%d < 0", fn, np); return PPM_ERR_INVALID_ARG; }
     if (np > 0 && (!points || !system || !phi)) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
+    if (system_bytes < ewald_system_bytes(n_layers, n_harm)) { set_error("%s: system buffer too small for n_layers, n_harm", fn); return PPM_ERR_INVALID_ARG; }
     return launch_ewald_points(mode, points, np, system, n_layers, n_harm, a, b, phi, (hipStream_t)stream);
 }
 
@@ -309,9 +310,9 @@ size_t ppm_ewald_grid_workspace_size(int n_harm, int nz) {
     return ewald_grid_workspace(n_harm, nz);
 }
 
-int ppm_ewald_tip_grid(const void* system, int n_layers, int n_harm, const double a[2], const double b[2],
-                       const PpmGridSpec* spec, const float Qs[4], const float QZs[4], int accumulate, void* workspace,
-                       size_t workspace_bytes, float* grid, void* stream) {
+int ppm_ewald_tip_grid(const void* system, size_t system_bytes, int n_layers, int n_harm, const double a[2],
+                       const double b[2], const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
+                       int accumulate, void* workspace, size_t workspace_bytes, float* grid, void* stream) {
     const char* fn = "ppm_ewald_tip_grid";
     g_err[0] = 0;
     if (int r = check_ewald_sys(fn, n_layers, n_harm)) return r;
@@ -325,6 +326,7 @@ int ppm_ewald_tip_grid(const void* system, int n_layers, int n_harm, const doubl
     }
     if (spec->nz > 65535 * 8) { set_error("%s: nz=%d too large", fn, spec->nz); return PPM_ERR_INVALID_ARG; }
     if (workspace_bytes < ewald_grid_workspace(n_harm, spec->nz)) { set_error("%s: workspace too small", fn); return PPM_ERR_INVALID_ARG; }
+    if (system_bytes < ewald_system_bytes(n_layers, n_harm)) { set_error("%s: system buffer too small for n_layers, n_harm", fn); return PPM_ERR_INVALID_ARG; }
     return launch_ewald_tip_grid(system, n_layers, n_harm, a, b, *spec, Qs, QZs, accumulate, workspace, grid,
                                  (hipStream_t)stream);
 }

@@ -209,7 +209,7 @@ __global__ __launch_bounds__(256) void k_plane_tables(SysView s, EwGeom a, int n
         return;
     }
     if (c >= min(s.cnt[0], a.n_g)) return;
-    const int g = s.list[c];
+    const int g = min(max(s.list[c], 0), a.n_g - 1);
     int h, k;
     g_of(a.n_harm, g, h, k);
     double Gx, Gy, Gn;
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(256) void k_ewald_points(const float4* __restrict__
         for (int l = 0; l < a.n_layers; ++l) acc -= (2.0 * kPi / a.area) * s.Ql[l] * fabs(z - s.zl[l]);
     const int cnt = min(s.cnt[0], a.n_g);
     for (int c = 0; c < cnt; ++c) {
-        const int g = s.list[c];
+        const int g = min(max(s.list[c], 0), a.n_g - 1);
         int h, k;
         g_of(a.n_harm, g, h, k);
         double Gx, Gy, Gn;

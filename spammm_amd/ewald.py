@@ -155,7 +155,7 @@ class SurfaceEwald:
         P = D._f32dev(P, "points", 4)
         phi = torch.empty(P.shape[0], dtype=torch.float32, device=P.device)
         _check(_lib.load().ppm_ewald_eval(int(mode), P.data_ptr(), int(P.shape[0]), self.system.data_ptr(),
-                                          self.n_layers, self.n_harm, ctypes.byref(_d2(self.a_vec)),
+                                          int(self.system.numel()), self.n_layers, self.n_harm, ctypes.byref(_d2(self.a_vec)),
                                           ctypes.byref(_d2(self.b_vec)), phi.data_ptr(), D._stream(P)),
                "ppm_ewald_eval")
         return phi
@@ -213,7 +213,7 @@ class SurfaceEwald:
         wsb = L.ppm_ewald_grid_workspace_size(self.n_harm, nz)
         ws = torch.empty(int(wsb), dtype=torch.uint8, device=self.dev)
         cs = spec.c()
-        _check(L.ppm_ewald_tip_grid(self.system.data_ptr(), self.n_layers, self.n_harm,
+        _check(L.ppm_ewald_tip_grid(self.system.data_ptr(), int(self.system.numel()), self.n_layers, self.n_harm,
                                     ctypes.byref(_d2(self.a_vec)), ctypes.byref(_d2(self.b_vec)), ctypes.byref(cs),
                                     _lib.f4(Qs), _lib.f4(QZs), int(bool(accumulate)), ws.data_ptr(), int(wsb),
                                     out.data_ptr(), D._stream(out)), "ppm_ewald_tip_grid")

@@ -179,11 +179,13 @@ def test_ewald_sizes_and_validation_on_host():
     assert rc == _lib.PPM_ERR_INVALID_ARG and b"degenerate" in L.ppm_last_error()
     rc = L.ppm_ewald_prepare(1, 1, 2, ctypes.byref(a), ctypes.byref(b), 3, 0.0, 1, 16, None)
     assert rc == _lib.PPM_ERR_INVALID_ARG and b"too small" in L.ppm_last_error()
-    rc = L.ppm_ewald_eval(7, 1, 1, 1, 2, 3, ctypes.byref(a), ctypes.byref(b), 1, None)
+    rc = L.ppm_ewald_eval(7, 1, 1, 1, 1 << 20, 2, 3, ctypes.byref(a), ctypes.byref(b), 1, None)
     assert rc == _lib.PPM_ERR_INVALID_ARG and b"mode" in L.ppm_last_error()
+    rc = L.ppm_ewald_eval(0, 1, 1, 1, 64, 2, 3, ctypes.byref(a), ctypes.byref(b), 1, None)
+    assert rc == _lib.PPM_ERR_INVALID_ARG and b"system buffer too small" in L.ppm_last_error()
     s = _lib.PpmGridSpec()
     s.nx, s.ny, s.nz = 4, 4, 4
     s.dA, s.dB, s.dC = _lib.f4((1, 0, 0)), _lib.f4((0, 1, 0)), _lib.f4((0.2, 0, 1))
-    rc = L.ppm_ewald_tip_grid(1, 2, 3, ctypes.byref(a), ctypes.byref(b), ctypes.byref(s), _lib.f4((1,)),
+    rc = L.ppm_ewald_tip_grid(1, 1 << 20, 2, 3, ctypes.byref(a), ctypes.byref(b), ctypes.byref(s), _lib.f4((1,)),
                               _lib.f4((0,)), 0, 1, 1 << 20, 1, None)
     assert rc == _lib.PPM_ERR_INVALID_ARG and b"dC along z" in L.ppm_last_error()
