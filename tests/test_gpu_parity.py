@@ -338,3 +338,60 @@ def test_errors_are_raised_not_swallowed():
     bad.n = (0, 4, 4)
     with pytest.raises(ValueError):
         D.grid_project(torch.as_tensor(w.atoms, device=d), torch.as_tensor(w.cLJ, device=d), 0, bad, w.Qs, w.QZs)
+
+
+def _abs_term_sums(w, P):
+    """Per point, the sum over atoms and images of |F| and |E| of the individual LJ and Coulomb
+    terms (FP64) — the scale of an FP32 sum's rounding where terms cancel."""
+    S = np.zeros(len(P))
+    qs = float(w.Qs[0]) * 14.3996448915
+    for kx in range(-w.npbc[0], w.npbc[0] + 1):
+        for ky in range(-w.npbc[1], w.npbc[1] + 1):
+            sh = kx * np.asarray(w.lvA, np.float64) + ky * np.asarray(w.lvB, np.float64)
+            A = w.atoms[:, :3].astype(np.float64) + sh
+            d = P[:, None, :] - A[None, :, :]
+            r2 = (d * d).sum(2) + 1e-4
+            ir2 = 1.0 / r2
+            ir6 = ir2 ** 3
+            c6, c12 = w.cLJ[:, 0].astype(np.float64), w.cLJ[:, 1].astype(np.float64)
+            ec = np.abs(w.atoms[:, 3].astype(np.float64) * qs) * np.sqrt(ir2)
+            S += (np.abs((c12 * ir6 - c6) * ir6) + np.abs((12 * c12 * ir6 - 6 * c6) * ir6 * ir2) * np.sqrt(r2)
+                  + ec + ec * np.sqrt(ir2)).sum(1)
+    return S
+
+
+def test_full_flake_grid_sampled_and_scan_subsample_bitexact():
+    """The 200-atom flake at its full size (300x300x240 grid, 256x256x60 scan): 2000 sampled voxels
+    against the FP64 oracle within 1e-5 |(F,E)| + 4e-6 sum|terms| + 1e-6 (the flake's ~1800 Coulomb
+    terms per voxel cancel to ~1e-2 of their magnitudes above the plane, where an FP32 sum's
+    rounding follows the terms, not the net value); 128 pixels of the relaxed scan bit-exact
+    against the FP32 oracle on the same grid."""
+    w = W.flake()
+    G, sp = grid_gpu(w)
+    Gh = G.cpu().numpy().reshape(-1, 4)
+    assert np.isfinite(Gh).all()
+    idx = np.random.default_rng(11).choice(Gh.shape[0], 2000, replace=False)
+    Go = O.grid_voxels(0, w.atoms, w.cLJ, w.n, sp.p0, sp.dA, sp.dB, sp.dC, w.Qs, w.QZs, idx, w.npbc, w.lvA,
+                       w.lvB).astype(np.float64)
+    nx, ny, nz = w.n
+    ic, ib, ia = np.unravel_index(idx, (nz, ny, nx))
+    P = (np.asarray(sp.p0, np.float64)[None, :] + np.outer(ia, sp.dA) + np.outer(ib, sp.dB) + np.outer(ic, sp.dC))
+    S = _abs_term_sums(w, P)
+    scale = np.minimum(1.0, 100.0 / np.maximum(np.abs(Go[:, 3]), 1e-300))     # capped voxels
+    bound = 1e-5 * np.linalg.norm(Go, axis=1) + 4e-6 * S * scale + 1e-6
+    err = np.abs(Gh[idx].astype(np.float64) - Go).max(1)
+    print(f"[full] flake grid: worst |d|/bound {(err / bound).max():.3e}", flush=True)
+    assert (err <= bound).all()
+    d = dev()
+    pts = D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy)
+    FEs, PPs = D.relax_tilted(G, sp, torch.as_tensor(pts, device=d), w.nz, tip_of(w), w.max_iter, w.f2conv,
+                              scan_ny=w.nxy[1])
+    FEs, PPs = FEs.cpu().numpy(), PPs.cpu().numpy()
+    assert np.isfinite(FEs).all() and np.isfinite(PPs).all()
+    sel = np.random.default_rng(4).choice(w.n_scan, 128, replace=False)
+    tip = tip_of(w)
+    FEo, PPo = O.relax_tilted(G.cpu().numpy(), sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pts[sel], w.nz, tip.tipA,
+                              tip.tipB, tip.tipC, tip.stiffness, tip.dpos0, tip.relax, tip.surfFF, w.max_iter,
+                              w.f2conv, False, "f32")
+    np.testing.assert_array_equal(PPs[sel], PPo)
+    np.testing.assert_array_equal(FEs[sel], FEo)
