@@ -17,6 +17,7 @@ times the C++ oracle (FP32 mode, OpenMP) on a bounded sample of the same workloa
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -651,6 +652,24 @@ def _diff_case(dev, z0, adam_steps, lr):
     tf, tb = float(np.mean(t_f)), float(np.mean(t_b))
     gx, gq = x.grad.detach(), q.grad.detach()
     status = dict(prob.last_status)
+    # the same forward and adjoint with the grid's voxel sums in FP64 (ppm_grid_project_acc64)
+    prob64 = dataclasses.replace(prob, grid_acc64=True, last_status={})
+    t_f64, t_b64 = [], []
+    for rep in range(4):
+        xa = xyz0.clone().requires_grad_(True)
+        qa = q0.clone().requires_grad_(True)
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        e0.record(stream)
+        dfi = diff.df_image(xa, qa, R0, E0, prob64)
+        loss = 0.5 * ((dfi.double() - target) ** 2).sum()
+        e1.record(stream)
+        loss.backward()
+        e2.record(stream)
+        torch.cuda.synchronize()
+        if rep > 0:
+            t_f64.append(e0.elapsed_time(e1))
+            t_b64.append(e1.elapsed_time(e2))
+    grads = {"fp32_sums": (gx, gq), "acc64": (xa.grad.detach(), qa.grad.detach())}
     # central FD on 32 random parameters (seed 3): positions h = 1e-3 A, charges h = 1e-4 e
     na = xyz0.shape[0]
     sel = np.random.default_rng(3).choice(4 * na, 32, replace=False)
@@ -675,8 +694,9 @@ def _diff_case(dev, z0, adam_steps, lr):
             ad, name = float(gq[i]), f"q[{i}]"
         rows.append((name, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30)))
     rel = np.array([r[3] for r in rows])
-    fd64 = _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, gx, gq, (1e-3, 1e-4))
-    fd64_small_h = _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, gx, gq, (1e-4, 1e-5))
+    fd64 = _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, grads, (1e-3, 1e-4))
+    fd64_small_h = _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, grads, (1e-4, 1e-5))
+    fd64_exact_heights = _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, grads, (1e-3, 1e-4), f32_anchors=False)
     # directional FD along -grad (all positions) at three step sizes
     gn = float(gx.norm())
     dvec = -gx / gn
@@ -701,6 +721,9 @@ def _diff_case(dev, z0, adam_steps, lr):
     return {
         "tip_z": [z0, z0 + (w.nz - 1) * float(w.dtip)], "probe_z_rest": [z0 - 4.0, z0 - 4.0 + (w.nz - 1) * float(w.dtip)],
         "forward_ms": tf, "adjoint_ms": tb, "adjoint_over_forward": tb / tf,
+        "acc64_grid": {"forward_ms": float(np.mean(t_f64)), "adjoint_ms": float(np.mean(t_b64)),
+                       "adjoint_over_forward": float(np.mean(t_b64) / np.mean(t_f64)),
+                       "adjoint_over_fp32_sum_forward": float(np.mean(t_b64) / tf)},
         "pixel_z_steps_per_s_forward": w.n_scan * w.nz / (tf * 1e-3),
         "adjoint_status": status,
         "fd_32_params": {"h": "1e-3 A / 1e-4 e", "within_1e-3": int((rel <= 1e-3).sum()),
@@ -708,6 +731,7 @@ def _diff_case(dev, z0, adam_steps, lr):
                          "rows": [(n, round(a, 6), round(f, 6), float(f"{r:.3g}")) for n, a, f, r in rows]},
         "fd_32_params_fp64_forward": fd64,
         "fd_32_params_fp64_forward_h_1e-4": fd64_small_h,
+        "fd_32_params_fp64_forward_exact_heights": fd64_exact_heights,
         "directional_fd_minus_grad": {"adjoint": -gn, **{f"fd_eps_{k}": v for k, v in dirfd.items()}},
         "adam": {"steps": adam_steps, "lr": lr, "schedule": "constant", "wall_s": t_adam,
                  "loss_first": hist[0] if hist else None, "loss_last": hist[-1] if hist else None,
@@ -716,11 +740,16 @@ def _diff_case(dev, z0, adam_steps, lr):
     }
 
 
-def _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, gx, gq, steps=(1e-3, 1e-4)):
+def _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, grads, steps=(1e-3, 1e-4), f32_anchors=True):
     """Central differences of the same loss with the forward in FP64 on the device (FP64 grid
     projection and FP64 relaxation to |F| < 1e-12 eV/A, FP64 Giessibl df), on the same 32
     parameters as the FP32 check, steps (position A, charge e): the reference-precision
-    comparison of the adjoint."""
+    comparison of the adjoint.  ``grads``: {"fp32_sums": (gx, gq), "acc64": (gx, gq)}, the
+    adjoint's gradients with the FP32-summed and the FP64-summed grid forward.  ``f32_anchors``:
+    the FP64 scan advances its tip anchors in float32 as the FP32 scan does (the same heights, which
+    drift from p0 + iz*dz by up to ~1e-5 A); False: anchors in FP64 (the exact heights)."""
+    gx, gq = grads["fp32_sums"]
+    gx_a, gq_a = grads["acc64"]
     import torch
     from spammm_amd import device as D, df as DF
     dev = xyz0.device
@@ -739,7 +768,7 @@ def _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, gx, gq, steps=(1e-3, 1e-
         buf = D.grid_project_f64(atoms, cpar, 0, spec, prob.Qs, prob.QZs, prob.npbc, prob.lvA, prob.lvB,
                                  cap=prob.cap, out=buf)
         FEs, _ = D.relax_tilted_f64(buf, spec, pts64, nz, prob.tip, max_iter=40000, f2conv=1e-24,
-                                    check_before=True, want_pps=False)
+                                    check_before=True, want_pps=False, f32_anchors=f32_anchors)
         return FEs[..., 2] @ M.T
 
     def loss64(xx, qq):
@@ -764,7 +793,7 @@ def _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, gx, gq, steps=(1e-3, 1e-
             xp[i, c] += h
             xm[i, c] -= h
             fd = (loss64(xp, q0) - loss64(xm, q0)) / (2 * h)
-            ad, ar, name = float(gx[i, c]), float(gx_r[i, c]), f"xyz[{i},{c}]"
+            ad, ar, aa, name = float(gx[i, c]), float(gx_r[i, c]), float(gx_a[i, c]), f"xyz[{i},{c}]"
         else:
             i = k - 3 * na
             h = steps[1]
@@ -772,26 +801,33 @@ def _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, gx, gq, steps=(1e-3, 1e-
             qp[i] += h
             qm[i] -= h
             fd = (loss64(xyz0, qp) - loss64(xyz0, qm)) / (2 * h)
-            ad, ar, name = float(gq[i]), float(gq_r[i]), f"q[{i}]"
-        rows.append((name, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30), ar, abs(ar - fd) / max(abs(fd), 1e-30)))
+            ad, ar, aa, name = float(gq[i]), float(gq_r[i]), float(gq_a[i]), f"q[{i}]"
+        rows.append((name, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30), ar, abs(ar - fd) / max(abs(fd), 1e-30),
+                     aa, abs(aa - fd) / max(abs(fd), 1e-30)))
     rel = np.array([r[3] for r in rows])
     rel_r = np.array([r[5] for r in rows])
+    rel_a = np.array([r[7] for r in rows])
     floor = 1e-2 * max(abs(r[2]) for r in rows)
     rel_floor = np.array([abs(r[1] - r[2]) / max(abs(r[2]), floor) for r in rows])
     torch.cuda.synchronize()
     return {"h": f"{steps[0]:g} A / {steps[1]:g} e",
-            "forward": "FP64 grid + FP64 relaxation (|F| < 1e-12) + FP64 df, on the device",
+            "forward": "FP64 grid + FP64 relaxation (|F| < 1e-12) + FP64 df, on the device; tip anchors "
+                       + ("advanced in float32 as the FP32 scan" if f32_anchors else "in FP64 (exact heights)"),
             "within_1e-3": int((rel <= 1e-3).sum()), "within_1e-2": int((rel <= 1e-2).sum()),
             "median_rel_err": float(np.median(rel)), "max_rel_err": float(rel.max()),
             "within_1e-3_floor_1pct_of_max": int((rel_floor <= 1e-3).sum()),
+            "adjoint_acc64_grid_forward": {"within_1e-3": int((rel_a <= 1e-3).sum()),
+                                           "within_1e-2": int((rel_a <= 1e-2).sum()),
+                                           "median_rel_err": float(np.median(rel_a)),
+                                           "max_rel_err": float(rel_a.max())},
             "adjoint_with_fp64_residual": {"within_1e-3": int((rel_r <= 1e-3).sum()),
                                            "within_1e-2": int((rel_r <= 1e-2).sum()),
                                            "median_rel_err": float(np.median(rel_r)),
                                            "max_rel_err": float(rel_r.max())},
             "wall_s": time.perf_counter() - t0,
-            "rows (name, adjoint, fd, rel, adjoint_fp64_residual, rel)":
-                [(n, round(a, 6), round(f, 6), float(f"{r:.3g}"), round(ar, 6), float(f"{rr:.3g}"))
-                 for n, a, f, r, ar, rr in rows]}
+            "rows (name, adjoint, fd, rel, adjoint_fp64_residual, rel, adjoint_acc64, rel)":
+                [(n, round(a, 6), round(f, 6), float(f"{r:.3g}"), round(ar, 6), float(f"{rr:.3g}"), round(aa, 6),
+                  float(f"{ra:.3g}")) for n, a, f, r, ar, rr, aa, ra in rows]}
 
 
 def diff_leg(dev, adam_steps=50):

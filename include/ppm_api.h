@@ -70,6 +70,9 @@ typedef struct PpmRelaxCtl {
                              1: test before it (relaxStrokesTiltedMorseDirect, kernels/AFM.cl:1478) */
     int   scan_ny;        /* >0: points form an ix-major raster with this many iy per ix
                              (enables 8x8 pixel tiles per wavefront); 0: no raster */
+    int   f32_anchors;    /* FP64 relaxation only: 1 = tip anchors advance as the FP32 kernels
+                             advance them (tip += dTip in float32, kernels/AFM.cl:556), so the
+                             FP64 scan follows the FP32 scan's exact heights; 0 = in FP64 */
 } PpmRelaxCtl;
 
 /* One molecule of a batch (an array of these lives in device memory).  The molecule's grid
@@ -135,6 +138,18 @@ int ppm_grid_project(int model, const float* atoms, const float* cpar, int na,
                      const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
                      int npbc_a, int npbc_b, const float lvA[3], const float lvB[3],
                      int cap, float* grid, void* stream);
+
+/* ppm_grid_project with each voxel's pair terms, evaluated in FP32 exactly as there, summed in
+ * FP64 and rounded once to the FP32 output.  Same arguments, pair laws, images and rescale
+ * (evalLJC_QZs_toImg / evalMorseC_QZs_toImg, kernels/AFM.cl:916-1056, whose FP32 sum it
+ * refines).  Above a neutral molecule the thousands of terms of a voxel cancel to ~1e-2 of
+ * their magnitude and the FP32 running sum dominates the field's error; this form removes that
+ * * part at ~1.55x the time (8 voxels per thread).  For gradient checks against FP64 finite
+ * differences and other accuracy-bound uses. */
+int ppm_grid_project_acc64(int model, const float* atoms, const float* cpar, int na,
+                           const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
+                           int npbc_a, int npbc_b, const float lvA[3], const float lvB[3],
+                           int cap, float* grid, void* stream);
 
 /* ppm_grid_project over a base: grid = cap(base + pair sum of `atoms`), the |E|>100 rescale
  * (if cap) applied to the total.  base: a raw (cap = 0) grid of other atoms on the same spec,

@@ -41,7 +41,7 @@ class PpmTip(ctypes.Structure):
 
 class PpmRelaxCtl(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int), ("f2conv", ctypes.c_float), ("check_before", ctypes.c_int),
-                ("scan_ny", ctypes.c_int)]
+                ("scan_ny", ctypes.c_int), ("f32_anchors", ctypes.c_int)]
 
 
 EXPORTS = (
@@ -49,7 +49,7 @@ EXPORTS = (
     "ppm_fe_in_strokes_tilted", "ppm_z_stencil", "ppm_axis_operator", "ppm_relax_adjoint",
     "ppm_grid_adjoint_workspace_size", "ppm_grid_adjoint", "ppm_relax_polish", "ppm_batch_workspace_size",
     "ppm_batch_forward", "ppm_relax_direct_morse", "ppm_direct_morse_vjp_workspace_size", "ppm_direct_morse_vjp",
-    "ppm_grid_project_add", "ppm_grid_adjoint_add", "ppm_grid_project_f64",
+    "ppm_grid_project_add", "ppm_grid_project_acc64", "ppm_grid_adjoint_add", "ppm_grid_project_f64",
     "ppm_relax_strokes_tilted_f64", "ppm_fdbm_workspace_size", "ppm_fdbm_fft_combine", "ppm_fdbm_pauli_scale",
     "ppm_fdbm_pad_roll", "ppm_fdbm_flip3", "ppm_fdbm_dispersion", "ppm_fdbm_compose_gradient",
     "ppm_ewald_num_g", "ppm_ewald_system_bytes", "ppm_ewald_prepare", "ppm_ewald_eval", "ppm_ewald_brute",
@@ -83,6 +83,8 @@ def load():
     L.ppm_grid_project.restype = i
     L.ppm_grid_project.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
                                    P(ctypes.c_float * 3), i, vp, vp]
+    L.ppm_grid_project_acc64.restype = i
+    L.ppm_grid_project_acc64.argtypes = L.ppm_grid_project.argtypes
     L.ppm_grid_project_add.restype = i
     L.ppm_grid_project_add.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
                                        P(ctypes.c_float * 3), i, vp, vp, vp]

@@ -87,7 +87,7 @@ const char* ppm_last_error(void) { return g_err; }
 static int grid_project_impl(const char* fn, int model, const float* atoms, const float* cpar, int na,
                              const PpmGridSpec* spec, const float Qs[4], const float QZs[4], int npbc_a, int npbc_b,
                              const float lvA[3], const float lvB[3], int cap, const float* base, float* grid,
-                             void* stream) {
+                             void* stream, bool acc64 = false) {
     g_err[0] = 0;
     if (int r = check_spec(fn, spec)) return r;
     if (model != PPM_MODEL_LJ && model != PPM_MODEL_MORSE) { set_error("%s: unknown model %d", fn, model); return PPM_ERR_INVALID_ARG; }
@@ -101,7 +101,7 @@ static int grid_project_impl(const char* fn, int model, const float* atoms, cons
     if ((long long)na * (2 * npbc_a + 1) * (2 * npbc_b + 1) > (1LL << 28)) { set_error("%s: too many image sites", fn); return PPM_ERR_INVALID_ARG; }
     if (!finite4(Qs, 4) || !finite4(QZs, 4)) { set_error("%s: non-finite tip charges", fn); return PPM_ERR_INVALID_ARG; }
     return launch_grid_project(model, atoms, cpar, na, *spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, base, grid,
-                               (hipStream_t)stream);
+                               acc64, (hipStream_t)stream);
 }
 
 int ppm_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
@@ -109,6 +109,13 @@ int ppm_grid_project(int model, const float* atoms, const float* cpar, int na, c
                      const float lvB[3], int cap, float* grid, void* stream) {
     return grid_project_impl("ppm_grid_project", model, atoms, cpar, na, spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap,
                              nullptr, grid, stream);
+}
+
+int ppm_grid_project_acc64(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
+                           const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
+                           const float lvB[3], int cap, float* grid, void* stream) {
+    return grid_project_impl("ppm_grid_project_acc64", model, atoms, cpar, na, spec, Qs, QZs, npbc_a, npbc_b, lvA,
+                             lvB, cap, nullptr, grid, stream, true);
 }
 
 int ppm_grid_project_add(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,

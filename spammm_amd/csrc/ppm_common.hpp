@@ -35,7 +35,8 @@ int check_launch(const char* what);
 // Per-translation-unit launchers (host side).
 int launch_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
-                        const float lvB[3], int cap, const float* base, float* grid, hipStream_t stream);
+                        const float lvB[3], int cap, const float* base, float* grid, bool acc64,
+                        hipStream_t stream);
 int launch_relax_tilted(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
                         const PpmTip& tip, const PpmRelaxCtl& ctl, float* FEs, float* PPs, hipStream_t stream);
 int launch_relax_strokes(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,

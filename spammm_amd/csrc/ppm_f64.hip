@@ -112,7 +112,7 @@ __device__ __forceinline__ D3 operator*(D3 a, double s) { return d3(a.x * s, a.y
 __device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
 struct Relax64Args {
-    int nx, ny, nz_grid, n_scan, nz, max_iter, check_before;
+    int nx, ny, nz_grid, n_scan, nz, max_iter, check_before, f32_anchors;
     double f2conv;
     double dinv[3][4];
     double A[3], B[3], C[3], dtip, k[4], d0[4], dt0, damp0, surf;
@@ -160,6 +160,10 @@ __global__ __launch_bounds__(64) void k_relax_tilted_f64(const double* __restric
     auto rotT = [&](D3 v) { return A * v.x + B * v.y + C * v.z; };               // tip frame -> world
     const D3 dTip = C * r.dtip;
     D3 tipPos = d3(pts[4 * (size_t)ip], pts[4 * (size_t)ip + 1], pts[4 * (size_t)ip + 2]);
+    // the FP32 kernels' anchor: float32 position advanced by the float32 step C * dtip
+    float tf[3] = {(float)tipPos.x, (float)tipPos.y, (float)tipPos.z};
+    const float dtf[3] = {(float)r.C[0] * (float)r.dtip, (float)r.C[1] * (float)r.dtip, (float)r.C[2] * (float)r.dtip};
+    if (r.f32_anchors) tipPos = d3(tf[0], tf[1], tf[2]);
     D3 pos = tipPos + rotT(d3(r.d0[0], r.d0[1], r.d0[2]));
     double dt = r.dt0, damp = r.damp0;
     const double dtmax = dt, dtmin = dtmax * (double)0.1f;
@@ -205,7 +209,12 @@ __global__ __launch_bounds__(64) void k_relax_tilted_f64(const double* __restric
             PPs[o] = pos.x; PPs[o + 1] = pos.y; PPs[o + 2] = pos.z;
             PPs[o + 3] = (conv && finite) ? (double)iters : -(double)iters;
         }
-        tipPos = tipPos + dTip;
+        if (r.f32_anchors) {
+            for (int q = 0; q < 3; ++q) tf[q] = tf[q] + dtf[q];
+            tipPos = d3(tf[0], tf[1], tf[2]);
+        } else {
+            tipPos = tipPos + dTip;
+        }
         pos = pos + dTip;
     }
 }
@@ -246,6 +255,7 @@ int launch_relax_tilted_f64(const double* grid, const PpmGridSpec& g, const doub
     Relax64Args r{};
     r.nx = g.nx; r.ny = g.ny; r.nz_grid = g.nz; r.n_scan = n_scan; r.nz = nz;
     r.max_iter = ctl.max_iter; r.check_before = ctl.check_before; r.f2conv = ctl.f2conv;
+    r.f32_anchors = ctl.f32_anchors != 0;
     for (int q = 0; q < 4; ++q) { r.dinv[0][q] = g.dinvA[q]; r.dinv[1][q] = g.dinvB[q]; r.dinv[2][q] = g.dinvC[q]; }
     for (int q = 0; q < 3; ++q) { r.A[q] = tip.tipA[q]; r.B[q] = tip.tipB[q]; r.C[q] = tip.tipC[q]; }
     r.dtip = tip.tipC[3];

@@ -57,11 +57,18 @@ __device__ __forceinline__ void load_site(const float4* __restrict__ atoms, cons
 constexpr int kOffNone = 0, kOffDynamic = 1, kOffStatic = 2;
 
 // Coulomb term of one tip charge site at z offset zo (charge fraction qf) on the V voxels.
-template <int V, bool DCZ>
+// Accumulation of one pair term into a voxel sum: FP32 (fused multiply-add) or, for ACC =
+// double, the FP32 term added in FP64 (the product of two floats is exact in double).
+__device__ __forceinline__ void acc_fma(float& s, float a, float b) { s = fmaf(a, b, s); }
+__device__ __forceinline__ void acc_fma(double& s, float a, float b) { s = fma((double)a, (double)b, s); }
+__device__ __forceinline__ void acc_add(float& s, float t) { s += t; }
+__device__ __forceinline__ void acc_add(double& s, float t) { s += (double)t; }
+
+template <int V, bool DCZ, typename ACC>
 __device__ __forceinline__ void offset_site(const Site& S, float qf, float zo, const float (&pz)[V],
                                             const float (&dxs)[DCZ ? 1 : V], const float (&dys)[DCZ ? 1 : V],
-                                            const float (&dxy2)[DCZ ? 1 : V], float (&fx)[V], float (&fy)[V],
-                                            float (&fz)[V], float (&fe)[V]) {
+                                            const float (&dxy2)[DCZ ? 1 : V], ACC (&fx)[V], ACC (&fy)[V],
+                                            ACC (&fz)[V], ACC (&fe)[V]) {
     const float qo = S.q * qf;
 #pragma unroll
     for (int k = 0; k < V; ++k) {
@@ -70,10 +77,10 @@ __device__ __forceinline__ void offset_site(const Site& S, float qf, float zo, c
         const float iro = rsq(fmaf(dzo, dzo, dxy2[kk]));
         const float eo = qo * iro;
         const float fro = eo * iro * iro;
-        fx[k] = fmaf(fro, dxs[kk], fx[k]);
-        fy[k] = fmaf(fro, dys[kk], fy[k]);
-        fz[k] = fmaf(fro, dzo, fz[k]);
-        fe[k] += eo;
+        acc_fma(fx[k], fro, dxs[kk]);
+        acc_fma(fy[k], fro, dys[kk]);
+        acc_fma(fz[k], fro, dzo);
+        acc_add(fe[k], eo);
     }
 }
 
@@ -90,7 +97,11 @@ __device__ __forceinline__ void offset_site(const Site& S, float qf, float zo, c
 // bound with an early exit (GridArgs in registers, as in the batched kernel, where a dynamic
 // index would put qoff/zoff in scratch); kOffNone: no offset sites (a.noff == 0).  The static
 // form schedules the pair loop ~20 % slower than the other two on gfx950.
-template <int MODEL, bool HASQ0, int V, bool DCZ, int OFF = kOffDynamic>
+// ACC: type of the per-voxel sums.  float: the FP32 sum in site order.  double: every pair term
+// is evaluated in FP32 exactly as for float and added in FP64, so the stored voxel carries only
+// the terms' own rounding, not the growth of an FP32 running sum over thousands of terms that
+// cancel (a field at a few A above a neutral molecule is ~1e-2 of its largest terms).
+template <int MODEL, bool HASQ0, int V, bool DCZ, int OFF = kOffDynamic, typename ACC = float>
 __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, const float* __restrict__ cpar,
                                           const GridArgs& a, Site* sites, int col, int ic0, int ic_end,
                                           float4* __restrict__ out, int out_klo, const float4* base = nullptr) {
@@ -108,9 +119,9 @@ __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, cons
             py[k] = a.p0[1] + a.dA[1] * fia + a.dB[1] * fib + a.dC[1] * fic;
         }
     }
-    float fx[V], fy[V], fz[V], fe[V];
+    ACC fx[V], fy[V], fz[V], fe[V];
 #pragma unroll
-    for (int k = 0; k < V; ++k) { fx[k] = 0.f; fy[k] = 0.f; fz[k] = 0.f; fe[k] = 0.f; }
+    for (int k = 0; k < V; ++k) { fx[k] = 0; fy[k] = 0; fz[k] = 0; fe[k] = 0; }
 
     for (int t0 = 0; t0 < a.nsites; t0 += kTile) {
         __syncthreads();
@@ -141,7 +152,32 @@ __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, cons
                 const float ir = rsq(r2);
                 const float ir2 = ir * ir;
                 float fr;
-                if (MODEL == PPM_MODEL_LJ && HASQ0) {
+                if constexpr (sizeof(ACC) == sizeof(double)) {
+                    float e;                                  // the pair's energy term
+                    if (MODEL == PPM_MODEL_LJ) {
+                        const float ir6 = ir2 * ir2 * ir2;
+                        if (HASQ0) {
+                            const float ec = qq0 * ir;
+                            e = fmaf(fmaf(S.c1, ir6, -S.c0), ir6, ec);
+                            fr = fmaf(fmaf(S.c3, ir6, -S.c2), ir6, ec) * ir2;
+                        } else {
+                            e = fmaf(S.c1, ir6, -S.c0) * ir6;
+                            fr = fmaf(S.c3, ir6, -S.c2) * ir6 * ir2;
+                        }
+                    } else {
+                        const float r = r2 * ir;
+                        const float ex = ex2(fmaf(S.c2, r, S.c0));
+                        const float Ae = S.c1 * ex;
+                        e = (ex - 2.0f) * Ae;
+                        fr = S.c3 * (Ae * (ex - 1.0f)) * ir;
+                        if (HASQ0) {
+                            const float ec = qq0 * ir;
+                            fr = fmaf(ec, ir2, fr);
+                            e += ec;
+                        }
+                    }
+                    acc_add(fe[k], e);
+                } else if (MODEL == PPM_MODEL_LJ && HASQ0) {
                     // LJ and the charge term share the final 1/r^2: 9 VALU instead of 10
                     const float ir6 = ir2 * ir2 * ir2;
                     const float ec = qq0 * ir;
@@ -158,25 +194,25 @@ __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, cons
                     fe[k] = fmaf(ex - 2.0f, Ae, fe[k]);                      // E = E0 ex (ex - 2)
                     fr = S.c3 * (Ae * (ex - 1.0f)) * ir;
                 }
-                if (HASQ0 && MODEL != PPM_MODEL_LJ) {
+                if (sizeof(ACC) == sizeof(float) && HASQ0 && MODEL != PPM_MODEL_LJ) {
                     const float ec = qq0 * ir;
                     fr = fmaf(ec, ir2, fr);
-                    fe[k] += ec;
+                    fe[k] += (float)ec;
                 }
-                fx[k] = fmaf(fr, dx, fx[k]);
-                fy[k] = fmaf(fr, dy, fy[k]);
-                fz[k] = fmaf(fr, dz, fz[k]);
+                acc_fma(fx[k], fr, dx);
+                acc_fma(fy[k], fr, dy);
+                acc_fma(fz[k], fr, dz);
             }
             // tip charge sites with a nonzero z offset (none for a single-charge probe)
             if constexpr (OFF == kOffStatic) {
 #pragma unroll
                 for (int o = 0; o < 4; ++o) {
                     if (o >= a.noff) break;
-                    offset_site<V, DCZ>(S, a.qoff[o], a.zoff[o], pz, dxs, dys, dxy2, fx, fy, fz, fe);
+                    offset_site<V, DCZ, ACC>(S, a.qoff[o], a.zoff[o], pz, dxs, dys, dxy2, fx, fy, fz, fe);
                 }
             } else if constexpr (OFF == kOffDynamic) {
                 for (int o = 0; o < a.noff; ++o)
-                    offset_site<V, DCZ>(S, a.qoff[o], a.zoff[o], pz, dxs, dys, dxy2, fx, fy, fz, fe);
+                    offset_site<V, DCZ, ACC>(S, a.qoff[o], a.zoff[o], pz, dxs, dys, dxy2, fx, fy, fz, fe);
             }
         }
     }
@@ -185,7 +221,7 @@ __device__ __forceinline__ void grid_tile(const float4* __restrict__ atoms, cons
     for (int k = 0; k < V; ++k) {
         const int ic = ic0 + k;
         if (ic >= ic_end) break;
-        float4 v = make_float4(fx[k], fy[k], fz[k], fe[k]);
+        float4 v = make_float4((float)fx[k], (float)fy[k], (float)fz[k], (float)fe[k]);
         if (base) {            // raw sum of other atoms (e.g. a substrate), same layout as out
             const float4 b = base[(size_t)(ic - out_klo) * a.ncol + col];
             v.x += b.x; v.y += b.y; v.z += b.z; v.w += b.w;

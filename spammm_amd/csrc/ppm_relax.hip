@@ -189,7 +189,7 @@ struct StateArgs {
 struct StateEval {
     size_t idx[8];
     float w[8];
-    float4 fe;          // trilinear sample at (float)q
+    float4 fe;          // trilinear sample at q, rounded to FP32
     double JF[3][3];    // dF_xyz/dq inside the cell
     double gE[3];       // dE/dq inside the cell
     double G[3];        // total force
@@ -205,17 +205,37 @@ __device__ __forceinline__ f3 tip_anchor(const float4* pts, const StateArgs& p, 
     return tip;
 }
 
+// REPEAT addressing of one axis in FP64: the cell (i0, i1) and the weight a of the upper corner,
+// the texel-centre convention of axis_repeat without its FP32 rounding of the coordinate.
+__device__ __forceinline__ void axis_repeat_d(double s, int n, int& i0, int& i1, double& a) {
+    const double u = (s - floor(s)) * (double)n;
+    const double t = u - 0.5;
+    double fl = floor(t);
+    a = t - fl;
+    fl = fmin(fmax(fl, -1.0), (double)n);      // keeps non-finite coordinates in range
+    int i = (int)fl;
+    int j = i + 1;
+    if (i < 0) i += n;
+    if (j > n - 1) j -= n;
+    i0 = min(max(i, 0), n - 1);
+    i1 = min(max(j, 0), n - 1);
+}
+
+// The trilinear model of the FP32 grid at an FP64 position q: sampling coordinates, weights and
+// interpolation in FP64 (the corner values are the grid's FP32 values), so the model is the
+// continuous piecewise-trilinear field whose equilibria the Newton polish solves for and whose
+// derivatives the adjoint uses (an FP32 sample would quantize q to ~1e-6 A through its texture
+// coordinates).
 __device__ void eval_state(const GridDev& g, const StateArgs& p, f3 tip, double qx, double qy, double qz,
                            StateEval& e) {
-    const float fx = (float)qx, fy = (float)qy, fz = (float)qz;
-    const float sa = fx * g.da[0] + fy * g.da[1] + fz * g.da[2] + g.da[3];
-    const float sb = fx * g.db[0] + fy * g.db[1] + fz * g.db[2] + g.db[3];
-    const float sc = fx * g.dc[0] + fy * g.dc[1] + fz * g.dc[2] + g.dc[3];
+    const double sa = qx * (double)g.da[0] + qy * (double)g.da[1] + qz * (double)g.da[2] + (double)g.da[3];
+    const double sb = qx * (double)g.db[0] + qy * (double)g.db[1] + qz * (double)g.db[2] + (double)g.db[3];
+    const double sc = qx * (double)g.dc[0] + qy * (double)g.dc[1] + qz * (double)g.dc[2] + (double)g.dc[3];
     int i0, i1, j0, j1, k0, k1;
-    float a, b, c;
-    axis_repeat(sa, g.nx, i0, i1, a);
-    axis_repeat(sb, g.ny, j0, j1, b);
-    axis_repeat(sc, g.nz, k0, k1, c);
+    double a, b, c;
+    axis_repeat_d(sa, g.nx, i0, i1, a);
+    axis_repeat_d(sb, g.ny, j0, j1, b);
+    axis_repeat_d(sc, g.nz, k0, k1, c);
     const size_t sx = (size_t)g.nx, sxy = (size_t)g.nx * g.ny;
     e.idx[0] = (size_t)k0 * sxy + (size_t)j0 * sx + i0; e.idx[1] = (size_t)k0 * sxy + (size_t)j0 * sx + i1;
     e.idx[2] = (size_t)k0 * sxy + (size_t)j1 * sx + i0; e.idx[3] = (size_t)k0 * sxy + (size_t)j1 * sx + i1;
@@ -224,23 +244,24 @@ __device__ void eval_state(const GridDev& g, const StateArgs& p, f3 tip, double 
     Cell C;
 #pragma unroll
     for (int m = 0; m < 8; ++m) C.c[m] = g.data[e.idx[m]];
-    e.fe = lerp8(C, a, b, c);
-    const float ma = 1.f - a, mb = 1.f - b, mc = 1.f - c;
-    e.w[0] = ma * mb * mc; e.w[1] = a * mb * mc; e.w[2] = ma * b * mc; e.w[3] = a * b * mc;
-    e.w[4] = ma * mb * c;  e.w[5] = a * mb * c;  e.w[6] = ma * b * c;  e.w[7] = a * b * c;
-    double dFa[4], dFb[4], dFc[4];
+    const double ma = 1.0 - a, mb = 1.0 - b, mc = 1.0 - c;
+    const double w[8] = {ma * mb * mc, a * mb * mc, ma * b * mc, a * b * mc,
+                         ma * mb * c,  a * mb * c,  ma * b * c,  a * b * c};
+#pragma unroll
+    for (int m = 0; m < 8; ++m) e.w[m] = (float)w[m];
+    double fd[4], dFa[4], dFb[4], dFc[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const float* f[8];
+        double f[8];
 #pragma unroll
-        for (int m = 0; m < 8; ++m) f[m] = &C.c[m].x;
-        dFa[q] = (double)(((f[1][q] - f[0][q]) * mb + (f[3][q] - f[2][q]) * b) * mc +
-                          ((f[5][q] - f[4][q]) * mb + (f[7][q] - f[6][q]) * b) * c);
-        dFb[q] = (double)(((f[2][q] - f[0][q]) * ma + (f[3][q] - f[1][q]) * a) * mc +
-                          ((f[6][q] - f[4][q]) * ma + (f[7][q] - f[5][q]) * a) * c);
-        dFc[q] = (double)(((f[4][q] - f[0][q]) * ma + (f[5][q] - f[1][q]) * a) * mb +
-                          ((f[6][q] - f[2][q]) * ma + (f[7][q] - f[3][q]) * a) * b);
+        for (int m = 0; m < 8; ++m) f[m] = (double)(&C.c[m].x)[q];
+        fd[q] = ((f[0] * ma + f[1] * a) * mb + (f[2] * ma + f[3] * a) * b) * mc +
+                ((f[4] * ma + f[5] * a) * mb + (f[6] * ma + f[7] * a) * b) * c;
+        dFa[q] = ((f[1] - f[0]) * mb + (f[3] - f[2]) * b) * mc + ((f[5] - f[4]) * mb + (f[7] - f[6]) * b) * c;
+        dFb[q] = ((f[2] - f[0]) * ma + (f[3] - f[1]) * a) * mc + ((f[6] - f[4]) * ma + (f[7] - f[5]) * a) * c;
+        dFc[q] = ((f[4] - f[0]) * ma + (f[5] - f[1]) * a) * mb + ((f[6] - f[2]) * ma + (f[7] - f[3]) * a) * b;
     }
+    e.fe = make_float4((float)fd[0], (float)fd[1], (float)fd[2], (float)fd[3]);
     double ta[3], tb[3], tc[3];
     for (int j = 0; j < 3; ++j) {
         ta[j] = (double)g.nx * (double)g.da[j];
@@ -263,7 +284,7 @@ __device__ void eval_state(const GridDev& g, const StateArgs& p, f3 tip, double 
         gt[i] = (d[i] - (double)p.d0[i]) * (double)p.k[i] + d[i] * phi;
         for (int j = 0; j < 3; ++j) Jt[i][j] = beta * d[i] * d[j] + (i == j ? (double)p.k[i] + phi : 0.0);
     }
-    const double fs[3] = {e.fe.x, e.fe.y, e.fe.z};
+    const double fs[3] = {fd[0], fd[1], fd[2]};
     for (int i = 0; i < 3; ++i) {
         e.G[i] = fs[i] + R[0][i] * gt[0] + R[1][i] * gt[1] + R[2][i] * gt[2] + (double)p.C[i] * (double)p.surf;
         for (int j = 0; j < 3; ++j) {

@@ -113,19 +113,21 @@ class Tip:
         return t
 
 
-def relax_ctl(max_iter=128, f2conv=1e-8, check_before=False, scan_ny=0) -> _lib.PpmRelaxCtl:
+def relax_ctl(max_iter=128, f2conv=1e-8, check_before=False, scan_ny=0, f32_anchors=False) -> _lib.PpmRelaxCtl:
     c = _lib.PpmRelaxCtl()
     c.max_iter = int(max_iter)
     c.f2conv = float(f2conv)
     c.check_before = int(bool(check_before))
     c.scan_ny = int(scan_ny)
+    c.f32_anchors = int(bool(f32_anchors))
     return c
 
 
 def grid_project(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 0), lvA=(0, 0, 0), lvB=(0, 0, 0),
-                 cap=True, out=None, base=None):
+                 cap=True, out=None, base=None, acc64=False):
     """Force-field grid (nz,ny,nx,4) on the device (evalLJC_QZs_toImg / evalMorseC_QZs_toImg).
-    ``base``: a raw (cap=False) grid of other atoms on the same spec, added before the cap."""
+    ``base``: a raw (cap=False) grid of other atoms on the same spec, added before the cap.
+    ``acc64``: sum each voxel's FP32 pair terms in FP64 (ppm_grid_project_acc64; no base)."""
     import torch
     _f32dev(atoms, "atoms", 4)
     _f32dev(cpar, "cpar", 2 if model == _lib.PPM_MODEL_LJ else 4)
@@ -144,7 +146,11 @@ def grid_project(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 0), 
     s = spec.c()
     args = (int(model), atoms.data_ptr(), cpar.data_ptr(), int(atoms.shape[0]), ctypes.byref(s), _lib.f4(Qs),
             _lib.f4(QZs), int(npbc[0]), int(npbc[1]), ctypes.byref(lva), ctypes.byref(lvb), int(bool(cap)))
-    if base is None:
+    if acc64:
+        if base is not None:
+            raise ValueError("acc64 takes no base grid")
+        rc = L.ppm_grid_project_acc64(*args, out.data_ptr(), _stream(atoms))
+    elif base is None:
         rc = L.ppm_grid_project(*args, out.data_ptr(), _stream(atoms))
     else:
         _f32dev(base, "base", 4)
@@ -220,9 +226,11 @@ def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, 
 
 
 def relax_tilted_f64(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, f2conv=1e-8,
-                     check_before=False, want_pps=True):
+                     check_before=False, want_pps=True, f32_anchors=False):
     """`relax_tilted` in FP64 on an FP64 grid and FP64 points (reference precision): returns
-    (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None) as float64 device tensors."""
+    (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None) as float64 device tensors.  ``f32_anchors``:
+    the tip anchors advance in float32 as in the FP32 scan (whose heights drift from p0 + iz*dz by
+    up to ~1e-5 A over tens of slices), so the FP64 scan relaxes at the FP32 scan's exact heights."""
     import torch
     _f64dev(grid, "grid", 4)
     _f64dev(points, "points", 4)
@@ -232,7 +240,7 @@ def relax_tilted_f64(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=1
     FEs = torch.empty((ns, nz, 4), dtype=torch.float64, device=grid.device)
     PPs = torch.empty((ns, nz, 4), dtype=torch.float64, device=grid.device) if want_pps else None
     L = _lib.load()
-    s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, 0)
+    s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, 0, f32_anchors)
     rc = L.ppm_relax_strokes_tilted_f64(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz),
                                         ctypes.byref(tp), ctypes.byref(ctl), FEs.data_ptr(),
                                         PPs.data_ptr() if PPs is not None else None, _stream(grid))

@@ -50,6 +50,7 @@ class Problem:
     amp: float = 1.0
     f0: float = 30000.0
     k_cant: float = 1800.0
+    grid_acc64: bool = False    # grid voxel sums in FP64 (ppm_grid_project_acc64): the accuracy form
     recompute: bool = False     # backward re-runs the relaxation instead of keeping probe positions
     strict: bool = True         # raise if any state is non-finite in the adjoint (gated states are counted)
     last_status: dict = field(default_factory=dict)
@@ -92,7 +93,7 @@ def _relax(grid, prob: Problem):
 
 def _forward_parts(atoms, cpar, prob: Problem):
     grid = D.grid_project(atoms, cpar, _lib.PPM_MODEL_LJ, prob.spec, prob.Qs, prob.QZs, prob.npbc, prob.lvA,
-                          prob.lvB, cap=prob.cap)
+                          prob.lvB, cap=prob.cap, acc64=prob.grid_acc64)
     FEs, PPs = _relax(grid, prob)
     op = DF.z_operator(prob.df_kind, prob.nz, prob.dz, atoms.device, prob.amp, prob.f0, prob.k_cant)
     df = op.apply(FEs, (prob.nz * 4, 4, 0), prob.n_scan, 1, offset=2)

@@ -67,6 +67,34 @@ def test_relax_f64_matches_fp64_oracle():
         assert dp <= 1e-9 and df <= 1e-9
 
 
+def test_relax_f64_f32_anchors_follow_fp32_scan_heights():
+    """f32_anchors: the FP64 scan's tip anchors advance in float32 exactly as the FP32 kernel's
+    (tip += C*dtip in float32).  On a zero grid the relaxed probe sits at anchor + dpos0, so the
+    probe heights expose the anchors: float32-accumulated heights with the flag, p0 + iz*dz in
+    FP64 without it (over 40 slices of -0.1 A from 9.5 A they differ by up to ~1e-5 A)."""
+    w, sp, *_ = _grid_case(0, ((-0.05, 0, 0, 0), (0, 0, 0, 0)))
+    nz = 40
+    tip = D.Tip(stiffness=w.stiffness, dpos0=w.dpos0, tipC=np.array([0., 0., 1., -0.1], np.float32))
+    pts = D.scan_points(w.scan_p0, w.scan_da, w.scan_db, 3, 2).astype(np.float64)
+    dev = D.require_gpu()
+    G0 = torch.zeros((sp.n[2], sp.n[1], sp.n[0], 4), dtype=torch.float64, device=dev)
+    z32 = np.empty(nz)
+    t = np.float32(pts[0, 2])
+    for k in range(nz):
+        z32[k] = float(t)
+        t = np.float32(t + np.float32(np.float32(1.0) * np.float32(-0.1)))
+    z64 = pts[0, 2] + np.arange(nz) * float(np.float32(-0.1))
+    d0z = float(np.float32(w.dpos0[2]))
+    for f32, ref in ((True, z32), (False, z64)):
+        _, PPs = D.relax_tilted_f64(G0, sp, torch.as_tensor(pts, device=dev), nz, tip, max_iter=40000, f2conv=1e-24,
+                                    check_before=True, f32_anchors=f32)
+        PPz = PPs[..., 2].cpu().numpy()
+        err = np.abs(PPz - (ref[None, :] + d0z)).max()
+        print(f"[f64] f32_anchors={f32}: max |z - (anchor + d0)| {err:.2e}")
+        assert err <= 1e-9
+    assert np.abs(z32 - z64).max() > 1e-6
+
+
 def test_f64_entry_points_reject_fp32_tensors():
     w, sp, atoms, cpar, Qs, QZs, npbc, lvA, lvB = _grid_case(0, ((-0.05, 0, 0, 0), (0, 0, 0, 0)))
     dev = D.require_gpu()

@@ -31,13 +31,13 @@ def spec_of(w):
     return D.axis_aligned_spec(w.p0, w.L, w.n)
 
 
-def grid_gpu(w, model=0, cpar=None, Qs=None, QZs=None, npbc=None, cap=True):
+def grid_gpu(w, model=0, cpar=None, Qs=None, QZs=None, npbc=None, cap=True, acc64=False, sp=None):
     d = dev()
     atoms = torch.as_tensor(w.atoms, device=d)
     cp = torch.as_tensor(w.cLJ if cpar is None else cpar, device=d)
-    sp = spec_of(w)
+    sp = spec_of(w) if sp is None else sp
     G = D.grid_project(atoms, cp, model, sp, w.Qs if Qs is None else Qs, w.QZs if QZs is None else QZs,
-                       w.npbc if npbc is None else npbc, w.lvA, w.lvB, cap=cap)
+                       w.npbc if npbc is None else npbc, w.lvA, w.lvB, cap=cap, acc64=acc64)
     torch.cuda.synchronize()
     return G, sp
 
@@ -60,18 +60,26 @@ def morse_params(w):
     return c
 
 
+@pytest.mark.parametrize("acc64", [False, True])
 @pytest.mark.parametrize("model", [0, 1])
-@pytest.mark.parametrize("case", ["pbc_q0", "nopbc_offsets", "pbc_offsets_nocap"])
-def test_grid_project_small_vs_oracle(model, case):
+@pytest.mark.parametrize("case", ["pbc_q0", "nopbc_offsets", "pbc_offsets_nocap", "tilted_c"])
+def test_grid_project_small_vs_oracle(model, case, acc64):
+    """Both voxel-sum forms (FP32, and FP32 terms summed in FP64) against the FP64 oracle."""
     w = W.small()
     cpar = w.cLJ if model == 0 else morse_params(w)
     Qs, QZs, npbc, cap = w.Qs, w.QZs, w.npbc, True
+    sp = spec_of(w)
     if case == "nopbc_offsets":
         Qs, QZs, npbc = np.array([0., -0.1, 0.1, 0.], np.float32), np.array([0., 1.8, 3.6, 0.], np.float32), (0, 0)
     elif case == "pbc_offsets_nocap":
         Qs, QZs, cap = np.array([-0.05, 0.2, 0., 0.], np.float32), np.array([0., 1.2, 0., 0.], np.float32), False
-    print(f"[grid] model={model} case={case} n={w.n}", flush=True)
-    G, sp = grid_gpu(w, model, cpar, Qs, QZs, npbc, cap)
+    elif case == "tilted_c":     # dC with an x,y component: the kernels' per-voxel x,y form
+        n = tuple(int(v) for v in w.n)
+        dC = np.array([0.02, -0.015, float(w.L[2]) / n[2]], np.float32)
+        L = np.array([sp.dA * n[0], sp.dB * n[1], dC * n[2]], np.float64)
+        sp = D.GridSpec(n, sp.p0, sp.dA, sp.dB, dC, *D.dinv_lvec(np.linalg.inv(L).T, n))
+    print(f"[grid] model={model} case={case} acc64={acc64} n={w.n}", flush=True)
+    G, sp = grid_gpu(w, model, cpar, Qs, QZs, npbc, cap, acc64=acc64, sp=sp)
     Go = O.grid_project(model, w.atoms, cpar, w.n, sp.p0, sp.dA, sp.dB, sp.dC, Qs, QZs, npbc, w.lvA, w.lvB,
                         cap=cap)
     worst = check_grid(G.cpu().numpy(), Go)
@@ -395,3 +403,33 @@ def test_full_flake_grid_sampled_and_scan_subsample_bitexact():
                               w.f2conv, False, "f32")
     np.testing.assert_array_equal(PPs[sel], PPo)
     np.testing.assert_array_equal(FEs[sel], FEo)
+
+
+def test_full_flake_grid_acc64_vs_fp64():
+    """ppm_grid_project_acc64 on the full flake grid: 2000 sampled voxels against the FP64 oracle
+    within a quarter of the FP32 sum's bound (1e-5 |(F,E)| + 1e-6 sum|terms| + 1e-6), and, on the
+    voxels whose net field cancels below 3 % of sum|terms| (above the molecule), a median error at
+    most half the FP32 sum's: what FP64 sums remove is the FP32 running sum's rounding."""
+    w = W.flake()
+    G32, sp = grid_gpu(w)
+    G64, _ = grid_gpu(w, acc64=True)
+    G32 = G32.cpu().numpy().reshape(-1, 4)
+    G64 = G64.cpu().numpy().reshape(-1, 4)
+    assert np.isfinite(G64).all()
+    idx = np.random.default_rng(12).choice(G64.shape[0], 2000, replace=False)
+    Go = O.grid_voxels(0, w.atoms, w.cLJ, w.n, sp.p0, sp.dA, sp.dB, sp.dC, w.Qs, w.QZs, idx, w.npbc, w.lvA,
+                       w.lvB).astype(np.float64)
+    nx, ny, nz = w.n
+    ic, ib, ia = np.unravel_index(idx, (nz, ny, nx))
+    P = (np.asarray(sp.p0, np.float64)[None, :] + np.outer(ia, sp.dA) + np.outer(ib, sp.dB) + np.outer(ic, sp.dC))
+    S = _abs_term_sums(w, P)
+    scale = np.minimum(1.0, 100.0 / np.maximum(np.abs(Go[:, 3]), 1e-300))
+    bound = 1e-5 * np.linalg.norm(Go, axis=1) + 1e-6 * S * scale + 1e-6
+    e64 = np.abs(G64[idx].astype(np.float64) - Go).max(1)
+    e32 = np.abs(G32[idx].astype(np.float64) - Go).max(1)
+    canc = np.linalg.norm(Go, axis=1) < 0.03 * S * scale
+    m64, m32 = float(np.median(e64[canc])), float(np.median(e32[canc]))
+    print(f"[acc64] worst |d|/bound {(e64 / bound).max():.3e}; cancelling voxels {int(canc.sum())}: "
+          f"median |d| {m64:.3e} (FP64 sums) vs {m32:.3e} (FP32 sums)", flush=True)
+    assert (e64 <= bound).all()
+    assert canc.sum() >= 200 and m64 <= 0.5 * m32

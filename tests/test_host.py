@@ -54,7 +54,7 @@ def test_invalid_arguments_rejected_on_host():
 def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.PpmGridSpec) == 16 + 7 * 16
     assert ctypes.sizeof(_lib.PpmTip) == 7 * 16
-    assert ctypes.sizeof(_lib.PpmRelaxCtl) == 16
+    assert ctypes.sizeof(_lib.PpmRelaxCtl) == 20
 
 
 def test_scan_points_match_reference_loop():
@@ -103,7 +103,7 @@ def test_batch_struct_layouts_match_header():
     from spammm_amd import batch as B
     assert ctypes.sizeof(B.PpmMolDesc) == 16 * 4 + 8 + 12 * 16
     assert B.PpmMolDesc.grid_off.offset == 64
-    assert ctypes.sizeof(B.PpmBatchParams) == 12 * 4 + 2 * 16 + 7 * 16 + 16
+    assert ctypes.sizeof(B.PpmBatchParams) == 12 * 4 + 2 * 16 + 7 * 16 + 20
 
 
 def test_batch_z_window_covers_probe_heights():
