@@ -144,8 +144,8 @@ int ppm_grid_project(int model, const float* atoms, const float* cpar, int na,
  * (evalLJC_QZs_toImg / evalMorseC_QZs_toImg, kernels/AFM.cl:916-1056, whose FP32 sum it
  * refines).  Above a neutral molecule the thousands of terms of a voxel cancel to ~1e-2 of
  * their magnitude and the FP32 running sum dominates the field's error; this form removes that
- * * part at ~1.55x the time (8 voxels per thread).  For gradient checks against FP64 finite
- * differences and other accuracy-bound uses. */
+ * part (3-6x lower error there) at ~1.55x the time (8 voxels per thread), for uses bound by
+ * the field's own accuracy. */
 int ppm_grid_project_acc64(int model, const float* atoms, const float* cpar, int na,
                            const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
                            int npbc_a, int npbc_b, const float lvA[3], const float lvB[3],
