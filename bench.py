@@ -869,6 +869,13 @@ def accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip):
         G32[k] = O.grid_project(0, w.atoms, w.cLJ, (1, 1, 1), pv.astype(np.float32), sp.dA, sp.dB, sp.dC, w.Qs,
                                 w.QZs, w.npbc, w.lvA, w.lvB, cap=True, precision="f32").reshape(4)
     gerr32 = np.abs(G32 - Go).max(1)
+    # the full-size parity bar: 1e-5 |(F,E)| + 4e-6 sum|terms| (rescaled on capped voxels) + 1e-6
+    ic, rem = np.divmod(idx, nx * ny)
+    ib, ia = np.divmod(rem, nx)
+    Pv = (np.asarray(sp.p0, np.float64)[None, :] + np.outer(ia, sp.dA) + np.outer(ib, sp.dB) + np.outer(ic, sp.dC))
+    S = O.abs_term_sums(w.atoms, w.cLJ, w.Qs[0], w.npbc, w.lvA, w.lvB, Pv)
+    cap = np.minimum(1.0, 100.0 / np.maximum(np.abs(Go[:, 3]), 1e-300))
+    tbar = 1e-5 * mag + 4e-6 * S * cap + 1e-6
     sel = rng.choice(w.n_scan, 64, replace=False)
     F = FEs.cpu().numpy()[sel]
     P = PPs.cpu().numpy()[sel]
@@ -884,14 +891,17 @@ def accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip):
     ddf = np.abs(dfg - dfo)
     rms = lambda x: float(np.sqrt(np.mean(np.square(x))))  # noqa: E731
     res = {
-        # per voxel: max channel error relative to |(F,E)|; the parity bar of the tests is
-        # |d| <= 1e-5 |(F,E)| + 1e-6.  High above the flake the net field (~1e-2) is a sum of
-        # ~1800 Coulomb terms of ~1e-2 each, so any FP32 sum errs by a few 1e-6 there: the FP32
-        # oracle's own error at the same voxels is reported beside the GPU's
+        # per voxel: max channel error relative to |(F,E)|.  The small-case parity bar is
+        # |d| <= 1e-5 |(F,E)| + 1e-6; high above the flake the net field (~1e-2) is a sum of
+        # ~1800 Coulomb terms of ~1e-2 each, so any FP32 sum errs by a few 1e-6 there (the FP32
+        # oracle's own error at the same voxels is reported beside the GPU's) and the full-size
+        # bar adds 4e-6 sum|terms|: `*_over_term_bar_max` <= 1 is the full-size parity test's pass
         "grid_rel_err_vs_fp64_max": float(grid_rel.max()), "grid_rel_err_vs_fp64_rms": rms(grid_rel),
         "grid_abs_err_vs_fp64_max": float(gerr.max()),
-        "grid_err_over_parity_bar_max": float((gerr / (1e-5 * mag + 1e-6)).max()),
-        "grid_fp32_oracle_err_over_parity_bar_max": float((gerr32 / (1e-5 * mag + 1e-6)).max()),
+        "grid_err_over_small_case_bar_max": float((gerr / (1e-5 * mag + 1e-6)).max()),
+        "grid_fp32_oracle_err_over_small_case_bar_max": float((gerr32 / (1e-5 * mag + 1e-6)).max()),
+        "grid_err_over_term_bar_max": float((gerr / tbar).max()),
+        "grid_fp32_oracle_err_over_term_bar_max": float((gerr32 / tbar).max()),
         "grid_fp32_oracle_abs_err_vs_fp64_max": float(gerr32.max()),
         "relax_bitexact_vs_fp32_oracle": bool(np.array_equal(F, F32) and np.array_equal(P, P32)),
         "pp_err_vs_fp64_max_A": float(dpp.max()), "pp_err_vs_fp64_rms_A": rms(dpp),

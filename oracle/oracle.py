@@ -356,3 +356,27 @@ def direct_vjp(atoms, cMs, Qs, QZs, pts, nz, tipA, tipB, tipC, stiffness, dpos0,
                                 int(nz), *[_p(v, _d) for v in vs], _p(PPs, _d), _p(U, _d), _p(lam, _d),
                                 _p(diag, _d), _p(grad, _d))
     return grad, lam, diag
+
+
+def abs_term_sums(atoms, cLJ, q_tip, npbc, lvA, lvB, P):
+    """Per query point (k,3), the FP64 sum over atoms and x,y images of |E| and |F| of the individual
+    LJ and Coulomb (tip charge q_tip at z offset 0) pair terms: the scale of an FP32 sum's rounding
+    where terms cancel (a bound helper for the parity tests, not a restatement of a reference
+    function)."""
+    P = np.asarray(P, np.float64)
+    S = np.zeros(len(P))
+    qs = float(q_tip) * 14.3996448915
+    c6, c12 = np.asarray(cLJ, np.float64)[:, 0], np.asarray(cLJ, np.float64)[:, 1]
+    q = np.asarray(atoms, np.float64)[:, 3]
+    for kx in range(-npbc[0], npbc[0] + 1):
+        for ky in range(-npbc[1], npbc[1] + 1):
+            sh = kx * np.asarray(lvA, np.float64) + ky * np.asarray(lvB, np.float64)
+            A = np.asarray(atoms, np.float64)[:, :3] + sh
+            d = P[:, None, :] - A[None, :, :]
+            r2 = (d * d).sum(2) + 1e-4
+            ir2 = 1.0 / r2
+            ir6 = ir2 ** 3
+            ec = np.abs(q * qs) * np.sqrt(ir2)
+            S += (np.abs((c12 * ir6 - c6) * ir6) + np.abs((12 * c12 * ir6 - 6 * c6) * ir6 * ir2) * np.sqrt(r2)
+                  + ec + ec * np.sqrt(ir2)).sum(1)
+    return S

@@ -351,21 +351,7 @@ def test_errors_are_raised_not_swallowed():
 def _abs_term_sums(w, P):
     """Per point, the sum over atoms and images of |F| and |E| of the individual LJ and Coulomb
     terms (FP64) — the scale of an FP32 sum's rounding where terms cancel."""
-    S = np.zeros(len(P))
-    qs = float(w.Qs[0]) * 14.3996448915
-    for kx in range(-w.npbc[0], w.npbc[0] + 1):
-        for ky in range(-w.npbc[1], w.npbc[1] + 1):
-            sh = kx * np.asarray(w.lvA, np.float64) + ky * np.asarray(w.lvB, np.float64)
-            A = w.atoms[:, :3].astype(np.float64) + sh
-            d = P[:, None, :] - A[None, :, :]
-            r2 = (d * d).sum(2) + 1e-4
-            ir2 = 1.0 / r2
-            ir6 = ir2 ** 3
-            c6, c12 = w.cLJ[:, 0].astype(np.float64), w.cLJ[:, 1].astype(np.float64)
-            ec = np.abs(w.atoms[:, 3].astype(np.float64) * qs) * np.sqrt(ir2)
-            S += (np.abs((c12 * ir6 - c6) * ir6) + np.abs((12 * c12 * ir6 - 6 * c6) * ir6 * ir2) * np.sqrt(r2)
-                  + ec + ec * np.sqrt(ir2)).sum(1)
-    return S
+    return O.abs_term_sums(w.atoms, w.cLJ, w.Qs[0], w.npbc, w.lvA, w.lvB, P)
 
 
 def test_full_flake_grid_sampled_and_scan_subsample_bitexact():
