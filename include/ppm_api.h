@@ -253,15 +253,18 @@ int ppm_ewald_num_g(int n_harm);
 /* Bytes of the prepared-system buffer of ppm_ewald_prepare (1 <= n_layers, 1 <= n_harm <= 64). */
 size_t ppm_ewald_system_bytes(int n_layers, int n_harm);
 
-/* Byte offset, inside the prepared-system buffer, of the int count of G kept for evaluation
- * (written by ppm_ewald_prepare; read it with a device-to-host copy). */
+/* Byte offset, inside the prepared-system buffer, of the int count of kept G (written by
+ * ppm_ewald_prepare; read it with a device-to-host copy).  Each kept G stands for the pair +-G,
+ * so the count is half the number of G vectors summed. */
 size_t ppm_ewald_kept_count_offset(int n_layers, int n_harm);
 
 /* Prepares a slab (SurfaceEwaldCL.prepare_system, SurfaceEwald.py:165-234, whose
  * compute_ewald_coefficients kernel is kernels/surface.cl:1036-1086): per-layer structure factors
  * W_l(G) = (2 pi/(A|G|)) sum_{i in l} q_i e^{-iG.rho_i} in FP64, layer heights and charges, and the
  * ordered list of G kept for evaluation: those with max_l |W_l(G)| > drop_tol * max |W| (drop_tol 0
- * keeps every G with a nonzero structure factor).  system: device, ppm_ewald_system_bytes. */
+ * keeps every G with a nonzero structure factor), one per pair +-G (h > 0, or h = 0 and k > 0:
+ * real charges give W_l(-G) = conj W_l(G), so the pair's two terms are equal and are evaluated
+ * once, doubled).  system: device, ppm_ewald_system_bytes. */
 int ppm_ewald_prepare(const float* ions, const int* layer_off, int n_layers, const double a[2],
                       const double b[2], int n_harm, double drop_tol, void* system, size_t system_bytes,
                       void* stream);

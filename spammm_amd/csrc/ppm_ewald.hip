@@ -138,6 +138,18 @@ __device__ inline double g_weight2(const double2* __restrict__ W, int n_layers, 
 }
 
 // Ordered compaction (one workgroup): keep G with max_l |W_l(G)| > drop_tol * max_{l,G} |W_l(G)|.
+// G and -G contribute equal terms: the charges are real, so W_l(-G) = conj(W_l(G)) and
+// Re(W_l(-G) e^{-iG.r}) = Re(W_l(G) e^{iG.r}), with the same -grad.  The kept list holds the half
+// plane h > 0 or (h = 0, k > 0) of the G above the drop threshold (|W(-G)| = |W(G)|, so pairs are
+// kept or dropped together) and every consumer counts each entry twice.
+__device__ inline bool kept_g(const double2* __restrict__ W, const EwGeom& a, int g, double thr) {
+    int h, k;
+    g_of(a.n_harm, g, h, k);
+    if (!(h > 0 || (h == 0 && k > 0))) return false;
+    const double mg = g_weight2(W, a.n_layers, a.n_g, g);
+    return mg > thr && mg > 0.0;
+}
+
 __global__ __launch_bounds__(kScanThreads) void k_compact(const double2* __restrict__ W, EwGeom a, double drop_tol,
                                                           int* __restrict__ cnt, int* __restrict__ list,
                                                           float4* __restrict__ ginfo) {
@@ -159,10 +171,7 @@ __global__ __launch_bounds__(kScanThreads) void k_compact(const double2* __restr
     const int per = (a.n_g + kScanThreads - 1) / kScanThreads;
     const int g0 = min(a.n_g, tid * per), g1 = min(a.n_g, g0 + per);
     int mine = 0;
-    for (int g = g0; g < g1; ++g) {
-        const double mg = g_weight2(W, a.n_layers, a.n_g, g);
-        mine += (mg > thr && mg > 0.0) ? 1 : 0;
-    }
+    for (int g = g0; g < g1; ++g) mine += kept_g(W, a, g, thr) ? 1 : 0;
     scan[tid] = mine;
     __syncthreads();
     for (int o = 1; o < kScanThreads; o <<= 1) {       // inclusive Hillis-Steele scan
@@ -173,8 +182,7 @@ __global__ __launch_bounds__(kScanThreads) void k_compact(const double2* __restr
     }
     int pos = scan[tid] - mine;
     for (int g = g0; g < g1; ++g) {
-        const double mg = g_weight2(W, a.n_layers, a.n_g, g);
-        if (mg > thr && mg > 0.0) {
+        if (kept_g(W, a, g, thr)) {
             int h, k;
             g_of(a.n_harm, g, h, k);
             double Gx, Gy, Gn;
@@ -228,7 +236,9 @@ __global__ __launch_bounds__(256) void k_plane_tables(SysView s, EwGeom a, int n
             di += sg * e * w.y;
         }
     }
-    tab[(size_t)iz * a.n_g + c] = make_float4((float)tr, (float)ti, (float)dr, (float)di);
+    // x2: the entry stands for the pair +-G
+    tab[(size_t)iz * a.n_g + c] = make_float4((float)(2.0 * tr), (float)(2.0 * ti), (float)(2.0 * dr),
+                                              (float)(2.0 * di));
 }
 
 // One thread = one (ia, ib) column x kVz planes.  Voxel position p0 + ia dA + ib dB + ic dC with
@@ -332,7 +342,7 @@ __global__ __launch_bounds__(256) void k_ewald_points(const float4* __restrict__
         for (int l = 0; l < a.n_layers; ++l) {
             const double2 w = s.W[(size_t)l * a.n_g + g];
             const double d = mode == 0 ? fabs(z - s.zl[l]) : z - s.zl[l];
-            acc += (w.x * cs - w.y * sn) * exp(-Gn * d);
+            acc += 2.0 * (w.x * cs - w.y * sn) * exp(-Gn * d);     // the pair +-G
         }
     }
     phi[i] = (float)(acc * kCOULOMB_D);

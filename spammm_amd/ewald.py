@@ -142,7 +142,8 @@ class SurfaceEwald:
             raise ValueError("System not prepared. Call prepare_system() first.")
 
     def n_kept(self) -> int:
-        """Number of G kept for evaluation (reads the count from the device)."""
+        """Number of kept G, one per pair +-G (reads the count from the device): the series sums
+        2 * n_kept() G vectors."""
         self._need()
         import torch
         off = int(_lib.load().ppm_ewald_kept_count_offset(self.n_layers, self.n_harm))
