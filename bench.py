@@ -716,8 +716,10 @@ def _diff_case(dev, z0, adam_steps, lr):
         loss = 0.5 * ((diff.df_image(x, q0, R0, E0, prob).double() - target) ** 2).sum()
         loss.backward()
         opt.step()
-        hist.append(float(loss))
+        hist.append(float(loss.detach()))
         rmsd.append(float(((x.detach() - xyz_t) ** 2).sum(1).mean().sqrt()))
+    dxe = (x.detach() - xyz_t) ** 2
+    dx0 = (xyz0 - xyz_t) ** 2
     torch.cuda.synchronize()
     t_adam = time.perf_counter() - t0
     rmsd0 = float(((xyz0 - xyz_t) ** 2).sum(1).mean().sqrt())
@@ -739,7 +741,9 @@ def _diff_case(dev, z0, adam_steps, lr):
         "adam": {"steps": adam_steps, "lr": lr, "schedule": "constant", "wall_s": t_adam,
                  "loss_first": hist[0] if hist else None, "loss_last": hist[-1] if hist else None,
                  "loss_every5": hist[::5], "rmsd_start_A": rmsd0, "rmsd_end_A": rmsd[-1] if rmsd else None,
-                 "rmsd_every5": rmsd[::5]},
+                 "rmsd_every5": rmsd[::5],
+                 "rmsd_xy_start_end_A": [float(dx0[:, :2].sum(1).mean().sqrt()), float(dxe[:, :2].sum(1).mean().sqrt())],
+                 "rmsd_z_start_end_A": [float(dx0[:, 2].mean().sqrt()), float(dxe[:, 2].mean().sqrt())]},
     }
 
 
