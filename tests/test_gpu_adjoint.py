@@ -68,7 +68,7 @@ def test_adjoint_matches_fp64_finite_differences():
     df, t = gpu_df(prob, th, requires_grad=True)
     loss = 0.5 * ((df.double() - target) ** 2).sum()
     loss.backward()
-    print(f"[adjoint] loss {float(loss):.6e} status {prob.last_status}")
+    print(f"[adjoint] loss {float(loss.detach()):.6e} status {prob.last_status}")
     assert prob.last_status["gated"] == 0
     grads = {k: t[k].grad.cpu().numpy() for k in t}
     tgt = target.cpu().numpy()
