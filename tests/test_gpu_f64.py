@@ -95,6 +95,35 @@ def test_relax_f64_f32_anchors_follow_fp32_scan_heights():
     assert np.abs(z32 - z64).max() > 1e-6
 
 
+def test_fp32_scan_with_polish_matches_fp64_scan_at_fp32_anchors():
+    """The FP32 scan, its states polished by FP64 Newton steps on the trilinear model, against
+    the FP64 scan of the same FP32 grid (widened) with f32_anchors: the same heights, so the
+    polished probes and their Fz agree to FP32 resolution; with FP64 anchors (exact heights, up
+    to ~1e-5 A away over 40 slices) Fz differs by the anchor drift times the force gradient."""
+    w = W.small()
+    sp = D.axis_aligned_spec(w.p0, w.L, w.n)
+    dev = D.require_gpu()
+    G = D.grid_project(torch.as_tensor(w.atoms, device=dev), torch.as_tensor(w.cLJ, device=dev), 0, sp, w.Qs,
+                       w.QZs, w.npbc, w.lvA, w.lvB, cap=True)
+    nz = 40
+    tip = D.Tip(stiffness=w.stiffness, dpos0=w.dpos0, tipC=np.array([0., 0., 1., -0.1], np.float32))
+    pts = torch.as_tensor(D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy), device=dev)
+    FEs, PPs = D.relax_tilted(G, sp, pts, nz, tip, 1000, 1e-12, True)
+    D.relax_polish(G, sp, pts, nz, tip, PPs, FEs, 3)
+    G64, pts64 = G.double().contiguous(), pts.double().contiguous()
+    out = {}
+    for f32 in (True, False):
+        F64, P64 = D.relax_tilted_f64(G64, sp, pts64, nz, tip, max_iter=40000, f2conv=1e-24, check_before=True,
+                                      f32_anchors=f32)
+        dz = (FEs[..., 2].double() - F64[..., 2]).abs().cpu().numpy()
+        dp = (PPs[..., :3].double() - P64[..., :3]).abs().amax(-1).cpu().numpy()
+        out[f32] = (float(np.median(dz)), float(np.median(dp)))
+        print(f"[f64] f32_anchors={f32}: median |dFz| {out[f32][0]:.2e} eV/A, median |dPP| {out[f32][1]:.2e} A")
+    assert (PPs[..., 3] > 0).float().mean().item() > 0.99
+    assert out[True][0] <= 1e-8 and out[True][1] <= 1e-6
+    assert out[False][0] >= 3 * out[True][0]
+
+
 def test_f64_entry_points_reject_fp32_tensors():
     w, sp, atoms, cpar, Qs, QZs, npbc, lvA, lvB = _grid_case(0, ((-0.05, 0, 0, 0), (0, 0, 0, 0)))
     dev = D.require_gpu()
