@@ -10,7 +10,9 @@ df along z.  ``value`` = relaxed pixel*z-steps per second of the whole step.
 Default workload (flake): 200-atom flat polycyclic flake, 30x30x24 A cell on a
 0.1 A grid (300x300x240 float4), 256x256 pixels x 60 z-steps.
 
-Prints ONE JSON line.  Roofline of the dominant kernel (grid projection) is
+Prints ONE compact JSON line (under 4 KB: the headline, roofline, CPU baseline and a summary
+per leg) and writes the full per-leg record to --detail (default gpurun_out/bench_detail.json).
+Roofline of the dominant kernel (grid projection) is
 measured live with HIP events on the stream the kernels run on; the CPU baseline
 times the C++ oracle (FP32 mode, OpenMP) on a bounded sample of the same workload.
 """
@@ -53,6 +55,8 @@ def main():
     ap.add_argument("--no-ewald", action="store_true", help="skip the 2D Ewald substrate electrostatics leg")
     ap.add_argument("--z-margin", default=None,
                     help="z window margins of the batched leg in A, 'below,above' (default: the library's)")
+    ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
+                    help="file for the full per-leg record (the printed line carries summaries)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scale of the HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -138,7 +142,8 @@ def main():
         f"mean iters {mean_iter:.2f} converged {conv_frac:.4f}")
 
     out = {
-        "metric": "relaxed pixel*z-steps/s on 1 MI355X (plain per-image path: grid + relax + df)",
+        "metric": "relaxed pixel*z-steps/s on 1 MI355X (batched vs plain); adjoint/forward ratio",
+        "value_is": "plain per-image path on the 200-atom flake: grid + relaxed scan + Giessibl df per step",
         "value": value,
         "unit": "pixel*z-steps/s",
         "n_gpus": 1,
@@ -159,7 +164,7 @@ def main():
         },
         "roofline": {
             "kernel": "k_grid_project (grid projection)",
-            "bound": "mfma",
+            "bound": "valu",
             "compute_unit": "FP32 vector ALU (no matrix instructions); peak = published MI355X FP32",
             "achieved": grid_tflops,
             "peak": PEAK_FP32_TFLOPS,
@@ -186,23 +191,90 @@ def main():
         },
     }
 
+    legs = {}
     if not args.no_diff:
-        out["differentiable"] = diff_leg(dev, args.adam_steps)
+        legs["differentiable"] = diff_leg(dev, args.adam_steps)
     if args.batch_mols > 0:
-        out["batched"] = batch_leg(dev, args.batch_mols, args.z_margin)
+        legs["batched"] = batch_leg(dev, args.batch_mols, args.z_margin)
     if not args.no_direct:
-        out["direct_morse"] = direct_leg(dev)
+        legs["direct_morse"] = direct_leg(dev)
     if args.poses > 0:
-        out["substrate_poses"] = substrate_leg(dev, args.poses)
+        legs["substrate_poses"] = substrate_leg(dev, args.poses)
     if not args.no_fdbm:
-        out["fdbm"] = fdbm_leg(dev)
+        legs["fdbm"] = fdbm_leg(dev)
     if not args.no_ewald:
-        out["ewald"] = ewald_leg(dev)
+        legs["ewald"] = ewald_leg(dev)
     if not args.no_check:
-        out["accuracy"] = accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip)
+        legs["accuracy"] = accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_leg(w, sp, grid, pts_h, tip, states)
-    print(json.dumps(out), flush=True)
+    detail = dict(out, **legs)
+    out["detail_file"] = write_detail(detail, args.detail)
+    out.update(summarize(legs))
+    line = compact_line(out)
+    log(f"[bench] result line: {len(line)} bytes")
+    print(line, flush=True)
+
+
+SUMMARY_KEYS = {
+    "batched": ("batched_ms", "plain_loop_ms", "plain_windowed_loop_ms", "speedup", "speedup_vs_windowed_plain",
+                "batched_pixel_z_steps_per_s", "launch", "autotune_fused_vs_two_phase_ms", "bitexact_vs_plain",
+                "window_fallback_molecules", "wait_timeout"),
+    "differentiable": ("forward_ms", "adjoint_ms", "adjoint_over_forward", "fd_fp64_configured_heights",
+                       "fd_fp64_smooth_heights", "gated_states_configured_heights", "adam_configured_heights",
+                       "adam_smooth_heights"),
+    "substrate_poses": ("forward_ms_median", "adjoint_ms_median", "adjoint_over_forward",
+                        "placements_per_s_forward_plus_adjoint", "window_fallbacks", "gated_states_total",
+                        "directional_fd_rel_err"),
+    "direct_morse": ("forward_ms", "vjp_ms", "vjp_over_forward", "pixel_z_steps_per_s"),
+    "fdbm": ("stage3_device_ms", "grid_points_per_s_stage3"),
+    "ewald": ("tip_grid_ms", "direct_3x3_coulomb_ms", "kept_g_pairs"),
+    "accuracy": ("relax_bitexact_vs_fp32_oracle", "pp_err_vs_fp64_max_A", "pp_err_vs_fp64_rms_A",
+                 "fz_err_vs_fp64_max", "fz_err_vs_fp64_rms", "df_max_rel_err_vs_fp64", "grid_err_over_term_bar_max"),
+}
+
+
+def summarize(legs):
+    """One small object per leg: the figures the result line carries (the full record of every leg
+    goes to the detail file)."""
+    out = {}
+    for name, res in legs.items():
+        keys = SUMMARY_KEYS.get(name, ())
+        out[name] = {k: res[k] for k in keys if k in res}
+    return out
+
+
+def _round_floats(x, sig=5):
+    if isinstance(x, float):
+        return float(f"{x:.{sig}g}")
+    if isinstance(x, dict):
+        return {k: _round_floats(v, sig) for k, v in x.items()}
+    if isinstance(x, (list, tuple)):
+        return [_round_floats(v, sig) for v in x]
+    return x
+
+
+def compact_line(out):
+    """The result line: floats to 5 significant digits, no whitespace.  It must stay short (the
+    record of every leg is in the detail file), so it is checked here."""
+    line = json.dumps(_round_floats(out), separators=(",", ":"))
+    if len(line) > 4000:
+        raise RuntimeError(f"bench result line is {len(line)} bytes; keep it under 4000")
+    return line
+
+
+def write_detail(detail, path):
+    """Writes the full per-leg record as JSON; returns the path written (or None)."""
+    if not path:
+        return None
+    try:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        with open(path, "w") as f:
+            json.dump(detail, f, indent=1, default=str)
+        return path
+    except OSError as e:
+        log(f"[bench] detail file not written: {e}")
+        return None
 
 
 def batch_leg(dev, n_mol, z_margin=None):
@@ -857,7 +929,7 @@ def diff_leg(dev, adam_steps=50):
 
 def accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip):
     """Error of every output against the oracle on a sample (FP64 and FP32-faithful)."""
-    from oracle import oracle as O, df_ref as R
+    from oracle import oracle as O, giessibl_ref as GQ
     t0 = time.perf_counter()
     rng = np.random.default_rng(11)
     G = grid.cpu().numpy()
@@ -893,7 +965,7 @@ def accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip):
     dpp = np.abs(P[..., :3] - P64[..., :3]).max(-1)
     dfz = np.abs(F[..., 2] - F64[..., 2])
     z = w.scan_p0[2] + w.dtip * np.arange(w.nz)
-    dfo = R.giessibl_df_piecewise(F[..., 2].astype(np.float64), z, w.amp, w.f0, w.k_cant)
+    dfo = GQ.giessibl_df_piecewise(F[..., 2].astype(np.float64), z, w.amp, w.f0, w.k_cant)
     dfg = df.cpu().numpy().reshape(w.n_scan, w.nz)[sel]
     ddf = np.abs(dfg - dfo)
     rms = lambda x: float(np.sqrt(np.mean(np.square(x))))  # noqa: E731
@@ -924,6 +996,20 @@ def accuracy_leg(w, sp, grid, pts_h, FEs, PPs, df, tip):
     return res
 
 
+def _cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo, as lscpu reports it) and its logical CPU count."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    name = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return f"{name} ({os.cpu_count()} logical CPUs on the host)"
+
+
 def cpu_leg(w, sp, grid, pts_h, tip, states):
     """C++ oracle (FP32 mode, OpenMP) on a bounded sample, scaled to the whole step."""
     from oracle import oracle as O
@@ -950,6 +1036,7 @@ def cpu_leg(w, sp, grid, pts_h, tip, states):
         "unit": "pixel*z-steps/s",
         "cores": cores,
         "kind": "port",
+        "cpu_model": _cpu_model(),
         "sample": (f"grid: {nslab} of {w.n[2]} z-planes; relax: {nsel} of {w.n_scan} pixels x {w.nz} z; "
                    f"scaled to the full step (grid {t_grid:.2f} s + relax {t_relax:.2f} s)"),
         "grid_points_per_s": w.nvox / t_grid,

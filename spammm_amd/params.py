@@ -102,3 +102,22 @@ def mix_R0E0(enames, elements, tip_R: float, tip_E: float):
     R0 = np.array([tip_R + elements[e].RvdW for e in enames], dtype=np.float64)
     E0 = np.array([np.sqrt(abs(tip_E * elements[e].EvdW)) for e in enames], dtype=np.float64)
     return R0, E0
+
+
+def morse_atoms_from_Z(atomPos, atomTypes, params_path=None, tip_R=1.452, tip_E=0.0006808, tip_alpha=-1.8):
+    """Atom rows and Morse pair rows from positions and atomic numbers (_morse_atoms_from_Z,
+    spammm/SPM/AFM_utils.py:3993-4017): atoms (na,4) float32 = (x, y, z, 0) and cMs (na,4) float32
+    = the Morse mixing rule of the element with each atomic number.  The element table is
+    params_path (ElementTypes.dat layout) or, without it, the built-in values of the reference's
+    data/ElementTypes.dat.  An atomic number without an entry raises KeyError."""
+    table = read_element_types(params_path) if params_path is not None else ELEMENTS_REFERENCE
+    by_z = {int(et.iZ): name for name, et in table.items()}
+    pos = np.asarray(atomPos, dtype=np.float32)
+    zs = np.asarray(atomTypes).reshape(-1)[:len(pos)].astype(np.int64)
+    missing = sorted({int(z) for z in zs if int(z) not in by_z})
+    if missing:
+        raise KeyError(f"morse_atoms_from_Z: no element entry for Z={missing}")
+    atoms = np.zeros((len(pos), 4), dtype=np.float32)
+    atoms[:, :3] = pos[:, :3]
+    return atoms, mix_morse([by_z[int(z)] for z in zs], table, tip_R, tip_E, tip_alpha)
+

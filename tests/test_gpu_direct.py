@@ -17,8 +17,8 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-from oracle import df_ref as R                               # noqa: E402
 from oracle import oracle as O                               # noqa: E402
+from spammm_amd import afm as afm_mod                        # noqa: E402
 from spammm_amd import device as D                           # noqa: E402
 from spammm_amd.afm import AFMulator                         # noqa: E402
 from tests.direct_cases import (fd_component_vjp, total_force, toy_system)   # noqa: E402
@@ -156,7 +156,7 @@ def test_vjp_vs_cpu64_oracle_at_gpu_pps_per_channel():
     nz, dtip = 5, -0.2
     FEs, pts, PPs = run(afm, atoms, (3, 3), nz, dtip)
     target = np.zeros((3, 3, nz))
-    _, _, dL = R.df_loss_seed(FEs, target, dtip)
+    _, _, dL = afm_mod.df_loss_seed(FEs, target, dtip)
     g, diag = afm.vjp_scan_morse_direct(PPs, pts, dtip, dL.astype(np.float32))
     assert (diag["status_codes"] == 0).all()
     pts4 = np.zeros((9, 4))
@@ -187,7 +187,7 @@ def loss_at(afm, atoms, cMs, v, h, target, nxy, nz, dtip):
     ap[:, 3] += h * v[:, 5]
     afm.atoms_arr, afm.cLJs_arr = ap.astype(np.float32), cp.astype(np.float32)
     FE, _, _ = run(afm, atoms, nxy, nz, dtip)
-    return R.df_loss_seed(FE, target, dtip)[0]
+    return afm_mod.df_loss_seed(FE, target, dtip)[0]
 
 
 def test_vjp_re_relax_directional_fd():
@@ -197,7 +197,7 @@ def test_vjp_re_relax_directional_fd():
     nxy, nz, dtip = (3, 3), 5, -0.2
     FEs, pts, PPs = run(afm, atoms, nxy, nz, dtip)
     target = np.zeros((*nxy, nz))
-    _, _, dL = R.df_loss_seed(FEs, target, dtip)
+    _, _, dL = afm_mod.df_loss_seed(FEs, target, dtip)
     g, _ = afm.vjp_scan_morse_direct(PPs, pts, dtip, dL.astype(np.float32))
     v = np.random.default_rng(55).standard_normal((2, 6)) * 0.01
     v[:, 5] = 0
@@ -221,7 +221,7 @@ def test_df_loss_end_to_end_per_channel():
     target = np.zeros_like(Fz)
     target[:, :, 1:-1] = -(Fz[:, :, 2:] - Fz[:, :, :-2]) / (2 * abs(dtip))
     target += np.random.default_rng(33).standard_normal(target.shape) * 0.01
-    _, _, dL = R.df_loss_seed(FEs, target, dtip)
+    _, _, dL = afm_mod.df_loss_seed(FEs, target, dtip)
     g, _ = afm.vjp_scan_morse_direct(PPs, pts, dtip, dL.astype(np.float32))
     h = {0: 1e-2, 1: 1e-2, 2: 1e-2, 3: 1e-2, 4: 1e-4, 5: 1e-3}
     worst = 0.0

@@ -18,7 +18,8 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 from oracle import oracle as O          # noqa: E402
-from oracle import df_ref as R          # noqa: E402
+from oracle import giessibl_ref as GQ   # noqa: E402
+from tests.golden_vectors import df_vectors   # noqa: E402
 from spammm_amd import device as D      # noqa: E402
 from spammm_amd import workloads as W   # noqa: E402
 
@@ -289,11 +290,11 @@ def _cmp(a, b, rel=1e-5):
 
 def test_df_gradient_and_amp9_vs_oracle():
     from spammm_amd import afm
-    rng = np.random.default_rng(42)
-    Fz = rng.normal(size=(5, 6, 41)).astype(np.float32)
-    _cmp(afm.compute_df(Fz, 0.1), R.compute_df(Fz.astype(np.float64), 0.1))
-    _cmp(afm.compute_df_amp(Fz, 0.1, 1.0), R.compute_df_amp(Fz, 0.1, 1.0))
-    _cmp(afm.compute_df_amp(Fz, -0.1, 0.37), R.compute_df_amp(Fz, -0.1, 0.37))
+    V = df_vectors()
+    Fz = V["grad__Fz"]
+    _cmp(afm.compute_df(Fz, 0.1), V["grad__df"])
+    _cmp(afm.compute_df_amp(Fz, 0.1, 1.0), V["amp1__df"])
+    _cmp(afm.compute_df_amp(Fz, -0.1, 0.37), V["amp2__df"])
 
 
 def test_df_giessibl_vs_quadrature():
@@ -302,35 +303,31 @@ def test_df_giessibl_vs_quadrature():
     z = 6.0 - 0.1 * np.arange(40)
     Fz = (0.3 * np.exp(-(z[None, None, :] - 3.0)) + 0.05 * rng.normal(size=(3, 2, 40))).astype(np.float32)
     df = afm.df_giessibl(Fz, -0.1, 1.0, 30000.0, 1800.0)
-    ref = R.giessibl_df_piecewise(Fz.astype(np.float64), z, 1.0, 30000.0, 1800.0)
+    ref = GQ.giessibl_df_piecewise(Fz.astype(np.float64), z, 1.0, 30000.0, 1800.0)
     _cmp(df, ref, 2e-5)
 
 
 def test_df_loss_seed_vs_oracle():
     from spammm_amd import afm
-    rng = np.random.default_rng(7)
-    FEs = rng.normal(size=(4, 3, 9, 4))
-    tgt = rng.normal(size=(4, 3, 9))
-    w = rng.uniform(0.5, 1.5, size=(4, 3, 9))
-    L, df, dL = afm.df_loss_seed(FEs, tgt, -0.1, w)
-    Lo, dfo, dLo = R.df_loss_seed(FEs.astype(np.float32).astype(np.float64), tgt, -0.1, w)
-    assert abs(L - Lo) <= 1e-5 * abs(Lo)
-    _cmp(df, dfo)
-    _cmp(dL, dLo)
+    V = df_vectors()
+    L, df, dL = afm.df_loss_seed(V["loss__FEs"], V["loss__target"], -0.1, V["loss__w"])
+    assert abs(L - float(V["loss__L"])) <= 1e-5 * abs(float(V["loss__L"]))
+    _cmp(df, V["loss__df"])
+    _cmp(dL, V["loss__dL"])
+    L, df, dL = afm.df_loss_seed(V["loss__FEs"], V["loss__target"], -0.1)
+    assert abs(L - float(V["lossd__L"])) <= 1e-5 * abs(float(V["lossd__L"]))
+    _cmp(df, V["lossd__df"])
+    _cmp(dL, V["lossd__dL"])
 
 
 def test_df_directional_and_dissipation_vs_oracle():
     from spammm_amd import afm
-    rng = np.random.default_rng(9)
-    FEs = rng.normal(size=(6, 5, 11, 4)).astype(np.float32)
-    n = np.array([1., 2., 3.]) / np.linalg.norm([1., 2., 3.])
-    sp = (0.4, 0.3, 0.2)
-    _cmp(afm.compute_df_dir(FEs, sp, n), R.compute_df_dir(FEs.astype(np.float64), sp, n))
-    _cmp(afm.compute_df_amp_dir(FEs, sp, n, 0.5), R.compute_df_amp_dir(FEs, sp, n, 0.5))
-    _cmp(afm.compute_df_amp_dir(FEs, sp, (0., 0., 1.), 0.5), R.compute_df_amp_dir(FEs, sp, (0., 0., 1.), 0.5))
-    h = 3.0 + 0.2 * np.arange(11)
-    bwd = FEs + rng.normal(scale=0.1, size=FEs.shape).astype(np.float32)
-    _cmp(afm.compute_dissipation(FEs, bwd, h, h[2:9:3], 0.4), R.compute_dissipation(FEs, bwd, h, h[2:9:3], 0.4))
+    V = df_vectors()
+    FEs, n, sp = V["dir__FEs"], V["dir__n"], tuple(V["dir__spacing"])
+    _cmp(afm.compute_df_dir(FEs, sp, n), V["dir__df_dir"])
+    _cmp(afm.compute_df_amp_dir(FEs, sp, n, 0.5), V["dir__df_amp_dir"])
+    _cmp(afm.compute_df_amp_dir(FEs, sp, (0., 0., 1.), 0.5), V["dir__df_amp_vertical"])
+    _cmp(afm.compute_dissipation(FEs, V["dir__bwd"], V["dir__h_scan"], V["dir__h_df"], 0.4), V["dir__E_diss"])
 
 
 def test_errors_are_raised_not_swallowed():

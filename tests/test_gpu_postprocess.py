@@ -13,8 +13,8 @@ pytestmark = pytest.mark.gpu
 
 pytest.importorskip("torch")
 
-from oracle import df_ref as R                 # noqa: E402
 from spammm_amd import postprocess as PP       # noqa: E402
+from tests.golden_vectors import df_vectors    # noqa: E402
 
 
 def spec(h_df, h_Fz, h_scan, amp, osc, nx=5, ny=4):
@@ -52,19 +52,17 @@ def test_lateral_oscillation_closed_form():
 
 
 def test_composition_matches_oracle():
-    rng = np.random.default_rng(17)
+    """shared_postprocess of the fixture volume against the oracle's compute_df_amp_dir and
+    compute_dissipation of it (tests/golden/df_vectors.npz, case post)."""
+    V = df_vectors()
     h_df, h_Fz, h_scan = PP.afm_df_height_stacks(3.0, 4.0, 0.1, amp=0.3, osc_dir=(1., 2., 3.))
-    nx, ny = 6, 5
-    FEs = rng.normal(size=(nx, ny, h_scan.size, 4)).astype(np.float32)
-    bwd = FEs + rng.normal(scale=0.1, size=FEs.shape).astype(np.float32)
+    FEs, bwd = V["post__FEs"], V["post__bwd"]
+    nx, ny = FEs.shape[:2]
     sp = spec(h_df, h_Fz, h_scan, 0.3, (1., 2., 3.), nx, ny)
     r = PP.shared_postprocess(FEs, sp, FEs_bwd=bwd)
-    n = np.array([1., 2., 3.]) / np.linalg.norm([1., 2., 3.])
-    spacing = (0.2, 0.3, float(h_scan[1] - h_scan[0]))
-    dfo = R.compute_df_amp_dir(FEs.astype(np.float64), spacing, n, 0.3)
     idf = [int(np.argmin(np.abs(h_scan - h))) for h in h_df]
     iFz = [int(np.argmin(np.abs(h_scan - h))) for h in h_Fz]
-    for a, b in ((r.df, dfo[:, :, idf]), (r.E_diss, R.compute_dissipation(FEs, bwd, h_scan, h_df, 0.3, n))):
+    for a, b in ((r.df, V["post__df_amp_dir"][:, :, idf]), (r.E_diss, V["post__E_diss"])):
         a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
         assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max()
     assert np.array_equal(r.Fz, FEs[..., 2][:, :, iFz])

@@ -16,8 +16,10 @@ Host operators (spammm_amd.df, product code, applied here on the CPU to check th
 import numpy as np
 import pytest
 
-from oracle import df_ref as R
+from oracle import giessibl_ref as GQ
 from spammm_amd import df as DF
+
+R = pytest.importorskip("oracle.df_ref")   # the restatement stays in the build container
 
 C = DF.EV_A2_TO_N_M
 
@@ -105,10 +107,10 @@ def test_df_loss_seed_boundary_weight_quirk():
 def test_giessibl_continuous_limits():
     f0, k = 30000.0, 1800.0
     for A in (0.1, 1.0, 3.0):
-        val = R.giessibl_df_continuous(lambda z: 0.7 - 0.25 * z, np.array([2.0, 5.0]), A, f0, k)
+        val = GQ.giessibl_df_continuous(lambda z: 0.7 - 0.25 * z, np.array([2.0, 5.0]), A, f0, k)
         np.testing.assert_allclose(val, -(f0 / (2 * k)) * (-0.25) * C, rtol=1e-10)
     A = 1e-3
-    val = R.giessibl_df_continuous(lambda z: np.exp(-z), np.array([1.5]), A, f0, k)
+    val = GQ.giessibl_df_continuous(lambda z: np.exp(-z), np.array([1.5]), A, f0, k)
     np.testing.assert_allclose(val, -(f0 / (2 * k)) * (-np.exp(-1.5)) * C, rtol=1e-6)
 
 
@@ -116,9 +118,70 @@ def test_giessibl_exponential_closed_form():
     """F = F0 exp(-z/l): int_{-1}^{1} e^{-A u/l} u/sqrt(1-u^2) du = -pi I1(A/l)."""
     from scipy.special import iv
     f0, k, F0, lam, A, zc = 30000.0, 1800.0, 0.8, 0.6, 1.0, 4.0
-    val = R.giessibl_df_continuous(lambda z: F0 * np.exp(-z / lam), np.array([zc]), A, f0, k)
+    val = GQ.giessibl_df_continuous(lambda z: F0 * np.exp(-z / lam), np.array([zc]), A, f0, k)
     ref = -(f0 / (np.pi * k * A)) * C * F0 * np.exp(-zc / lam) * (-np.pi * iv(1, A / lam))
     np.testing.assert_allclose(val, ref, rtol=1e-9)
+
+
+def _library_df_amp(Fz, dz, amp):
+    """compute_df_amp as the reference composes it from library routines: map_coordinates (order 1,
+    mode 'nearest') at z + u_k A / |dz|, numpy.gradient along z, nine equal weights."""
+    from scipy.ndimage import map_coordinates
+    nx, ny, nz = Fz.shape
+    I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    out = np.zeros(Fz.shape)
+    for u in np.cos((2 * np.arange(9) + 1) / 18 * np.pi):
+        s = map_coordinates(Fz.astype(np.float64), np.array([I, J, K + u * amp / abs(dz)], np.float64), order=1,
+                            mode="nearest")
+        out -= np.gradient(s, abs(dz), axis=2) / 9
+    return out
+
+
+@pytest.mark.parametrize("dz,amp", [(0.1, 1.0), (-0.1, 0.37), (0.07, 2.5), (0.2, 0.05)])
+def test_df_edges_match_library_composition(dz, amp):
+    """The whole volume, end slices included, against numpy.gradient / scipy map_coordinates: pins the
+    edge handling (one-sided end differences, clamped sample indices) that the reference's own
+    tests leave open (they check the interior only, tests/SPM/test_afm_morse.py:75-76)."""
+    from scipy.ndimage import map_coordinates
+    rng = np.random.default_rng(21)
+    Fz = rng.normal(size=(3, 4, 23))
+    np.testing.assert_allclose(R.compute_df(Fz, dz), -np.gradient(Fz, abs(dz), axis=2), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(R.compute_df_amp(Fz, dz, amp), _library_df_amp(Fz, dz, amp), rtol=0, atol=2e-5)
+    FEs = rng.normal(size=(5, 6, 7, 4))
+    n = np.array([0.3, -0.5, 0.8]) / np.linalg.norm([0.3, -0.5, 0.8])
+    sp = (0.3, 0.2, abs(dz))
+    Fn = FEs[..., :3] @ n
+    lib = np.zeros(Fn.shape)
+    for a in range(3):
+        lib -= n[a] * np.gradient(Fn, sp[a], axis=a)
+    np.testing.assert_allclose(R.compute_df_dir(FEs, sp, n), lib, rtol=0, atol=1e-12)
+    base = np.indices(Fn.shape, dtype=np.float64)
+    lib = np.zeros(Fn.shape)
+    for u in np.cos((2 * np.arange(9) + 1) / 18 * np.pi):
+        s = map_coordinates(Fn, base + (u * amp * n / np.array(sp)).reshape(3, 1, 1, 1), order=1, mode="nearest")
+        for a in range(3):
+            lib -= n[a] * np.gradient(s, sp[a], axis=a) / 9
+    np.testing.assert_allclose(R.compute_df_amp_dir(FEs, sp, n, amp), lib, rtol=0, atol=2e-5)
+
+
+def test_dissipation_trapezoid_matches_library():
+    """compute_dissipation's trapezoid sums against numpy's trapezoid rule, ascending and descending
+    heights, a window holding one sample (all heights used) and an oblique oscillation."""
+    rng = np.random.default_rng(22)
+    trapz = getattr(np, "trapezoid", None) or np.trapz
+    f, b = rng.normal(size=(3, 2, 15, 4)), rng.normal(size=(3, 2, 15, 4))
+    for h_scan in (3.0 + 0.1 * np.arange(15), 4.4 - 0.1 * np.arange(15)):
+        for osc, amp in (((0., 0., 1.), 0.3), ((1., 0., 1.), 0.25), ((0., 0., 1.), 0.01)):
+            n = np.array(osc) / np.linalg.norm(osc)
+            dF = f[..., :3] @ n - b[..., :3] @ n
+            h_df = np.array([3.5, 3.8, 4.1])
+            got = R.compute_dissipation(f, b, h_scan, h_df, amp, osc)
+            for i, h in enumerate(h_df):
+                m = (h_scan >= h - amp * abs(n[2])) & (h_scan <= h + amp * abs(n[2]))
+                if m.sum() < 2:
+                    m[:] = True
+                ref = np.abs(trapz(dF[:, :, m], x=h_scan[m], axis=2))
+                np.testing.assert_allclose(got[:, :, i], ref, rtol=1e-6, atol=1e-7)
 
 
 # ── host-built operators (spammm_amd.df) vs the oracle functions ──────────────
@@ -147,7 +210,7 @@ def test_host_giessibl_operator_matches_quadrature(dz, amp):
     z = 6.0 + dz * np.arange(nz)
     Fz = 0.5 * np.exp(-(z - 2.5) / 0.7) + 0.02 * rng.normal(size=(2, nz))
     M = DF.giessibl_matrix(nz, dz, amp, 30000.0, 1800.0)
-    ref = R.giessibl_df_piecewise(Fz, z, amp, 30000.0, 1800.0)
+    ref = GQ.giessibl_df_piecewise(Fz, z, amp, 30000.0, 1800.0)
     np.testing.assert_allclose(Fz @ M.T, ref, rtol=1e-8, atol=1e-8 * np.abs(ref).max())
 
 
@@ -168,3 +231,19 @@ def test_band_compression_roundtrip():
     for j in range(30):
         R2[j, start[j]:start[j] + W.shape[1]] = W[j]
     np.testing.assert_array_equal(R2, M)
+
+
+def test_committed_vectors_equal_the_restatement():
+    """tests/golden/df_vectors.npz (read by the GPU tests) is what make_df_vectors.py writes from this
+    restatement now."""
+    import importlib.util
+    import os
+    from tests.golden_vectors import df_vectors
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "make_df_vectors.py")
+    spec = importlib.util.spec_from_file_location("make_df_vectors", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    fresh, stored = mod.vectors(), df_vectors()
+    assert set(fresh) == set(stored)
+    for k in fresh:
+        np.testing.assert_array_equal(np.asarray(fresh[k]), stored[k], err_msg=k)

@@ -47,18 +47,14 @@ def test_nearest_picks_the_first_on_a_tie():
     assert PP._nearest(np.array([0.0, 1.0, 2.0]), [0.5, 1.6, -3.0, 9.0]) == [0, 2, 0, 2]
 
 
-def test_morse_atoms_from_Z_and_padded_axes():
-    """morse_atoms_from_Z equals the element-name mixing rule (zero charges); the padded scan axis
-    continues the first scan coordinate in grid steps and holds the scan axis in its middle."""
-    from spammm_amd import params as P, runners as RN
+def test_morse_atoms_from_Z():
+    """morse_atoms_from_Z equals the element-name mixing rule (zero charges) and rejects an atomic
+    number without an element entry."""
+    from spammm_amd import params as P
     pos = np.array([[0., 0., 0.], [1.1, 0., 0.], [0., 1.2, 0.3]])
-    atoms, cMs = RN.morse_atoms_from_Z(pos, [6, 1, 8])
+    atoms, cMs = P.morse_atoms_from_Z(pos, [6, 1, 8])
     ref = P.mix_morse(["C", "H", "O"], P.ELEMENTS_REFERENCE, 1.452, 0.0006808, -1.8)
     assert np.array_equal(cMs, ref) and np.array_equal(atoms[:, :3], pos.astype(np.float32))
     assert not atoms[:, 3].any()
     with pytest.raises(KeyError):
-        RN.morse_atoms_from_Z(pos[:1], [92])
-    xs = np.arange(-2.0, 2.01, 0.2).astype(np.float32)
-    full = RN._padded_axis(xs, 3, 0.2)
-    assert len(full) == len(xs) + 6 and np.allclose(full[3:-3], xs, atol=1e-6)
-    assert np.allclose(np.diff(full), 0.2, atol=1e-6) and np.isclose(full[0], xs[0] - 0.6, atol=1e-6)
+        P.morse_atoms_from_Z(pos[:1], [92])
