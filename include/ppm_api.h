@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define PPM_API_VERSION 1
+#define PPM_API_VERSION 2
 
 /* status codes */
 #define PPM_OK                 0
@@ -73,6 +73,10 @@ typedef struct PpmRelaxCtl {
     int   f32_anchors;    /* FP64 relaxation only: 1 = tip anchors advance as the FP32 kernels
                              advance them (tip += dTip in float32, kernels/AFM.cl:556), so the
                              FP64 scan follows the FP32 scan's exact heights; 0 = in FP64 */
+    int   damped;         /* integrator of the relaxation loops: 0 = FIRE (update_FIRE, the
+                             reference's default OPT_FIRE=1 build); 1 = damped velocity
+                             v *= (1 - damp) with constant dt, damp (its OPT_FIRE=0 build,
+                             AFMulator(use_fire=False), kernels/AFM.cl:378-382, :536-540) */
 } PpmRelaxCtl;
 
 /* One molecule of a batch (an array of these lives in device memory).  The molecule's grid
@@ -323,6 +327,20 @@ int ppm_relax_strokes(const float* grid, const PpmGridSpec* spec,
                       const float relax[4], const PpmRelaxCtl* ctl,
                       float* FEs, float* disps, float* PPs, void* stream);
 
+/* Lateral-only relaxed scan at fixed heights.
+ * Replaces relaxStrokes2D (kernels/AFM.cl:396-450) as launched by AFMulator.scan_fdbm_2d
+ * (spammm/SPM/AFM.py:644-701): for point k with anchor (ax, ay, az) = points[k].xyz, slice iz
+ * fixes the probe height pz = az - iz*dh and relaxes (x, y) from the anchor by damped velocity
+ * steps (v = v (1 - damp) + F dt; p += v dt) in the grid force plus the spring -k_lat (p - anchor),
+ * at most ctl->max_iter steps, stopping when Fx^2 + Fy^2 < ctl->f2conv after a step
+ * (ctl->check_before and ctl->damped are ignored).
+ *   FEs (n_scan,nz,4): sample at the final position; disps (n_scan,nz,4) or NULL:
+ *   (x - ax, y - ay, 0, 0); PPs or NULL: (x, y, pz, +/-iterations) as ppm_relax_strokes_tilted. */
+int ppm_relax_strokes_2d(const float* grid, const PpmGridSpec* spec,
+                         const float* points, int n_scan, int nz,
+                         float k_lat, float dh, float dt, float damp, const PpmRelaxCtl* ctl,
+                         float* FEs, float* disps, float* PPs, void* stream);
+
 /* Unrelaxed sampling along strokes.
  * Replaces getFEinStrokesTilted (kernels/AFM.cl:210-233) as launched by
  * AFMulator.get_raw_FE (spammm/SPM/AFM.py:2523-2580).  tip supplies tipA/B/C;
@@ -349,6 +367,18 @@ int ppm_relax_adjoint(const float* grid, const PpmGridSpec* spec,
                       const float* points, int n_scan, int nz, const PpmTip* tip,
                       const float* PPs, const float* dL_dFEs,
                       float* grid_cot, float* lambdas, void* stream);
+
+/* Per-state diagnostics of relaxed states for the trilinear grid model: the adjoint_diag record
+ * of the reference's direct-path state adjoint (morseDirectStateAdjoint, kernels/AFM.cl:1774-1844)
+ * on the grid path, without a cotangent.  diag (n_scan,nz,4) = (|G(q*)|, smallest eigenvalue of
+ * the symmetric part of -J, infinity-norm condition number of J, status bits of
+ * ppm_relax_adjoint: 1 residual, 2 non-finite, 4 singular/ill-conditioned, 8 unstable).  A small
+ * smallest eigenvalue marks a probe near a fold of its equilibrium branch, where a small change
+ * of the field moves it to another branch; callers exclude such probes from a loss before they
+ * differentiate it.  points/tip/PPs as for ppm_relax_adjoint. */
+int ppm_relax_state_diag(const float* grid, const PpmGridSpec* spec,
+                         const float* points, int n_scan, int nz, const PpmTip* tip,
+                         const float* PPs, float* diag, void* stream);
 
 /* Newton polish of converged states (no reference counterpart; used by the differentiable
  * path so that the forward sits on the equilibrium the implicit-function adjoint assumes).

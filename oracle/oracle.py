@@ -89,10 +89,13 @@ def lib():
         L.oracle_interp_f64.argtypes = [_f, _i, _f, _f, _f, _d, c_int, _d]
         for nm in ("oracle_relax_tilted_f64", "oracle_relax_tilted_f32"):
             getattr(L, nm).argtypes = [_f, _i, _f, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, _f, _f, _f,
-                                       c_int, c_float, c_int, _f, _f]
+                                       c_int, c_float, c_int, _f, _f, c_int]
         for nm in ("oracle_relax_strokes_f64", "oracle_relax_strokes_f32"):
             getattr(L, nm).argtypes = [_f, _i, _f, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, c_int, c_float,
-                                       _f, _f, _f]
+                                       _f, _f, _f, c_int]
+        for nm in ("oracle_relax_2d_f64", "oracle_relax_2d_f32"):
+            getattr(L, nm).argtypes = [_f, _i, _f, _f, _f, _f, c_int, c_int, c_float, c_float, c_float, c_float,
+                                       c_int, c_float, _f, _f, _f]
         for nm in ("oracle_fe_in_strokes_tilted_f64", "oracle_fe_in_strokes_tilted_f32"):
             getattr(L, nm).argtypes = [_f, _i, _f, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, _f, _f]
         L.oracle_grid_project_f64g.argtypes = [c_int, _d, _d, c_int, c_int, _d, _d, c_int, c_int, _d, _d, _i,
@@ -216,8 +219,9 @@ def interp(grid, n, dinvA, dinvB, dinvC, pos):
 
 
 def relax_tilted(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF,
-                 max_iter=128, f2conv=1e-8, check_before=False, precision="f64"):
-    """Returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4)) float32."""
+                 max_iter=128, f2conv=1e-8, check_before=False, precision="f64", damped=False):
+    """Returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4)) float32.  ``damped``: the damped-velocity
+    integrator of the OPT_FIRE=0 build (kernels/AFM.cl:536-540) instead of update_FIRE."""
     g, nn, a, b, c = _grid_args(grid, n, dinvA, dinvB, dinvC)
     pts = _cf(pts).reshape(-1, 4)
     ns = pts.shape[0]
@@ -226,13 +230,15 @@ def relax_tilted(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC, stiffn
     vs = [_vec(x, 4) for x in (tipA, tipB, tipC, stiffness, dpos0, relax, surfFF)]
     fn = lib().oracle_relax_tilted_f64 if precision == "f64" else lib().oracle_relax_tilted_f32
     fn(_p(g, _f), _p(nn, _i), _p(a, _f), _p(b, _f), _p(c, _f), _p(pts, _f), ns, int(nz),
-       *[_p(v, _f) for v in vs], int(max_iter), float(f2conv), int(bool(check_before)), _p(FEs, _f), _p(PPs, _f))
+       *[_p(v, _f) for v in vs], int(max_iter), float(f2conv), int(bool(check_before)), _p(FEs, _f), _p(PPs, _f),
+       int(bool(damped)))
     return FEs, PPs
 
 
 def relax_strokes(grid, n, dinvA, dinvB, dinvC, pts, nz, dTip, stiffness, dpos0, relax, max_iter=128,
-                  f2conv=1e-8, precision="f64"):
-    """Returns (FEs, disps, PPs), each (n_scan,nz,4) float32."""
+                  f2conv=1e-8, precision="f64", damped=False):
+    """Returns (FEs, disps, PPs), each (n_scan,nz,4) float32.  ``damped``: the damped-velocity
+    integrator of the OPT_FIRE=0 build (kernels/AFM.cl:378-382) instead of update_FIRE."""
     g, nn, a, b, c = _grid_args(grid, n, dinvA, dinvB, dinvC)
     pts = _cf(pts).reshape(-1, 4)
     ns = pts.shape[0]
@@ -242,7 +248,22 @@ def relax_strokes(grid, n, dinvA, dinvB, dinvC, pts, nz, dTip, stiffness, dpos0,
     vs = [_vec(x, 4) for x in (dTip, stiffness, dpos0, relax)]
     fn = lib().oracle_relax_strokes_f64 if precision == "f64" else lib().oracle_relax_strokes_f32
     fn(_p(g, _f), _p(nn, _i), _p(a, _f), _p(b, _f), _p(c, _f), _p(pts, _f), ns, int(nz), *[_p(v, _f) for v in vs],
-       int(max_iter), float(f2conv), _p(FEs, _f), _p(disps, _f), _p(PPs, _f))
+       int(max_iter), float(f2conv), _p(FEs, _f), _p(disps, _f), _p(PPs, _f), int(bool(damped)))
+    return FEs, disps, PPs
+
+
+def relax_2d(grid, n, dinvA, dinvB, dinvC, pts, nz, k_lat, dh, dt, damp, max_iter=128, f2conv=1e-8,
+             precision="f64"):
+    """relaxStrokes2D (kernels/AFM.cl:396-450): returns (FEs, disps, PPs), each (n_scan,nz,4) float32."""
+    g, nn, a, b, c = _grid_args(grid, n, dinvA, dinvB, dinvC)
+    pts = _cf(pts).reshape(-1, 4)
+    ns = pts.shape[0]
+    FEs = np.zeros((ns, nz, 4), dtype=np.float32)
+    disps = np.zeros((ns, nz, 4), dtype=np.float32)
+    PPs = np.zeros((ns, nz, 4), dtype=np.float32)
+    fn = lib().oracle_relax_2d_f64 if precision == "f64" else lib().oracle_relax_2d_f32
+    fn(_p(g, _f), _p(nn, _i), _p(a, _f), _p(b, _f), _p(c, _f), _p(pts, _f), ns, int(nz), float(k_lat), float(dh),
+       float(dt), float(damp), int(max_iter), float(f2conv), _p(FEs, _f), _p(disps, _f), _p(PPs, _f))
     return FEs, disps, PPs
 
 

@@ -142,6 +142,14 @@ template <class R> inline V3<R> update_FIRE(V3<R> f, V3<R> v, R& dt, R& damp, R 
     return v;
 }
 
+// The integrator step before `v += f dt; pos += v dt`: update_FIRE (OPT_FIRE=1), or the damped
+// velocity `v *= (1 - damp)` of the OPT_FIRE=0 build (kernels/AFM.cl:378-382, :536-540), in which
+// dt and damp keep their initial values.
+template <class R> inline V3<R> integrate(int damped, V3<R> f, V3<R> v, R& dt, R& damp, R dtmin, R dtmax, R damp0) {
+    if (damped) return mul(v, (R)1.0f - damp);
+    return update_FIRE<R>(f, v, dt, damp, dtmin, dtmax, damp0);
+}
+
 // relaxStrokesTilted   kernels/AFM.cl:478-562, plus PP telemetry (pos, +iters on
 // convergence / -iters otherwise) in the convention of relaxStrokesTiltedMorseDirect
 // (kernels/AFM.cl:1494-1505).  check_before=1 moves the |f|^2 test before the update
@@ -151,7 +159,7 @@ template <class R> inline V3<R> update_FIRE(V3<R> f, V3<R> v, R& dt, R& damp, R 
 template <class R, class SRC, class P>
 void relax_tilted_src(const SRC& sample, const P* pts, int n_scan, int nz, const P* tipA, const P* tipB,
                       const P* tipC, const P* stiffness, const P* dpos0, const P* relax, const P* surfFF, int max_iter,
-                      double f2conv_f, int check_before, P* FEs, P* PPs) {
+                      double f2conv_f, int check_before, P* FEs, P* PPs, int damped = 0) {
     const V3<R> A = ld3<R>(tipA), B = ld3<R>(tipB), C = ld3<R>(tipC);
     const R f2conv = (R)f2conv_f;
 #pragma omp parallel for schedule(dynamic, 8)
@@ -176,7 +184,7 @@ void relax_tilted_src(const SRC& sample, const P* pts, int n_scan, int nz, const
                 f = add(f, mul(C, (R)surfFF[0]));
                 R ff = dot(f, f);
                 if (check_before && ff < f2conv) { conv = true; break; }
-                v = update_FIRE<R>(f, v, dt, damp, dtmin, dtmax, damp0);
+                v = integrate<R>(damped, f, v, dt, damp, dtmin, dtmax, damp0);
                 v = add(v, mul(f, dt));
                 pos = add(pos, mul(v, dt));
                 if (!check_before && ff < f2conv) { conv = true; break; }
@@ -201,16 +209,17 @@ void relax_tilted_src(const SRC& sample, const P* pts, int n_scan, int nz, const
 template <class R, class GT, class P>
 void relax_tilted(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P* tipA, const P* tipB, const P* tipC,
                   const P* stiffness, const P* dpos0, const P* relax, const P* surfFF, int max_iter, double f2conv_f,
-                  int check_before, P* FEs, P* PPs) {
+                  int check_before, P* FEs, P* PPs, int damped = 0) {
     relax_tilted_src<R>([&g](V3<R> p) { return interp<R>(g, p); }, pts, n_scan, nz, tipA, tipB, tipC, stiffness,
-                        dpos0, relax, surfFF, max_iter, f2conv_f, check_before, FEs, PPs);
+                        dpos0, relax, surfFF, max_iter, f2conv_f, check_before, FEs, PPs, damped);
 }
 
 // relaxStrokes   kernels/AFM.cl:345-394.  FE output is the sample taken at the start
 // of the last iteration (before the final update); disps = pos - (tipPos + dpos0).
 template <class R, class GT, class P>
 void relax_strokes(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P* dTip_, const P* stiffness,
-                   const P* dpos0, const P* relax, int max_iter, double f2conv_f, P* FEs, P* disps, P* PPs) {
+                   const P* dpos0, const P* relax, int max_iter, double f2conv_f, P* FEs, P* disps, P* PPs,
+                   int damped = 0) {
     const R f2conv = (R)f2conv_f;
 #pragma omp parallel for schedule(dynamic, 8)
     for (int ip = 0; ip < n_scan; ++ip) {
@@ -230,7 +239,7 @@ void relax_strokes(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P
                 V3<R> f = v3<R>(fe.x, fe.y, fe.z);
                 f = add(f, tipForce<R>(sub(pos, tipPos), stiffness, dpos0));
                 R ff = dot(f, f);
-                v = update_FIRE<R>(f, v, dt, damp, dtmin, dtmax, damp0);
+                v = integrate<R>(damped, f, v, dt, damp, dtmin, dtmax, damp0);
                 v = add(v, mul(f, dt));
                 pos = add(pos, mul(v, dt));
                 if (ff < f2conv) { conv = true; break; }
@@ -250,6 +259,47 @@ void relax_strokes(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P
             }
             tipPos = add(tipPos, dTip);
             pos = add(pos, dTip);
+        }
+    }
+}
+
+// relaxStrokes2D   kernels/AFM.cl:396-450: z fixed per slice at az - iz*dh, x,y relaxed from the
+// anchor by damped velocity steps in the grid force plus -k_lat (p - anchor); the |F_xy|^2 test
+// follows the update.  FEs = sample at the final position, disps = (x - ax, y - ay, 0, 0), PPs =
+// (x, y, pz, +/-iterations).
+template <class R, class GT, class P>
+void relax_2d(const GridT<GT>& g, const P* pts, int n_scan, int nz, double k_lat_f, double dh_f, double dt_f,
+              double damp_f, int max_iter, double f2conv_f, P* FEs, P* disps, P* PPs) {
+    const R k_lat = (R)k_lat_f, dh = (R)dh_f, dt = (R)dt_f, damp = (R)damp_f, f2conv = (R)f2conv_f;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int ip = 0; ip < n_scan; ++ip) {
+        const R ax = (R)pts[4 * (size_t)ip], ay = (R)pts[4 * (size_t)ip + 1], az = (R)pts[4 * (size_t)ip + 2];
+        for (int iz = 0; iz < nz; ++iz) {
+            const R pz = az - (R)iz * dh;
+            R px = ax, py = ay, vx = 0, vy = 0;
+            int it = 0;
+            bool conv = false;
+            for (it = 0; it < max_iter; ++it) {
+                FE<R> fe = interp<R>(g, v3<R>(px, py, pz));
+                const R fx = fe.x - k_lat * (px - ax);
+                const R fy = fe.y - k_lat * (py - ay);
+                vx = vx * ((R)1.0f - damp) + fx * dt;
+                vy = vy * ((R)1.0f - damp) + fy * dt;
+                px += vx * dt;
+                py += vy * dt;
+                if ((fx * fx + fy * fy) < f2conv) { conv = true; break; }
+            }
+            const int iters = conv ? it + 1 : max_iter;
+            FE<R> fe = interp<R>(g, v3<R>(px, py, pz));
+            const size_t o = 4 * ((size_t)ip * nz + iz);
+            FEs[o + 0] = (P)fe.x; FEs[o + 1] = (P)fe.y; FEs[o + 2] = (P)fe.z; FEs[o + 3] = (P)fe.w;
+            if (disps) { disps[o + 0] = (P)(px - ax); disps[o + 1] = (P)(py - ay); disps[o + 2] = 0; disps[o + 3] = 0; }
+            const bool finite = std::isfinite(fe.x) && std::isfinite(fe.y) && std::isfinite(fe.z) &&
+                                std::isfinite(fe.w) && std::isfinite(px) && std::isfinite(py);
+            if (PPs) {
+                PPs[o + 0] = (P)px; PPs[o + 1] = (P)py; PPs[o + 2] = (P)pz;
+                PPs[o + 3] = (P)((conv && finite) ? iters : -iters);
+            }
         }
     }
 }
@@ -549,10 +599,10 @@ void oracle_interp_f64(const float* grid, const int* n, const float* dinvA, cons
     void NAME(const float* grid, const int* n, const float* dinvA, const float* dinvB, const float* dinvC,          \
               const float* pts, int n_scan, int nz, const float* tipA, const float* tipB, const float* tipC,        \
               const float* stiffness, const float* dpos0, const float* relax, const float* surfFF, int max_iter,    \
-              float f2conv, int check_before, float* FEs, float* PPs) {                                             \
+              float f2conv, int check_before, float* FEs, float* PPs, int damped) {                                 \
         Grid g = make_grid(grid, n, dinvA, dinvB, dinvC);                                                           \
         relax_tilted<R>(g, pts, n_scan, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF, max_iter, f2conv,    \
-                        check_before, FEs, PPs);                                                                    \
+                        check_before, FEs, PPs, damped);                                                            \
     }
 RELAX_TILTED(oracle_relax_tilted_f64, double)
 RELAX_TILTED(oracle_relax_tilted_f32, float)
@@ -560,12 +610,23 @@ RELAX_TILTED(oracle_relax_tilted_f32, float)
 #define RELAX_STROKES(NAME, R)                                                                                     \
     void NAME(const float* grid, const int* n, const float* dinvA, const float* dinvB, const float* dinvC,          \
               const float* pts, int n_scan, int nz, const float* dTip, const float* stiffness, const float* dpos0,  \
-              const float* relax, int max_iter, float f2conv, float* FEs, float* disps, float* PPs) {               \
+              const float* relax, int max_iter, float f2conv, float* FEs, float* disps, float* PPs, int damped) {   \
         Grid g = make_grid(grid, n, dinvA, dinvB, dinvC);                                                           \
-        relax_strokes<R>(g, pts, n_scan, nz, dTip, stiffness, dpos0, relax, max_iter, f2conv, FEs, disps, PPs);     \
+        relax_strokes<R>(g, pts, n_scan, nz, dTip, stiffness, dpos0, relax, max_iter, f2conv, FEs, disps, PPs,      \
+                         damped);                                                                                   \
     }
 RELAX_STROKES(oracle_relax_strokes_f64, double)
 RELAX_STROKES(oracle_relax_strokes_f32, float)
+
+#define RELAX_2D(NAME, R)                                                                                          \
+    void NAME(const float* grid, const int* n, const float* dinvA, const float* dinvB, const float* dinvC,          \
+              const float* pts, int n_scan, int nz, float k_lat, float dh, float dt, float damp, int max_iter,      \
+              float f2conv, float* FEs, float* disps, float* PPs) {                                                 \
+        Grid g = make_grid(grid, n, dinvA, dinvB, dinvC);                                                           \
+        relax_2d<R>(g, pts, n_scan, nz, k_lat, dh, dt, damp, max_iter, f2conv, FEs, disps, PPs);                    \
+    }
+RELAX_2D(oracle_relax_2d_f64, double)
+RELAX_2D(oracle_relax_2d_f32, float)
 
 #define FE_STROKES(NAME, R)                                                                                        \
     void NAME(const float* grid, const int* n, const float* dinvA, const float* dinvB, const float* dinvC,          \

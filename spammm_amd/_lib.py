@@ -41,7 +41,7 @@ class PpmTip(ctypes.Structure):
 
 class PpmRelaxCtl(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int), ("f2conv", ctypes.c_float), ("check_before", ctypes.c_int),
-                ("scan_ny", ctypes.c_int), ("f32_anchors", ctypes.c_int)]
+                ("scan_ny", ctypes.c_int), ("f32_anchors", ctypes.c_int), ("damped", ctypes.c_int)]
 
 
 EXPORTS = (
@@ -54,6 +54,7 @@ EXPORTS = (
     "ppm_fdbm_pad_roll", "ppm_fdbm_flip3", "ppm_fdbm_dispersion", "ppm_fdbm_compose_gradient",
     "ppm_ewald_num_g", "ppm_ewald_system_bytes", "ppm_ewald_prepare", "ppm_ewald_eval", "ppm_ewald_brute",
     "ppm_ewald_cluster", "ppm_ewald_grid_workspace_size", "ppm_ewald_tip_grid", "ppm_ewald_kept_count_offset",
+    "ppm_relax_state_diag", "ppm_relax_strokes_2d",
 )
 
 _lib = None
@@ -146,6 +147,10 @@ def load():
     L.ppm_axis_operator.argtypes = [vp, ll, ll, ll, _F4, i, i, i, i, vp, vp, i, i, f, i, vp, vp]
     L.ppm_relax_adjoint.restype = i
     L.ppm_relax_adjoint.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), vp, vp, vp, vp, vp]
+    L.ppm_relax_strokes_2d.restype = i
+    L.ppm_relax_strokes_2d.argtypes = [vp, P(PpmGridSpec), vp, i, i, f, f, f, f, P(PpmRelaxCtl), vp, vp, vp, vp]
+    L.ppm_relax_state_diag.restype = i
+    L.ppm_relax_state_diag.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), vp, vp, vp]
     L.ppm_batch_workspace_size.restype = ctypes.c_size_t
     L.ppm_batch_workspace_size.argtypes = [i]
     L.ppm_batch_forward.restype = i

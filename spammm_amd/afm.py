@@ -108,9 +108,11 @@ class AFMulator:
     F2CONV = 1e-8
 
     def __init__(self, cl_src_dir=None, use_morse=False, nloc=32, use_fire=True, device=None):
-        if not use_fire:
-            raise NotImplementedError("only the FIRE integrator (OPT_FIRE=1) is built")
+        # use_fire picks the integrator of every relaxation this instance runs (run_scan, scan_fdbm,
+        # run_scan_morse_direct), as the reference compiles its program with -D OPT_FIRE=<use_fire>
+        # (spammm/SPM/AFM.py:320-321): FIRE, or the damped velocity v *= (1 - damp).
         self.device = D.require_gpu(device)
+        self.use_fire = bool(use_fire)
         self.use_morse = use_morse
         self.nloc = nloc
         self.mol = self.elem_types = None
@@ -317,7 +319,7 @@ class AFMulator:
         pts = D.scan_points(scan_p0, scan_da, scan_db, nx_s, ny_s)
         pts_d = torch.as_tensor(pts, device=self.device)
         FEs, PPs = D.relax_tilted(self.FF, self.spec, pts_d, int(nz), self._tip(dtip), self.max_iter, self.f2conv,
-                                  check_before, scan_ny=ny_s)
+                                  check_before, scan_ny=ny_s, damped=not self.use_fire)
         self.scan_FEs_dev, self.scan_PPs_dev = FEs, PPs
         FEs_h = FEs.cpu().numpy().reshape(nx_s, ny_s, nz, 4)
         return FEs_h, pts[:, :3].reshape(nx_s, ny_s, 3)
@@ -388,7 +390,8 @@ class AFMulator:
         atoms = torch.as_tensor(self.atoms_arr, device=dev)
         cMs = torch.as_tensor(self.cLJs_arr, device=dev)
         FEs, PPs = D.relax_direct_morse(atoms, cMs, torch.as_tensor(pts, device=dev), nz, self._tip(dtip), self.tipQs,
-                                        self.tipQZs, self.max_iter, self.f2conv, scan_ny=ny_s)
+                                        self.tipQZs, self.max_iter, self.f2conv, scan_ny=ny_s,
+                                        damped=not self.use_fire)
         FEs_h = FEs.cpu().numpy().reshape(nx_s, ny_s, nz, 4)
         PPs_h = PPs.cpu().numpy().reshape(nx_s, ny_s, nz, 4)
         points_xyz = pts[:, :3].reshape(nx_s, ny_s, 3)
@@ -543,11 +546,12 @@ class AFMulator:
     def scan_fdbm(self, scan_xs, scan_ys, probe_heights, mol_z=0.0, ppm_mode=True, use_fire=True, K_LAT=0.03,
                   K_RAD=20.0, bond_length=3.0, stiffness=None, dpos0=None, relax_pars=None, osc_dir=(0., 0., 1.),
                   base_pos=(0., 0., 0.), reverse=False):
-        """relaxStrokes over scan_xs x scan_ys; FEs (nx,ny,nz,4) with iz=0 the lowest height, and tip_disp."""
+        """relaxStrokes over scan_xs x scan_ys; FEs (nx,ny,nz,4) with iz=0 the lowest height, and tip_disp.
+        ``use_fire`` chooses the default relax_pars only, as in the reference (spammm/SPM/AFM.py:572-580):
+        the instance's (FIRE) or (dt, damp) = (0.3, 0.2) of the damped CPU twin; the integrator itself is
+        the one the instance was built with (AFMulator(use_fire=...))."""
         import torch
         assert self.img_FF_fdbm is not None, "call setup_fdbm_grid() first"
-        if not use_fire:
-            raise NotImplementedError("only the FIRE integrator (OPT_FIRE=1) is built")
         nx_s, ny_s, nz_s = len(scan_xs), len(scan_ys), len(probe_heights)
         self._check_osc(osc_dir)
         base = np.asarray(base_pos, dtype=np.float32)
@@ -557,7 +561,12 @@ class AFMulator:
                        else np.array(stiffness, dtype=np.float32))
         dpos0_v = (np.array([0., 0., -bond_length, bond_length], dtype=np.float32) if dpos0 is None
                    else np.array(dpos0, dtype=np.float32))
-        relax_v = self.relax_pars.copy() if relax_pars is None else np.array(relax_pars, dtype=np.float32)
+        if relax_pars is not None:
+            relax_v = np.array(relax_pars, dtype=np.float32)
+        elif use_fire:
+            relax_v = self.relax_pars.copy()
+        else:
+            relax_v = np.array([0.3, 0.2, 0.03, 0.3], dtype=np.float32)
         hs = np.sort(probe_heights)
         dh = float(hs[-1] - hs[0]) / max(nz_s - 1, 1) if nz_s > 1 else 0.
         if reverse:
@@ -573,11 +582,33 @@ class AFMulator:
         pts[:, 2] = z_start
         pts_d = torch.as_tensor(pts, device=self.device)
         FEs, disps, _ = D.relax_strokes(self.img_FF_fdbm, self.fdbm_spec, pts_d, nz_s, dTip, stiffness_v, dpos0_v,
-                                        relax_v, self.max_iter, self.f2conv, scan_ny=ny_s, want_pps=False)
+                                        relax_v, self.max_iter, self.f2conv, scan_ny=ny_s, want_pps=False,
+                                        damped=not self.use_fire)
         FEs = FEs.cpu().numpy().reshape(nx_s, ny_s, nz_s, 4)
         disps = disps.cpu().numpy().reshape(nx_s, ny_s, nz_s, 4)
         if not reverse:
             FEs, disps = FEs[:, :, ::-1, :], disps[:, :, ::-1, :]
+        return FEs, {"dx": disps[..., 0], "dy": disps[..., 1], "dz": disps[..., 2]}
+
+    def scan_fdbm_2d(self, scan_xs, scan_ys, probe_heights, mol_z=0.0, K_LAT=0.03, dt=0.3, damp=0.2):
+        """relaxStrokes2D over scan_xs x scan_ys (spammm/SPM/AFM.py:644-701): lateral-only damped
+        relaxation at fixed probe heights; FEs (nx,ny,nz,4) with iz=0 the lowest height, and tip_disp
+        (dz = 0)."""
+        import torch
+        assert self.img_FF_fdbm is not None, "call setup_fdbm_grid() first"
+        nx_s, ny_s, nz_s = len(scan_xs), len(scan_ys), len(probe_heights)
+        hd = np.sort(probe_heights)[::-1]
+        dh = float(hd[0] - hd[-1]) / max(nz_s - 1, 1) if nz_s > 1 else 0.
+        gx, gy = np.meshgrid(scan_xs, scan_ys, indexing="ij")
+        pts = np.zeros((nx_s * ny_s, 4), dtype=np.float32)
+        pts[:, 0] = gx.ravel()
+        pts[:, 1] = gy.ravel()
+        pts[:, 2] = float(hd[0]) + mol_z
+        pts_d = torch.as_tensor(pts, device=self.device)
+        FEs, disps, _ = D.relax_strokes_2d(self.img_FF_fdbm, self.fdbm_spec, pts_d, nz_s, K_LAT, dh, dt, damp,
+                                           self.max_iter, self.f2conv, scan_ny=ny_s)
+        FEs = FEs.cpu().numpy().reshape(nx_s, ny_s, nz_s, 4)[:, :, ::-1, :]
+        disps = disps.cpu().numpy().reshape(nx_s, ny_s, nz_s, 4)[:, :, ::-1, :]
         return FEs, {"dx": disps[..., 0], "dy": disps[..., 1], "dz": disps[..., 2]}
 
     # ── frequency shift ──────────────────────────────────────────────────────

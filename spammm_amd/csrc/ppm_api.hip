@@ -67,6 +67,7 @@ static int check_ctl(const char* fn, const PpmRelaxCtl* c, int n_scan) {
         return PPM_ERR_INVALID_ARG;
     }
     if (!std::isfinite(c->f2conv) || c->f2conv < 0.f) { set_error("%s: invalid f2conv", fn); return PPM_ERR_INVALID_ARG; }
+    if (c->damped != 0 && c->damped != 1) { set_error("%s: damped=%d is not 0 or 1", fn, c->damped); return PPM_ERR_INVALID_ARG; }
     if (c->scan_ny < 0 || (c->scan_ny > 0 && n_scan % c->scan_ny != 0)) {
         set_error("%s: scan_ny=%d does not divide n_scan=%d", fn, c->scan_ny, n_scan);
         return PPM_ERR_INVALID_ARG;
@@ -362,6 +363,23 @@ int ppm_relax_strokes(const float* grid, const PpmGridSpec* spec, const float* p
                                 (hipStream_t)stream);
 }
 
+int ppm_relax_strokes_2d(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
+                         float k_lat, float dh, float dt, float damp, const PpmRelaxCtl* ctl, float* FEs, float* disps,
+                         float* PPs, void* stream) {
+    const char* fn = "ppm_relax_strokes_2d";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (int r = check_ctl(fn, ctl, n_scan)) return r;
+    if (!grid) { set_error("%s: NULL grid", fn); return PPM_ERR_INVALID_ARG; }
+    if (!std::isfinite(k_lat) || !std::isfinite(dh) || !std::isfinite(dt) || !std::isfinite(damp)) {
+        set_error("%s: non-finite k_lat/dh/dt/damp", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return launch_relax_strokes_2d(grid, *spec, points, n_scan, nz, k_lat, dh, dt, damp, *ctl, FEs, disps, PPs,
+                                   (hipStream_t)stream);
+}
+
 int ppm_fe_in_strokes_tilted(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
                              const PpmTip* tip, const float dTip[4], const float dpos0[4], float* FEs, void* stream) {
     const char* fn = "ppm_fe_in_strokes_tilted";
@@ -393,6 +411,16 @@ int ppm_relax_polish(const float* grid, const PpmGridSpec* spec, const float* po
     if (!grid || !tip) { set_error("%s: NULL grid/tip", fn); return PPM_ERR_INVALID_ARG; }
     if (steps < 0 || steps > 64) { set_error("%s: steps=%d out of [0,64]", fn, steps); return PPM_ERR_INVALID_ARG; }
     return launch_relax_polish(grid, *spec, points, n_scan, nz, *tip, steps, PPs, FEs, (hipStream_t)stream);
+}
+
+int ppm_relax_state_diag(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
+                         const PpmTip* tip, const float* PPs, float* diag, void* stream) {
+    const char* fn = "ppm_relax_state_diag";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (int r = check_scan(fn, points, n_scan, nz, PPs)) return r;
+    if (!grid || !tip || !diag) { set_error("%s: NULL argument", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_relax_state_diag(grid, *spec, points, n_scan, nz, *tip, PPs, diag, (hipStream_t)stream);
 }
 
 static int check_direct(const char* fn, const float* atoms, const float* cMs, int na, const float* points,

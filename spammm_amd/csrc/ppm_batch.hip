@@ -76,6 +76,7 @@ __device__ GridArgs mol_grid_args(const PpmMolDesc& d, const PpmBatchParams& P) 
 __device__ rc::TiltArgs tilt_args(const PpmBatchParams& P) {
     rc::TiltArgs t{};
     t.nz = P.nz; t.max_iter = P.ctl.max_iter; t.check_before = P.ctl.check_before; t.f2conv = P.ctl.f2conv;
+    t.damped = P.ctl.damped != 0;
     for (int q = 0; q < 3; ++q) { t.A[q] = P.tip.tipA[q]; t.B[q] = P.tip.tipB[q]; t.C[q] = P.tip.tipC[q]; }
     t.dtip = P.tip.tipC[3];
     for (int q = 0; q < 4; ++q) { t.k[q] = P.tip.stiffness[q]; t.d0[q] = P.tip.dpos0[q]; }

@@ -45,6 +45,11 @@ int launch_relax_strokes(const float* grid, const PpmGridSpec& g, const float* p
 int launch_relax_adjoint(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
                          const PpmTip& tip, const float* PPs, const float* dL_dFEs, float* grid_cot, float* lambdas,
                          hipStream_t stream);
+int launch_relax_strokes_2d(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
+                            float k_lat, float dh, float dt, float damp, const PpmRelaxCtl& ctl, float* FEs,
+                            float* disps, float* PPs, hipStream_t stream);
+int launch_relax_state_diag(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
+                            const PpmTip& tip, const float* PPs, float* diag, hipStream_t stream);
 int launch_relax_polish(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
                         const PpmTip& tip, int steps, float* PPs, float* FEs, hipStream_t stream);
 int launch_grid_project_f64(int model, const double* atoms, const double* cpar, int na, const PpmGridSpec& g,

@@ -384,7 +384,7 @@ int launch_direct_relax(const float* atoms, const float* cms, int na, const floa
         }
     rc::TiltArgs p{};
     p.n_scan = n_scan; p.nz = nz; p.max_iter = ctl.max_iter; p.check_before = 1;
-    p.scan_ny = ctl.scan_ny; p.f2conv = ctl.f2conv;
+    p.scan_ny = ctl.scan_ny; p.f2conv = ctl.f2conv; p.damped = ctl.damped != 0;
     for (int q = 0; q < 3; ++q) { p.A[q] = tip.tipA[q]; p.B[q] = tip.tipB[q]; p.C[q] = tip.tipC[q]; }
     p.dtip = tip.tipC[3];
     for (int q = 0; q < 4; ++q) { p.k[q] = tip.stiffness[q]; p.d0[q] = tip.dpos0[q]; }

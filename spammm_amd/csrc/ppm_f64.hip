@@ -112,7 +112,7 @@ __device__ __forceinline__ D3 operator*(D3 a, double s) { return d3(a.x * s, a.y
 __device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
 struct Relax64Args {
-    int nx, ny, nz_grid, n_scan, nz, max_iter, check_before, f32_anchors;
+    int nx, ny, nz_grid, n_scan, nz, max_iter, check_before, f32_anchors, damped;
     double f2conv;
     double dinv[3][4];
     double A[3], B[3], C[3], dtip, k[4], d0[4], dt0, damp0, surf;
@@ -182,9 +182,11 @@ __global__ __launch_bounds__(64) void k_relax_tilted_f64(const double* __restric
             const D3 f = d3(fe[0], fe[1], fe[2]) + rotT(ft) + C * r.surf;
             const double ff = dot(f, f);
             if (r.check_before && ff < r.f2conv) { conv = true; break; }
-            // update_FIRE (kernels/AFM.cl:121-157)
+            // update_FIRE (kernels/AFM.cl:121-157), or v *= (1 - damp) (OPT_FIRE=0, kernels/AFM.cl:536-540)
             const double vf = dot(v, f);
-            if (vf < 0.0) {
+            if (r.damped) {
+                v = v * (1.0 - damp);
+            } else if (vf < 0.0) {
                 v = d3(0, 0, 0);
                 dt = fmax(dtmin, dt * (double)0.5f);
                 damp = r.damp0;
@@ -256,6 +258,7 @@ int launch_relax_tilted_f64(const double* grid, const PpmGridSpec& g, const doub
     r.nx = g.nx; r.ny = g.ny; r.nz_grid = g.nz; r.n_scan = n_scan; r.nz = nz;
     r.max_iter = ctl.max_iter; r.check_before = ctl.check_before; r.f2conv = ctl.f2conv;
     r.f32_anchors = ctl.f32_anchors != 0;
+    r.damped = ctl.damped != 0;
     for (int q = 0; q < 4; ++q) { r.dinv[0][q] = g.dinvA[q]; r.dinv[1][q] = g.dinvB[q]; r.dinv[2][q] = g.dinvC[q]; }
     for (int q = 0; q < 3; ++q) { r.A[q] = tip.tipA[q]; r.B[q] = tip.tipB[q]; r.C[q] = tip.tipC[q]; }
     r.dtip = tip.tipC[3];

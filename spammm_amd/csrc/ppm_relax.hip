@@ -75,6 +75,7 @@ int group_waves(int n_scan, int scan_ny, int g) {
 
 struct StrokeArgs {
     int n_scan, nz, max_iter, scan_ny;
+    int damped;             // 1: damped-velocity integrator (OPT_FIRE=0), 0: FIRE
     float f2conv;
     float dTip[3];
     float k[4], d0[4];
@@ -102,7 +103,7 @@ __device__ __forceinline__ void strokes_lane(SAMPLE& smp, int ip, const float4* 
         const f3 ft = tipForce(mk3(pos.x - tip.x, pos.y - tip.y, pos.z - tip.z), p.k, p.d0);
         const f3 f = mk3(fe.x + ft.x, fe.y + ft.y, fe.z + ft.z);
         const float ff = dot3(f, f);
-        v = fire(f, v, dt, damp, dtmin, dtmax, damp0);
+        v = integrate(p.damped, f, v, dt, damp, dtmin, dtmax, damp0);
         v = mk3(v.x + f.x * dt, v.y + f.y * dt, v.z + f.z * dt);
         pos = mk3(pos.x + v.x * dt, pos.y + v.y * dt, pos.z + v.z * dt);
         const bool conv = ff < p.f2conv;
@@ -146,6 +147,73 @@ __global__ __launch_bounds__(64) void k_relax_strokes(GridDev g, const float4* _
         cell.i = -1; cell.j = -1; cell.k = -1;
         auto smp = [&](float x, float y, float z) { return sample_group<G>(g, cell, sub, x, y, z); };
         strokes_lane(smp, ip, pts, p, FEs, disps, PPs, sub == 0);
+    }
+}
+
+struct Strokes2DArgs {
+    int n_scan, nz, max_iter, scan_ny;
+    float f2conv, k_lat, dh, dt, damp;
+};
+
+// relaxStrokes2D (kernels/AFM.cl:396-450) for one scan point: the probe height is fixed per slice
+// (pz = az - iz*dh), only x,y relax, from the anchor at every slice, by damped velocity steps
+// against the lateral spring -k_lat (p - anchor); the |F_xy|^2 test follows the update.  FEs: the
+// sample at the final position; disps: (x - ax, y - ay, 0, 0); PPs (telemetry): position and
+// +iterations (converged) / -iterations.
+template <class SAMPLE>
+__device__ __forceinline__ void strokes2d_lane(SAMPLE& smp, int ip, const float4* __restrict__ pts,
+                                               const Strokes2DArgs& p, float4* __restrict__ FEs,
+                                               float4* __restrict__ disps, float4* __restrict__ PPs, bool writer) {
+    const float4 P = pts[ip];
+    const float ax = P.x, ay = P.y, az = P.z;
+    const float cv = 1.0f - p.damp;
+    size_t o = (size_t)ip * p.nz;
+    for (int iz = 0; iz < p.nz; ++iz, ++o) {
+        const float pz = az - (float)iz * p.dh;
+        float px = ax, py = ay, vx = 0.f, vy = 0.f;
+        int it = 0;
+        bool conv = false;
+        for (it = 0; it < p.max_iter; ++it) {
+            const float4 fe = smp(px, py, pz);
+            const float fx = fe.x - p.k_lat * (px - ax);
+            const float fy = fe.y - p.k_lat * (py - ay);
+            vx = vx * cv + fx * p.dt;
+            vy = vy * cv + fy * p.dt;
+            px += vx * p.dt;
+            py += vy * p.dt;
+            if ((fx * fx + fy * fy) < p.f2conv) { conv = true; break; }
+        }
+        const int iters = conv ? it + 1 : p.max_iter;
+        const float4 fe = smp(px, py, pz);
+        if (writer) {
+            FEs[o] = fe;
+            if (disps) disps[o] = make_float4(px - ax, py - ay, 0.f, 0.f);
+            if (PPs) {
+                const bool fin = finite4(fe) && isfinite(px) && isfinite(py);
+                PPs[o] = make_float4(px, py, pz, (conv && fin) ? (float)iters : -(float)iters);
+            }
+        }
+    }
+}
+
+// relaxStrokes2D; G lanes per scan point as k_relax_tilted (results identical to G = 1).
+template <int G>
+__global__ __launch_bounds__(64) void k_relax_strokes_2d(GridDev g, const float4* __restrict__ pts, Strokes2DArgs p,
+                                                         float4* __restrict__ FEs, float4* __restrict__ disps,
+                                                         float4* __restrict__ PPs) {
+    int sub;
+    const int ip = group_point<G>(p.n_scan, p.scan_ny, sub);
+    if (ip < 0) return;                       // whole groups leave together
+    if constexpr (G == 1) {
+        Cell cell;
+        cell.i = -1; cell.j = -1; cell.k = -1;
+        auto smp = [&](float x, float y, float z) { return sample(g, cell, x, y, z); };
+        strokes2d_lane(smp, ip, pts, p, FEs, disps, PPs, true);
+    } else {
+        GroupCell<G> cell;
+        cell.i = -1; cell.j = -1; cell.k = -1;
+        auto smp = [&](float x, float y, float z) { return sample_group<G>(g, cell, sub, x, y, z); };
+        strokes2d_lane(smp, ip, pts, p, FEs, disps, PPs, sub == 0);
     }
 }
 
@@ -440,6 +508,38 @@ __global__ __launch_bounds__(256) void k_relax_adjoint(GridDev g, const float4* 
         atomic_add4(gcot + 4 * e.idx[m], make_float4(e.w[m] * gx, e.w[m] * gy, e.w[m] * gz, e.w[m] * u.w));
 }
 
+// Per-state diagnostics of relaxed states without a cotangent: the adjoint_diag record of the
+// reference's direct-path state adjoint (kernels/AFM.cl:1774-1844) for the trilinear grid model:
+// (|G(q*)|, smallest eigenvalue of the symmetric part of -J, infinity-norm condition number of J,
+// status bits of k_relax_adjoint).  A caller uses it to find states the adjoint would gate, or
+// probes close to a fold of their equilibrium branch (small smallest eigenvalue), before it
+// forms a loss.
+__global__ __launch_bounds__(256) void k_relax_state_diag(GridDev g, const float4* __restrict__ pts, StateArgs p,
+                                                          const float4* __restrict__ PPs, float4* __restrict__ diag) {
+    const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (size_t)p.n_scan * p.nz) return;
+    const int ip = (int)(s / p.nz);
+    const int iz = (int)(s - (size_t)ip * p.nz);
+    const f3 tip = tip_anchor(pts, p, ip, iz);
+    const float4 PP = PPs[s];
+    int status = 0;
+    StateEval e;
+    eval_state(g, p, tip, PP.x, PP.y, PP.z, e);
+    const double res = norm3(e.G);
+    if (!(PP.w > 0.f) || !(res < kResidualGate)) status |= 1;
+    double cof[3][3], det;
+    double cond = INFINITY;
+    if (cofactors(e.J, cof, det)) cond = cond_inf(e.J, cof, det);
+    if (!(cond < kCondGate)) status |= 4;
+    double S[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) S[i][j] = -0.5 * (e.J[i][j] + e.J[j][i]);
+    const double lmin = eigmin_sym3(S);
+    if (!(lmin > kStableGate)) status |= 8;
+    if (!(isfinite(PP.x) && isfinite(PP.y) && isfinite(PP.z) && isfinite(res))) status |= 2;
+    diag[s] = make_float4((float)res, (float)lmin, (float)cond, (float)status);
+}
+
 GridDev make_grid(const float* grid, const PpmGridSpec& s) {
     GridDev g;
     g.data = reinterpret_cast<const float4*>(grid);
@@ -465,7 +565,7 @@ int launch_relax_tilted(const float* grid, const PpmGridSpec& s, const float* pt
                         const PpmTip& tip, const PpmRelaxCtl& ctl, float* FEs, float* PPs, hipStream_t stream) {
     TiltArgs p{};
     p.n_scan = n_scan; p.nz = nz; p.max_iter = ctl.max_iter; p.check_before = ctl.check_before;
-    p.scan_ny = ctl.scan_ny; p.f2conv = ctl.f2conv;
+    p.scan_ny = ctl.scan_ny; p.f2conv = ctl.f2conv; p.damped = ctl.damped != 0;
     for (int q = 0; q < 3; ++q) { p.A[q] = tip.tipA[q]; p.B[q] = tip.tipB[q]; p.C[q] = tip.tipC[q]; }
     p.dtip = tip.tipC[3];
     for (int q = 0; q < 4; ++q) { p.k[q] = tip.stiffness[q]; p.d0[q] = tip.dpos0[q]; }
@@ -487,6 +587,7 @@ int launch_relax_strokes(const float* grid, const PpmGridSpec& s, const float* p
                          const PpmRelaxCtl& ctl, float* FEs, float* disps, float* PPs, hipStream_t stream) {
     StrokeArgs p{};
     p.n_scan = n_scan; p.nz = nz; p.max_iter = ctl.max_iter; p.scan_ny = ctl.scan_ny; p.f2conv = ctl.f2conv;
+    p.damped = ctl.damped != 0;
     for (int q = 0; q < 3; ++q) p.dTip[q] = dTip[q];
     for (int q = 0; q < 4; ++q) { p.k[q] = stiffness[q]; p.d0[q] = dpos0[q]; }
     p.dt0 = relax[0]; p.damp0 = relax[1];
@@ -500,6 +601,24 @@ int launch_relax_strokes(const float* grid, const PpmGridSpec& s, const float* p
     else if (G == 2) hipLaunchKernelGGL(k_relax_strokes<2>, blocks, dim3(64), 0, stream, gd, P, p, F, Dp, Q);
     else hipLaunchKernelGGL(k_relax_strokes<1>, blocks, dim3(64), 0, stream, gd, P, p, F, Dp, Q);
     return check_launch("relax_strokes");
+}
+
+int launch_relax_strokes_2d(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
+                            float k_lat, float dh, float dt, float damp, const PpmRelaxCtl& ctl, float* FEs,
+                            float* disps, float* PPs, hipStream_t stream) {
+    Strokes2DArgs p{};
+    p.n_scan = n_scan; p.nz = nz; p.max_iter = ctl.max_iter; p.scan_ny = ctl.scan_ny; p.f2conv = ctl.f2conv;
+    p.k_lat = k_lat; p.dh = dh; p.dt = dt; p.damp = damp;
+    const int G = relax_group(n_scan);
+    const dim3 blocks((unsigned)group_waves(n_scan, ctl.scan_ny, G));
+    const GridDev gd = make_grid(grid, s);
+    const float4* P = reinterpret_cast<const float4*>(pts);
+    float4 *F = reinterpret_cast<float4*>(FEs), *Dp = reinterpret_cast<float4*>(disps),
+           *Q = reinterpret_cast<float4*>(PPs);
+    if (G == 4) hipLaunchKernelGGL(k_relax_strokes_2d<4>, blocks, dim3(64), 0, stream, gd, P, p, F, Dp, Q);
+    else if (G == 2) hipLaunchKernelGGL(k_relax_strokes_2d<2>, blocks, dim3(64), 0, stream, gd, P, p, F, Dp, Q);
+    else hipLaunchKernelGGL(k_relax_strokes_2d<1>, blocks, dim3(64), 0, stream, gd, P, p, F, Dp, Q);
+    return check_launch("relax_strokes_2d");
 }
 
 static StateArgs state_args(int n_scan, int nz, const PpmTip& tip, int steps) {
@@ -521,6 +640,15 @@ int launch_relax_adjoint(const float* grid, const PpmGridSpec& s, const float* p
                        reinterpret_cast<const float4*>(PPs), reinterpret_cast<const float4*>(dL_dFEs), grid_cot,
                        reinterpret_cast<float4*>(lambdas));
     return check_launch("relax_adjoint");
+}
+
+int launch_relax_state_diag(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
+                            const PpmTip& tip, const float* PPs, float* diag, hipStream_t stream) {
+    const size_t n = (size_t)n_scan * nz;
+    hipLaunchKernelGGL(k_relax_state_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream,
+                       make_grid(grid, s), reinterpret_cast<const float4*>(pts), state_args(n_scan, nz, tip, 0),
+                       reinterpret_cast<const float4*>(PPs), reinterpret_cast<float4*>(diag));
+    return check_launch("relax_state_diag");
 }
 
 int launch_relax_polish(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,

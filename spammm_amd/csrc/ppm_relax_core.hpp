@@ -168,6 +168,17 @@ __device__ __forceinline__ f3 fire(f3 f, f3 v, float& dt, float& damp, float dtm
     damp = damp * kFDAMP;
     return mk3(v.x * cv + f.x * cf, v.y * cv + f.y * cf, v.z * cv + f.z * cf);
 }
+// The integrator step before `v += f dt; pos += v dt`: update_FIRE, or the damped-velocity
+// update `v *= (1 - damp)` of the reference's OPT_FIRE=0 build (kernels/AFM.cl:378-382), whose dt
+// and damp stay at their initial values.
+__device__ __forceinline__ f3 integrate(bool damped, f3 f, f3 v, float& dt, float& damp, float dtmin, float dtmax,
+                                        float damp0) {
+    if (damped) {
+        const float cv = 1.0f - damp;
+        return mk3(v.x * cv, v.y * cv, v.z * cv);
+    }
+    return fire(f, v, dt, damp, dtmin, dtmax, damp0);
+}
 
 __device__ __forceinline__ bool finite4(float4 v) {
     return isfinite(v.x) && isfinite(v.y) && isfinite(v.z) && isfinite(v.w);
@@ -210,6 +221,7 @@ __device__ __forceinline__ int group_point(int n_scan, int scan_ny, int& sub) {
 
 struct TiltArgs {
     int n_scan, nz, max_iter, check_before, scan_ny;
+    int damped;             // 1: damped-velocity integrator (OPT_FIRE=0), 0: FIRE
     float f2conv;
     float A[3], B[3], C[3], dtip;
     float k[4], d0[4];
@@ -250,7 +262,7 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
         if (p.check_before && ff < p.f2conv) {
             conv = done = true;
         } else {
-            v = fire(f, v, dt, damp, dtmin, dtmax, damp0);
+            v = integrate(p.damped, f, v, dt, damp, dtmin, dtmax, damp0);
             v = mk3(v.x + f.x * dt, v.y + f.y * dt, v.z + f.z * dt);
             pos = mk3(pos.x + v.x * dt, pos.y + v.y * dt, pos.z + v.z * dt);
             if (!p.check_before && ff < p.f2conv) conv = done = true;

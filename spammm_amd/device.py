@@ -113,13 +113,17 @@ class Tip:
         return t
 
 
-def relax_ctl(max_iter=128, f2conv=1e-8, check_before=False, scan_ny=0, f32_anchors=False) -> _lib.PpmRelaxCtl:
+def relax_ctl(max_iter=128, f2conv=1e-8, check_before=False, scan_ny=0, f32_anchors=False,
+              damped=False) -> _lib.PpmRelaxCtl:
+    """Relaxation control record; ``damped``: the damped-velocity integrator of the reference's
+    OPT_FIRE=0 build instead of FIRE."""
     c = _lib.PpmRelaxCtl()
     c.max_iter = int(max_iter)
     c.f2conv = float(f2conv)
     c.check_before = int(bool(check_before))
     c.scan_ny = int(scan_ny)
     c.f32_anchors = int(bool(f32_anchors))
+    c.damped = int(bool(damped))
     return c
 
 
@@ -199,7 +203,7 @@ def grid_project_f64(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 
 
 
 def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, f2conv=1e-8, check_before=False,
-                 scan_ny=0, FEs=None, PPs=None, want_pps=True):
+                 scan_ny=0, FEs=None, PPs=None, want_pps=True, damped=False):
     """relaxStrokesTilted on the device: returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None)."""
     import torch
     _f32dev(grid, "grid", 4)
@@ -217,7 +221,7 @@ def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, 
             if t.numel() != ns * nz * 4:
                 raise ValueError(f"{nm} has the wrong size")
     L = _lib.load()
-    s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, scan_ny)
+    s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, scan_ny, damped=damped)
     rc = L.ppm_relax_strokes_tilted(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz),
                                     ctypes.byref(tp), ctypes.byref(ctl), FEs.data_ptr(),
                                     PPs.data_ptr() if PPs is not None else None, _stream(grid))
@@ -226,7 +230,7 @@ def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, 
 
 
 def relax_tilted_f64(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, f2conv=1e-8,
-                     check_before=False, want_pps=True, f32_anchors=False):
+                     check_before=False, want_pps=True, f32_anchors=False, damped=False):
     """`relax_tilted` in FP64 on an FP64 grid and FP64 points (reference precision): returns
     (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None) as float64 device tensors.  ``f32_anchors``:
     the tip anchors advance in float32 as in the FP32 scan (whose heights drift from p0 + iz*dz by
@@ -240,7 +244,7 @@ def relax_tilted_f64(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=1
     FEs = torch.empty((ns, nz, 4), dtype=torch.float64, device=grid.device)
     PPs = torch.empty((ns, nz, 4), dtype=torch.float64, device=grid.device) if want_pps else None
     L = _lib.load()
-    s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, 0, f32_anchors)
+    s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, 0, f32_anchors, damped)
     rc = L.ppm_relax_strokes_tilted_f64(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz),
                                         ctypes.byref(tp), ctypes.byref(ctl), FEs.data_ptr(),
                                         PPs.data_ptr() if PPs is not None else None, _stream(grid))
@@ -249,8 +253,33 @@ def relax_tilted_f64(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=1
 
 
 def relax_strokes(grid, spec: GridSpec, points, nz: int, dTip, stiffness, dpos0, relax, max_iter=128, f2conv=1e-8,
-                  scan_ny=0, want_disps=True, want_pps=True):
-    """relaxStrokes on the device: returns (FEs, disps or None, PPs or None), each (n_scan,nz,4)."""
+                  scan_ny=0, want_disps=True, want_pps=True, damped=False):
+    """relaxStrokes on the device: returns (FEs, disps or None, PPs or None), each (n_scan,nz,4).
+    ``damped``: damped-velocity integrator (the reference's OPT_FIRE=0 build) instead of FIRE."""
+    import torch
+    _f32dev(grid, "grid", 4)
+    _f32dev(points, "points", 4)
+    if grid.numel() != 4 * spec.nvox:
+        raise ValueError("grid size does not match spec")
+    ns = int(points.shape[0])
+    dev = grid.device
+    FEs = torch.empty((ns, nz, 4), dtype=torch.float32, device=dev)
+    disps = torch.empty((ns, nz, 4), dtype=torch.float32, device=dev) if want_disps else None
+    PPs = torch.empty((ns, nz, 4), dtype=torch.float32, device=dev) if want_pps else None
+    L = _lib.load()
+    s, ctl = spec.c(), relax_ctl(max_iter, f2conv, False, scan_ny, damped=damped)
+    rc = L.ppm_relax_strokes(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz), _lib.f4(dTip),
+                             _lib.f4(stiffness), _lib.f4(dpos0), _lib.f4(relax), ctypes.byref(ctl), FEs.data_ptr(),
+                             disps.data_ptr() if disps is not None else None,
+                             PPs.data_ptr() if PPs is not None else None, _stream(grid))
+    _lib.check(rc)
+    return FEs, disps, PPs
+
+
+def relax_strokes_2d(grid, spec: GridSpec, points, nz: int, k_lat, dh, dt, damp, max_iter=128, f2conv=1e-8,
+                     scan_ny=0, want_disps=True, want_pps=False):
+    """relaxStrokes2D on the device (lateral-only damped relaxation at fixed heights az - iz*dh):
+    returns (FEs, disps or None, PPs or None), each (n_scan,nz,4)."""
     import torch
     _f32dev(grid, "grid", 4)
     _f32dev(points, "points", 4)
@@ -263,10 +292,10 @@ def relax_strokes(grid, spec: GridSpec, points, nz: int, dTip, stiffness, dpos0,
     PPs = torch.empty((ns, nz, 4), dtype=torch.float32, device=dev) if want_pps else None
     L = _lib.load()
     s, ctl = spec.c(), relax_ctl(max_iter, f2conv, False, scan_ny)
-    rc = L.ppm_relax_strokes(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz), _lib.f4(dTip),
-                             _lib.f4(stiffness), _lib.f4(dpos0), _lib.f4(relax), ctypes.byref(ctl), FEs.data_ptr(),
-                             disps.data_ptr() if disps is not None else None,
-                             PPs.data_ptr() if PPs is not None else None, _stream(grid))
+    rc = L.ppm_relax_strokes_2d(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz), float(k_lat),
+                                float(dh), float(dt), float(damp), ctypes.byref(ctl), FEs.data_ptr(),
+                                disps.data_ptr() if disps is not None else None,
+                                PPs.data_ptr() if PPs is not None else None, _stream(grid))
     _lib.check(rc)
     return FEs, disps, PPs
 
@@ -312,6 +341,25 @@ def relax_adjoint(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, dL_dFEs,
                              PPs.data_ptr(), dL_dFEs.data_ptr(), grid_cot.data_ptr(), lam.data_ptr(), _stream(grid))
     _lib.check(rc)
     return grid_cot, lam
+
+
+def relax_state_diag(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs):
+    """Per-state (|G|, smallest eigenvalue of sym(-J), condition number, status bits) of relaxed
+    states, (n_scan, nz, 4) float32; status bits as relax_adjoint's."""
+    import torch
+    _f32dev(grid, "grid", 4)
+    _f32dev(points, "points", 4)
+    _f32dev(PPs, "PPs", 4)
+    ns = int(points.shape[0])
+    if PPs.numel() != ns * nz * 4:
+        raise ValueError("PPs must be (n_scan, nz, 4)")
+    diag = torch.empty((ns, nz, 4), dtype=torch.float32, device=grid.device)
+    L = _lib.load()
+    s, tp = spec.c(), tip.c()
+    rc = L.ppm_relax_state_diag(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz), ctypes.byref(tp),
+                                PPs.data_ptr(), diag.data_ptr(), _stream(grid))
+    _lib.check(rc)
+    return diag
 
 
 def relax_polish(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, FEs=None, steps=3):
@@ -368,7 +416,7 @@ def grid_adjoint(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, grid, grid_co
 
 
 def relax_direct_morse(atoms, cMs, points, nz: int, tip: Tip, Qs, QZs, max_iter=128, f2conv=1e-8, scan_ny=0,
-                       FEs=None, PPs=None, want_pps=True):
+                       FEs=None, PPs=None, want_pps=True, damped=False):
     """Gridless direct Morse relaxed scan (relaxStrokesTiltedMorseDirect): (FEs, PPs or None), (n_scan,nz,4)."""
     import torch
     _f32dev(atoms, "atoms", 4)
@@ -387,7 +435,7 @@ def relax_direct_morse(atoms, cMs, points, nz: int, tip: Tip, Qs, QZs, max_iter=
             if t.numel() != ns * nz * 4:
                 raise ValueError(f"{nm} has the wrong size")
     L = _lib.load()
-    tp, ctl = tip.c(), relax_ctl(max_iter, f2conv, True, scan_ny)
+    tp, ctl = tip.c(), relax_ctl(max_iter, f2conv, True, scan_ny, damped=damped)
     rc = L.ppm_relax_direct_morse(atoms.data_ptr(), cMs.data_ptr(), na, points.data_ptr(), ns, int(nz),
                                   ctypes.byref(tp), _lib.f4(Qs), _lib.f4(QZs), ctypes.byref(ctl), FEs.data_ptr(),
                                   PPs.data_ptr() if PPs is not None else None, _stream(atoms))

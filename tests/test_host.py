@@ -29,7 +29,7 @@ def test_library_exports_every_header_symbol():
 
 def test_version_and_error_string_without_gpu():
     L = _lib.load()
-    assert L.ppm_version() == 1
+    assert L.ppm_version() == 2
     assert isinstance(L.ppm_last_error(), bytes)
 
 
@@ -54,7 +54,7 @@ def test_invalid_arguments_rejected_on_host():
 def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.PpmGridSpec) == 16 + 7 * 16
     assert ctypes.sizeof(_lib.PpmTip) == 7 * 16
-    assert ctypes.sizeof(_lib.PpmRelaxCtl) == 20
+    assert ctypes.sizeof(_lib.PpmRelaxCtl) == 24
 
 
 def test_scan_points_match_reference_loop():
@@ -103,7 +103,7 @@ def test_batch_struct_layouts_match_header():
     from spammm_amd import batch as B
     assert ctypes.sizeof(B.PpmMolDesc) == 16 * 4 + 8 + 12 * 16
     assert B.PpmMolDesc.grid_off.offset == 64
-    assert ctypes.sizeof(B.PpmBatchParams) == 12 * 4 + 2 * 16 + 7 * 16 + 20
+    assert ctypes.sizeof(B.PpmBatchParams) == 12 * 4 + 2 * 16 + 7 * 16 + 24
 
 
 def test_batch_z_window_covers_probe_heights():
