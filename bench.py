@@ -19,7 +19,6 @@ times the C++ oracle (FP32 mode, OpenMP) on a bounded sample of the same workloa
 from __future__ import annotations
 
 import argparse
-import dataclasses
 import json
 import os
 import sys
@@ -219,9 +218,9 @@ def main():
 SUMMARY_KEYS = {
     "batched": ("batched_ms", "plain_loop_ms", "plain_windowed_loop_ms", "speedup", "speedup_vs_windowed_plain",
                 "batched_pixel_z_steps_per_s", "launch", "autotune_fused_vs_two_phase_ms", "bitexact_vs_plain",
-                "window_fallback_molecules", "wait_timeout"),
+                "bitexact_windowed_plain", "window_fallback_molecules", "wait_timeout"),
     "differentiable": ("forward_ms", "adjoint_ms", "adjoint_over_forward", "fd_fp64_configured_heights",
-                       "fd_fp64_smooth_heights", "gated_states_configured_heights", "adam_configured_heights",
+                       "fd_fp64_smooth_heights", "gated_columns_configured_heights", "adam_configured_heights",
                        "adam_smooth_heights"),
     "substrate_poses": ("forward_ms_median", "adjoint_ms_median", "adjoint_over_forward",
                         "placements_per_s_forward_plus_adjoint", "window_fallbacks", "gated_states_total",
@@ -315,22 +314,34 @@ def batch_leg(dev, n_mol, z_margin=None):
         bf.recompute_plain(fallback)
         torch.cuda.synchronize(dev)
     same = bool(torch.equal(bf.df, df_plain) and torch.equal(bf.FEs, fe_plain))
+    # the plain per-image loop on the same z windows (equal pair evaluations): separates what the
+    # persistent launch, fusion and graph replay buy from what the windows buy
+    fe_w = torch.empty_like(bf.FEs)
+    df_w = torch.empty_like(bf.df)
+    tw = timed(lambda: bf.plain_loop(fe_w, df_w, windowed=True), 2)
+    same_w = bool(torch.equal(bf.df, df_w) and torch.equal(bf.FEs, fe_w))
+    win_fallback = int(bf.last_plain_fallback.size)
     states = bf.states
-    t_b, t_p = min(tb), min(tp)
+    t_b, t_p, t_w = min(tb), min(tp), min(tw)
     res = {
         "workload": f"{n_mol} molecules, U(10..150) atoms, cells U(16,32) A x 20 A, 0.1 A grids, 128x128 px x 40 z",
-        "batched_ms": t_b, "plain_loop_ms": t_p,
+        "batched_ms": t_b, "plain_loop_ms": t_p, "plain_windowed_loop_ms": t_w,
         "batched_pixel_z_steps_per_s": states / (t_b * 1e-3),
         "plain_pixel_z_steps_per_s": states / (t_p * 1e-3),
-        "speedup": t_p / t_b,
+        "plain_windowed_pixel_z_steps_per_s": states / (t_w * 1e-3),
+        "speedup": t_p / t_b, "speedup_vs_windowed_plain": t_w / t_b,
+        "bitexact_windowed_plain": same_w, "windowed_plain_fallback_molecules": win_fallback,
         "launch": best.key(), "autotune_ms": timings,
+        "autotune_fused_vs_two_phase_ms": [min((v for k, v in timings.items() if k.endswith("_fused")), default=None),
+                                           min((v for k, v in timings.items() if k.endswith("_two_phase")),
+                                               default=None)],
         "projected_pair_evals": bf.pair_evals, "z_margin_A": list(bf.z_margin),
         "full_grid_pair_evals": int(sum(w.pair_evals for w in mols)),
         "window_fallback_molecules": int(fallback.size), "wait_timeout": int(err),
         "bitexact_vs_plain": same,
     }
     log(f"[bench] batched: {res}")
-    del bf, fe_plain, df_plain
+    del bf, fe_plain, df_plain, fe_w, df_w
     torch.cuda.empty_cache()
     return res
 
@@ -682,31 +693,21 @@ def direct_leg(dev, reps=5):
     return out
 
 
-def _diff_case(dev, z0, adam_steps, lr):
+def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False):
     """One height range of the differentiable flake case (see diff_leg)."""
     import torch
-    from spammm_amd import diff, params as P, workloads as WL
-    w = WL.flake()
-    step = 30.0 / 128
-    w.nxy, w.nz = (128, 128), 40
-    w.scan_p0 = np.array([-15. + step / 2, -15. + step / 2, z0], np.float32)
-    w.scan_da = np.array([step, 0., 0.], np.float32)
-    w.scan_db = np.array([0., step, 0.], np.float32)
+    from spammm_amd import diff, gradcheck as GC, params as P, workloads as WL
+    w = WL.flake_differentiable(z0)
     prob = diff.Problem.from_workload(w, dev, max_iter=1000, f2conv=1e-12, check_before=True, polish=3,
-                                      strict=False)
+                                      strict=False, gate_eig_frac=0.5, gate_max_jump=0.3)
     R0, E0 = P.mix_R0E0(w.enames, P.ELEMENTS_BENCH, 1.66, 0.0091)
+    R0, E0 = torch.tensor(R0, device=dev), torch.tensor(E0, device=dev)
     xyz0 = torch.tensor(w.atoms[:, :3], dtype=torch.float64, device=dev)
     q0 = torch.tensor(w.atoms[:, 3], dtype=torch.float64, device=dev)
-    R0 = torch.tensor(R0, device=dev)
-    E0 = torch.tensor(E0, device=dev)
-    rng = np.random.default_rng(3)
-    xyz_t = xyz0 + torch.tensor(rng.normal(0, 0.1, w.atoms[:, :3].shape), device=dev)
+    xyz_t = torch.tensor(WL.perturbed_positions(w), device=dev)
     with torch.no_grad():
         target = diff.df_image(xyz_t, q0, R0, E0, prob).double()
-
-    def loss_at(xx, qq):
-        with torch.no_grad():
-            return float(0.5 * ((diff.df_image(xx, qq, R0, E0, prob).double() - target) ** 2).sum())
+    tcols, tgate = prob.last_columns.clone(), dict(prob.last_gate)
 
     stream = torch.cuda.current_stream(dev)
     t_f, t_b = [], []
@@ -716,7 +717,8 @@ def _diff_case(dev, z0, adam_steps, lr):
         e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         e0.record(stream)
         dfi = diff.df_image(x, q, R0, E0, prob)
-        loss = 0.5 * ((dfi.double() - target) ** 2).sum()
+        cols = prob.last_columns & tcols
+        loss = diff.masked_l2(dfi, target, cols)
         e1.record(stream)
         loss.backward()
         e2.record(stream)
@@ -726,203 +728,77 @@ def _diff_case(dev, z0, adam_steps, lr):
             t_b.append(e1.elapsed_time(e2))
     tf, tb = float(np.mean(t_f)), float(np.mean(t_b))
     gx, gq = x.grad.detach(), q.grad.detach()
+    gating = {"forward": dict(prob.last_gate), "target": tgate, "columns_kept_both": int(cols.sum()),
+              "columns": int(cols.numel())}
     status = dict(prob.last_status)
-    # the same forward and adjoint with the grid's voxel sums in FP64 (ppm_grid_project_acc64)
-    prob64 = dataclasses.replace(prob, grid_acc64=True, last_status={})
-    t_f64, t_b64 = [], []
-    for rep in range(4):
-        xa = xyz0.clone().requires_grad_(True)
-        qa = q0.clone().requires_grad_(True)
-        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-        e0.record(stream)
-        dfi = diff.df_image(xa, qa, R0, E0, prob64)
-        loss = 0.5 * ((dfi.double() - target) ** 2).sum()
-        e1.record(stream)
-        loss.backward()
-        e2.record(stream)
-        torch.cuda.synchronize()
-        if rep > 0:
-            t_f64.append(e0.elapsed_time(e1))
-            t_b64.append(e1.elapsed_time(e2))
-    grads = {"fp32_sums": (gx, gq), "acc64": (xa.grad.detach(), qa.grad.detach())}
-    # central FD on 32 random parameters (seed 3): positions h = 1e-3 A, charges h = 1e-4 e
+    # central differences of the FP64 device forward on 32 random parameters (seed 3), same gated loss
     na = xyz0.shape[0]
     sel = np.random.default_rng(3).choice(4 * na, 32, replace=False)
-    rows = []
-    for k in sel:
-        k = int(k)
-        if k < 3 * na:
-            i, c = divmod(k, 3)
-            h = 1e-3
-            xp, xm = xyz0.clone(), xyz0.clone()
-            xp[i, c] += h
-            xm[i, c] -= h
-            fd = (loss_at(xp, q0) - loss_at(xm, q0)) / (2 * h)
-            ad, name = float(gx[i, c]), f"xyz[{i},{c}]"
-        else:
-            i = k - 3 * na
-            h = 1e-4
-            qp, qm = q0.clone(), q0.clone()
-            qp[i] += h
-            qm[i] -= h
-            fd = (loss_at(xyz0, qp) - loss_at(xyz0, qm)) / (2 * h)
-            ad, name = float(gq[i]), f"q[{i}]"
-        rows.append((name, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30)))
-    rel = np.array([r[3] for r in rows])
-    fd64 = _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, grads, (1e-3, 1e-4))
-    fd64_small_h = _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, grads, (1e-4, 1e-5))
-    fd64_exact_heights = _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, grads, (1e-3, 1e-4), f32_anchors=False)
-    # directional FD along -grad (all positions) at three step sizes
-    gn = float(gx.norm())
-    dvec = -gx / gn
-    dirfd = {}
-    for eps in (1e-2, 1e-3, 1e-4):
-        dirfd[f"{eps:g}"] = (loss_at(xyz0 + eps * dvec, q0) - loss_at(xyz0 - eps * dvec, q0)) / (2 * eps)
-    # Adam on positions (constant learning rate)
+    t0 = time.perf_counter()
+    fd = GC.fd_check(GC.Fp64Forward(prob, R0, E0), target.reshape(prob.n_scan, prob.nz), xyz0, q0, gx, gq, sel,
+                     steps=fd_steps, columns=cols, order4_step=fd_steps[0] if order4 else None)
+    fd["wall_s"] = time.perf_counter() - t0
+    # Adam on the positions, the loss gated on the columns stable in the forward and in the target
     x = xyz0.clone().requires_grad_(True)
     opt = torch.optim.Adam([x], lr=lr)
-    hist, rmsd = [], []
+    hist = []
     t0 = time.perf_counter()
     for it in range(adam_steps):
         opt.zero_grad()
-        loss = 0.5 * ((diff.df_image(x, q0, R0, E0, prob).double() - target) ** 2).sum()
+        loss = diff.masked_l2(diff.df_image(x, q0, R0, E0, prob), target, prob.last_columns & tcols)
         loss.backward()
         opt.step()
         hist.append(float(loss.detach()))
-        rmsd.append(float(((x.detach() - xyz_t) ** 2).sum(1).mean().sqrt()))
-    dxe = (x.detach() - xyz_t) ** 2
-    dx0 = (xyz0 - xyz_t) ** 2
     torch.cuda.synchronize()
     t_adam = time.perf_counter() - t0
-    rmsd0 = float(((xyz0 - xyz_t) ** 2).sum(1).mean().sqrt())
+    d0, d1 = xyz0 - xyz_t, x.detach() - xyz_t
+    rxy = [float(d[:, :2].pow(2).sum(1).mean().sqrt()) for d in (d0, d1)]
+    rz = [float(d[:, 2].pow(2).mean().sqrt()) for d in (d0, d1)]
     return {
         "tip_z": [z0, z0 + (w.nz - 1) * float(w.dtip)], "probe_z_rest": [z0 - 4.0, z0 - 4.0 + (w.nz - 1) * float(w.dtip)],
         "forward_ms": tf, "adjoint_ms": tb, "adjoint_over_forward": tb / tf,
-        "acc64_grid": {"forward_ms": float(np.mean(t_f64)), "adjoint_ms": float(np.mean(t_b64)),
-                       "adjoint_over_forward": float(np.mean(t_b64) / np.mean(t_f64)),
-                       "adjoint_over_fp32_sum_forward": float(np.mean(t_b64) / tf)},
         "pixel_z_steps_per_s_forward": w.n_scan * w.nz / (tf * 1e-3),
-        "adjoint_status": status,
-        "fd_32_params": {"h": "1e-3 A / 1e-4 e", "within_1e-3": int((rel <= 1e-3).sum()),
-                         "within_1e-2": int((rel <= 1e-2).sum()), "median_rel_err": float(np.median(rel)),
-                         "rows": [(n, round(a, 6), round(f, 6), float(f"{r:.3g}")) for n, a, f, r in rows]},
-        "fd_32_params_fp64_forward": fd64,
-        "fd_32_params_fp64_forward_h_1e-4": fd64_small_h,
-        "fd_32_params_fp64_forward_exact_heights": fd64_exact_heights,
-        "directional_fd_minus_grad": {"adjoint": -gn, **{f"fd_eps_{k}": v for k, v in dirfd.items()}},
-        "adam": {"steps": adam_steps, "lr": lr, "schedule": "constant", "wall_s": t_adam,
-                 "loss_first": hist[0] if hist else None, "loss_last": hist[-1] if hist else None,
-                 "loss_every5": hist[::5], "rmsd_start_A": rmsd0, "rmsd_end_A": rmsd[-1] if rmsd else None,
-                 "rmsd_every5": rmsd[::5],
-                 "rmsd_xy_start_end_A": [float(dx0[:, :2].sum(1).mean().sqrt()), float(dxe[:, :2].sum(1).mean().sqrt())],
-                 "rmsd_z_start_end_A": [float(dx0[:, 2].mean().sqrt()), float(dxe[:, 2].mean().sqrt())]},
+        "adjoint_status": status, "gating": gating, "fd_fp64": fd,
+        "adam": {"steps": adam_steps, "lr": lr, "wall_s": t_adam, "loss_first": hist[0] if hist else None,
+                 "loss_last": hist[-1] if hist else None, "loss_every5": hist[::5],
+                 "rmsd_xy_start_end_A": rxy, "rmsd_z_start_end_A": rz},
     }
-
-
-def _fd64_check(w, prob, R0, E0, target, xyz0, q0, sel, grads, steps=(1e-3, 1e-4), f32_anchors=True):
-    """Central differences of the same loss with the forward in FP64 on the device (FP64 grid
-    projection and FP64 relaxation to |F| < 1e-12 eV/A, FP64 Giessibl df), on the same 32
-    parameters as the FP32 check, steps (position A, charge e): the reference-precision
-    comparison of the adjoint.  ``grads``: {"fp32_sums": (gx, gq), "acc64": (gx, gq)}, the
-    adjoint's gradients with the FP32-summed and the FP64-summed grid forward.  ``f32_anchors``:
-    the FP64 scan advances its tip anchors in float32 as the FP32 scan does (the same heights, which
-    drift from p0 + iz*dz by up to ~1e-5 A); False: anchors in FP64 (the exact heights)."""
-    gx, gq = grads["fp32_sums"]
-    gx_a, gq_a = grads["acc64"]
-    import torch
-    from spammm_amd import device as D, df as DF
-    dev = xyz0.device
-    spec, nz = prob.spec, prob.nz
-    pts64 = prob.pts.double().contiguous()
-    M = torch.as_tensor(DF.giessibl_matrix(nz, prob.dz, prob.amp, prob.f0, prob.k_cant), device=dev)
-    r6 = R0.double() ** 6
-    cpar = torch.stack([2.0 * E0.double() * r6, E0.double() * r6 * r6], 1).contiguous()
-    tgt = target.reshape(prob.n_scan, nz)
-    buf = None
-    t0 = time.perf_counter()
-
-    def df64(xx, qq):
-        nonlocal buf
-        atoms = torch.cat([xx, qq[:, None]], 1).contiguous()
-        buf = D.grid_project_f64(atoms, cpar, 0, spec, prob.Qs, prob.QZs, prob.npbc, prob.lvA, prob.lvB,
-                                 cap=prob.cap, out=buf)
-        FEs, _ = D.relax_tilted_f64(buf, spec, pts64, nz, prob.tip, max_iter=40000, f2conv=1e-24,
-                                    check_before=True, want_pps=False, f32_anchors=f32_anchors)
-        return FEs[..., 2] @ M.T
-
-    def loss64(xx, qq):
-        return float(0.5 * ((df64(xx, qq) - tgt) ** 2).sum())
-
-    # the adjoint of the FP32 forward with the FP64 forward's residual as its upstream: separates
-    # the accuracy of the adjoint's derivatives from the FP32 forward's own error in df - target
-    from spammm_amd import diff
-    res64 = (df64(xyz0, q0) - tgt).reshape(int(prob.nxy[0]), int(prob.nxy[1]), nz)
-    xr, qr = xyz0.clone().requires_grad_(True), q0.clone().requires_grad_(True)
-    diff.df_image(xr, qr, R0, E0, prob).backward(res64.float())
-    gx_r, gq_r = xr.grad.detach(), qr.grad.detach()
-
-    na = xyz0.shape[0]
-    rows = []
-    for k in sel:
-        k = int(k)
-        if k < 3 * na:
-            i, c = divmod(k, 3)
-            h = steps[0]
-            xp, xm = xyz0.clone(), xyz0.clone()
-            xp[i, c] += h
-            xm[i, c] -= h
-            fd = (loss64(xp, q0) - loss64(xm, q0)) / (2 * h)
-            ad, ar, aa, name = float(gx[i, c]), float(gx_r[i, c]), float(gx_a[i, c]), f"xyz[{i},{c}]"
-        else:
-            i = k - 3 * na
-            h = steps[1]
-            qp, qm = q0.clone(), q0.clone()
-            qp[i] += h
-            qm[i] -= h
-            fd = (loss64(xyz0, qp) - loss64(xyz0, qm)) / (2 * h)
-            ad, ar, aa, name = float(gq[i]), float(gq_r[i]), float(gq_a[i]), f"q[{i}]"
-        rows.append((name, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30), ar, abs(ar - fd) / max(abs(fd), 1e-30),
-                     aa, abs(aa - fd) / max(abs(fd), 1e-30)))
-    rel = np.array([r[3] for r in rows])
-    rel_r = np.array([r[5] for r in rows])
-    rel_a = np.array([r[7] for r in rows])
-    floor = 1e-2 * max(abs(r[2]) for r in rows)
-    rel_floor = np.array([abs(r[1] - r[2]) / max(abs(r[2]), floor) for r in rows])
-    torch.cuda.synchronize()
-    return {"h": f"{steps[0]:g} A / {steps[1]:g} e",
-            "forward": "FP64 grid + FP64 relaxation (|F| < 1e-12) + FP64 df, on the device; tip anchors "
-                       + ("advanced in float32 as the FP32 scan" if f32_anchors else "in FP64 (exact heights)"),
-            "within_1e-3": int((rel <= 1e-3).sum()), "within_1e-2": int((rel <= 1e-2).sum()),
-            "median_rel_err": float(np.median(rel)), "max_rel_err": float(rel.max()),
-            "within_1e-3_floor_1pct_of_max": int((rel_floor <= 1e-3).sum()),
-            "adjoint_acc64_grid_forward": {"within_1e-3": int((rel_a <= 1e-3).sum()),
-                                           "within_1e-2": int((rel_a <= 1e-2).sum()),
-                                           "median_rel_err": float(np.median(rel_a)),
-                                           "max_rel_err": float(rel_a.max())},
-            "adjoint_with_fp64_residual": {"within_1e-3": int((rel_r <= 1e-3).sum()),
-                                           "within_1e-2": int((rel_r <= 1e-2).sum()),
-                                           "median_rel_err": float(np.median(rel_r)),
-                                           "max_rel_err": float(rel_r.max())},
-            "wall_s": time.perf_counter() - t0,
-            "rows (name, adjoint, fd, rel, adjoint_fp64_residual, rel, adjoint_acc64, rel)":
-                [(n, round(a, 6), round(f, 6), float(f"{r:.3g}"), round(ar, 6), float(f"{rr:.3g}"), round(aa, 6),
-                  float(f"{ra:.3g}")) for n, a, f, r, ar, rr, aa, ra in rows]}
 
 
 def diff_leg(dev, adam_steps=50):
     """Differentiable df image of the flake molecule, 128x128 pixels x 40 z-steps, L2 loss to a target
-    made from positions perturbed by N(0, 0.1 A): forward and adjoint timed with HIP events
-    (adjoint/forward ratio), central FD on 32 random parameters (seed 3), directional FD along -grad
-    at three step sizes, and Adam on the positions (learning rate 1e-3 and 1e-2, the best of
-    {1e-3, 3e-3, 1e-2} x {constant, cosine} for each height range).  Run at the configured tip heights (10 -> 6.1 A:
-    probe 6 -> 2.1 A, where probes flip between bistable positions and the loss is only piecewise
-    smooth) and 1 A higher (probe 7 -> 3.1 A, smooth)."""
-    res = {"workload": "flake 200 atoms, 300x300x240 grid, 128x128 pixels x 40 z, Giessibl df, L2 loss",
-           "configured_heights": _diff_case(dev, 10.0, adam_steps, 1e-3),
-           "smooth_heights": _diff_case(dev, 11.0, adam_steps, 1e-2)}
+    made from positions perturbed by N(0, 0.1 A), taken over the pixel columns whose probes stay on one
+    equilibrium branch in the forward and in the target (diff.stable_columns): forward and adjoint
+    timed with HIP events (adjoint/forward ratio), central differences of the FP64 device forward on 32
+    random parameters (seed 3) at h = 1e-3 and 1e-4 A (1e-4, 1e-5 e), a fourth-order stencil at 1e-3,
+    and 50 Adam steps (lr 1e-2) on the positions.  Run at the configured tip heights (10 -> 6.1 A:
+    probe 6 -> 2.1 A) and 1 A higher (probe 7 -> 3.1 A)."""
+    steps = ((1e-3, 1e-4), (1e-4, 1e-5))
+    res = {"workload": "flake 200 atoms, 300x300x240 grid, 128x128 pixels x 40 z, Giessibl df, gated L2 loss",
+           "configured_heights": _diff_case(dev, 10.0, adam_steps, 1e-2, steps, order4=True),
+           "smooth_heights": _diff_case(dev, 11.0, adam_steps, 1e-2, steps)}
     c = res["configured_heights"]
     res["forward_ms"], res["adjoint_ms"], res["adjoint_over_forward"] = (c["forward_ms"], c["adjoint_ms"],
                                                                        c["adjoint_over_forward"])
+
+    def fd_summary(f):
+        out = {p["h"]: p["within_tol"] for p in f["per_step"]}
+        if "order4" in f:
+            out[f["order4"]["h"] + " order4"] = f["order4"]["within_tol"]
+        out["plateau"] = f["plateau_within_tol"]
+        out["of"] = f["n"]
+        return out
+
+    def adam_summary(a):
+        return {"loss": [a["loss_first"], a["loss_last"]], "rmsd_xy_A": a["rmsd_xy_start_end_A"],
+                "rmsd_z_A": a["rmsd_z_start_end_A"]}
+
+    res["fd_fp64_configured_heights"] = fd_summary(c["fd_fp64"])
+    res["fd_fp64_smooth_heights"] = fd_summary(res["smooth_heights"]["fd_fp64"])
+    res["gated_columns_configured_heights"] = [c["gating"]["columns"] - c["gating"]["columns_kept_both"],
+                                               c["gating"]["columns"]]
+    res["adam_configured_heights"] = adam_summary(c["adam"])
+    res["adam_smooth_heights"] = adam_summary(res["smooth_heights"]["adam"])
     log(f"[bench] differentiable: {res}")
     return res
 

@@ -155,6 +155,18 @@ int ppm_grid_project_acc64(int model, const float* atoms, const float* cpar, int
                            int npbc_a, int npbc_b, const float lvA[3], const float lvB[3],
                            int cap, float* grid, void* stream);
 
+/* ppm_grid_project of the z planes [klo, klo + nzw) only: grid (nzw,ny,nx,4) holds the voxels
+ * (ia, ib, klo + k) of the spec's grid, bit for bit the values ppm_grid_project stores there
+ * (same pair laws, images and rescale, evalLJC_QZs_toImg / evalMorseC_QZs_toImg,
+ * kernels/AFM.cl:916-1056).  No reference counterpart: the reference always projects the whole
+ * image (AFMulator.make_forcefield, spammm/SPM/AFM.py:897-946); a scan that descends over a few
+ * A of a tall cell samples only the planes around its probe heights, and
+ * ppm_relax_strokes_tilted_window relaxes on such a window.  0 <= klo, klo + nzw <= nz. */
+int ppm_grid_project_window(int model, const float* atoms, const float* cpar, int na,
+                            const PpmGridSpec* spec, int klo, int nzw, const float Qs[4], const float QZs[4],
+                            int npbc_a, int npbc_b, const float lvA[3], const float lvB[3],
+                            int cap, float* grid, void* stream);
+
 /* ppm_grid_project over a base: grid = cap(base + pair sum of `atoms`), the |E|>100 rescale
  * (if cap) applied to the total.  base: a raw (cap = 0) grid of other atoms on the same spec,
  * e.g. a substrate computed once and shared by many molecule placements; it must not be the
@@ -314,6 +326,18 @@ int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec,
                              const float* points, int n_scan, int nz,
                              const PpmTip* tip, const PpmRelaxCtl* ctl,
                              float* FEs, float* PPs, void* stream);
+
+/* ppm_relax_strokes_tilted on a z window of the grid: `grid` holds only planes [klo, klo + nzw)
+ * (as ppm_grid_project_window stores them); sampling coordinates and REPEAT addressing are those
+ * of the whole grid `spec`, so wherever every probe samples inside the window the results equal
+ * ppm_relax_strokes_tilted on the whole grid bit for bit.  A sample that needs a plane outside
+ * the window reads the nearest stored plane and sets *flag (device int, caller-zeroed) nonzero:
+ * the results are then not those of the whole grid and the caller recomputes without a window.
+ * relaxStrokesTilted semantics (kernels/AFM.cl:478-562); no reference counterpart for the
+ * window.  nzw >= 2, 0 <= klo, klo + nzw <= nz. */
+int ppm_relax_strokes_tilted_window(const float* grid, const PpmGridSpec* spec, int klo, int nzw,
+                                    const float* points, int n_scan, int nz, const PpmTip* tip,
+                                    const PpmRelaxCtl* ctl, float* FEs, float* PPs, int* flag, void* stream);
 
 /* Relaxed scan, untilted tip, with displacement output.
  * Replaces relaxStrokes (kernels/AFM.cl:345-394) as launched by

@@ -54,7 +54,7 @@ EXPORTS = (
     "ppm_fdbm_pad_roll", "ppm_fdbm_flip3", "ppm_fdbm_dispersion", "ppm_fdbm_compose_gradient",
     "ppm_ewald_num_g", "ppm_ewald_system_bytes", "ppm_ewald_prepare", "ppm_ewald_eval", "ppm_ewald_brute",
     "ppm_ewald_cluster", "ppm_ewald_grid_workspace_size", "ppm_ewald_tip_grid", "ppm_ewald_kept_count_offset",
-    "ppm_relax_state_diag", "ppm_relax_strokes_2d",
+    "ppm_relax_state_diag", "ppm_relax_strokes_2d", "ppm_grid_project_window", "ppm_relax_strokes_tilted_window",
 )
 
 _lib = None
@@ -147,6 +147,12 @@ def load():
     L.ppm_axis_operator.argtypes = [vp, ll, ll, ll, _F4, i, i, i, i, vp, vp, i, i, f, i, vp, vp]
     L.ppm_relax_adjoint.restype = i
     L.ppm_relax_adjoint.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), vp, vp, vp, vp, vp]
+    L.ppm_grid_project_window.restype = i
+    L.ppm_grid_project_window.argtypes = [i, vp, vp, i, P(PpmGridSpec), i, i, _F4, _F4, i, i, P(ctypes.c_float * 3),
+                                          P(ctypes.c_float * 3), i, vp, vp]
+    L.ppm_relax_strokes_tilted_window.restype = i
+    L.ppm_relax_strokes_tilted_window.argtypes = [vp, P(PpmGridSpec), i, i, vp, i, i, P(PpmTip), P(PpmRelaxCtl), vp,
+                                                  vp, vp, vp]
     L.ppm_relax_strokes_2d.restype = i
     L.ppm_relax_strokes_2d.argtypes = [vp, P(PpmGridSpec), vp, i, i, f, f, f, f, P(PpmRelaxCtl), vp, vp, vp, vp]
     L.ppm_relax_state_diag.restype = i

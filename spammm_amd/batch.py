@@ -250,9 +250,13 @@ class BatchForward:
         n = len(self.molecules)
         return int(ws[1]), ws[4 + n:4 + 2 * n].copy()
 
-    def recompute_plain(self, indices):
-        """Plain per-image path (whole grid) for the given molecules, written into the batch outputs."""
+    def recompute_plain(self, indices, FEs=None, df=None, PPs=None, with_pps=True):
+        """Plain per-image path (whole grid) for the given molecules, written into the batch outputs
+        (or into FEs / df / PPs of the same layout; with_pps=False writes no PPs)."""
         import torch
+        FEs = self.FEs if FEs is None else FEs
+        df = self.df if df is None else df
+        PPs = (self.PPs if PPs is None else PPs) if with_pps else None
         for i in indices:
             w, sp, d = self.molecules[i], self.specs[i], self.descs_host[i]
             a0, a1 = d.atom_off, d.atom_off + d.na
@@ -260,12 +264,12 @@ class BatchForward:
                                self.Qs, self.QZs, w.npbc, w.lvA, w.lvB, cap=self.cap)
             pts = torch.as_tensor(D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy), device=self.device)
             p0, p1 = d.pix_off, d.pix_off + d.scan_nx * d.scan_ny
-            FEs, PPs = D.relax_tilted(G, sp, pts, self.nz, self.tip, self.max_iter, self.f2conv, self.check_before,
-                                      scan_ny=d.scan_ny, FEs=self.FEs[p0:p1], PPs=None if self.PPs is None
-                                      else self.PPs[p0:p1], want_pps=self.PPs is not None)
-            if self.dfop is not None:
-                self.dfop.apply(self.FEs, (self.nz * 4, 4, 0), p1 - p0, 1, offset=2 + p0 * self.nz * 4,
-                                out=self.df[p0:p1].view(p1 - p0, self.dfop.n_out, 1))
+            D.relax_tilted(G, sp, pts, self.nz, self.tip, self.max_iter, self.f2conv, self.check_before,
+                           scan_ny=d.scan_ny, FEs=FEs[p0:p1], PPs=None if PPs is None else PPs[p0:p1],
+                           want_pps=PPs is not None)
+            if self.dfop is not None and df is not None:
+                self.dfop.apply(FEs, (self.nz * 4, 4, 0), p1 - p0, 1, offset=2 + p0 * self.nz * 4,
+                                out=df[p0:p1].view(p1 - p0, self.dfop.n_out, 1))
 
     def forward(self, cfg: LaunchConfig = LaunchConfig()):
         """Run the batch; recompute molecules whose probes left the projected window; return df."""
@@ -298,30 +302,44 @@ class BatchForward:
 
     # ── the plain per-image path (baseline) ───────────────────────────────────
 
-    def plain_loop(self, FEs_out=None, df_out=None):
-        """The same work as a loop of plain per-image launches (whole grids): returns (FEs, df)."""
+    def plain_loop(self, FEs_out=None, df_out=None, windowed=False):
+        """The same work as a loop of plain per-image launches: returns (FEs, df).  windowed=False:
+        whole grids; True: each molecule's grid projected and relaxed on the z window of its batch
+        descriptor (ppm_grid_project_window + ppm_relax_strokes_tilted_window), the same pair
+        evaluations as the batched launch; a molecule whose probes leave the window is recomputed
+        with its whole grid (as the batched path does)."""
         import torch
         FEs = FEs_out if FEs_out is not None else torch.empty_like(self.FEs)
         df = df_out if df_out is not None else (torch.empty_like(self.df) if self.df is not None else None)
         if getattr(self, "_pts", None) is None:
             self._pts = [torch.as_tensor(D.scan_points(w.scan_p0, w.scan_da, w.scan_db, *w.nxy), device=self.device)
                          for w in self.molecules]
+        if windowed:
+            flags = torch.zeros(len(self.molecules), dtype=torch.int32, device=self.device)
         cache = {}
         for i, (w, sp, d) in enumerate(zip(self.molecules, self.specs, self.descs_host)):
-            key = tuple(sp.n)
+            nzg = int(d.nzw) if windowed else int(sp.n[2])
+            key = (int(sp.n[0]), int(sp.n[1]), nzg)
             if key not in cache:
                 cache.clear()
-                cache[key] = torch.empty((int(sp.n[2]), int(sp.n[1]), int(sp.n[0]), 4), dtype=torch.float32,
+                cache[key] = torch.empty((nzg, int(sp.n[1]), int(sp.n[0]), 4), dtype=torch.float32,
                                          device=self.device)
             G = cache[key]
             a0, a1 = d.atom_off, d.atom_off + d.na
+            win = (int(d.klo), int(d.nzw)) if windowed else None
             D.grid_project(self.atoms[a0:a1], self.cpar[a0:a1], self.model, sp, self.Qs, self.QZs, w.npbc, w.lvA,
-                           w.lvB, cap=self.cap, out=G)
+                           w.lvB, cap=self.cap, out=G, window=win)
             pts = self._pts[i]
             p0, p1 = d.pix_off, d.pix_off + d.scan_nx * d.scan_ny
             D.relax_tilted(G, sp, pts, self.nz, self.tip, self.max_iter, self.f2conv, self.check_before,
-                           scan_ny=d.scan_ny, FEs=FEs[p0:p1], want_pps=False)
+                           scan_ny=d.scan_ny, FEs=FEs[p0:p1], want_pps=False, window=win,
+                           flag=flags[i:i + 1] if windowed else None)
             if df is not None:
                 self.dfop.apply(FEs, (self.nz * 4, 4, 0), p1 - p0, 1, offset=2 + p0 * self.nz * 4,
                                 out=df[p0:p1].view(p1 - p0, self.dfop.n_out, 1))
+        if windowed:
+            bad = np.flatnonzero(flags.cpu().numpy())
+            self.last_plain_fallback = bad
+            if bad.size:
+                self.recompute_plain(bad, FEs=FEs, df=df, with_pps=False)
         return FEs, df

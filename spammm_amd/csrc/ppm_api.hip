@@ -88,9 +88,13 @@ const char* ppm_last_error(void) { return g_err; }
 static int grid_project_impl(const char* fn, int model, const float* atoms, const float* cpar, int na,
                              const PpmGridSpec* spec, const float Qs[4], const float QZs[4], int npbc_a, int npbc_b,
                              const float lvA[3], const float lvB[3], int cap, const float* base, float* grid,
-                             void* stream, bool acc64 = false) {
+                             void* stream, bool acc64 = false, int klo = 0, int nzw = 0) {
     g_err[0] = 0;
     if (int r = check_spec(fn, spec)) return r;
+    if (nzw > 0 && (klo < 0 || (long long)klo + nzw > spec->nz)) {
+        set_error("%s: window [%d, %d) outside the grid's %d planes", fn, klo, klo + nzw, spec->nz);
+        return PPM_ERR_INVALID_ARG;
+    }
     if (model != PPM_MODEL_LJ && model != PPM_MODEL_MORSE) { set_error("%s: unknown model %d", fn, model); return PPM_ERR_INVALID_ARG; }
     if (na < 1 || na > (1 << 20)) { set_error("%s: invalid atom count %d", fn, na); return PPM_ERR_INVALID_ARG; }
     if (!atoms || !cpar || !grid || !Qs || !QZs) { set_error("%s: NULL pointer argument", fn); return PPM_ERR_INVALID_ARG; }
@@ -102,7 +106,7 @@ static int grid_project_impl(const char* fn, int model, const float* atoms, cons
     if ((long long)na * (2 * npbc_a + 1) * (2 * npbc_b + 1) > (1LL << 28)) { set_error("%s: too many image sites", fn); return PPM_ERR_INVALID_ARG; }
     if (!finite4(Qs, 4) || !finite4(QZs, 4)) { set_error("%s: non-finite tip charges", fn); return PPM_ERR_INVALID_ARG; }
     return launch_grid_project(model, atoms, cpar, na, *spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, base, grid,
-                               acc64, (hipStream_t)stream);
+                               acc64, (hipStream_t)stream, klo, nzw);
 }
 
 int ppm_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
@@ -110,6 +114,18 @@ int ppm_grid_project(int model, const float* atoms, const float* cpar, int na, c
                      const float lvB[3], int cap, float* grid, void* stream) {
     return grid_project_impl("ppm_grid_project", model, atoms, cpar, na, spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap,
                              nullptr, grid, stream);
+}
+
+int ppm_grid_project_window(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
+                            int klo, int nzw, const float Qs[4], const float QZs[4], int npbc_a, int npbc_b,
+                            const float lvA[3], const float lvB[3], int cap, float* grid, void* stream) {
+    if (nzw < 1) {
+        g_err[0] = 0;
+        set_error("ppm_grid_project_window: empty window (nzw=%d)", nzw);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return grid_project_impl("ppm_grid_project_window", model, atoms, cpar, na, spec, Qs, QZs, npbc_a, npbc_b, lvA,
+                             lvB, cap, nullptr, grid, stream, false, klo, nzw);
 }
 
 int ppm_grid_project_acc64(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
@@ -348,6 +364,23 @@ int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec, const f
     if (int r = check_ctl(fn, ctl, n_scan)) return r;
     if (!grid || !tip) { set_error("%s: NULL grid/tip", fn); return PPM_ERR_INVALID_ARG; }
     return launch_relax_tilted(grid, *spec, points, n_scan, nz, *tip, *ctl, FEs, PPs, (hipStream_t)stream);
+}
+
+int ppm_relax_strokes_tilted_window(const float* grid, const PpmGridSpec* spec, int klo, int nzw,
+                                    const float* points, int n_scan, int nz, const PpmTip* tip,
+                                    const PpmRelaxCtl* ctl, float* FEs, float* PPs, int* flag, void* stream) {
+    const char* fn = "ppm_relax_strokes_tilted_window";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (nzw < 2 || klo < 0 || (long long)klo + nzw > spec->nz) {
+        set_error("%s: window [%d, %d) outside the grid's %d planes (at least 2 planes)", fn, klo, klo + nzw, spec->nz);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (int r = check_ctl(fn, ctl, n_scan)) return r;
+    if (!grid || !tip || !flag) { set_error("%s: NULL grid/tip/flag", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_relax_tilted(grid, *spec, points, n_scan, nz, *tip, *ctl, FEs, PPs, (hipStream_t)stream, klo, nzw,
+                               flag);
 }
 
 int ppm_relax_strokes(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,

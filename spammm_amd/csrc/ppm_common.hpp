@@ -36,9 +36,12 @@ int check_launch(const char* what);
 int launch_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
                         const float lvB[3], int cap, const float* base, float* grid, bool acc64,
-                        hipStream_t stream);
+                        hipStream_t stream, int klo = 0, int nzw = 0);
+// klo/nzw/flag: the grid buffer holds planes [klo, klo + nzw) only (nzw = 0: the whole grid); a probe
+// that samples outside them sets *flag.
 int launch_relax_tilted(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
-                        const PpmTip& tip, const PpmRelaxCtl& ctl, float* FEs, float* PPs, hipStream_t stream);
+                        const PpmTip& tip, const PpmRelaxCtl& ctl, float* FEs, float* PPs, hipStream_t stream,
+                        int klo = 0, int nzw = 0, int* flag = nullptr);
 int launch_relax_strokes(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
                          const float dTip[4], const float stiffness[4], const float dpos0[4], const float relax[4],
                          const PpmRelaxCtl& ctl, float* FEs, float* disps, float* PPs, hipStream_t stream);

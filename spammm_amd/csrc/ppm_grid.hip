@@ -25,13 +25,16 @@ namespace {
 
 using namespace gc;
 
+// Planes [klo, kend) of the grid (klo = 0, kend = nz: the whole grid), stored from plane klo on:
+// a z window has the voxel values of the same planes of the whole grid, bit for bit.
 template <int MODEL, bool HASQ0, int V, bool DCZ, typename ACC = float>
 __global__ __launch_bounds__(256) void k_grid_project(const float4* __restrict__ atoms,
                                                       const float* __restrict__ cpar, GridArgs a,
-                                                      float4* __restrict__ out, const float4* base) {
+                                                      float4* __restrict__ out, const float4* base, int klo,
+                                                      int kend) {
     __shared__ Site sites[kTile + 1];
     grid_tile<MODEL, HASQ0, V, DCZ, kOffDynamic, ACC>(atoms, cpar, a, sites, blockIdx.x * 256 + threadIdx.x,
-                                                      blockIdx.y * V, a.nz, out, 0, base);
+                                                      klo + blockIdx.y * V, kend, out, klo, base);
 }
 
 // ── grid -> atom adjoint ─────────────────────────────────────────────────────────────
@@ -290,34 +293,39 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
 
 template <int MODEL, bool HASQ0>
 int dispatch(const float4* atoms, const float* cpar, const GridArgs& a, float4* out, const float4* base,
-             bool acc64, hipStream_t st) {
+             bool acc64, hipStream_t st, int klo, int nzw) {
     const bool dcz = (a.dC[0] == 0.f && a.dC[1] == 0.f);
+    const int kend = klo + nzw;
     dim3 blk(256);
     if (acc64) {          // FP64 sums: 8 voxels per thread (16 would need 128 VGPRs of sums alone)
         const int V = dcz ? 8 : 4;
-        dim3 grd((a.ncol + 255) / 256, (a.nz + V - 1) / V);
+        dim3 grd((a.ncol + 255) / 256, (nzw + V - 1) / V);
         if (dcz)
             hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 8, true, double>), grd, blk, 0, st, atoms, cpar, a, out,
-                               base);
+                               base, klo, kend);
         else
             hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 4, false, double>), grd, blk, 0, st, atoms, cpar, a,
-                               out, base);
+                               out, base, klo, kend);
         return check_launch("grid_project_acc64");
     }
     if (dcz) {
-        if (MODEL == PPM_MODEL_LJ && a.nz >= 16) {   // 16 voxels per thread: -10 % on LJ, +2 % on Morse
-            dim3 grd((a.ncol + 255) / 256, (a.nz + 15) / 16);
-            hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 16, true>), grd, blk, 0, st, atoms, cpar, a, out, base);
-        } else if (a.nz >= 8) {
-            dim3 grd((a.ncol + 255) / 256, (a.nz + 7) / 8);
-            hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 8, true>), grd, blk, 0, st, atoms, cpar, a, out, base);
+        if (MODEL == PPM_MODEL_LJ && nzw >= 16) {   // 16 voxels per thread: -10 % on LJ, +2 % on Morse
+            dim3 grd((a.ncol + 255) / 256, (nzw + 15) / 16);
+            hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 16, true>), grd, blk, 0, st, atoms, cpar, a, out, base,
+                               klo, kend);
+        } else if (nzw >= 8) {
+            dim3 grd((a.ncol + 255) / 256, (nzw + 7) / 8);
+            hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 8, true>), grd, blk, 0, st, atoms, cpar, a, out, base,
+                               klo, kend);
         } else {
-            dim3 grd((a.ncol + 255) / 256, (a.nz + 3) / 4);
-            hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 4, true>), grd, blk, 0, st, atoms, cpar, a, out, base);
+            dim3 grd((a.ncol + 255) / 256, (nzw + 3) / 4);
+            hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 4, true>), grd, blk, 0, st, atoms, cpar, a, out, base,
+                               klo, kend);
         }
     } else {
-        dim3 grd((a.ncol + 255) / 256, (a.nz + 3) / 4);
-        hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 4, false>), grd, blk, 0, st, atoms, cpar, a, out, base);
+        dim3 grd((a.ncol + 255) / 256, (nzw + 3) / 4);
+        hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 4, false>), grd, blk, 0, st, atoms, cpar, a, out, base, klo,
+                           kend);
     }
     return check_launch("grid_project");
 }
@@ -362,17 +370,18 @@ static int adjoint_V(const PpmGridSpec& g) { return (g.dC[0] == 0.f && g.dC[1] =
 int launch_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
                         const float lvB[3], int cap, const float* base, float* grid, bool acc64,
-                        hipStream_t stream) {
+                        hipStream_t stream, int klo, int nzw) {
     const GridArgs a = make_args(na, g, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap);
     const float4* at = reinterpret_cast<const float4*>(atoms);
     float4* o = reinterpret_cast<float4*>(grid);
     const float4* b = reinterpret_cast<const float4*>(base);
     const bool hasq0 = a.q0 != 0.f;
+    if (nzw <= 0) { klo = 0; nzw = g.nz; }
     if (model == PPM_MODEL_LJ)
-        return hasq0 ? dispatch<PPM_MODEL_LJ, true>(at, cpar, a, o, b, acc64, stream)
-                     : dispatch<PPM_MODEL_LJ, false>(at, cpar, a, o, b, acc64, stream);
-    return hasq0 ? dispatch<PPM_MODEL_MORSE, true>(at, cpar, a, o, b, acc64, stream)
-                 : dispatch<PPM_MODEL_MORSE, false>(at, cpar, a, o, b, acc64, stream);
+        return hasq0 ? dispatch<PPM_MODEL_LJ, true>(at, cpar, a, o, b, acc64, stream, klo, nzw)
+                     : dispatch<PPM_MODEL_LJ, false>(at, cpar, a, o, b, acc64, stream, klo, nzw);
+    return hasq0 ? dispatch<PPM_MODEL_MORSE, true>(at, cpar, a, o, b, acc64, stream, klo, nzw)
+                 : dispatch<PPM_MODEL_MORSE, false>(at, cpar, a, o, b, acc64, stream, klo, nzw);
 }
 
 size_t grid_adjoint_workspace(int na, const PpmGridSpec& g) {

@@ -562,7 +562,8 @@ int n_waves(int n_scan, int scan_ny) {
 }  // namespace
 
 int launch_relax_tilted(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
-                        const PpmTip& tip, const PpmRelaxCtl& ctl, float* FEs, float* PPs, hipStream_t stream) {
+                        const PpmTip& tip, const PpmRelaxCtl& ctl, float* FEs, float* PPs, hipStream_t stream,
+                        int klo, int nzw, int* flag) {
     TiltArgs p{};
     p.n_scan = n_scan; p.nz = nz; p.max_iter = ctl.max_iter; p.check_before = ctl.check_before;
     p.scan_ny = ctl.scan_ny; p.f2conv = ctl.f2conv; p.damped = ctl.damped != 0;
@@ -572,7 +573,8 @@ int launch_relax_tilted(const float* grid, const PpmGridSpec& s, const float* pt
     p.dt0 = tip.relax[0]; p.damp0 = tip.relax[1]; p.surf = tip.surfFF[0];
     const int G = relax_group(n_scan);
     const dim3 blocks((unsigned)group_waves(n_scan, ctl.scan_ny, G));
-    const GridDev gd = make_grid(grid, s);
+    GridDev gd = make_grid(grid, s);
+    if (nzw > 0) { gd.klo = klo; gd.nzw = nzw; gd.flag = flag; }
     const float4* P = reinterpret_cast<const float4*>(pts);
     float4* F = reinterpret_cast<float4*>(FEs);
     float4* Q = reinterpret_cast<float4*>(PPs);

@@ -128,16 +128,22 @@ def relax_ctl(max_iter=128, f2conv=1e-8, check_before=False, scan_ny=0, f32_anch
 
 
 def grid_project(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 0), lvA=(0, 0, 0), lvB=(0, 0, 0),
-                 cap=True, out=None, base=None, acc64=False):
+                 cap=True, out=None, base=None, acc64=False, window=None):
     """Force-field grid (nz,ny,nx,4) on the device (evalLJC_QZs_toImg / evalMorseC_QZs_toImg).
     ``base``: a raw (cap=False) grid of other atoms on the same spec, added before the cap.
-    ``acc64``: sum each voxel's FP32 pair terms in FP64 (ppm_grid_project_acc64; no base)."""
+    ``acc64``: sum each voxel's FP32 pair terms in FP64 (ppm_grid_project_acc64; no base).
+    ``window``: (klo, nzw) -> only planes [klo, klo + nzw), returned as (nzw,ny,nx,4)
+    (ppm_grid_project_window; no base, no acc64)."""
     import torch
     _f32dev(atoms, "atoms", 4)
     _f32dev(cpar, "cpar", 2 if model == _lib.PPM_MODEL_LJ else 4)
     if atoms.shape[0] != cpar.shape[0]:
         raise ValueError("atoms and cpar must have the same number of rows")
     nx, ny, nz = (int(v) for v in spec.n)
+    if window is not None:
+        if base is not None or acc64:
+            raise ValueError("a z window takes no base grid and no acc64")
+        nz = int(window[1])
     if out is None:
         out = torch.empty((nz, ny, nx, 4), dtype=torch.float32, device=atoms.device)
     else:
@@ -150,7 +156,10 @@ def grid_project(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 0), 
     s = spec.c()
     args = (int(model), atoms.data_ptr(), cpar.data_ptr(), int(atoms.shape[0]), ctypes.byref(s), _lib.f4(Qs),
             _lib.f4(QZs), int(npbc[0]), int(npbc[1]), ctypes.byref(lva), ctypes.byref(lvb), int(bool(cap)))
-    if acc64:
+    if window is not None:
+        rc = L.ppm_grid_project_window(*args[:5], int(window[0]), int(window[1]), *args[5:], out.data_ptr(),
+                                       _stream(atoms))
+    elif acc64:
         if base is not None:
             raise ValueError("acc64 takes no base grid")
         rc = L.ppm_grid_project_acc64(*args, out.data_ptr(), _stream(atoms))
@@ -203,13 +212,19 @@ def grid_project_f64(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 
 
 
 def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, f2conv=1e-8, check_before=False,
-                 scan_ny=0, FEs=None, PPs=None, want_pps=True, damped=False):
-    """relaxStrokesTilted on the device: returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None)."""
+                 scan_ny=0, FEs=None, PPs=None, want_pps=True, damped=False, window=None, flag=None):
+    """relaxStrokesTilted on the device: returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None).
+    ``window``: (klo, nzw) -> ``grid`` holds only planes [klo, klo + nzw) of the spec's grid
+    (ppm_relax_strokes_tilted_window); ``flag`` (int32 device tensor, zeroed by the caller) turns
+    nonzero if a probe sampled outside the window."""
     import torch
     _f32dev(grid, "grid", 4)
     _f32dev(points, "points", 4)
-    if grid.numel() != 4 * spec.nvox:
+    nplanes = spec.n[2] if window is None else int(window[1])
+    if grid.numel() != 4 * spec.n[0] * spec.n[1] * nplanes:
         raise ValueError("grid size does not match spec")
+    if window is not None and (flag is None or flag.dtype != torch.int32 or not flag.is_cuda):
+        raise ValueError("a windowed relaxation needs an int32 device flag")
     ns = int(points.shape[0])
     if FEs is None:
         FEs = torch.empty((ns, nz, 4), dtype=torch.float32, device=grid.device)
@@ -222,9 +237,15 @@ def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, 
                 raise ValueError(f"{nm} has the wrong size")
     L = _lib.load()
     s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, scan_ny, damped=damped)
-    rc = L.ppm_relax_strokes_tilted(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz),
-                                    ctypes.byref(tp), ctypes.byref(ctl), FEs.data_ptr(),
-                                    PPs.data_ptr() if PPs is not None else None, _stream(grid))
+    if window is not None:
+        rc = L.ppm_relax_strokes_tilted_window(grid.data_ptr(), ctypes.byref(s), int(window[0]), int(window[1]),
+                                               points.data_ptr(), ns, int(nz), ctypes.byref(tp), ctypes.byref(ctl),
+                                               FEs.data_ptr(), PPs.data_ptr() if PPs is not None else None,
+                                               flag.data_ptr(), _stream(grid))
+    else:
+        rc = L.ppm_relax_strokes_tilted(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz),
+                                        ctypes.byref(tp), ctypes.byref(ctl), FEs.data_ptr(),
+                                        PPs.data_ptr() if PPs is not None else None, _stream(grid))
     _lib.check(rc)
     return FEs, PPs
 

@@ -53,6 +53,16 @@ class Problem:
     grid_acc64: bool = False    # grid voxel sums in FP64 (ppm_grid_project_acc64): the accuracy form
     recompute: bool = False     # backward re-runs the relaxation instead of keeping probe positions
     strict: bool = True         # raise if any state is non-finite in the adjoint (gated states are counted)
+    # Column gating (off when gate_eig_frac <= 0): the forward flags each pixel column whose probe
+    # descent comes near a fold of its equilibrium branch -- a state with smallest Hessian
+    # eigenvalue below gate_eig_frac * k_lat, a jump of the probe (relative to the tip) of
+    # gate_max_jump A or more between consecutive slices, or a state the adjoint gates -- where a
+    # small parameter change moves the probe to another branch and the df column jumps.
+    # last_columns: (nx, ny) bool device tensor, True = column kept; callers weight the loss by it.
+    gate_eig_frac: float = 0.0
+    gate_max_jump: float = 0.3
+    last_columns: object = None
+    last_gate: dict = field(default_factory=dict)
     last_status: dict = field(default_factory=dict)
 
     @property
@@ -91,10 +101,43 @@ def _relax(grid, prob: Problem):
     return FEs, PPs
 
 
+def stable_columns(grid, PPs, prob: Problem):
+    """(nx, ny) bool device tensor of the pixel columns that stay on one equilibrium branch under
+    small parameter changes (see Problem.gate_eig_frac), and a dict of counts."""
+    import torch
+    dg = D.relax_state_diag(grid, prob.spec, prob.pts, prob.nz, prob.tip, PPs)
+    k_lat = float(np.min(np.abs(np.asarray(prob.tip.stiffness[:3], np.float64))))
+    ns, nz = prob.n_scan, prob.nz
+    dev = PPs.device
+    iz = torch.arange(nz, device=dev, dtype=torch.float32)
+    C = torch.as_tensor(np.asarray(prob.tip.tipC[:3], np.float32) * np.float32(prob.dz), device=dev)
+    rel = PPs[..., :3] - (prob.pts[:, None, :3] + iz[None, :, None] * C)       # probe - tip anchor
+    jump = (rel[:, 1:] - rel[:, :-1]).norm(dim=-1).amax(dim=1) if nz > 1 else torch.zeros(ns, device=dev)
+    gated = (dg[..., 3] != 0).any(dim=1)
+    soft = dg[..., 1].amin(dim=1) < prob.gate_eig_frac * k_lat
+    jumped = jump >= prob.gate_max_jump
+    keep = ~(gated | soft | jumped)
+    stats = {"columns": ns, "kept": int(keep.sum()), "adjoint_gated": int(gated.sum()), "near_fold": int(soft.sum()),
+             "jumped": int(jumped.sum()), "eig_frac": prob.gate_eig_frac, "max_jump_A": prob.gate_max_jump}
+    return keep.reshape(int(prob.nxy[0]), int(prob.nxy[1])), stats
+
+
+def masked_l2(df, target, columns=None):
+    """1/2 sum w (df - target)^2 (float64) over pixel columns weighted by `columns` ((nx, ny) bool or
+    float; None: all): the L2 loss of the differentiable image restricted to stable columns (the
+    intersection of the forward's and the target's Problem.last_columns)."""
+    r = df.double() - target.double().reshape(df.shape)
+    if columns is not None:
+        r = r * columns.reshape(*columns.shape, 1).to(r.dtype)
+    return 0.5 * (r * r).sum()
+
+
 def _forward_parts(atoms, cpar, prob: Problem):
     grid = D.grid_project(atoms, cpar, _lib.PPM_MODEL_LJ, prob.spec, prob.Qs, prob.QZs, prob.npbc, prob.lvA,
                           prob.lvB, cap=prob.cap, acc64=prob.grid_acc64)
     FEs, PPs = _relax(grid, prob)
+    if prob.gate_eig_frac > 0:
+        prob.last_columns, prob.last_gate = stable_columns(grid, PPs, prob)
     op = DF.z_operator(prob.df_kind, prob.nz, prob.dz, atoms.device, prob.amp, prob.f0, prob.k_cant)
     df = op.apply(FEs, (prob.nz * 4, 4, 0), prob.n_scan, 1, offset=2)
     return grid, FEs, PPs, df.reshape(int(prob.nxy[0]), int(prob.nxy[1]), prob.nz)

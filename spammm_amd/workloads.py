@@ -218,6 +218,26 @@ def flake(seed=1, n_atoms=200) -> Workload:
                     nxy=nxy, nz=60)
 
 
+def flake_differentiable(tip_z0=10.0, seed=1) -> Workload:
+    """The flake on its 300x300x240 grid scanned at 128x128 pixels x 40 z-steps from tip height
+    tip_z0 (10 A: probe 6 -> 2.1 A), the differentiable-image workload."""
+    w = flake(seed)
+    step = 30.0 / 128
+    w.name = "flake_differentiable"
+    w.nxy, w.nz = (128, 128), 40
+    w.scan_p0 = np.array([-15. + step / 2, -15. + step / 2, tip_z0], np.float32)
+    w.scan_da = np.array([step, 0., 0.], np.float32)
+    w.scan_db = np.array([0., step, 0.], np.float32)
+    return w
+
+
+def perturbed_positions(w: Workload, sigma=0.1, seed=3) -> np.ndarray:
+    """(na, 3) float64 atom positions displaced by N(0, sigma) per coordinate (the target of the
+    differentiable workload)."""
+    rng = np.random.default_rng(seed)
+    return w.atoms[:, :3].astype(np.float64) + rng.normal(0, sigma, w.atoms[:, :3].shape)
+
+
 def batch_molecule(i: int, base_seed=100) -> Workload:
     rng = np.random.default_rng(base_seed + i)
     na = int(rng.integers(10, 151))
