@@ -476,6 +476,28 @@ int ppm_batch_forward(const PpmMolDesc* mols, const PpmBatchParams* params,
                       void* workspace, size_t workspace_bytes, int workers, int reset,
                       void* stream);
 
+/* Largest nmol of ppm_batch_forward_noalloc. */
+#define PPM_BATCH_NOALLOC_MAX_MOL 65536
+
+/* ppm_batch_forward without a workspace argument and without any allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+ * results as ppm_batch_forward, except that the work queue, per-molecule counters and window
+ * flags live in a library-owned scratch array in device memory (one per GPU, 512 KiB, kept until
+ * the process ends; its address is looked up on the first call on each device, which must be the
+ * current device) instead of a caller workspace; params->nmol <= PPM_BATCH_NOALLOC_MAX_MOL.
+ *   status: device int[1 + nmol], written after the launch: status[0] = error word (bit 1: a
+ *     wait hit its spin limit), status[1 + m] = window flag of molecule m (nonzero: recompute m
+ *     with the whole grid);
+ *   reset: 1 (clear queue, counters and flags first) or 2 (clear the queue head only: the
+ *     RELAX_ONLY launch that follows a GRID_ONLY launch); 0 is rejected.
+ * Every call on one GPU shares that GPU's scratch array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialized by the caller. */
+int ppm_batch_forward_noalloc(const PpmMolDesc* mols, const PpmBatchParams* params,
+                              const float* atoms, const float* cpar, float* grids,
+                              float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
+                              int* status, int workers, int reset, void* stream);
+
 /* Bytes of device workspace ppm_grid_adjoint needs for na atoms on this grid (0 on bad input). */
 size_t ppm_grid_adjoint_workspace_size(int na, const PpmGridSpec* spec);
 

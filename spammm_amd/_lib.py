@@ -18,6 +18,7 @@ PPM_ERR_UNSUPPORTED = 3
 PPM_MODEL_LJ = 0
 PPM_MODEL_MORSE = 1
 PPM_DIRECT_MAX_ATOMS = 128
+PPM_BATCH_NOALLOC_MAX_MOL = 65536
 PPM_FFT_CORRELATE = 0
 PPM_FFT_CONVOLVE = 1
 PPM_FFT_POISSON = 2
@@ -55,6 +56,7 @@ EXPORTS = (
     "ppm_ewald_num_g", "ppm_ewald_system_bytes", "ppm_ewald_prepare", "ppm_ewald_eval", "ppm_ewald_brute",
     "ppm_ewald_cluster", "ppm_ewald_grid_workspace_size", "ppm_ewald_tip_grid", "ppm_ewald_kept_count_offset",
     "ppm_relax_state_diag", "ppm_relax_strokes_2d", "ppm_grid_project_window", "ppm_relax_strokes_tilted_window",
+    "ppm_batch_forward_noalloc",
 )
 
 _lib = None
@@ -161,6 +163,8 @@ def load():
     L.ppm_batch_workspace_size.argtypes = [i]
     L.ppm_batch_forward.restype = i
     L.ppm_batch_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, i, i, vp]
+    L.ppm_batch_forward_noalloc.restype = i
+    L.ppm_batch_forward_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, i, vp]
     L.ppm_relax_direct_morse.restype = i
     L.ppm_relax_direct_morse.argtypes = [vp, vp, i, vp, i, i, P(PpmTip), _F4, _F4, P(PpmRelaxCtl), vp, vp, vp]
     L.ppm_direct_morse_vjp_workspace_size.restype = ctypes.c_size_t

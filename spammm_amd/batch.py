@@ -163,6 +163,8 @@ class BatchForward:
         L = _lib.load()
         self.ws_bytes = int(L.ppm_batch_workspace_size(len(mols)))
         self.workspace = torch.zeros(self.ws_bytes, dtype=torch.uint8, device=dev)
+        self.status_buf = torch.zeros(1 + len(mols), dtype=torch.int32, device=dev)   # allocation-free launches
+        self._last_noalloc = False
         self.descs_dev = torch.empty(ctypes.sizeof(descs), dtype=torch.uint8, device=dev)
         self.n_cu = torch.cuda.get_device_properties(dev).multi_processor_count
         self._items_for = None
@@ -219,8 +221,10 @@ class BatchForward:
 
     # ── launches ──────────────────────────────────────────────────────────────
 
-    def launch(self, cfg: LaunchConfig = LaunchConfig()):
-        """Enqueue the batch on the current stream (no synchronization)."""
+    def launch(self, cfg: LaunchConfig = LaunchConfig(), noalloc: bool = False):
+        """Enqueue the batch on the current stream (no synchronization).  noalloc: through
+        ppm_batch_forward_noalloc (the library's scratch instead of this batch's workspace; the
+        status lands in status_buf)."""
         import torch
         self._set_items(cfg)
         L = _lib.load()
@@ -230,24 +234,33 @@ class BatchForward:
         W = self.dfop.W.data_ptr() if self.dfop is not None else None
         S = self.dfop.start.data_ptr() if self.dfop is not None else None
         pp = self.PPs.data_ptr() if self.PPs is not None else None
-        args = (self.descs_dev.data_ptr(), None, self.atoms.data_ptr(), self.cpar.data_ptr(), self.grids.data_ptr(),
-                self.FEs.data_ptr(), pp, df_ptr, W, S, self.workspace.data_ptr(), self.ws_bytes, workers)
+        bufs = (self.atoms.data_ptr(), self.cpar.data_ptr(), self.grids.data_ptr(), self.FEs.data_ptr(), pp, df_ptr,
+                W, S)
+        if noalloc:
+            def call(P, reset):
+                return L.ppm_batch_forward_noalloc(self.descs_dev.data_ptr(), ctypes.addressof(P), *bufs,
+                                                   self.status_buf.data_ptr(), workers, reset, stream)
+        else:
+            def call(P, reset):
+                return L.ppm_batch_forward(self.descs_dev.data_ptr(), ctypes.addressof(P), *bufs,
+                                           self.workspace.data_ptr(), self.ws_bytes, workers, reset, stream)
         if cfg.schedule == "fused":
-            P = self._params(cfg, MODES["fused"])
-            _lib.check(L.ppm_batch_forward(args[0], ctypes.addressof(P), *args[2:], 1, stream))
+            _lib.check(call(self._params(cfg, MODES["fused"]), 1))
         elif cfg.schedule == "two_phase":
-            P = self._params(cfg, MODES["grid_only"])
-            _lib.check(L.ppm_batch_forward(args[0], ctypes.addressof(P), *args[2:], 1, stream))
-            P = self._params(cfg, MODES["relax_only"])
-            _lib.check(L.ppm_batch_forward(args[0], ctypes.addressof(P), *args[2:], 2, stream))
+            _lib.check(call(self._params(cfg, MODES["grid_only"]), 1))
+            _lib.check(call(self._params(cfg, MODES["relax_only"]), 2))
         else:
             raise ValueError(f"unknown schedule {cfg.schedule!r}")
+        self._last_noalloc = bool(noalloc)
 
     def status(self):
         """(error word, per-molecule window flags) of the last launch (synchronizes)."""
         import torch
-        ws = self.workspace.view(torch.int32).cpu().numpy()
         n = len(self.molecules)
+        if self._last_noalloc:
+            st = self.status_buf.cpu().numpy()
+            return int(st[0]), st[1:1 + n].copy()
+        ws = self.workspace.view(torch.int32).cpu().numpy()
         return int(ws[1]), ws[4 + n:4 + 2 * n].copy()
 
     def recompute_plain(self, indices, FEs=None, df=None, PPs=None, with_pps=True):

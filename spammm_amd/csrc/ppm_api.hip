@@ -511,12 +511,10 @@ int ppm_direct_morse_vjp(const float* atoms, const float* cMs, int na, const flo
 
 size_t ppm_batch_workspace_size(int nmol) { return nmol < 1 ? 0 : batch_workspace(nmol); }
 
-int ppm_batch_forward(const PpmMolDesc* mols, const PpmBatchParams* params, const float* atoms, const float* cpar,
-                      float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
-                      void* workspace, size_t workspace_bytes, int workers, int reset, void* stream) {
-    const char* fn = "ppm_batch_forward";
-    g_err[0] = 0;
-    if (!mols || !params || !atoms || !cpar || !grids || !FEs || !workspace) {
+static int check_batch(const char* fn, const PpmMolDesc* mols, const PpmBatchParams* params, const float* atoms,
+                       const float* cpar, const float* grids, const float* FEs, const float* df, const float* dfW,
+                       const int* dfStart, int workers, int reset) {
+    if (!mols || !params || !atoms || !cpar || !grids || !FEs) {
         set_error("%s: NULL argument", fn);
         return PPM_ERR_INVALID_ARG;
     }
@@ -533,9 +531,39 @@ int ppm_batch_forward(const PpmMolDesc* mols, const PpmBatchParams* params, cons
     }
     if (workers < 1 || workers > (1 << 16)) { set_error("%s: workers=%d out of range", fn, workers); return PPM_ERR_INVALID_ARG; }
     if (reset < 0 || reset > 2) { set_error("%s: reset=%d not in {0,1,2}", fn, reset); return PPM_ERR_INVALID_ARG; }
-    if (workspace_bytes < batch_workspace(P.nmol)) { set_error("%s: workspace too small", fn); return PPM_ERR_INVALID_ARG; }
-    return launch_batch(mols, P, atoms, cpar, grids, FEs, PPs, df, dfW, dfStart, workspace, workers, reset,
+    return PPM_OK;
+}
+
+int ppm_batch_forward(const PpmMolDesc* mols, const PpmBatchParams* params, const float* atoms, const float* cpar,
+                      float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
+                      void* workspace, size_t workspace_bytes, int workers, int reset, void* stream) {
+    const char* fn = "ppm_batch_forward";
+    g_err[0] = 0;
+    if (int r = check_batch(fn, mols, params, atoms, cpar, grids, FEs, df, dfW, dfStart, workers, reset)) return r;
+    if (!workspace) { set_error("%s: NULL workspace", fn); return PPM_ERR_INVALID_ARG; }
+    if (workspace_bytes < batch_workspace(params->nmol)) { set_error("%s: workspace too small", fn); return PPM_ERR_INVALID_ARG; }
+    return launch_batch(mols, *params, atoms, cpar, grids, FEs, PPs, df, dfW, dfStart, workspace, workers, reset,
                         (hipStream_t)stream);
+}
+
+int ppm_batch_forward_noalloc(const PpmMolDesc* mols, const PpmBatchParams* params, const float* atoms,
+                              const float* cpar, float* grids, float* FEs, float* PPs, float* df, const float* dfW,
+                              const int* dfStart, int* status, int workers, int reset, void* stream) {
+    const char* fn = "ppm_batch_forward_noalloc";
+    g_err[0] = 0;
+    if (int r = check_batch(fn, mols, params, atoms, cpar, grids, FEs, df, dfW, dfStart, workers, reset)) return r;
+    if (!status) { set_error("%s: NULL status", fn); return PPM_ERR_INVALID_ARG; }
+    if (reset == 0) {
+        set_error("%s: reset=0 would reuse the previous call's queue; use 1 (or 2 after a GRID_ONLY launch)", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    if (params->nmol > kBatchNoallocMaxMol) {
+        set_error("%s: nmol=%d exceeds the allocation-free maximum of %d molecules (use ppm_batch_forward)", fn,
+                  params->nmol, kBatchNoallocMaxMol);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return launch_batch_noalloc(mols, *params, atoms, cpar, grids, FEs, PPs, df, dfW, dfStart, status, workers, reset,
+                                (hipStream_t)stream);
 }
 
 size_t ppm_grid_adjoint_workspace_size(int na, const PpmGridSpec* spec) {

@@ -15,6 +15,8 @@
 // (ppm_grid_core.hpp, ppm_relax_core.hpp), so each molecule's results equal the plain path's
 // on the same inputs, bit for bit, whenever no probe leaves the stored z window (a probe that
 // does sets the molecule's flag; the caller recomputes that molecule on the plain path).
+#include <mutex>
+
 #include "ppm_grid_core.hpp"
 #include "ppm_relax_core.hpp"
 
@@ -41,6 +43,14 @@ __global__ void k_batch_reset(Queue* q, int nmol, int head_only) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) { q->head = 0; if (!head_only) q->error = 0; }
     if (!head_only && i < 2 * nmol) q_done(q)[i] = 0;
+}
+
+// status[0] = error word, status[1 + m] = window flag of molecule m (the allocation-free launch's
+// status, copied out of its scratch queue).
+__global__ void k_batch_status(Queue* q, int nmol, int* __restrict__ status) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) status[0] = q->error;
+    if (i < nmol) status[1 + i] = q_flag(q, nmol)[i];
 }
 
 __device__ GridArgs mol_grid_args(const PpmMolDesc& d, const PpmBatchParams& P) {
@@ -229,6 +239,40 @@ void launch_q(bool hasq0, bool offq, int V, dim3 grd, hipStream_t st, const PpmM
 
 size_t batch_workspace(int nmol) { return sizeof(Queue) + 2 * (size_t)nmol * sizeof(int); }
 
+// Scratch of the allocation-free batch entry point: the work queue, per-molecule tile counters and
+// window flags of up to kBatchNoallocMaxMol molecules, one copy per GPU for the life of the process.
+// In host code `scratch` names a host-side stand-in; kernels get its device address from
+// hipGetSymbolAddress, looked up once per device.
+static __device__ __align__(256) unsigned char scratch[sizeof(Queue) + 2 * (size_t)kBatchNoallocMaxMol * sizeof(int)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first call per device).
+int noalloc_scratch(void** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("batch_forward_noalloc: no current HIP device (or device id %d >= %d)", dev, kMaxDevices);
+        return PPM_ERR_HIP;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("batch_forward_noalloc: hipGetSymbolAddress of the scratch array failed on device %d: %s", dev,
+                      hipGetErrorString(e));
+            return PPM_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PPM_OK;
+}
+}  // namespace
+
 int launch_batch(const PpmMolDesc* mols, const PpmBatchParams& P, const float* atoms, const float* cpar,
                  float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
                  void* workspace, int workers, int reset, hipStream_t stream) {
@@ -253,6 +297,18 @@ int launch_batch(const PpmMolDesc* mols, const PpmBatchParams& P, const float* a
     else
         launch_q<PPM_MODEL_MORSE>(hasq0, offq, P.vz, grd, stream, mols, P, at, cpar, gr, fe, pp, df, dfW, dfStart, q);
     return check_launch("batch");
+}
+
+int launch_batch_noalloc(const PpmMolDesc* mols, const PpmBatchParams& P, const float* atoms, const float* cpar,
+                         float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
+                         int* status, int workers, int reset, hipStream_t stream) {
+    void* ws = nullptr;
+    if (int r = noalloc_scratch(&ws)) return r;
+    if (int r = launch_batch(mols, P, atoms, cpar, grids, FEs, PPs, df, dfW, dfStart, ws, workers, reset, stream))
+        return r;
+    hipLaunchKernelGGL(k_batch_status, dim3((P.nmol + 255) / 256), dim3(256), 0, stream, reinterpret_cast<Queue*>(ws),
+                       P.nmol, status);
+    return check_launch("batch_status");
 }
 
 }  // namespace ppm

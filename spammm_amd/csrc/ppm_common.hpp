@@ -87,6 +87,12 @@ int launch_ewald_tip_grid(const void* system, int n_layers, int n_harm, const do
                           const PpmGridSpec& gs, const float Qs[4], const float QZs[4], int accumulate,
                           void* workspace, float* grid, hipStream_t stream);
 size_t batch_workspace(int nmol);
+// Largest batch of the allocation-free batch entry point (its file-scope scratch holds the queue,
+// counters and flags of this many molecules: 512 KiB + 16 B per GPU).
+constexpr int kBatchNoallocMaxMol = PPM_BATCH_NOALLOC_MAX_MOL;
+int launch_batch_noalloc(const PpmMolDesc* mols, const PpmBatchParams& P, const float* atoms, const float* cpar,
+                         float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
+                         int* status, int workers, int reset, hipStream_t stream);
 int launch_batch(const PpmMolDesc* mols, const PpmBatchParams& P, const float* atoms, const float* cpar,
                  float* grids, float* FEs, float* PPs, float* df, const float* dfW, const int* dfStart,
                  void* workspace, int workers, int reset, hipStream_t stream);
