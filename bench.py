@@ -224,7 +224,7 @@ SUMMARY_KEYS = {
                        "adam_smooth_heights"),
     "substrate_poses": ("forward_ms_median", "adjoint_ms_median", "adjoint_over_forward",
                         "placements_per_s_forward_plus_adjoint", "window_fallbacks", "gated_states_total",
-                        "directional_fd_rel_err"),
+                        "gated_columns_total", "directional_fd_rel_err"),
     "direct_morse": ("forward_ms", "vjp_ms", "vjp_over_forward", "pixel_z_steps_per_s"),
     "fdbm": ("stage3_device_ms", "grid_points_per_s_stage3"),
     "ewald": ("tip_grid_ms", "direct_3x3_coulomb_ms", "kept_g_pairs"),
@@ -368,7 +368,8 @@ def substrate_leg(dev, n_poses=64):
     scene = PS.Scene(spec=spec, sub_atoms=t(sc["sub_atoms"]), sub_cpar=t(sc["sub_cLJ"]),
                      mol_xyz0=sc["mol_atoms"][:, :3].astype(np.float64), mol_cpar=t(sc["mol_cLJ"]), pts=t(pts),
                      nxy=sc["nxy"], nz=sc["nz"], tip=tip, Qs=sc["Qs"], QZs=sc["QZs"], npbc=sc["npbc"],
-                     lvA=sc["lvA"], lvB=sc["lvB"], max_iter=1000, f2conv=1e-12, strict=False)
+                     lvA=sc["lvA"], lvB=sc["lvB"], max_iter=1000, f2conv=1e-12, strict=False,
+                     gate_eig_frac=0.5, gate_max_jump=0.3)
     stream = torch.cuda.current_stream(dev)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
@@ -379,18 +380,20 @@ def substrate_leg(dev, n_poses=64):
     log(f"[bench] substrate base grid {t_base:.1f} ms")
     q0 = torch.tensor(sc["mol_atoms"][:, 3], dtype=torch.float64, device=dev)
     dpose = torch.tensor([0.1, -0.1, 0.0, 0.02], dtype=torch.float64, device=dev)
-    t_f, t_b, gnorm, gated = [], [], [], 0
+    t_f, t_b, gnorm, gated, cols_out = [], [], [], 0, 0
     checks = []
     for k in range(n_poses):
         pose_k = torch.tensor(sc["poses"][k], dtype=torch.float64, device=dev)
         with torch.no_grad():
             target = PS.pose_df(pose_k + dpose, q0, scene).double()
+        tcols = scene.last_columns.clone()
         pose = pose_k.clone().requires_grad_(True)
         q = q0.clone().requires_grad_(True)
         a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         a.record(stream)
         dfi = PS.pose_df(pose, q, scene)
-        loss = 0.5 * ((dfi.double() - target) ** 2).sum()
+        cols = scene.last_columns & tcols
+        loss = 0.5 * (((dfi.double() - target) * cols[..., None]) ** 2).sum()
         b.record(stream)
         loss.backward()
         c.record(stream)
@@ -398,6 +401,7 @@ def substrate_leg(dev, n_poses=64):
         t_f.append(a.elapsed_time(b))
         t_b.append(b.elapsed_time(c))
         gated += scene.last_status.get("gated", 0)
+        cols_out += int(cols.numel() - cols.sum())
         gnorm.append(float(pose.grad.norm()))
         if k < 2:       # directional finite difference along -grad(pose) on two placements
             gp = pose.grad.detach()
@@ -405,9 +409,13 @@ def substrate_leg(dev, n_poses=64):
             row = {"pose": k, "adjoint": -float(gp.norm())}
             for eps in (1e-3, 1e-4):
                 with torch.no_grad():
-                    lp = float(0.5 * ((PS.pose_df(pose_k + eps * d, q0, scene).double() - target) ** 2).sum())
-                    lm = float(0.5 * ((PS.pose_df(pose_k - eps * d, q0, scene).double() - target) ** 2).sum())
+                    lp = float(0.5 * (((PS.pose_df(pose_k + eps * d, q0, scene).double() - target)
+                                       * cols[..., None]) ** 2).sum())
+                    lm = float(0.5 * (((PS.pose_df(pose_k - eps * d, q0, scene).double() - target)
+                                       * cols[..., None]) ** 2).sum())
                 row[f"fd_eps_{eps:g}"] = (lp - lm) / (2 * eps)
+            row["rel_err_best"] = min(abs(row["adjoint"] - row[f"fd_eps_{e:g}"]) / abs(row[f"fd_eps_{e:g}"])
+                                      for e in (1e-3, 1e-4))
             checks.append(row)
         if (k + 1) % 16 == 0:
             log(f"[bench] substrate placements {k + 1}/{n_poses}")
@@ -422,7 +430,8 @@ def substrate_leg(dev, n_poses=64):
         "pixel_z_steps_per_s_forward": states / (np.mean(t_f) * 1e-3),
         "placements_per_s_forward_plus_adjoint": 1e3 / (np.mean(t_f) + np.mean(t_b)),
         "gated_states_total": int(gated), "pose_grad_norm_median": float(np.median(gnorm)),
-        "directional_fd": checks,
+        "gated_columns_total": [cols_out, n_poses * scene.n_scan],
+        "directional_fd": checks, "directional_fd_rel_err": [r["rel_err_best"] for r in checks],
         "scan_window": list(scene.spec_run.n), "window_fallbacks": int(scene.window_fallbacks),
         "grid_bytes": int(scene.base.numel() * 4), "peak_mem_GB": torch.cuda.max_memory_allocated(dev) / 1e9,
     }

@@ -89,7 +89,7 @@ def lib():
         L.oracle_interp_f64.argtypes = [_f, _i, _f, _f, _f, _d, c_int, _d]
         for nm in ("oracle_relax_tilted_f64", "oracle_relax_tilted_f32"):
             getattr(L, nm).argtypes = [_f, _i, _f, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, _f, _f, _f,
-                                       c_int, c_float, c_int, _f, _f, c_int]
+                                       c_int, c_float, c_int, _f, _f, c_int, c_int]
         for nm in ("oracle_relax_strokes_f64", "oracle_relax_strokes_f32"):
             getattr(L, nm).argtypes = [_f, _i, _f, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, c_int, c_float,
                                        _f, _f, _f, c_int]
@@ -101,7 +101,7 @@ def lib():
         L.oracle_grid_project_f64g.argtypes = [c_int, _d, _d, c_int, c_int, _d, _d, c_int, c_int, _d, _d, _i,
                                                _d, _d, _d, _d, c_int, _d]
         L.oracle_relax_tilted_f64g.argtypes = [_d, _i, _d, _d, _d, _d, c_int, c_int, _d, _d, _d, _d, _d, _d, _d,
-                                               c_int, ctypes.c_double, c_int, _d, _d]
+                                               c_int, ctypes.c_double, c_int, _d, _d, c_int]
         for nm in ("oracle_relax_direct_f64", "oracle_relax_direct_f32"):
             getattr(L, nm).argtypes = [_f, _f, c_int, _f, _f, _f, c_int, c_int, _f, _f, _f, _f, _f, _f, _f,
                                        c_int, c_float, _f, _f]
@@ -219,9 +219,11 @@ def interp(grid, n, dinvA, dinvB, dinvC, pos):
 
 
 def relax_tilted(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF,
-                 max_iter=128, f2conv=1e-8, check_before=False, precision="f64", damped=False):
+                 max_iter=128, f2conv=1e-8, check_before=False, precision="f64", damped=False, f32_anchors=False):
     """Returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4)) float32.  ``damped``: the damped-velocity
-    integrator of the OPT_FIRE=0 build (kernels/AFM.cl:536-540) instead of update_FIRE."""
+    integrator of the OPT_FIRE=0 build (kernels/AFM.cl:536-540) instead of update_FIRE.
+    ``f32_anchors``: the tip anchors advance in float32 as in the kernel (kernels/AFM.cl:556) also in
+    the FP64 mode (the FP32 scan's exact heights)."""
     g, nn, a, b, c = _grid_args(grid, n, dinvA, dinvB, dinvC)
     pts = _cf(pts).reshape(-1, 4)
     ns = pts.shape[0]
@@ -231,7 +233,7 @@ def relax_tilted(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC, stiffn
     fn = lib().oracle_relax_tilted_f64 if precision == "f64" else lib().oracle_relax_tilted_f32
     fn(_p(g, _f), _p(nn, _i), _p(a, _f), _p(b, _f), _p(c, _f), _p(pts, _f), ns, int(nz),
        *[_p(v, _f) for v in vs], int(max_iter), float(f2conv), int(bool(check_before)), _p(FEs, _f), _p(PPs, _f),
-       int(bool(damped)))
+       int(bool(damped)), int(bool(f32_anchors)))
     return FEs, PPs
 
 
@@ -304,8 +306,9 @@ def grid_project_d(model, atoms, cpar, n, p0, dA, dB, dC, Qs, QZs, npbc=(0, 0), 
 
 
 def relax_tilted_d(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF,
-                   max_iter=128, f2conv=1e-8, check_before=False):
-    """relaxStrokesTilted in double on a double grid: (FEs, PPs) float64 (n_scan,nz,4)."""
+                   max_iter=128, f2conv=1e-8, check_before=False, f32_anchors=False):
+    """relaxStrokesTilted in double on a double grid: (FEs, PPs) float64 (n_scan,nz,4);
+    ``f32_anchors`` as in relax_tilted."""
     grid = _cd(grid)
     nn = np.asarray(n, dtype=np.int32).ravel()[:3].copy()
     assert grid.size == 4 * int(np.prod(nn.astype(np.int64)))
@@ -317,7 +320,7 @@ def relax_tilted_d(grid, n, dinvA, dinvB, dinvC, pts, nz, tipA, tipB, tipC, stif
     ts = [_dvec(x, 4) for x in (tipA, tipB, tipC, stiffness, dpos0, relax, surfFF)]
     lib().oracle_relax_tilted_f64g(_p(grid, _d), _p(nn, _i), *[_p(v, _d) for v in vs], _p(pts, _d), ns, int(nz),
                                    *[_p(t, _d) for t in ts], int(max_iter), float(f2conv), int(bool(check_before)),
-                                   _p(FEs, _d), _p(PPs, _d))
+                                   _p(FEs, _d), _p(PPs, _d), int(bool(f32_anchors)))
     return FEs, PPs
 
 

@@ -159,7 +159,7 @@ template <class R> inline V3<R> integrate(int damped, V3<R> f, V3<R> v, R& dt, R
 template <class R, class SRC, class P>
 void relax_tilted_src(const SRC& sample, const P* pts, int n_scan, int nz, const P* tipA, const P* tipB,
                       const P* tipC, const P* stiffness, const P* dpos0, const P* relax, const P* surfFF, int max_iter,
-                      double f2conv_f, int check_before, P* FEs, P* PPs, int damped = 0) {
+                      double f2conv_f, int check_before, P* FEs, P* PPs, int damped = 0, int f32_anchors = 0) {
     const V3<R> A = ld3<R>(tipA), B = ld3<R>(tipB), C = ld3<R>(tipC);
     const R f2conv = (R)f2conv_f;
 #pragma omp parallel for schedule(dynamic, 8)
@@ -167,6 +167,13 @@ void relax_tilted_src(const SRC& sample, const P* pts, int n_scan, int nz, const
         const V3<R> dTip = mul(C, (R)tipC[3]);
         const V3<R> d0r = rotMatT(ld3<R>(dpos0), A, B, C);
         V3<R> tipPos = ld3<R>(pts + 4 * (size_t)ip);
+        // f32_anchors: the anchor advanced as the FP32 kernel advances it (float32 position plus the
+        // float32 step tipC.xyz * tipC.w, kernels/AFM.cl:556), so an FP64 relaxation runs at the FP32
+        // scan's exact heights
+        V3<float> tipf = v3<float>((float)pts[4 * (size_t)ip], (float)pts[4 * (size_t)ip + 1],
+                                   (float)pts[4 * (size_t)ip + 2]);
+        const V3<float> dTipf = mul(v3<float>((float)tipC[0], (float)tipC[1], (float)tipC[2]), (float)tipC[3]);
+        if (f32_anchors) tipPos = v3<R>((R)tipf.x, (R)tipf.y, (R)tipf.z);
         V3<R> pos = add(tipPos, d0r);
         R dt = (R)relax[0], damp = (R)relax[1];
         const R dtmax = dt, dtmin = dtmax * (R)0.1f, damp0 = damp;
@@ -201,6 +208,10 @@ void relax_tilted_src(const SRC& sample, const P* pts, int n_scan, int nz, const
                 PPs[o + 3] = (P)((conv && finite) ? iters : -iters);
             }
             tipPos = add(tipPos, dTip);
+            if (f32_anchors) {
+                tipf = add(tipf, dTipf);
+                tipPos = v3<R>((R)tipf.x, (R)tipf.y, (R)tipf.z);
+            }
             pos = add(pos, dTip);
         }
     }
@@ -209,9 +220,9 @@ void relax_tilted_src(const SRC& sample, const P* pts, int n_scan, int nz, const
 template <class R, class GT, class P>
 void relax_tilted(const GridT<GT>& g, const P* pts, int n_scan, int nz, const P* tipA, const P* tipB, const P* tipC,
                   const P* stiffness, const P* dpos0, const P* relax, const P* surfFF, int max_iter, double f2conv_f,
-                  int check_before, P* FEs, P* PPs, int damped = 0) {
+                  int check_before, P* FEs, P* PPs, int damped = 0, int f32_anchors = 0) {
     relax_tilted_src<R>([&g](V3<R> p) { return interp<R>(g, p); }, pts, n_scan, nz, tipA, tipB, tipC, stiffness,
-                        dpos0, relax, surfFF, max_iter, f2conv_f, check_before, FEs, PPs, damped);
+                        dpos0, relax, surfFF, max_iter, f2conv_f, check_before, FEs, PPs, damped, f32_anchors);
 }
 
 // relaxStrokes   kernels/AFM.cl:345-394.  FE output is the sample taken at the start
@@ -599,10 +610,10 @@ void oracle_interp_f64(const float* grid, const int* n, const float* dinvA, cons
     void NAME(const float* grid, const int* n, const float* dinvA, const float* dinvB, const float* dinvC,          \
               const float* pts, int n_scan, int nz, const float* tipA, const float* tipB, const float* tipC,        \
               const float* stiffness, const float* dpos0, const float* relax, const float* surfFF, int max_iter,    \
-              float f2conv, int check_before, float* FEs, float* PPs, int damped) {                                 \
+              float f2conv, int check_before, float* FEs, float* PPs, int damped, int f32_anchors) {                \
         Grid g = make_grid(grid, n, dinvA, dinvB, dinvC);                                                           \
         relax_tilted<R>(g, pts, n_scan, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF, max_iter, f2conv,    \
-                        check_before, FEs, PPs, damped);                                                            \
+                        check_before, FEs, PPs, damped, f32_anchors);                                               \
     }
 RELAX_TILTED(oracle_relax_tilted_f64, double)
 RELAX_TILTED(oracle_relax_tilted_f32, float)
@@ -670,13 +681,13 @@ void oracle_relax_tilted_f64g(const double* grid, const int* n, const double* di
                               const double* dinvC, const double* pts, int n_scan, int nz, const double* tipA,
                               const double* tipB, const double* tipC, const double* stiffness, const double* dpos0,
                               const double* relax, const double* surfFF, int max_iter, double f2conv,
-                              int check_before, double* FEs, double* PPs) {
+                              int check_before, double* FEs, double* PPs, int f32_anchors) {
     GridT<double> g;
     g.data = grid;
     for (int k = 0; k < 3; ++k) g.n[k] = n[k];
     for (int q = 0; q < 4; ++q) { g.dinv[0][q] = dinvA[q]; g.dinv[1][q] = dinvB[q]; g.dinv[2][q] = dinvC[q]; }
     relax_tilted<double>(g, pts, n_scan, nz, tipA, tipB, tipC, stiffness, dpos0, relax, surfFF, max_iter, f2conv,
-                         check_before, FEs, PPs);
+                         check_before, FEs, PPs, 0, f32_anchors);
 }
 
 // relaxStrokesTiltedMorseDirect  (kernels/AFM.cl:1419-1511); cMs rows (R0,E0,K,0).

@@ -101,25 +101,34 @@ def _relax(grid, prob: Problem):
     return FEs, PPs
 
 
-def stable_columns(grid, PPs, prob: Problem):
+def column_gate(grid, spec, pts, nxy, nz, tip, PPs, eig_frac, max_jump):
     """(nx, ny) bool device tensor of the pixel columns that stay on one equilibrium branch under
-    small parameter changes (see Problem.gate_eig_frac), and a dict of counts."""
+    small parameter changes -- every state passes the adjoint's gates, has a smallest Hessian
+    eigenvalue (of the symmetric part of -J, ppm_relax_state_diag) of at least eig_frac * k_lat, and
+    the probe moves by less than max_jump A relative to the tip between consecutive slices -- and a
+    dict of counts."""
     import torch
-    dg = D.relax_state_diag(grid, prob.spec, prob.pts, prob.nz, prob.tip, PPs)
-    k_lat = float(np.min(np.abs(np.asarray(prob.tip.stiffness[:3], np.float64))))
-    ns, nz = prob.n_scan, prob.nz
+    dg = D.relax_state_diag(grid, spec, pts, nz, tip, PPs)
+    k_lat = float(np.min(np.abs(np.asarray(tip.stiffness[:3], np.float64))))
+    ns = int(nxy[0]) * int(nxy[1])
     dev = PPs.device
     iz = torch.arange(nz, device=dev, dtype=torch.float32)
-    C = torch.as_tensor(np.asarray(prob.tip.tipC[:3], np.float32) * np.float32(prob.dz), device=dev)
-    rel = PPs[..., :3] - (prob.pts[:, None, :3] + iz[None, :, None] * C)       # probe - tip anchor
+    C = torch.as_tensor(np.asarray(tip.tipC[:3], np.float32) * np.float32(tip.tipC[3]), device=dev)
+    rel = PPs[..., :3] - (pts[:, None, :3] + iz[None, :, None] * C)       # probe - tip anchor
     jump = (rel[:, 1:] - rel[:, :-1]).norm(dim=-1).amax(dim=1) if nz > 1 else torch.zeros(ns, device=dev)
     gated = (dg[..., 3] != 0).any(dim=1)
-    soft = dg[..., 1].amin(dim=1) < prob.gate_eig_frac * k_lat
-    jumped = jump >= prob.gate_max_jump
+    soft = dg[..., 1].amin(dim=1) < eig_frac * k_lat
+    jumped = jump >= max_jump
     keep = ~(gated | soft | jumped)
     stats = {"columns": ns, "kept": int(keep.sum()), "adjoint_gated": int(gated.sum()), "near_fold": int(soft.sum()),
-             "jumped": int(jumped.sum()), "eig_frac": prob.gate_eig_frac, "max_jump_A": prob.gate_max_jump}
-    return keep.reshape(int(prob.nxy[0]), int(prob.nxy[1])), stats
+             "jumped": int(jumped.sum()), "eig_frac": eig_frac, "max_jump_A": max_jump}
+    return keep.reshape(int(nxy[0]), int(nxy[1])), stats
+
+
+def stable_columns(grid, PPs, prob: Problem):
+    """column_gate for a Problem (its gate_eig_frac, gate_max_jump)."""
+    return column_gate(grid, prob.spec, prob.pts, prob.nxy, prob.nz, prob.tip, PPs, prob.gate_eig_frac,
+                       prob.gate_max_jump)
 
 
 def masked_l2(df, target, columns=None):

@@ -69,6 +69,12 @@ class Scene:
     window_faces: tuple = None  # (lo, hi) bool[3]: window faces that are faces of the full grid
     full: object = None         # (base, grid, grid_cot, serial) of the full grid, built on first fallback
     keep_states: bool = True    # the backward reuses the forward's probe positions (else recomputes them)
+    # column gating (diff.column_gate; off when gate_eig_frac <= 0): last_columns (nx, ny) bool, True =
+    # the column's probes stay on one equilibrium branch; callers weight the loss by it
+    gate_eig_frac: float = 0.0
+    gate_max_jump: float = 0.3
+    last_columns: object = None
+    last_gate: dict = field(default_factory=dict)
 
     @property
     def n_scan(self):
@@ -243,6 +249,12 @@ def _make_function():
                 scene.window_fallbacks += 1
                 full = True
                 FEs, PPs = _forward(scene, atoms, full=True)
+            if scene.gate_eig_frac > 0:
+                from .diff import column_gate
+                spec, _, grid, _ = scene.buffers(full)
+                scene.last_columns, scene.last_gate = column_gate(grid, spec, scene.pts, scene.nxy, scene.nz,
+                                                                  scene.tip, PPs, scene.gate_eig_frac,
+                                                                  scene.gate_max_jump)
             ctx.full = full
             ctx.PPs = PPs if scene.keep_states else None
             ctx.grid_serial = _serial(scene, full)

@@ -30,10 +30,12 @@ def make_scene(sc, **kw):
                 dpos0=np.array([0, 0, -sc["bond"], sc["bond"]], np.float32),
                 tipC=np.array([0., 0., 1., sc["dtip"]], np.float32))
     pts = D.scan_points(sc["scan_p0"], sc["scan_da"], sc["scan_db"], *sc["nxy"])
+    args = dict(max_iter=3000, f2conv=1e-12)
+    args.update(kw)
     scene = PS.Scene(spec=spec, sub_atoms=t(sc["sub_atoms"]), sub_cpar=t(sc["sub_cLJ"]),
                      mol_xyz0=sc["mol_atoms"][:, :3].astype(np.float64), mol_cpar=t(sc["mol_cLJ"]), pts=t(pts),
                      nxy=sc["nxy"], nz=sc["nz"], tip=tip, Qs=sc["Qs"], QZs=sc["QZs"], npbc=sc["npbc"],
-                     lvA=sc["lvA"], lvB=sc["lvB"], max_iter=3000, f2conv=1e-12, **kw)
+                     lvA=sc["lvA"], lvB=sc["lvB"], **args)
     return scene.prepare(), pts
 
 

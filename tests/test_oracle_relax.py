@@ -121,6 +121,33 @@ def test_uniform_field_equilibrium_closed_form(precision):
             np.testing.assert_allclose(FEs[i, iz, :3], F0, atol=1e-6)
 
 
+def test_f32_anchors_follow_the_float32_tip_accumulation():
+    """relax_tilted_d with f32_anchors: in a uniform field the relaxed probe sits at the closed-form
+    offset from anchors accumulated in float32 (tip += dTip, kernels/AFM.cl:556); without it, from the
+    exact anchors p0 + iz*dtip.  Over 60 slices of -0.1 A the two anchor sequences differ by more than
+    the 1e-9 A tolerance, so each mode is told apart."""
+    F0 = np.array([0.01, -0.004, 0.02])
+    G, n, dinv, st, d0 = uniform_field_case(F0)
+    pts = np.array([[0.1, -0.2, 6.3, 0]], np.float32)
+    nz = 60
+    tipC = np.array([0, 0, 1, -0.1], np.float32)
+    dref = analytic_equilibrium(F0.astype(np.float32).astype(np.float64), float(np.float32(0.05)), 1.5, 4.0)  # FP32 inputs
+    anch32 = np.zeros((nz, 3), np.float32)
+    cur = pts[0, :3].copy()
+    for iz in range(nz):
+        anch32[iz] = cur
+        cur = cur + tipC[:3] * tipC[3]
+    anch64 = pts[0, :3].astype(np.float64) + np.outer(np.arange(nz), [0, 0, float(tipC[3])])
+    assert np.abs(anch32 - anch64).max() > 1e-7
+    print(f"anchor drift of the float32 accumulation: {np.abs(anch32 - anch64).max():.2e} A")
+    for f32, anch in ((True, anch32.astype(np.float64)), (False, anch64)):
+        FEs, PPs = O.relax_tilted_d(G.astype(np.float64), n, *dinv, pts, nz, [1, 0, 0, 0], [0, 1, 0, 0], tipC, st, d0,
+                                    [0.5, 0.1, 0, 0], [0, 0, 0, 0], max_iter=20000, f2conv=1e-26, check_before=True,
+                                    f32_anchors=f32)
+        assert (PPs[..., 3] > 0).all()
+        np.testing.assert_allclose(PPs[0, :, :3] - anch, np.tile(dref, (nz, 1)), rtol=0, atol=1e-9)
+
+
 def test_mirror_symmetric_pixel_has_no_lateral_displacement():
     """One LJ atom on the z axis, pixel on the axis: x,y of the probe stay exactly on the axis."""
     from oracle.oracle import grid_project
