@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable leg")
+    ap.add_argument("--no-fd", action="store_true",
+                    help="skip the finite-difference checks of the differentiable and substrate legs (timing only)")
     ap.add_argument("--adam-steps", type=int, default=50)
     ap.add_argument("--batch-mols", type=int, default=512, help="molecules of the batched leg (0: skip)")
     ap.add_argument("--no-direct", action="store_true", help="skip the gridless direct Morse leg")
@@ -84,24 +86,23 @@ def main():
     df = torch.empty((w.n_scan, nz, 1), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream(dev)
 
-    ev = {k: [] for k in ("grid", "relax", "df")}
+    ev = {k: [] for k in ("grid", "relax")}
+    df2 = df.view(w.n_scan, dfop.n_out)
 
     def step(record=False):
-        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)] if record else None
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)] if record else None
         if record:
             e[0].record(stream)
         D.grid_project(atoms, cpar, _lib.PPM_MODEL_LJ, sp, w.Qs, w.QZs, w.npbc, w.lvA, w.lvB, True, out=grid)
         if record:
             e[1].record(stream)
-        D.relax_tilted(grid, sp, pts, nz, tip, w.max_iter, w.f2conv, False, w.nxy[1], FEs=FEs, PPs=PPs)
+        # relaxation with the df operator applied to each Fz column in the same launch
+        D.relax_tilted(grid, sp, pts, nz, tip, w.max_iter, w.f2conv, False, w.nxy[1], FEs=FEs, PPs=PPs,
+                       df_op=dfop, df=df2)
         if record:
             e[2].record(stream)
-        dfop.apply(FEs, (nz * 4, 4, 0), w.n_scan, 1, offset=2, out=df)
-        if record:
-            e[3].record(stream)
             ev["grid"].append((e[0], e[1]))
             ev["relax"].append((e[1], e[2]))
-            ev["df"].append((e[2], e[3]))
 
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
@@ -125,7 +126,7 @@ def main():
     def avg_ms(k):
         return float(np.mean([a.elapsed_time(b) for a, b in ev[k]]))
 
-    t_grid, t_relax, t_df = avg_ms("grid"), avg_ms("relax"), avg_ms("df")
+    t_grid, t_relax = avg_ms("grid"), avg_ms("relax")
     states = w.n_scan * nz
     value = states / (wall / args.steps)
     pp = PPs.cpu().numpy()
@@ -136,8 +137,7 @@ def main():
     grid_tflops = flop / (t_grid * 1e-3) / 1e12
     relax_bytes = states * (128.0 * (mean_iter + 1.0) + 32.0)
     relax_gbs = relax_bytes / (t_relax * 1e-3) / 1e9
-    df_gbs = w.n_scan * nz * 8.0 / (t_df * 1e-3) / 1e9
-    log(f"[bench] step {ms_step:.3f} ms (grid {t_grid:.3f} relax {t_relax:.3f} df {t_df:.4f}); "
+    log(f"[bench] step {ms_step:.3f} ms (grid {t_grid:.3f} relax+df {t_relax:.3f}); "
         f"mean iters {mean_iter:.2f} converged {conv_frac:.4f}")
 
     out = {
@@ -185,20 +185,19 @@ def main():
             "relax_algorithmic_GBps": relax_gbs,
             "relax_algorithmic_frac_of_hbm_peak": relax_gbs / PEAK_HBM_GBS,
             "relax_hbm_bytes_per_launch": _pmc_traffic("k_relax_tilted", w.name),
-            "df_ms": t_df,
-            "df_GBps": df_gbs,
+            "df": "applied to each Fz column inside the relaxation launch (ppm_relax_strokes_tilted_df)",
         },
     }
 
     legs = {}
     if not args.no_diff:
-        legs["differentiable"] = diff_leg(dev, args.adam_steps)
+        legs["differentiable"] = diff_leg(dev, args.adam_steps, fd=not args.no_fd)
     if args.batch_mols > 0:
         legs["batched"] = batch_leg(dev, args.batch_mols, args.z_margin)
     if not args.no_direct:
         legs["direct_morse"] = direct_leg(dev)
     if args.poses > 0:
-        legs["substrate_poses"] = substrate_leg(dev, args.poses)
+        legs["substrate_poses"] = substrate_leg(dev, args.poses, fd=not args.no_fd)
     if not args.no_fdbm:
         legs["fdbm"] = fdbm_leg(dev)
     if not args.no_ewald:
@@ -346,7 +345,7 @@ def batch_leg(dev, n_mol, z_margin=None):
     return res
 
 
-def substrate_leg(dev, n_poses=64):
+def substrate_leg(dev, n_poses=64, fd=True):
     """Placements of a 60-atom molecule 3 A above a 4-layer rock-salt slab (1024 ions, periodic
     x,y, 902x902x300 grid at 0.05 A, 192x192 pixels x 60 z-steps): the slab's raw grid once, then
     per placement the forward (grid over the slab base, relaxation, df) and the adjoint of an L2
@@ -403,7 +402,7 @@ def substrate_leg(dev, n_poses=64):
         gated += scene.last_status.get("gated", 0)
         cols_out += int(cols.numel() - cols.sum())
         gnorm.append(float(pose.grad.norm()))
-        if k < 2:       # directional finite difference along -grad(pose) on two placements
+        if fd and k < 2:    # directional finite difference along -grad(pose) on two placements
             gp = pose.grad.detach()
             d = -gp / gp.norm()
             row = {"pose": k, "adjoint": -float(gp.norm())}
@@ -420,7 +419,15 @@ def substrate_leg(dev, n_poses=64):
         if (k + 1) % 16 == 0:
             log(f"[bench] substrate placements {k + 1}/{n_poses}")
     states = scene.n_scan * scene.nz
+    with torch.no_grad():       # work of the grid->atom adjoint of placement 0 (its active workgroups)
+        x0 = PS.place(torch.as_tensor(scene.mol_xyz0, device=dev), torch.as_tensor(scene.center, device=dev),
+                      torch.tensor(sc["poses"][0], dtype=torch.float64, device=dev))
+        _, PPs0 = PS._forward(scene, torch.cat([x0.float(), q0.float()[:, None]], 1).contiguous())
+    nimg = (2 * scene.npbc[0] + 1) * (2 * scene.npbc[1] + 1)
+    adj_pairs, adj_blocks = grid_adjoint_active_pairs(scene.spec_run, PPs0, sc["mol_atoms"].shape[0] * nimg)
     res = {
+        "grid_adjoint_work_placement0": {"active_workgroups": adj_blocks, "pair_evals": adj_pairs,
+                                         "flop": adj_pairs * ADJ_FLOP_PER_PAIR},
         "workload": f"60-atom molecule x {n_poses} placements over a 1024-ion rock-salt slab, "
                     f"{sc['n'][0]}x{sc['n'][1]}x{sc['n'][2]} grid (0.05 A), {sc['nxy'][0]}x{sc['nxy'][1]} px x {sc['nz']} z",
         "base_grid_ms": t_base, "forward_ms_mean": float(np.mean(t_f)), "adjoint_ms_mean": float(np.mean(t_b)),
@@ -702,7 +709,7 @@ def direct_leg(dev, reps=5):
     return out
 
 
-def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False):
+def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False, fd=True):
     """One height range of the differentiable flake case (see diff_leg)."""
     import torch
     from spammm_amd import diff, gradcheck as GC, params as P, workloads as WL
@@ -739,14 +746,21 @@ def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False):
     gx, gq = x.grad.detach(), q.grad.detach()
     gating = {"forward": dict(prob.last_gate), "target": tgate, "columns_kept_both": int(cols.sum()),
               "columns": int(cols.numel())}
+    with torch.no_grad():
+        atoms = torch.cat([xyz0.float(), q0.float()[:, None]], 1).contiguous()
+        _, _, PPs, _ = diff._forward_parts(atoms, diff.lj_cpar(R0, E0), prob)
+    nimg = (2 * prob.npbc[0] + 1) * (2 * prob.npbc[1] + 1)
+    adj_pairs, adj_blocks = grid_adjoint_active_pairs(prob.spec, PPs, xyz0.shape[0] * nimg)
     status = dict(prob.last_status)
     # central differences of the FP64 device forward on 32 random parameters (seed 3), same gated loss
     na = xyz0.shape[0]
     sel = np.random.default_rng(3).choice(4 * na, 32, replace=False)
     t0 = time.perf_counter()
-    fd = GC.fd_check(GC.Fp64Forward(prob, R0, E0), target.reshape(prob.n_scan, prob.nz), xyz0, q0, gx, gq, sel,
-                     steps=fd_steps, columns=cols, order4_step=fd_steps[0] if order4 else None)
-    fd["wall_s"] = time.perf_counter() - t0
+    fdres = None
+    if fd:
+        fdres = GC.fd_check(GC.Fp64Forward(prob, R0, E0), target.reshape(prob.n_scan, prob.nz), xyz0, q0, gx, gq,
+                            sel, steps=fd_steps, columns=cols, order4_step=fd_steps[0] if order4 else None)
+        fdres["wall_s"] = time.perf_counter() - t0
     # Adam on the positions, the loss gated on the columns stable in the forward and in the target
     x = xyz0.clone().requires_grad_(True)
     opt = torch.optim.Adam([x], lr=lr)
@@ -767,14 +781,39 @@ def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False):
         "tip_z": [z0, z0 + (w.nz - 1) * float(w.dtip)], "probe_z_rest": [z0 - 4.0, z0 - 4.0 + (w.nz - 1) * float(w.dtip)],
         "forward_ms": tf, "adjoint_ms": tb, "adjoint_over_forward": tb / tf,
         "pixel_z_steps_per_s_forward": w.n_scan * w.nz / (tf * 1e-3),
-        "adjoint_status": status, "gating": gating, "fd_fp64": fd,
+        "adjoint_status": status, "gating": gating, "fd_fp64": fdres,
+        "grid_adjoint_work": {"active_workgroups": adj_blocks, "pair_evals": adj_pairs,
+                              "flop": adj_pairs * ADJ_FLOP_PER_PAIR, "flop_per_pair": ADJ_FLOP_PER_PAIR},
         "adam": {"steps": adam_steps, "lr": lr, "wall_s": t_adam, "loss_first": hist[0] if hist else None,
                  "loss_last": hist[-1] if hist else None, "loss_every5": hist[::5],
                  "rmsd_xy_start_end_A": rxy, "rmsd_z_start_end_A": rz},
     }
 
 
-def diff_leg(dev, adam_steps=50):
+ADJ_FLOP_PER_PAIR = 60   # grid->atom adjoint, LJ + tip charge at z offset 0 (pair_adjoint), counted as the forward's 52
+
+
+def grid_adjoint_active_pairs(spec, PPs, n_sites, V=8):
+    """Pair evaluations of k_grid_adjoint (V voxels per thread, 256 columns per workgroup) over the
+    workgroups it does not skip: those holding a corner of the trilinear cell of some relaxed probe
+    (the only voxels with a nonzero cotangent), x 256 columns x V planes x atom sites."""
+    nx, ny, nz = (int(v) for v in spec.n)
+    step = np.array([spec.dA[0], spec.dB[1], spec.dC[2]], np.float64)
+    p = PPs[..., :3].reshape(-1, 3).double().cpu().numpy()
+    i0 = np.floor((p - np.asarray(spec.p0[:3], np.float64)) / step).astype(np.int64)
+    blocks = set()
+    for cx in (0, 1):
+        for cy in (0, 1):
+            for cz in (0, 1):
+                ia = (i0[:, 0] + cx) % nx
+                ib = (i0[:, 1] + cy) % ny
+                ic = (i0[:, 2] + cz) % nz
+                key = ((ib * nx + ia) // 256) * ((nz + V - 1) // V) + ic // V
+                blocks.update(np.unique(key).tolist())
+    return len(blocks) * 256 * V * n_sites, len(blocks)
+
+
+def diff_leg(dev, adam_steps=50, fd=True):
     """Differentiable df image of the flake molecule, 128x128 pixels x 40 z-steps, L2 loss to a target
     made from positions perturbed by N(0, 0.1 A), taken over the pixel columns whose probes stay on one
     equilibrium branch in the forward and in the target (diff.stable_columns): forward and adjoint
@@ -784,13 +823,15 @@ def diff_leg(dev, adam_steps=50):
     probe 6 -> 2.1 A) and 1 A higher (probe 7 -> 3.1 A)."""
     steps = ((1e-3, 1e-4), (1e-4, 1e-5))
     res = {"workload": "flake 200 atoms, 300x300x240 grid, 128x128 pixels x 40 z, Giessibl df, gated L2 loss",
-           "configured_heights": _diff_case(dev, 10.0, adam_steps, 1e-2, steps, order4=True),
-           "smooth_heights": _diff_case(dev, 11.0, adam_steps, 1e-2, steps)}
+           "configured_heights": _diff_case(dev, 10.0, adam_steps, 1e-2, steps, order4=True, fd=fd),
+           "smooth_heights": _diff_case(dev, 11.0, adam_steps, 1e-2, steps, fd=fd)}
     c = res["configured_heights"]
     res["forward_ms"], res["adjoint_ms"], res["adjoint_over_forward"] = (c["forward_ms"], c["adjoint_ms"],
                                                                        c["adjoint_over_forward"])
 
     def fd_summary(f):
+        if f is None:
+            return None
         out = {p["h"]: p["within_tol"] for p in f["per_step"]}
         if "order4" in f:
             out[f["order4"]["h"] + " order4"] = f["order4"]["within_tol"]

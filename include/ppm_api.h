@@ -83,15 +83,19 @@ typedef struct PpmRelaxCtl {
  * geometry is that of PpmGridSpec; only the z planes [klo, klo + nzw) are computed and stored
  * (at grid_off float4s into the batch grid buffer, plane-major, x fastest).  Its scan is the
  * ix-major raster start(ix,iy) = (scan_p0 + scan_da*ix) + scan_db*iy, ix < scan_nx,
- * iy < scan_ny; per-pixel outputs start at pixel pix_off.  Work items item_off .. are the
- * molecule's grid tiles followed by its relaxation blocks (ppm_batch_items computes them). */
+ * iy < scan_ny; per-pixel outputs start at pixel pix_off.  Work queue items
+ * [item_off, item_off + n_grid_items) are the molecule's grid tiles and
+ * [relax_item_off, relax_item_off + n_relax_items) its relaxation blocks (relax_item_off = 0:
+ * right after the tiles).  Across molecules both kinds of segment start in increasing order of the
+ * molecule index and every relaxation segment comes after its own molecule's tiles; the segments
+ * of different molecules may interleave (e.g. tiles of m + k before relaxation blocks of m). */
 typedef struct PpmMolDesc {
     int atom_off, na;
     int nx, ny, nz;
     int klo, nzw;
     int npbc_a, npbc_b;
     int pix_off, scan_nx, scan_ny;
-    int item_off, n_grid_items, n_relax_items, reserved;
+    int item_off, n_grid_items, n_relax_items, relax_item_off;
     long long grid_off;
     float p0[4], dA[4], dB[4], dC[4];
     float dinvA[4], dinvB[4], dinvC[4];
@@ -221,10 +225,16 @@ size_t ppm_fdbm_workspace_size(int nx, int ny, int nz);
  * (es_fused_from_rho_cl :3543-3565, fdbm_mul_poisson_tip_c64 kernels/AFM.cl:1287-1303);
  * electrostatic convolution = CONVOLVE(V, flip(tip_del)) with scale dV
  * (compute_es_conv_field :4153-4175).  a, b, out: device (nx,ny,nz); out may not alias a or b.
- * hipFFT plans are created on the first call per shape and device and kept (plan creation
- * allocates hipFFT's own work area). */
+ * hipFFT plans are created on the first call per shape and device and kept until
+ * ppm_fdbm_release_plans (plan creation allocates hipFFT's own work area); a call's stream
+ * binding and transforms run under one lock, so calls from several threads on different
+ * streams each run on their own stream. */
 int ppm_fdbm_fft_combine(int mode, int nx, int ny, int nz, float step, const float* a, const float* b,
                          float scale, float* out, void* workspace, size_t workspace_bytes, void* stream);
+
+/* Destroys every cached hipFFT plan of ppm_fdbm_fft_combine (on every device; the next call
+ * creates them again).  Call it with no FFT work of the library in flight, e.g. at shutdown. */
+int ppm_fdbm_release_plans(void);
 
 /* field[i] = A * max(field[i], clip)^beta over n values (fdbm_scale_pauli_pow_f32,
  * kernels/AFM.cl:1258-1267; scale_pauli_field spammm/SPM/AFM.py:4106-4128). */
@@ -326,6 +336,18 @@ int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec,
                              const float* points, int n_scan, int nz,
                              const PpmTip* tip, const PpmRelaxCtl* ctl,
                              float* FEs, float* PPs, void* stream);
+
+/* ppm_relax_strokes_tilted with the df operator of ppm_z_stencil applied to each point's Fz
+ * column in the same launch (the Fz -> df conversion of compute_df / compute_df_amp / get_df,
+ * spammm/SPM/AFM.py:3048-3089, :2591-2599, on the device without a second pass over FEs):
+ *   df[k*nz_out + j] = sum_{t<taps} W[j*taps + t] * FEs[k, clamp(start[j] + t, 0, nz-1)].z,
+ * accumulated as ppm_z_stencil does (its results bit for bit).  W (nz_out*taps) float and
+ * start (nz_out) int: device arrays; df (n_scan, nz_out). */
+int ppm_relax_strokes_tilted_df(const float* grid, const PpmGridSpec* spec,
+                                const float* points, int n_scan, int nz,
+                                const PpmTip* tip, const PpmRelaxCtl* ctl,
+                                float* FEs, float* PPs, const float* W, const int* start, int taps,
+                                int nz_out, float* df, void* stream);
 
 /* ppm_relax_strokes_tilted on a z window of the grid: `grid` holds only planes [klo, klo + nzw)
  * (as ppm_grid_project_window stores them); sampling coordinates and REPEAT addressing are those

@@ -56,7 +56,7 @@ EXPORTS = (
     "ppm_ewald_num_g", "ppm_ewald_system_bytes", "ppm_ewald_prepare", "ppm_ewald_eval", "ppm_ewald_brute",
     "ppm_ewald_cluster", "ppm_ewald_grid_workspace_size", "ppm_ewald_tip_grid", "ppm_ewald_kept_count_offset",
     "ppm_relax_state_diag", "ppm_relax_strokes_2d", "ppm_grid_project_window", "ppm_relax_strokes_tilted_window",
-    "ppm_batch_forward_noalloc",
+    "ppm_batch_forward_noalloc", "ppm_fdbm_release_plans", "ppm_relax_strokes_tilted_df",
 )
 
 _lib = None
@@ -163,6 +163,11 @@ def load():
     L.ppm_batch_workspace_size.argtypes = [i]
     L.ppm_batch_forward.restype = i
     L.ppm_batch_forward.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_size_t, i, i, vp]
+    L.ppm_relax_strokes_tilted_df.restype = i
+    L.ppm_relax_strokes_tilted_df.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), P(PpmRelaxCtl), vp, vp, vp,
+                                              vp, i, i, vp, vp]
+    L.ppm_fdbm_release_plans.restype = i
+    L.ppm_fdbm_release_plans.argtypes = []
     L.ppm_batch_forward_noalloc.restype = i
     L.ppm_batch_forward_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, i, vp]
     L.ppm_relax_direct_morse.restype = i

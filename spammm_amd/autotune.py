@@ -1,8 +1,10 @@
 """Autotuner of the batched forward's launch shape.
 
 Candidates: grid voxels per thread along z (4, 8), the pixel tile of a wavefront (16x4, 8x8,
-4x16 pixels), persistent workgroups per compute unit (2, 3, 4) and the schedule (one fused
-launch, or a grid launch followed by a relaxation launch).  Each candidate runs the real batch
+4x16 pixels), persistent workgroups per compute unit (2, 3, 4), the schedule (one fused
+launch, or a grid launch followed by a relaxation launch) and, for the fused launch, the queue
+lag (0: each molecule's relaxation blocks right after its tiles; k: after the tiles of molecule
+m + k).  Each candidate runs the real batch
 (or a prefix of it) and is timed with HIP events on the launch stream; every candidate's df
 must equal the first candidate's bit for bit (they compute the same arithmetic, only the
 work decomposition differs).  The winner is cached in a JSON file keyed by device name and a
@@ -23,6 +25,7 @@ CANDIDATES = {
     "tile_w": (4, 8, 16),
     "workers_per_cu": (2, 3, 4),
     "schedule": ("fused", "two_phase"),
+    "lag": (0, 2, 8),
 }
 
 
@@ -90,6 +93,8 @@ def tune(batch: BatchForward, candidates=None, reps: int = 2, use_cache: bool = 
     best, best_ms = None, float("inf")
     for vals in itertools.product(*(cand[k] for k in keys)):
         cfg = LaunchConfig(**dict(zip(keys, vals)))
+        if cfg.schedule == "two_phase" and cfg.lag:
+            continue        # the lag orders fused work only (two phases separate the kinds anyway)
         ms = time_config(batch, cfg, reps)
         err, _ = batch.status()
         if err:

@@ -32,7 +32,7 @@ _F4 = ctypes.c_float * 4
 class PpmMolDesc(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in (
         "atom_off", "na", "nx", "ny", "nz", "klo", "nzw", "npbc_a", "npbc_b", "pix_off", "scan_nx", "scan_ny",
-        "item_off", "n_grid_items", "n_relax_items", "reserved")] + [("grid_off", ctypes.c_longlong)] + \
+        "item_off", "n_grid_items", "n_relax_items", "relax_item_off")] + [("grid_off", ctypes.c_longlong)] + \
         [(n, _F4) for n in ("p0", "dA", "dB", "dC", "dinvA", "dinvB", "dinvC", "lvA", "lvB", "scan_p0", "scan_da",
                             "scan_db")]
 
@@ -53,9 +53,11 @@ class LaunchConfig:
     tile_w: int = 8             # pixels along iy of a wavefront's 64-pixel tile
     workers_per_cu: int = 3     # persistent workgroups per compute unit
     schedule: str = "fused"     # "fused" (one launch) or "two_phase" (grid launch, then relax launch)
+    lag: int = 0                # queue order: the tiles of molecule m + lag come before the blocks of m
 
     def key(self):
-        return f"vz{self.vz}_tw{self.tile_w}_w{self.workers_per_cu}_{self.schedule}"
+        k = f"vz{self.vz}_tw{self.tile_w}_w{self.workers_per_cu}_{self.schedule}"
+        return k + (f"_lag{self.lag}" if self.lag else "")
 
 
 def z_window(w, sp, tip, nz, margin):
@@ -189,17 +191,35 @@ class BatchForward:
         return self.n_pix * self.nz
 
     def _set_items(self, cfg: LaunchConfig):
-        key = (cfg.vz, cfg.tile_w)
+        """Work items of every molecule and the queue order of their segments: the grid tiles of
+        molecule m, then (lag = k) the relaxation blocks of molecule m - k, so that a block of m is
+        dequeued k molecules' tiles after the last tile of m (it rarely waits)."""
+        key = (cfg.vz, cfg.tile_w, cfg.lag)
         if self._items_for == key:
             return
-        item = 0
+        if cfg.lag < 0:
+            raise ValueError("lag must be >= 0")
+        th = 64 // cfg.tile_w
         for d in self.descs_host:
-            d.item_off = item
             d.n_grid_items = ((d.nx * d.ny + 255) // 256) * ((d.nzw + cfg.vz - 1) // cfg.vz)
-            th = 64 // cfg.tile_w
             tiles = ((d.scan_nx + th - 1) // th) * ((d.scan_ny + cfg.tile_w - 1) // cfg.tile_w)
             d.n_relax_items = (tiles + 3) // 4
-            item += d.n_grid_items + d.n_relax_items
+        n = len(self.descs_host)
+        order = []
+        for m in range(n):
+            order.append(("grid", m))
+            if m - cfg.lag >= 0:
+                order.append(("relax", m - cfg.lag))
+        order += [("relax", m) for m in range(max(0, n - cfg.lag), n)]
+        item = 0
+        for kind, m in order:
+            d = self.descs_host[m]
+            if kind == "grid":
+                d.item_off = item
+                item += d.n_grid_items
+            else:
+                d.relax_item_off = item
+                item += d.n_relax_items
         self.nitems = item
         raw = np.frombuffer(ctypes.string_at(ctypes.addressof(self.descs_host), ctypes.sizeof(self.descs_host)),
                             dtype=np.uint8)

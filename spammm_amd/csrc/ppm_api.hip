@@ -199,6 +199,11 @@ size_t ppm_fdbm_workspace_size(int nx, int ny, int nz) {
     return 2 * fdbm_spectrum_bytes(nx, ny, nz);
 }
 
+int ppm_fdbm_release_plans(void) {
+    g_err[0] = 0;
+    return release_fft_plans();
+}
+
 int ppm_fdbm_fft_combine(int mode, int nx, int ny, int nz, float step, const float* a, const float* b, float scale,
                          float* out, void* workspace, size_t workspace_bytes, void* stream) {
     const char* fn = "ppm_fdbm_fft_combine";
@@ -364,6 +369,23 @@ int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec, const f
     if (int r = check_ctl(fn, ctl, n_scan)) return r;
     if (!grid || !tip) { set_error("%s: NULL grid/tip", fn); return PPM_ERR_INVALID_ARG; }
     return launch_relax_tilted(grid, *spec, points, n_scan, nz, *tip, *ctl, FEs, PPs, (hipStream_t)stream);
+}
+
+int ppm_relax_strokes_tilted_df(const float* grid, const PpmGridSpec* spec, const float* points, int n_scan, int nz,
+                                const PpmTip* tip, const PpmRelaxCtl* ctl, float* FEs, float* PPs, const float* W,
+                                const int* start, int taps, int nz_out, float* df, void* stream) {
+    const char* fn = "ppm_relax_strokes_tilted_df";
+    g_err[0] = 0;
+    if (int r = check_spec(fn, spec)) return r;
+    if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (int r = check_ctl(fn, ctl, n_scan)) return r;
+    if (!grid || !tip) { set_error("%s: NULL grid/tip", fn); return PPM_ERR_INVALID_ARG; }
+    if (!W || !start || !df || taps < 1 || taps > nz || nz_out < 1) {
+        set_error("%s: invalid df operator (taps=%d, nz_out=%d)", fn, taps, nz_out);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return launch_relax_tilted(grid, *spec, points, n_scan, nz, *tip, *ctl, FEs, PPs, (hipStream_t)stream, 0, 0,
+                               nullptr, W, start, taps, nz_out, df);
 }
 
 int ppm_relax_strokes_tilted_window(const float* grid, const PpmGridSpec* spec, int klo, int nzw,

@@ -1,9 +1,11 @@
 // ppm_batch.hip — batched forward: many molecules (uneven atom counts, cell sizes and grid
 // shapes) through grid projection -> relaxed scan -> df in ONE persistent launch.
 //
-// Work items, in queue order, are for each molecule m: its grid tiles (256 columns x vz planes
-// of the stored z window), then its relaxation blocks (4 wavefronts x 64 pixels; each lane
-// relaxes one pixel through all z-steps and then applies the df operator to its own Fz column).
+// Work items: for each molecule m its grid tiles (256 columns x vz planes of the stored z window)
+// and its relaxation blocks (4 wavefronts x 64 pixels; each lane relaxes one pixel through all
+// z-steps and then applies the df operator to its own Fz column), as two queue segments; the host
+// orders the segments (by default each molecule's blocks right after its tiles; with a lag k the
+// tiles of m + k come before the blocks of m, so a block rarely waits for its tiles).
 // Persistent workgroups take items from an atomic counter.  A relaxation block of m waits (with
 // a bounded spin) until every grid tile of m has been published: the tile's stores are released
 // at agent scope before the per-molecule counter is incremented, and the waiting workgroup
@@ -83,6 +85,11 @@ __device__ GridArgs mol_grid_args(const PpmMolDesc& d, const PpmBatchParams& P) 
     return a;
 }
 
+// First queue item of a molecule's relaxation blocks (right after its tiles when relax_item_off = 0).
+__device__ __forceinline__ int relax_start(const PpmMolDesc& d) {
+    return d.relax_item_off > 0 ? d.relax_item_off : d.item_off + d.n_grid_items;
+}
+
 __device__ rc::TiltArgs tilt_args(const PpmBatchParams& P) {
     rc::TiltArgs t{};
     t.nz = P.nz; t.max_iter = P.ctl.max_iter; t.check_before = P.ctl.check_before; t.f2conv = P.ctl.f2conv;
@@ -128,19 +135,9 @@ __device__ void do_relax_item(const PpmMolDesc& d, const PpmBatchParams& P, int 
     const size_t pix = (size_t)d.pix_off + (size_t)ix * d.scan_ny + iy;
     const size_t o = pix * P.nz;
     const rc::TiltArgs t = tilt_args(P);
-    rc::relax_tilted_lane(g, start, t, FEs + o, PPs ? PPs + o : nullptr);
-    if (df) {
-        // the lane's own Fz column, just written by this lane
-        for (int j = 0; j < P.df_nz_out; ++j) {
-            const int s0 = dfStart[j];
-            float acc = 0.f;
-            for (int k = 0; k < P.df_taps; ++k) {
-                const int iz = min(max(s0 + k, 0), P.nz - 1);
-                acc = fmaf(dfW[j * P.df_taps + k], FEs[o + iz].z, acc);
-            }
-            df[pix * P.df_nz_out + j] = acc;
-        }
-    }
+    const rc::DfOut dfo{dfW, dfStart, P.df_taps, P.df_nz_out, false};
+    rc::relax_tilted_lane(g, start, t, FEs + o, PPs ? PPs + o : nullptr, df ? &dfo : nullptr,
+                          df ? df + pix * P.df_nz_out : nullptr);
 }
 
 // OFFQ: the tip has charge sites at a nonzero z offset (compiled out when it has none).
@@ -152,7 +149,7 @@ __global__ __launch_bounds__(256) void k_batch(const PpmMolDesc* __restrict__ mo
                                                const float* __restrict__ dfW, const int* __restrict__ dfStart,
                                                Queue* q) {
     __shared__ Site sites[gc::kTile + 1];
-    __shared__ int s_item, s_mol;
+    __shared__ int s_item, s_mol, s_local;
     int* done = q_done(q);
     int* flags = q_flag(q, P.nmol);
     // Molecule whose grid tile this workgroup finished in the previous iteration (-1: none).
@@ -165,24 +162,37 @@ __global__ __launch_bounds__(256) void k_batch(const PpmMolDesc* __restrict__ mo
         if (threadIdx.x == 0) {
             if (signal_m >= 0) __hip_atomic_fetch_add(&done[signal_m], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             int item = atomicAdd(&q->head, 1);
-            int m = -1;
+            int m = -1, local = 0;
             if (item < P.nitems) {
                 int lo = 0, hi = P.nmol - 1;            // last molecule with item_off <= item
                 while (lo < hi) {
                     const int mid = (lo + hi + 1) >> 1;
                     if (mols[mid].item_off <= item) lo = mid; else hi = mid - 1;
                 }
-                m = lo;
+                if (item < mols[lo].item_off + mols[lo].n_grid_items) {
+                    m = lo;
+                    local = item - mols[lo].item_off;
+                } else {                                // a relaxation block: last molecule whose
+                    lo = 0;                             // relaxation segment starts at or before item
+                    hi = P.nmol - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (relax_start(mols[mid]) <= item) lo = mid; else hi = mid - 1;
+                    }
+                    m = lo;
+                    local = mols[lo].n_grid_items + (item - relax_start(mols[lo]));
+                }
             }
             s_item = item;
             s_mol = m;
+            s_local = local;
         }
         __syncthreads();
         const int item = __builtin_amdgcn_readfirstlane(s_item), m = __builtin_amdgcn_readfirstlane(s_mol);
+        const int local = __builtin_amdgcn_readfirstlane(s_local);
         __syncthreads();
         if (item >= P.nitems) break;
         const PpmMolDesc& d = mols[m];
-        const int local = item - d.item_off;
         signal_m = -1;
         if (local < d.n_grid_items) {
             if (P.mode != PPM_BATCH_RELAX_ONLY) {

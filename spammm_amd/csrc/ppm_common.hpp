@@ -41,7 +41,8 @@ int launch_grid_project(int model, const float* atoms, const float* cpar, int na
 // that samples outside them sets *flag.
 int launch_relax_tilted(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
                         const PpmTip& tip, const PpmRelaxCtl& ctl, float* FEs, float* PPs, hipStream_t stream,
-                        int klo = 0, int nzw = 0, int* flag = nullptr);
+                        int klo = 0, int nzw = 0, int* flag = nullptr, const float* dfW = nullptr,
+                        const int* dfStart = nullptr, int taps = 0, int nout = 0, float* df = nullptr);
 int launch_relax_strokes(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
                          const float dTip[4], const float stiffness[4], const float dpos0[4], const float relax[4],
                          const PpmRelaxCtl& ctl, float* FEs, float* disps, float* PPs, hipStream_t stream);
@@ -62,6 +63,7 @@ int launch_relax_tilted_f64(const double* grid, const PpmGridSpec& g, const doub
                             const PpmTip& tip, const PpmRelaxCtl& ctl, double* FEs, double* PPs,
                             hipStream_t stream);
 size_t fdbm_spectrum_bytes(int nx, int ny, int nz);
+int release_fft_plans();
 int launch_fft_combine(int mode, int nx, int ny, int nz, float step, const float* a, const float* b, float scale,
                        float* out, void* workspace, hipStream_t stream);
 int launch_pauli_scale(float* field, long long n, float A, float beta, float clip, hipStream_t stream);

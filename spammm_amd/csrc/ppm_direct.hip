@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_direct_relax(const float4* __restrict__
     const float4 P = pts[ip];
     auto smp = [&](float x, float y, float z) { return direct_fe_group<G>(A, M, d, sub, x, y, z); };
     const size_t o = (size_t)ip * p.nz;
-    rc::relax_tilted_lane_src(smp, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, sub == 0);
+    rc::relax_tilted_lane_src<G>(smp, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, sub);
 }
 
 // Lanes per scan point: enough groups for about 4 waves per SIMD (1024 SIMDs), at most one

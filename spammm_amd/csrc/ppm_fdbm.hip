@@ -137,7 +137,9 @@ __global__ __launch_bounds__(256) void k_compose_gradient(const float* __restric
     grid[((size_t)iz * ny + iy) * nx + ix] = make_float4(fx, fy, fz, E(ix, iy, iz));
 }
 
-// hipFFT plans per (device, shape), created on first use and kept for the process.
+// hipFFT plans per (device, shape), created on first use and kept until ppm_fdbm_release_plans.
+// A plan's stream is state of the plan, so one caller's hipfftSetStream and its hipfftExec* calls
+// run under g_plan_mu as a unit (calls on other streams wait for the enqueue, not for the GPU).
 struct Plans {
     int dev, nx, ny, nz;
     hipfftHandle r2c, c2r;
@@ -145,10 +147,10 @@ struct Plans {
 std::mutex g_plan_mu;
 std::vector<Plans> g_plans;
 
+// The caller holds g_plan_mu.
 int get_plans(int nx, int ny, int nz, hipStream_t stream, hipfftHandle* r2c, hipfftHandle* c2r) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) { set_error("fdbm: hipGetDevice failed"); return PPM_ERR_HIP; }
-    std::lock_guard<std::mutex> lock(g_plan_mu);
     for (const Plans& p : g_plans)
         if (p.dev == dev && p.nx == nx && p.ny == ny && p.nz == nz) {
             *r2c = p.r2c;
@@ -182,8 +184,24 @@ inline unsigned blocks_for(long long n) { return (unsigned)((n + 255) / 256); }
 
 size_t fdbm_spectrum_bytes(int nx, int ny, int nz) { return (size_t)nx * ny * (nz / 2 + 1) * sizeof(float2); }
 
+int release_fft_plans() {
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    int cur = 0;
+    const bool have_cur = hipGetDevice(&cur) == hipSuccess;
+    int rc = PPM_OK;
+    for (const Plans& p : g_plans) {
+        if (hipSetDevice(p.dev) != hipSuccess) { rc = PPM_ERR_HIP; continue; }
+        if (hipfftDestroy(p.r2c) != HIPFFT_SUCCESS || hipfftDestroy(p.c2r) != HIPFFT_SUCCESS) rc = PPM_ERR_HIP;
+    }
+    g_plans.clear();
+    if (have_cur) (void)hipSetDevice(cur);
+    if (rc) set_error("fdbm: destroying a cached hipFFT plan failed");
+    return rc;
+}
+
 int launch_fft_combine(int mode, int nx, int ny, int nz, float step, const float* a, const float* b, float scale,
                        float* out, void* workspace, hipStream_t stream) {
+    std::lock_guard<std::mutex> lock(g_plan_mu);
     hipfftHandle r2c, c2r;
     if (int r = get_plans(nx, ny, nz, stream, &r2c, &c2r)) return r;
     const size_t half = fdbm_spectrum_bytes(nx, ny, nz);

@@ -36,21 +36,27 @@ constexpr double kResidualGate = 5e-4, kCondGate = 1e6, kStableGate = 1e-5;
 
 // relaxStrokesTilted with PP telemetry; G lanes per scan point (G = 1, 2, 4): with G > 1 each
 // lane interpolates 4/G channels and the group shares the FIRE logic, results identical to G = 1.
+// df (optional, with dfo): the df operator applied to each point's Fz column, (n_scan, dfo.nout).
 template <int G>
 __global__ __launch_bounds__(64) void k_relax_tilted(GridDev g, const float4* __restrict__ pts, TiltArgs p,
-                                                     float4* __restrict__ FEs, float4* __restrict__ PPs) {
+                                                     float4* __restrict__ FEs, float4* __restrict__ PPs,
+                                                     DfOut dfo, float* __restrict__ df) {
     int sub;
     const int ip = group_point<G>(p.n_scan, p.scan_ny, sub);
     if (ip < 0) return;                       // whole groups leave together
     const float4 P = pts[ip];
     const size_t o = (size_t)ip * p.nz;
+    const DfOut* dp = df ? &dfo : nullptr;
+    float* row = df ? df + (size_t)ip * dfo.nout : nullptr;
+    extern __shared__ float s_fz[];              // (nz, 64) when the launch gives it LDS, else unused
+    float* col = (df && dfo.lds) ? s_fz + (threadIdx.x & 63) : nullptr;
     if constexpr (G == 1) {
-        relax_tilted_lane(g, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr);
+        relax_tilted_lane(g, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, dp, row, col);
     } else {
         GroupCell<G> cell;
         cell.i = -1; cell.j = -1; cell.k = -1;
         auto smp = [&](float x, float y, float z) { return sample_group<G>(g, cell, sub, x, y, z); };
-        relax_tilted_lane_src(smp, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, sub == 0);
+        relax_tilted_lane_src<G>(smp, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, sub, dp, row, col);
     }
 }
 
@@ -563,7 +569,8 @@ int n_waves(int n_scan, int scan_ny) {
 
 int launch_relax_tilted(const float* grid, const PpmGridSpec& s, const float* pts, int n_scan, int nz,
                         const PpmTip& tip, const PpmRelaxCtl& ctl, float* FEs, float* PPs, hipStream_t stream,
-                        int klo, int nzw, int* flag) {
+                        int klo, int nzw, int* flag, const float* dfW, const int* dfStart, int taps, int nout,
+                        float* df) {
     TiltArgs p{};
     p.n_scan = n_scan; p.nz = nz; p.max_iter = ctl.max_iter; p.check_before = ctl.check_before;
     p.scan_ny = ctl.scan_ny; p.f2conv = ctl.f2conv; p.damped = ctl.damped != 0;
@@ -578,9 +585,12 @@ int launch_relax_tilted(const float* grid, const PpmGridSpec& s, const float* pt
     const float4* P = reinterpret_cast<const float4*>(pts);
     float4* F = reinterpret_cast<float4*>(FEs);
     float4* Q = reinterpret_cast<float4*>(PPs);
-    if (G == 4) hipLaunchKernelGGL(k_relax_tilted<4>, blocks, dim3(64), 0, stream, gd, P, p, F, Q);
-    else if (G == 2) hipLaunchKernelGGL(k_relax_tilted<2>, blocks, dim3(64), 0, stream, gd, P, p, F, Q);
-    else hipLaunchKernelGGL(k_relax_tilted<1>, blocks, dim3(64), 0, stream, gd, P, p, F, Q);
+    // the Fz columns of a wave in LDS (nz x 64 floats) up to 64 KiB per workgroup, else read back
+    const size_t lds = (df && (size_t)nz * 64 * sizeof(float) <= 65536) ? (size_t)nz * 64 * sizeof(float) : 0;
+    DfOut dfo{dfW, dfStart, taps, nout, lds > 0};
+    if (G == 4) hipLaunchKernelGGL(k_relax_tilted<4>, blocks, dim3(64), lds, stream, gd, P, p, F, Q, dfo, df);
+    else if (G == 2) hipLaunchKernelGGL(k_relax_tilted<2>, blocks, dim3(64), lds, stream, gd, P, p, F, Q, dfo, df);
+    else hipLaunchKernelGGL(k_relax_tilted<1>, blocks, dim3(64), lds, stream, gd, P, p, F, Q, dfo, df);
     return check_launch("relax_strokes_tilted");
 }
 

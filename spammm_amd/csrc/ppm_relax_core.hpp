@@ -228,17 +228,50 @@ struct TiltArgs {
     float dt0, damp0, surf;
 };
 
+// Output staging of one lane: the newest (up to 4) states of its column are held in registers
+// and stored as one run of back-to-back float4 stores when the run reaches a 64 B boundary of the
+// output array or the column ends.  One float4 store per z-step at the column stride (the
+// reference's (pixel, z) layout) left 64 B segments of L2 lines partly written for many
+// microseconds; evicted in between, they were written back to memory several times over.
+struct Stage4 {
+    float4 b0, b1, b2, b3;   // oldest .. newest
+    __device__ __forceinline__ void push(float4 v) { b0 = b1; b1 = b2; b2 = b3; b3 = v; }
+    // stores the newest n (1..4) states ending at `last`
+    __device__ __forceinline__ void flush(float4* last, int n) const {
+        if (n >= 4) last[-3] = b0;
+        if (n >= 3) last[-2] = b1;
+        if (n >= 2) last[-1] = b2;
+        last[0] = b3;
+    }
+};
+
+// A linear operator along z applied to the lane's own Fz column after its relaxation (the df
+// image): out[j] = sum_t W[j*taps + t] * Fz[clamp(start[j] + t)], accumulated with fmaf in t order
+// (the accumulation of k_axis_operator for a single channel, so the results are those of the
+// separate operator launch bit for bit).
+struct DfOut {
+    const float* W;
+    const int* start;
+    int taps, nout;
+    bool lds;           // the launch holds each lane's Fz column in LDS
+};
+
 // relaxStrokesTilted (kernels/AFM.cl:478-562) for one scan point starting at tip anchor
 // `start`; writes FEs[iz] (tip-rotated sample FE) and PPs[iz] (position, +/-iterations).
 // `sample(x, y, z)` is the sample FE at a world point (trilinear grid sample, or the direct
 // pair sum of relaxStrokesTiltedMorseDirect).  With the convergence test before the update
 // the final resample is at the position just sampled, so its value is reused.
-// `writer` = false: the lane takes part in the samples (a lane group sharing one pixel) but
-// leaves the stores to the group's writer lane.
-template <class SAMPLE>
+// G lanes share the point (same state in every lane): G = 1 stores FEs and PPs; otherwise lane
+// sub 0 stores FEs and lane sub 1 PPs (the others only take part in the samples).  df (optional):
+// the operator applied to the column after the relaxation (dfo, dfrow).  fz_lds (optional): this
+// lane's Fz column in LDS (element iz at fz_lds[iz * 64]), kept by every lane of the group so that
+// the G lanes split the df outputs; without it the FE-storing lane reads its stored column back.
+template <int G, class SAMPLE>
 __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, const TiltArgs& p,
                                                       float4* __restrict__ FEs, float4* __restrict__ PPs,
-                                                      bool writer = true) {
+                                                      int sub = 0, const DfOut* dfo = nullptr,
+                                                      float* __restrict__ dfrow = nullptr,
+                                                      float* __restrict__ fz_lds = nullptr) {
     const f3 A = ld3(p.A), B = ld3(p.B), C = ld3(p.C);
     const f3 dTip = mk3(C.x * p.dtip, C.y * p.dtip, C.z * p.dtip);
     const f3 d0r = rotMatT(ld3(p.d0), A, B, C);
@@ -250,6 +283,8 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
     f3 v = mk3(0.f, 0.f, 0.f);
     int it = 0, iz = 0;
     int o = 0;
+    Stage4 s1, s2;       // G = 1: FEs, PPs; G > 1: the lane's own output
+    int pend = 0;
     while (true) {
         const float4 fe = sample(pos.x, pos.y, pos.z);
         f3 f = mk3(fe.x, fe.y, fe.z);
@@ -272,10 +307,28 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
         if (done) {
             const float4 fs = (p.check_before && conv) ? fe : sample(pos.x, pos.y, pos.z);
             const f3 fr = rotMat(mk3(fs.x, fs.y, fs.z), A, B, C);
-            if (writer) FEs[o] = make_float4(fr.x, fr.y, fr.z, fs.w);
-            if (PPs && writer) {
-                const bool fin = finite4(fs) && isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z);
-                PPs[o] = make_float4(pos.x, pos.y, pos.z, (conv && fin) ? (float)it : -(float)it);
+            const float4 feo = make_float4(fr.x, fr.y, fr.z, fs.w);
+            const bool fin = finite4(fs) && isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z);
+            const float4 ppo = make_float4(pos.x, pos.y, pos.z, (conv && fin) ? (float)it : -(float)it);
+            if (fz_lds) fz_lds[iz * 64] = feo.z;
+            if (G == 1) {
+                s1.push(feo);
+                s2.push(ppo);
+            } else {
+                s1.push(sub == 1 ? ppo : feo);
+            }
+            ++pend;
+            const bool last = iz + 1 >= p.nz;
+            if (last || (reinterpret_cast<uintptr_t>(FEs + o + 1) & 63) == 0) {
+                if (G == 1) {
+                    s1.flush(FEs + o, pend);
+                    if (PPs) s2.flush(PPs + o, pend);
+                } else if (sub == 0) {
+                    s1.flush(FEs + o, pend);
+                } else if (sub == 1 && PPs) {
+                    s1.flush(PPs + o, pend);
+                }
+                pend = 0;
             }
             ++o;
             tip = mk3(tip.x + dTip.x, tip.y + dTip.y, tip.z + dTip.z);
@@ -285,14 +338,37 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
             if (++iz >= p.nz) break;
         }
     }
+    if (dfo && fz_lds) {         // the lane's own copy of the column; the group splits the outputs
+        for (int j = sub; j < dfo->nout; j += G) {
+            const int s0 = dfo->start[j];
+            float acc = 0.f;
+            for (int t = 0; t < dfo->taps; ++t) {
+                const int m = min(max(s0 + t, 0), p.nz - 1);
+                acc = fmaf(dfo->W[j * dfo->taps + t], fz_lds[m * 64], acc);
+            }
+            dfrow[j] = acc;
+        }
+    } else if (dfo && sub == 0) {  // the column just stored by this lane
+        for (int j = 0; j < dfo->nout; ++j) {
+            const int s0 = dfo->start[j];
+            float acc = 0.f;
+            for (int t = 0; t < dfo->taps; ++t) {
+                const int m = min(max(s0 + t, 0), p.nz - 1);
+                acc = fmaf(dfo->W[j * dfo->taps + t], FEs[m].z, acc);
+            }
+            dfrow[j] = acc;
+        }
+    }
 }
 
 __device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, const TiltArgs& p,
-                                                  float4* __restrict__ FEs, float4* __restrict__ PPs) {
+                                                  float4* __restrict__ FEs, float4* __restrict__ PPs,
+                                                  const DfOut* dfo = nullptr, float* __restrict__ dfrow = nullptr,
+                                                  float* __restrict__ fz_lds = nullptr) {
     Cell cell;
     cell.i = -1; cell.j = -1; cell.k = -1;
     auto smp = [&](float x, float y, float z) { return sample(g, cell, x, y, z); };
-    relax_tilted_lane_src(smp, start, p, FEs, PPs);
+    relax_tilted_lane_src<1>(smp, start, p, FEs, PPs, 0, dfo, dfrow, fz_lds);
 }
 
 }  // namespace rc

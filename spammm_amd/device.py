@@ -212,11 +212,14 @@ def grid_project_f64(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 
 
 
 def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, f2conv=1e-8, check_before=False,
-                 scan_ny=0, FEs=None, PPs=None, want_pps=True, damped=False, window=None, flag=None):
+                 scan_ny=0, FEs=None, PPs=None, want_pps=True, damped=False, window=None, flag=None, df_op=None,
+                 df=None):
     """relaxStrokesTilted on the device: returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None).
     ``window``: (klo, nzw) -> ``grid`` holds only planes [klo, klo + nzw) of the spec's grid
     (ppm_relax_strokes_tilted_window); ``flag`` (int32 device tensor, zeroed by the caller) turns
-    nonzero if a probe sampled outside the window."""
+    nonzero if a probe sampled outside the window.  ``df_op`` (a df.AxisOperator over nz slices) with
+    ``df`` ((n_scan, n_out) float32 device tensor): the operator applied to each Fz column in the
+    same launch (ppm_relax_strokes_tilted_df)."""
     import torch
     _f32dev(grid, "grid", 4)
     _f32dev(points, "points", 4)
@@ -237,7 +240,18 @@ def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, 
                 raise ValueError(f"{nm} has the wrong size")
     L = _lib.load()
     s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, scan_ny, damped=damped)
-    if window is not None:
+    if df_op is not None:
+        if window is not None:
+            raise ValueError("the fused df takes no z window")
+        _f32dev(df, "df", None)
+        if df.numel() != ns * int(df_op.n_out):
+            raise ValueError("df must be (n_scan, n_out)")
+        rc = L.ppm_relax_strokes_tilted_df(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz),
+                                           ctypes.byref(tp), ctypes.byref(ctl), FEs.data_ptr(),
+                                           PPs.data_ptr() if PPs is not None else None, df_op.W.data_ptr(),
+                                           df_op.start.data_ptr(), int(df_op.taps), int(df_op.n_out), df.data_ptr(),
+                                           _stream(grid))
+    elif window is not None:
         rc = L.ppm_relax_strokes_tilted_window(grid.data_ptr(), ctypes.byref(s), int(window[0]), int(window[1]),
                                                points.data_ptr(), ns, int(nz), ctypes.byref(tp), ctypes.byref(ctl),
                                                FEs.data_ptr(), PPs.data_ptr() if PPs is not None else None,

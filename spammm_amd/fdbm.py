@@ -87,6 +87,15 @@ def _field(t, name):
     return t
 
 
+_release_at_exit = False
+
+
+def release_plans():
+    """Destroys the library's cached hipFFT plans (ppm_fdbm_release_plans); the next FFT recreates
+    them.  Registered to run at interpreter exit after the first FFT."""
+    _lib.check(_lib.load().ppm_fdbm_release_plans())
+
+
 def fft_combine(mode, a, b, step, scale, out=None):
     """out = scale * IFFT(FFT(a) (op) FFT(b) [/ k^2]) for the PPM_FFT_* modes (normalised inverse)."""
     import torch
@@ -105,6 +114,11 @@ def fft_combine(mode, a, b, step, scale, out=None):
     _lib.check(L.ppm_fdbm_fft_combine(int(mode), nx, ny, nz, float(step), a.data_ptr(),
                                       b.data_ptr() if b is not None else None, float(scale), out.data_ptr(),
                                       ws.data_ptr(), nb, D._stream(a)))
+    global _release_at_exit
+    if not _release_at_exit:
+        import atexit
+        atexit.register(release_plans)
+        _release_at_exit = True
     return out
 
 

@@ -32,7 +32,8 @@ def plain_reference(bf):
 
 @pytest.mark.parametrize("cfg", [
     B.LaunchConfig(8, 8, 3, "fused"), B.LaunchConfig(4, 16, 2, "fused"), B.LaunchConfig(8, 4, 1, "two_phase"),
-    B.LaunchConfig(4, 8, 4, "two_phase")])
+    B.LaunchConfig(4, 8, 4, "two_phase"), B.LaunchConfig(4, 8, 2, "fused", lag=1),
+    B.LaunchConfig(8, 4, 3, "fused", lag=3), B.LaunchConfig(4, 8, 3, "two_phase", lag=2)])
 def test_batch_equals_plain_loop(cfg):
     bf = make_batch()
     print(f"[batch] {cfg.key()} windows {bf.windows}", flush=True)

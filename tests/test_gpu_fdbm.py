@@ -47,6 +47,41 @@ def test_fft_modes_match_numpy(shape):
     assert torch.equal(ta, a0), "the forward transform modified its input"
 
 
+def test_fft_plans_shared_by_threads_on_their_own_streams_and_released():
+    """Four host threads, each on its own (default-priority) stream, run FFT products of the same
+    shape at once: each result equals the single-threaded one bit for bit (a thread's stream binding
+    and transforms run as a unit under the plan lock); after ppm_fdbm_release_plans the next call
+    recreates the plans with the same results."""
+    import threading
+    shape = (24, 20, 30)
+    dev = D.require_gpu()
+    rng = np.random.default_rng(3)
+    ins = [(torch.as_tensor(rng.normal(size=shape).astype(np.float32), device=dev),
+            torch.as_tensor(rng.normal(size=shape).astype(np.float32), device=dev)) for _ in range(4)]
+    ref = [fdbm.fft_combine(_lib.PPM_FFT_POISSON_CONVOLVE, a, b, 0.25, 1.3) for a, b in ins]
+    torch.cuda.synchronize()
+    outs = [None] * 4
+
+    def work(i):
+        st = torch.cuda.Stream(device=dev)
+        with torch.cuda.stream(st):
+            for _ in range(5):
+                outs[i] = fdbm.fft_combine(_lib.PPM_FFT_POISSON_CONVOLVE, ins[i][0], ins[i][1], 0.25, 1.3)
+        st.synchronize()
+
+    th = [threading.Thread(target=work, args=(i,)) for i in range(4)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    for r, o in zip(ref, outs):
+        assert torch.equal(r, o)
+    fdbm.release_plans()
+    again = fdbm.fft_combine(_lib.PPM_FFT_POISSON_CONVOLVE, ins[0][0], ins[0][1], 0.25, 1.3)
+    torch.cuda.synchronize()
+    assert torch.equal(again, ref[0])
+
+
 def test_fields_match_oracle():
     c = case()
     st, dV = c["step"], c["step"] ** 3

@@ -280,6 +280,37 @@ def test_full_naphthalene_scan_subsample_bitexact():
     np.testing.assert_array_equal(FEs[sel], FEo)
 
 
+@pytest.mark.parametrize("nxy", [(12, 10), (200, 200), (280, 260)])
+def test_relax_with_fused_df_equals_separate_operator(nxy):
+    """ppm_relax_strokes_tilted_df: FEs and PPs equal ppm_relax_strokes_tilted's and df equals the
+    separate operator launch on its FEs, bit for bit, for each lane-group width (these scans run 4,
+    2 and 1 lanes per point); FEs also bit-exact against the FP32 oracle on a subsample."""
+    from spammm_amd import df as DF
+    w = W.small()
+    G, sp = grid_gpu(w)
+    d = dev()
+    da = np.array([(w.nxy[0] - 1) * w.scan_da[0] / max(nxy[0] - 1, 1), 0, 0], np.float32)
+    db = np.array([0, (w.nxy[1] - 1) * w.scan_db[1] / max(nxy[1] - 1, 1), 0], np.float32)
+    pts_h = D.scan_points(w.scan_p0, da, db, *nxy)
+    pts = torch.as_tensor(pts_h, device=d)
+    tip = tip_of(w)
+    ns = nxy[0] * nxy[1]
+    op = DF.z_operator("giessibl", w.nz, w.dtip, d, w.amp, w.f0, w.k_cant)
+    FE1, PP1 = D.relax_tilted(G, sp, pts, w.nz, tip, 100, 1e-8, scan_ny=nxy[1])
+    df1 = op.apply(FE1, (w.nz * 4, 4, 0), ns, 1, offset=2)
+    df2 = torch.empty((ns, op.n_out), dtype=torch.float32, device=d)
+    FE2, PP2 = D.relax_tilted(G, sp, pts, w.nz, tip, 100, 1e-8, scan_ny=nxy[1], df_op=op, df=df2)
+    torch.cuda.synchronize()
+    assert torch.equal(FE1, FE2) and torch.equal(PP1, PP2)
+    assert torch.equal(df1.reshape(ns, op.n_out), df2)
+    sel = np.random.default_rng(1).choice(ns, min(ns, 64), replace=False)
+    FEo, PPo = O.relax_tilted(G.cpu().numpy(), sp.n, sp.dinvA, sp.dinvB, sp.dinvC, pts_h[sel], w.nz, tip.tipA,
+                              tip.tipB, tip.tipC, tip.stiffness, tip.dpos0, tip.relax, tip.surfFF, 100, 1e-8, False,
+                              "f32")
+    np.testing.assert_array_equal(FE2.cpu().numpy()[sel], FEo)
+    np.testing.assert_array_equal(PP2.cpu().numpy()[sel], PPo)
+
+
 # ── df operators ─────────────────────────────────────────────────────────────
 
 def _cmp(a, b, rel=1e-5):
