@@ -331,7 +331,7 @@ def batch_leg(dev, n_mol, z_margin=None):
         "speedup": t_p / t_b, "speedup_vs_windowed_plain": t_w / t_b,
         "bitexact_windowed_plain": same_w, "windowed_plain_fallback_molecules": win_fallback,
         "launch": best.key(), "autotune_ms": timings,
-        "autotune_fused_vs_two_phase_ms": [min((v for k, v in timings.items() if k.endswith("_fused")), default=None),
+        "autotune_fused_vs_two_phase_ms": [min((v for k, v in timings.items() if "_fused" in k), default=None),
                                            min((v for k, v in timings.items() if k.endswith("_two_phase")),
                                                default=None)],
         "projected_pair_evals": bf.pair_evals, "z_margin_A": list(bf.z_margin),
@@ -402,6 +402,8 @@ def substrate_leg(dev, n_poses=64, fd=True):
         gated += scene.last_status.get("gated", 0)
         cols_out += int(cols.numel() - cols.sum())
         gnorm.append(float(pose.grad.norm()))
+        if k == 0:
+            cols0 = cols.clone()
         if fd and k < 2:    # directional finite difference along -grad(pose) on two placements
             gp = pose.grad.detach()
             d = -gp / gp.norm()
@@ -424,9 +426,9 @@ def substrate_leg(dev, n_poses=64, fd=True):
                       torch.tensor(sc["poses"][0], dtype=torch.float64, device=dev))
         _, PPs0 = PS._forward(scene, torch.cat([x0.float(), q0.float()[:, None]], 1).contiguous())
     nimg = (2 * scene.npbc[0] + 1) * (2 * scene.npbc[1] + 1)
-    adj_pairs, adj_blocks = grid_adjoint_active_pairs(scene.spec_run, PPs0, sc["mol_atoms"].shape[0] * nimg)
+    adj_pairs, adj_octets = grid_adjoint_active_pairs(scene.spec_run, PPs0, sc["mol_atoms"].shape[0] * nimg, cols0)
     res = {
-        "grid_adjoint_work_placement0": {"active_workgroups": adj_blocks, "pair_evals": adj_pairs,
+        "grid_adjoint_work_placement0": {"octets": adj_octets, "pair_evals": adj_pairs,
                                          "flop": adj_pairs * ADJ_FLOP_PER_PAIR},
         "workload": f"60-atom molecule x {n_poses} placements over a 1024-ion rock-salt slab, "
                     f"{sc['n'][0]}x{sc['n'][1]}x{sc['n'][2]} grid (0.05 A), {sc['nxy'][0]}x{sc['nxy'][1]} px x {sc['nz']} z",
@@ -750,7 +752,7 @@ def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False, fd=True):
         atoms = torch.cat([xyz0.float(), q0.float()[:, None]], 1).contiguous()
         _, _, PPs, _ = diff._forward_parts(atoms, diff.lj_cpar(R0, E0), prob)
     nimg = (2 * prob.npbc[0] + 1) * (2 * prob.npbc[1] + 1)
-    adj_pairs, adj_blocks = grid_adjoint_active_pairs(prob.spec, PPs, xyz0.shape[0] * nimg)
+    adj_pairs, adj_octets = grid_adjoint_active_pairs(prob.spec, PPs, xyz0.shape[0] * nimg, cols)
     status = dict(prob.last_status)
     # central differences of the FP64 device forward on 32 random parameters (seed 3), same gated loss
     na = xyz0.shape[0]
@@ -782,7 +784,7 @@ def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False, fd=True):
         "forward_ms": tf, "adjoint_ms": tb, "adjoint_over_forward": tb / tf,
         "pixel_z_steps_per_s_forward": w.n_scan * w.nz / (tf * 1e-3),
         "adjoint_status": status, "gating": gating, "fd_fp64": fdres,
-        "grid_adjoint_work": {"active_workgroups": adj_blocks, "pair_evals": adj_pairs,
+        "grid_adjoint_work": {"octets": adj_octets, "pair_evals": adj_pairs,
                               "flop": adj_pairs * ADJ_FLOP_PER_PAIR, "flop_per_pair": ADJ_FLOP_PER_PAIR},
         "adam": {"steps": adam_steps, "lr": lr, "wall_s": t_adam, "loss_first": hist[0] if hist else None,
                  "loss_last": hist[-1] if hist else None, "loss_every5": hist[::5],
@@ -793,24 +795,26 @@ def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False, fd=True):
 ADJ_FLOP_PER_PAIR = 60   # grid->atom adjoint, LJ + tip charge at z offset 0 (pair_adjoint), counted as the forward's 52
 
 
-def grid_adjoint_active_pairs(spec, PPs, n_sites, V=8):
-    """Pair evaluations of k_grid_adjoint (V voxels per thread, 256 columns per workgroup) over the
-    workgroups it does not skip: those holding a corner of the trilinear cell of some relaxed probe
-    (the only voxels with a nonzero cotangent), x 256 columns x V planes x atom sites."""
+def grid_adjoint_active_pairs(spec, PPs, n_sites, columns=None, V=8):
+    """Pair evaluations of k_grid_adjoint: it evaluates the octets (a grid column x V planes, V = 8)
+    that hold a corner of the trilinear cell of some relaxed probe whose pixel column is in the loss
+    (the only voxels with a nonzero cotangent), x V voxels x atom sites.  Returns (pairs, octets)."""
     nx, ny, nz = (int(v) for v in spec.n)
     step = np.array([spec.dA[0], spec.dB[1], spec.dC[2]], np.float64)
     p = PPs[..., :3].reshape(-1, 3).double().cpu().numpy()
+    if columns is not None:
+        keep = np.repeat(columns.reshape(-1).cpu().numpy().astype(bool), p.shape[0] // columns.numel())
+        p = p[keep]
     i0 = np.floor((p - np.asarray(spec.p0[:3], np.float64)) / step).astype(np.int64)
-    blocks = set()
+    octets = set()
     for cx in (0, 1):
         for cy in (0, 1):
             for cz in (0, 1):
                 ia = (i0[:, 0] + cx) % nx
                 ib = (i0[:, 1] + cy) % ny
                 ic = (i0[:, 2] + cz) % nz
-                key = ((ib * nx + ia) // 256) * ((nz + V - 1) // V) + ic // V
-                blocks.update(np.unique(key).tolist())
-    return len(blocks) * 256 * V * n_sites, len(blocks)
+                octets.update(np.unique((ic // V) * (nx * ny) + ib * nx + ia).tolist())
+    return len(octets) * V * n_sites, len(octets)
 
 
 def diff_leg(dev, adam_steps=50, fd=True):

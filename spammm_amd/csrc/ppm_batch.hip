@@ -136,7 +136,7 @@ __device__ void do_relax_item(const PpmMolDesc& d, const PpmBatchParams& P, int 
     const size_t o = pix * P.nz;
     const rc::TiltArgs t = tilt_args(P);
     const rc::DfOut dfo{dfW, dfStart, P.df_taps, P.df_nz_out, false};
-    rc::relax_tilted_lane(g, start, t, FEs + o, PPs ? PPs + o : nullptr, df ? &dfo : nullptr,
+    rc::relax_tilted_lane<false>(g, start, t, FEs + o, PPs ? PPs + o : nullptr, df ? &dfo : nullptr,
                           df ? df + pix * P.df_nz_out : nullptr);
 }
 

@@ -47,9 +47,79 @@ __global__ __launch_bounds__(256) void k_grid_project(const float4* __restrict__
 // and the parameter derivatives follow from the pair law.  Capped voxels (|E| = 100 after
 // the rescale) are differentiated through the rescale: g_raw = s g - s (g.fe_raw)/E_raw e_w,
 // s = 100/|E_raw|, with fe_raw recomputed for the workgroup.
-// Workgroups whose voxels all have zero cotangent exit at once (flag 0); the others write
-// one FP64 partial sum per atom (the V voxels of one site image summed in FP32, the rest in
-// FP64), reduced over workgroups by k_reduce_partials in FP64.
+// Only octets (a column x V consecutive planes) with a nonzero cotangent are evaluated: the
+// probes touch a small part of the grid, and within the z blocks they reach whole columns stay
+// untouched (pixel and grid spacings differ), so the octets are first listed in octet order
+// (k_adj_mark, k_adj_offsets, k_adj_list) and k_grid_adjoint takes 256 listed octets per
+// workgroup.  Each workgroup writes one FP64 partial sum per atom (the V voxels of one site
+// image summed in FP32, the rest in FP64), reduced over workgroups by k_reduce_partials in FP64.
+// The list order is fixed, so the result does not depend on scheduling.
+
+// Octet o = (z block zb, padded column col) = zb * (gridDim.x * 256) + col, the workgroup layout
+// of the grid projection: flag[o] = any voxel of the octet has a nonzero cotangent, count[b] =
+// flags set in workgroup b.
+template <int V>
+__global__ __launch_bounds__(256) void k_adj_mark(GridArgs a, const float4* __restrict__ gcot,
+                                                  unsigned char* __restrict__ flag, int* __restrict__ count) {
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    const int ic0 = blockIdx.y * V;
+    bool any = false;
+    if (col < a.ncol) {
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            if (ic0 + k < a.nz) {
+                const float4 g = gcot[(size_t)(ic0 + k) * a.ncol + col];
+                any |= (g.x != 0.f) | (g.y != 0.f) | (g.z != 0.f) | (g.w != 0.f);
+            }
+        }
+    }
+    flag[(size_t)bid * 256 + threadIdx.x] = any ? 1 : 0;
+    const int n = __syncthreads_count(any);
+    if (threadIdx.x == 0) count[bid] = n;
+}
+
+// Exclusive offsets of the per-workgroup counts (one workgroup, each thread a contiguous run of
+// counts) and their total.
+__global__ __launch_bounds__(1024) void k_adj_offsets(const int* __restrict__ count, int nb, int* __restrict__ off,
+                                                      int* __restrict__ total) {
+    __shared__ int s[1024];
+    const int per = (nb + 1023) / 1024;
+    const int b0 = min(nb, (int)threadIdx.x * per), b1 = min(nb, b0 + per);
+    int sum = 0;
+    for (int b = b0; b < b1; ++b) sum += count[b];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = (int)threadIdx.x >= d ? s[threadIdx.x - d] : 0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int run = s[threadIdx.x] - sum;
+    for (int b = b0; b < b1; ++b) {
+        off[b] = run;
+        run += count[b];
+    }
+    if (threadIdx.x == 1023) *total = s[1023];
+}
+
+// list[off[b] + rank] = octet id of every flagged octet, in octet order.
+__global__ __launch_bounds__(256) void k_adj_list(const unsigned char* __restrict__ flag, const int* __restrict__ off,
+                                                  int* __restrict__ list) {
+    __shared__ int wc[4];
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+    const int o = bid * 256 + (int)threadIdx.x;
+    const bool f = flag[o] != 0;
+    const unsigned long long m = __ballot(f);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int below = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wc[w] = __popcll(m);
+    __syncthreads();
+    int base = off[bid];
+    for (int i = 0; i < w; ++i) base += wc[i];
+    if (f) list[base + below] = o;
+}
 template <int MODEL>
 __device__ __forceinline__ void pair_fe_raw(float dx, float dy, float dz, float dxy2, float q, float p0, float p1,
                                             float K, const GridArgs& a, float4& fe) {
@@ -144,28 +214,31 @@ template <int MODEL, int V, bool DCZ, bool OFFS>
 __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__ atoms, const float* __restrict__ cpar,
                                                       GridArgs a, const float4* __restrict__ grid,
                                                       const float4* __restrict__ gcot, const float4* __restrict__ base,
+                                                      const int* __restrict__ list, const int* __restrict__ total,
                                                       double* __restrict__ partials, int* __restrict__ active) {
 #pragma clang fp contract(fast)
     __shared__ double red[4][6];
     const int bid = blockIdx.y * gridDim.x + blockIdx.x;
-    const int col = blockIdx.x * 256 + threadIdx.x;
-    const bool colok = col < a.ncol;
-    const int cc = colok ? col : a.ncol - 1;
-    const int ia = cc % a.nx, ib = cc / a.nx;
-    const int ic0 = blockIdx.y * V;
-    float4 g[V];
-    bool any = false;
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
-        const bool ok = colok && ic0 + k < a.nz;
-        g[k] = ok ? gcot[(size_t)(ic0 + k) * a.ncol + col] : make_float4(0.f, 0.f, 0.f, 0.f);
-        any |= (g[k].x != 0.f) | (g[k].y != 0.f) | (g[k].z != 0.f) | (g[k].w != 0.f);
-    }
-    if (!__syncthreads_or(any)) {
+    const int n_list = *total;
+    if (bid * 256 >= n_list) {           // the same for every thread of the workgroup
         if (threadIdx.x == 0) active[bid] = 0;
         return;
     }
     if (threadIdx.x == 0) active[bid] = 1;
+    const int e = bid * 256 + (int)threadIdx.x;
+    const bool colok = e < n_list;
+    const int oct = list[colok ? e : n_list - 1];
+    const int ncolp = (int)gridDim.x * 256;
+    const int zb = oct / ncolp;
+    const int col = oct - zb * ncolp;     // < a.ncol: only octets holding voxels are listed
+    const int ia = col % a.nx, ib = col / a.nx;
+    const int ic0 = zb * V;
+    float4 g[V];
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        const bool ok = colok && ic0 + k < a.nz;
+        g[k] = ok ? gcot[(size_t)(ic0 + k) * a.ncol + col] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 
     float px[DCZ ? 1 : V], py[DCZ ? 1 : V], pz[V];
 #pragma unroll
@@ -384,11 +457,41 @@ int launch_grid_project(int model, const float* atoms, const float* cpar, int na
                  : dispatch<PPM_MODEL_MORSE, false>(at, cpar, a, o, b, acc64, stream, klo, nzw);
 }
 
-size_t grid_adjoint_workspace(int na, const PpmGridSpec& g) {
+// Workspace of the grid adjoint: per-workgroup FP64 partial rows, active flags, octet flags,
+// per-workgroup octet counts and offsets, the octet count and the octet list, each 256-B aligned.
+struct AdjWorkspace {
+    double* partials;
+    int* active;
+    unsigned char* flag;
+    int *count, *off, *total, *list;
+};
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+static size_t adjoint_layout(int na, const PpmGridSpec& g, void* base, AdjWorkspace* w) {
     const dim3 b = grid_blocks(g, adjoint_V(g));
-    const size_t nb = (size_t)b.x * b.y;
-    return nb * (size_t)na * 6 * sizeof(double) + nb * sizeof(int) + 256;
+    const size_t nb = (size_t)b.x * b.y, noct = nb * 256;
+    const size_t sizes[7] = {nb * (size_t)na * 6 * sizeof(double), nb * sizeof(int), noct, nb * sizeof(int),
+                             nb * sizeof(int), sizeof(int), noct * sizeof(int)};
+    size_t off[7], at = 0;
+    for (int i = 0; i < 7; ++i) {
+        off[i] = at;
+        at += align256(sizes[i]);
+    }
+    if (w) {
+        char* p = reinterpret_cast<char*>(base);
+        w->partials = reinterpret_cast<double*>(p + off[0]);
+        w->active = reinterpret_cast<int*>(p + off[1]);
+        w->flag = reinterpret_cast<unsigned char*>(p + off[2]);
+        w->count = reinterpret_cast<int*>(p + off[3]);
+        w->off = reinterpret_cast<int*>(p + off[4]);
+        w->total = reinterpret_cast<int*>(p + off[5]);
+        w->list = reinterpret_cast<int*>(p + off[6]);
+    }
+    return at + 256;
 }
+
+size_t grid_adjoint_workspace(int na, const PpmGridSpec& g) { return adjoint_layout(na, g, nullptr, nullptr); }
 
 int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
@@ -398,16 +501,27 @@ int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na
     const int V = adjoint_V(g);
     const dim3 blocks = grid_blocks(g, V);
     const size_t nb = (size_t)blocks.x * blocks.y;
-    double* partials = reinterpret_cast<double*>(workspace);
-    int* active = reinterpret_cast<int*>(reinterpret_cast<char*>(workspace) + nb * (size_t)na * 6 * sizeof(double));
+    if (nb * 256 > (size_t)INT32_MAX) {
+        set_error("grid_adjoint: %zu voxel octets exceed the int range", nb * 256);
+        return PPM_ERR_INVALID_ARG;
+    }
+    AdjWorkspace w;
+    adjoint_layout(na, g, workspace, &w);
+    double* partials = w.partials;
+    int* active = w.active;
     const float4* at = reinterpret_cast<const float4*>(atoms);
     const float4* G = reinterpret_cast<const float4*>(grid);
     const float4* gc = reinterpret_cast<const float4*>(grid_cot);
     const float4* B = reinterpret_cast<const float4*>(base);
     const bool offs = a.noff > 0;
+    if (V == 8) hipLaunchKernelGGL(k_adj_mark<8>, blocks, dim3(256), 0, stream, a, gc, w.flag, w.count);
+    else hipLaunchKernelGGL(k_adj_mark<4>, blocks, dim3(256), 0, stream, a, gc, w.flag, w.count);
+    hipLaunchKernelGGL(k_adj_offsets, dim3(1), dim3(1024), 0, stream, w.count, (int)nb, w.off, w.total);
+    hipLaunchKernelGGL(k_adj_list, blocks, dim3(256), 0, stream, w.flag, w.off, w.list);
+    if (int r = check_launch("grid_adjoint_list")) return r;
 #define PPM_ADJ(M, VV, DCZ, OF)                                                                                  \
     hipLaunchKernelGGL((k_grid_adjoint<M, VV, DCZ, OF>), blocks, dim3(256), 0, stream, at, cpar, a, G, gc, B, \
-                       partials, active)
+                       w.list, w.total, partials, active)
     if (model == PPM_MODEL_LJ) {
         if (V == 8) { if (offs) PPM_ADJ(PPM_MODEL_LJ, 8, true, true); else PPM_ADJ(PPM_MODEL_LJ, 8, true, false); }
         else { if (offs) PPM_ADJ(PPM_MODEL_LJ, 4, false, true); else PPM_ADJ(PPM_MODEL_LJ, 4, false, false); }

@@ -266,7 +266,10 @@ struct DfOut {
 // the operator applied to the column after the relaxation (dfo, dfrow).  fz_lds (optional): this
 // lane's Fz column in LDS (element iz at fz_lds[iz * 64]), kept by every lane of the group so that
 // the G lanes split the df outputs; without it the FE-storing lane reads its stored column back.
-template <int G, class SAMPLE>
+// STAGE: outputs go through Stage4 runs (8 float4 registers for G = 1); false: one store per
+// state, for kernels whose occupancy those registers would cost (the batched kernel shares its
+// register budget with the grid tiles).
+template <int G, class SAMPLE, bool STAGE = true>
 __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, const TiltArgs& p,
                                                       float4* __restrict__ FEs, float4* __restrict__ PPs,
                                                       int sub = 0, const DfOut* dfo = nullptr,
@@ -311,7 +314,16 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
             const bool fin = finite4(fs) && isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z);
             const float4 ppo = make_float4(pos.x, pos.y, pos.z, (conv && fin) ? (float)it : -(float)it);
             if (fz_lds) fz_lds[iz * 64] = feo.z;
-            if (G == 1) {
+            if (!STAGE) {
+                if (G == 1) {
+                    FEs[o] = feo;
+                    if (PPs) PPs[o] = ppo;
+                } else if (sub == 0) {
+                    FEs[o] = feo;
+                } else if (sub == 1 && PPs) {
+                    PPs[o] = ppo;
+                }
+            } else if (G == 1) {
                 s1.push(feo);
                 s2.push(ppo);
             } else {
@@ -319,7 +331,7 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
             }
             ++pend;
             const bool last = iz + 1 >= p.nz;
-            if (last || (reinterpret_cast<uintptr_t>(FEs + o + 1) & 63) == 0) {
+            if (STAGE && (last || (reinterpret_cast<uintptr_t>(FEs + o + 1) & 63) == 0)) {
                 if (G == 1) {
                     s1.flush(FEs + o, pend);
                     if (PPs) s2.flush(PPs + o, pend);
@@ -361,6 +373,7 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
     }
 }
 
+template <bool STAGE = true>
 __device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, const TiltArgs& p,
                                                   float4* __restrict__ FEs, float4* __restrict__ PPs,
                                                   const DfOut* dfo = nullptr, float* __restrict__ dfrow = nullptr,
@@ -368,7 +381,7 @@ __device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, co
     Cell cell;
     cell.i = -1; cell.j = -1; cell.k = -1;
     auto smp = [&](float x, float y, float z) { return sample(g, cell, x, y, z); };
-    relax_tilted_lane_src<1>(smp, start, p, FEs, PPs, 0, dfo, dfrow, fz_lds);
+    relax_tilted_lane_src<1, decltype(smp), STAGE>(smp, start, p, FEs, PPs, 0, dfo, dfrow, fz_lds);
 }
 
 }  // namespace rc
