@@ -470,48 +470,105 @@ __device__ __forceinline__ void atomic_add4(float* p, float4 v) {
     atomicAdd(p + 3, v.w);
 }
 
+// Consecutive states of a scan point sit in consecutive lanes; when the probe descends by one grid
+// plane per step, the lower corner plane of a state is the upper corner plane of the next one.
+// The next lane then adds those 4 corner values to its own before it scatters, and this lane
+// skips them: half the atomics of such a scan (the atomics, executed at the memory side so that
+// the 8 XCDs see one sum, bound the kernel).
 __global__ __launch_bounds__(256) void k_relax_adjoint(GridDev g, const float4* __restrict__ pts, StateArgs p,
                                                        const float4* __restrict__ PPs, const float4* __restrict__ U,
                                                        float* __restrict__ gcot, float4* __restrict__ lam) {
+    const size_t n = (size_t)p.n_scan * p.nz;
     const size_t s = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= (size_t)p.n_scan * p.nz) return;
-    const int ip = (int)(s / p.nz);
-    const int iz = (int)(s - (size_t)ip * p.nz);
-    const f3 tip = tip_anchor(pts, p, ip, iz);
-    const float4 PP = PPs[s];
-    const float4 u = U[s];
-    int status = 0;
+    const bool inrange = s < n;        // every lane stays to the end: the corner exchange is a wave shuffle
+    int status = 1;
     StateEval e;
-    eval_state(g, p, tip, PP.x, PP.y, PP.z, e);
-    if (!(PP.w > 0.f) || !(norm3(e.G) < kResidualGate)) status |= 1;
-    const double R[3][3] = {{p.A[0], p.A[1], p.A[2]}, {p.B[0], p.B[1], p.B[2]}, {p.C[0], p.C[1], p.C[2]}};
-    double ru[3];
-    for (int j = 0; j < 3; ++j) ru[j] = R[0][j] * u.x + R[1][j] * u.y + R[2][j] * u.z;
-    double bv[3];
-    for (int j = 0; j < 3; ++j)
-        bv[j] = e.JF[0][j] * ru[0] + e.JF[1][j] * ru[1] + e.JF[2][j] * ru[2] + (double)u.w * e.gE[j];
-    double cof[3][3], det;
-    double l[3] = {0.0, 0.0, 0.0};
-    if (!cofactors(e.J, cof, det) || !(cond_inf(e.J, cof, det) < kCondGate)) {
-        status |= 4;
-    } else {
-        // J^T l = b:  (J^T)^{-1} = adj(J^T)/det = cof(J)/det
-        for (int i = 0; i < 3; ++i) l[i] = (cof[i][0] * bv[0] + cof[i][1] * bv[1] + cof[i][2] * bv[2]) / det;
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        e.idx[m] = 0;
+        e.w[m] = 0.f;
     }
-    double S[3][3];                      // symmetric part of the Hessian -J
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) S[i][j] = -0.5 * (e.J[i][j] + e.J[j][i]);
-    if (!(eigmin_sym3(S) > kStableGate)) status |= 8;
-    if (!(isfinite(l[0]) && isfinite(l[1]) && isfinite(l[2]) && isfinite(PP.x) && isfinite(PP.y) &&
-          isfinite(PP.z) && isfinite(bv[0]) && isfinite(bv[1]) && isfinite(bv[2])))
-        status |= 2;
-    if (status) l[0] = l[1] = l[2] = 0.0;
-    lam[s] = make_float4((float)l[0], (float)l[1], (float)l[2], (float)status);
-    if (status) return;
-    const float gx = (float)(ru[0] - l[0]), gy = (float)(ru[1] - l[1]), gz = (float)(ru[2] - l[2]);
+    float4 u = make_float4(0.f, 0.f, 0.f, 0.f);
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    if (inrange) {
+        const int ip = (int)(s / p.nz);
+        const int iz = (int)(s - (size_t)ip * p.nz);
+        const f3 tip = tip_anchor(pts, p, ip, iz);
+        const float4 PP = PPs[s];
+        u = U[s];
+        status = 0;
+        eval_state(g, p, tip, PP.x, PP.y, PP.z, e);
+        if (!(PP.w > 0.f) || !(norm3(e.G) < kResidualGate)) status |= 1;
+        const double R[3][3] = {{p.A[0], p.A[1], p.A[2]}, {p.B[0], p.B[1], p.B[2]}, {p.C[0], p.C[1], p.C[2]}};
+        double ru[3];
+        for (int j = 0; j < 3; ++j) ru[j] = R[0][j] * u.x + R[1][j] * u.y + R[2][j] * u.z;
+        double bv[3];
+        for (int j = 0; j < 3; ++j)
+            bv[j] = e.JF[0][j] * ru[0] + e.JF[1][j] * ru[1] + e.JF[2][j] * ru[2] + (double)u.w * e.gE[j];
+        double cof[3][3], det;
+        double l[3] = {0.0, 0.0, 0.0};
+        if (!cofactors(e.J, cof, det) || !(cond_inf(e.J, cof, det) < kCondGate)) {
+            status |= 4;
+        } else {
+            // J^T l = b:  (J^T)^{-1} = adj(J^T)/det = cof(J)/det
+            for (int i = 0; i < 3; ++i) l[i] = (cof[i][0] * bv[0] + cof[i][1] * bv[1] + cof[i][2] * bv[2]) / det;
+        }
+        double S[3][3];                      // symmetric part of the Hessian -J
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) S[i][j] = -0.5 * (e.J[i][j] + e.J[j][i]);
+        if (!(eigmin_sym3(S) > kStableGate)) status |= 8;
+        if (!(isfinite(l[0]) && isfinite(l[1]) && isfinite(l[2]) && isfinite(PP.x) && isfinite(PP.y) &&
+              isfinite(PP.z) && isfinite(bv[0]) && isfinite(bv[1]) && isfinite(bv[2])))
+            status |= 2;
+        if (status) l[0] = l[1] = l[2] = 0.0;
+        lam[s] = make_float4((float)l[0], (float)l[1], (float)l[2], (float)status);
+        gx = (float)(ru[0] - l[0]);
+        gy = (float)(ru[1] - l[1]);
+        gz = (float)(ru[2] - l[2]);
+    }
+    const bool give = status == 0;           // this lane scatters its corners
+    float4 c[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const float w = give ? e.w[m] : 0.f;
+        c[m] = make_float4(w * gx, w * gy, w * gz, w * u.w);
+    }
+    unsigned long long key[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) key[m] = give ? (unsigned long long)e.idx[m] : ~0ull;
+    // lower plane of the previous lane (lane - 1 of the same wave); every lane executes every
+    // shuffle (a shuffle under a divergent condition would read lanes that did not take part)
+    const int lane = threadIdx.x & 63;
+    const int prev_give = __shfl_up((int)give, 1, 64);
+    bool take = give && lane > 0 && prev_give != 0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const unsigned long long prev = __shfl_up(key[m], 1, 64);
+        take = take && prev == (unsigned long long)e.idx[4 + m];
+    }
+    float4 prevc[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        prevc[m].x = __shfl_up(c[m].x, 1, 64);
+        prevc[m].y = __shfl_up(c[m].y, 1, 64);
+        prevc[m].z = __shfl_up(c[m].z, 1, 64);
+        prevc[m].w = __shfl_up(c[m].w, 1, 64);
+    }
+    const int next_take = __shfl_down((int)take, 1, 64);
+    const bool next_took = lane < 63 && next_take != 0;
+    if (!give) return;
+    if (take) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            c[4 + m].x += prevc[m].x;
+            c[4 + m].y += prevc[m].y;
+            c[4 + m].z += prevc[m].z;
+            c[4 + m].w += prevc[m].w;
+        }
+    }
 #pragma unroll
     for (int m = 0; m < 8; ++m)
-        atomic_add4(gcot + 4 * e.idx[m], make_float4(e.w[m] * gx, e.w[m] * gy, e.w[m] * gz, e.w[m] * u.w));
+        if (m >= 4 || !next_took) atomic_add4(gcot + 4 * e.idx[m], c[m]);
 }
 
 // Per-state diagnostics of relaxed states without a cotangent: the adjoint_diag record of the
