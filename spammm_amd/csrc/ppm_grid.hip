@@ -18,6 +18,8 @@
 //   * the tip charge at z offset 0 shares r^2 with the pair law (one rsqrt for both);
 //     charge sites with a nonzero offset are evaluated separately;
 //   * stores are float4 per voxel, consecutive lanes -> consecutive x (1 KiB per wave).
+#include <algorithm>
+
 #include "ppm_grid_core.hpp"
 
 namespace ppm {
@@ -50,9 +52,12 @@ __global__ __launch_bounds__(256) void k_grid_project(const float4* __restrict__
 // Only octets (a column x V consecutive planes) with a nonzero cotangent are evaluated: the
 // probes touch a small part of the grid, and within the z blocks they reach whole columns stay
 // untouched (pixel and grid spacings differ), so the octets are first listed in octet order
-// (k_adj_mark, k_adj_offsets, k_adj_list) and k_grid_adjoint takes 256 listed octets per
-// workgroup.  Each workgroup writes one FP64 partial sum per atom (the V voxels of one site
-// image summed in FP32, the rest in FP64), reduced over workgroups by k_reduce_partials in FP64.
+// (k_adj_mark, k_adj_offsets, k_adj_list) and k_grid_adjoint takes 256 listed octets and one of
+// up to gridDim.z shares of the atoms per workgroup (when few octets are listed, the shares keep
+// ~4096 workgroups in flight, so the last ones do not run alone on the chip; every share re-reads
+// its octets' cotangent, so large lists are not split).  Each workgroup
+// writes one FP64 partial sum per atom of its share (the V voxels of one site image summed in
+// FP32, the rest in FP64), reduced over workgroups by k_reduce_partials in FP64.
 // The list order is fixed, so the result does not depend on scheduling.
 
 // Octet o = (z block zb, padded column col) = zb * (gridDim.x * 256) + col, the workgroup layout
@@ -221,10 +226,17 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
     const int bid = blockIdx.y * gridDim.x + blockIdx.x;
     const int n_list = *total;
     if (bid * 256 >= n_list) {           // the same for every thread of the workgroup
-        if (threadIdx.x == 0) active[bid] = 0;
+        if (threadIdx.x == 0 && blockIdx.z == 0) active[bid] = 0;
         return;
     }
-    if (threadIdx.x == 0) active[bid] = 1;
+    if (threadIdx.x == 0 && blockIdx.z == 0) active[bid] = 1;
+    // atoms [j_begin, j_end): share blockIdx.z of n_share, n_share = the number of shares (at most
+    // gridDim.z) that brings the listed workgroups to ~4096 (16 per CU)
+    const int n_wg = (n_list + 255) / 256;
+    const int n_share = min((int)gridDim.z, max(1, (4096 + n_wg - 1) / n_wg));
+    if ((int)blockIdx.z >= n_share) return;
+    const int per = (a.na + n_share - 1) / n_share;
+    const int j_begin = min(a.na, (int)blockIdx.z * per), j_end = min(a.na, j_begin + per);
     const int e = bid * 256 + (int)threadIdx.x;
     const bool colok = e < n_list;
     const int oct = list[colok ? e : n_list - 1];
@@ -305,7 +317,7 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
     }
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int j = 0; j < a.na; ++j) {
+    for (int j = j_begin; j < j_end; ++j) {
         const float4 at = atoms[j];
         const float p0 = cpar[j * cs], p1 = cpar[j * cs + 1], K = MODEL == PPM_MODEL_LJ ? 0.f : cpar[j * cs + 2];
         // The V voxels of one image are summed in FP32, everything above in FP64: the grid
@@ -501,6 +513,8 @@ int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na
     const int V = adjoint_V(g);
     const dim3 blocks = grid_blocks(g, V);
     const size_t nb = (size_t)blocks.x * blocks.y;
+    // atom shares per listed workgroup: 4 while each share keeps >= 8 atoms
+    const dim3 adj_blocks(blocks.x, blocks.y, (unsigned)std::max(1, std::min(4, na / 8)));
     if (nb * 256 > (size_t)INT32_MAX) {
         set_error("grid_adjoint: %zu voxel octets exceed the int range", nb * 256);
         return PPM_ERR_INVALID_ARG;
@@ -520,7 +534,7 @@ int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na
     hipLaunchKernelGGL(k_adj_list, blocks, dim3(256), 0, stream, w.flag, w.off, w.list);
     if (int r = check_launch("grid_adjoint_list")) return r;
 #define PPM_ADJ(M, VV, DCZ, OF)                                                                                  \
-    hipLaunchKernelGGL((k_grid_adjoint<M, VV, DCZ, OF>), blocks, dim3(256), 0, stream, at, cpar, a, G, gc, B, \
+    hipLaunchKernelGGL((k_grid_adjoint<M, VV, DCZ, OF>), adj_blocks, dim3(256), 0, stream, at, cpar, a, G, gc, B, \
                        w.list, w.total, partials, active)
     if (model == PPM_MODEL_LJ) {
         if (V == 8) { if (offs) PPM_ADJ(PPM_MODEL_LJ, 8, true, true); else PPM_ADJ(PPM_MODEL_LJ, 8, true, false); }
