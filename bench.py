@@ -252,12 +252,28 @@ def _round_floats(x, sig=5):
     return x
 
 
+LINE_LIMIT = 4000
+# leg summaries dropped first when the line would exceed LINE_LIMIT (the detail file keeps them)
+DROP_ORDER = ("accuracy", "ewald", "fdbm", "direct_morse", "substrate_poses", "differentiable", "batched")
+
+
 def compact_line(out):
-    """The result line: floats to 5 significant digits, no whitespace.  It must stay short (the
-    record of every leg is in the detail file), so it is checked here."""
-    line = json.dumps(_round_floats(out), separators=(",", ":"))
-    if len(line) > 4000:
-        raise RuntimeError(f"bench result line is {len(line)} bytes; keep it under 4000")
+    """The result line: floats to 5 significant digits, no whitespace, at most LINE_LIMIT bytes.
+    The headline keys always stay; leg summaries are dropped in DROP_ORDER if the line would be
+    longer (the record of every leg is in the detail file)."""
+    out = _round_floats(out)
+    line = json.dumps(out, separators=(",", ":"))
+    dropped = []
+    for k in DROP_ORDER:
+        if len(line) <= LINE_LIMIT:
+            break
+        if k in out:
+            out = {kk: v for kk, v in out.items() if kk != k}
+            dropped.append(k)
+            out["dropped_from_line"] = dropped
+            line = json.dumps(out, separators=(",", ":"))
+    if dropped:
+        log(f"[bench] result line over {LINE_LIMIT} bytes: dropped {dropped} (kept in the detail file)")
     return line
 
 
