@@ -37,7 +37,9 @@ constexpr double kResidualGate = 5e-4, kCondGate = 1e6, kStableGate = 1e-5;
 // relaxStrokesTilted with PP telemetry; G lanes per scan point (G = 1, 2, 4): with G > 1 each
 // lane interpolates 4/G channels and the group shares the FIRE logic, results identical to G = 1.
 // df (optional, with dfo): the df operator applied to each point's Fz column, (n_scan, dfo.nout).
-template <int G>
+// FAST: identity tip frame and grid axes along the world axes (the form of every untilted scan),
+// which skips the rotations and the off-axis texture-coordinate terms (same results).
+template <int G, bool FAST>
 __global__ __launch_bounds__(64) void k_relax_tilted(GridDev g, const float4* __restrict__ pts, TiltArgs p,
                                                      float4* __restrict__ FEs, float4* __restrict__ PPs,
                                                      DfOut dfo, float* __restrict__ df) {
@@ -51,13 +53,21 @@ __global__ __launch_bounds__(64) void k_relax_tilted(GridDev g, const float4* __
     extern __shared__ float s_fz[];              // (nz, 64) when the launch gives it LDS, else unused
     float* col = (df && dfo.lds) ? s_fz + (threadIdx.x & 63) : nullptr;
     if constexpr (G == 1) {
-        relax_tilted_lane(g, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, dp, row, col);
+        relax_tilted_lane<true, FAST>(g, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, dp, row, col);
     } else {
         GroupCell<G> cell;
         cell.i = -1; cell.j = -1; cell.k = -1;
-        auto smp = [&](float x, float y, float z) { return sample_group<G>(g, cell, sub, x, y, z); };
-        relax_tilted_lane_src<G>(smp, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, sub, dp, row, col);
+        auto smp = [&](float x, float y, float z) { return sample_group<G, FAST>(g, cell, sub, x, y, z); };
+        relax_tilted_lane_src<G, decltype(smp), true, FAST>(smp, mk3(P.x, P.y, P.z), p, FEs + o,
+                                                            PPs ? PPs + o : nullptr, sub, dp, row, col);
     }
+}
+
+// The tip frame is the identity and the grid's inverse cell has no off-diagonal terms.
+bool fast_frame(const TiltArgs& p, const GridDev& g) {
+    return p.A[0] == 1.f && p.A[1] == 0.f && p.A[2] == 0.f && p.B[0] == 0.f && p.B[1] == 1.f && p.B[2] == 0.f &&
+           p.C[0] == 0.f && p.C[1] == 0.f && p.C[2] == 1.f && g.da[1] == 0.f && g.da[2] == 0.f && g.db[0] == 0.f &&
+           g.db[2] == 0.f && g.dc[0] == 0.f && g.dc[1] == 0.f;
 }
 
 // Lanes per scan point of k_relax_tilted / k_relax_strokes: the largest G <= 4 that keeps the
@@ -645,9 +655,12 @@ int launch_relax_tilted(const float* grid, const PpmGridSpec& s, const float* pt
     // the Fz columns of a wave in LDS (nz x 64 floats) up to 64 KiB per workgroup, else read back
     const size_t lds = (df && (size_t)nz * 64 * sizeof(float) <= 65536) ? (size_t)nz * 64 * sizeof(float) : 0;
     DfOut dfo{dfW, dfStart, taps, nout, lds > 0};
-    if (G == 4) hipLaunchKernelGGL(k_relax_tilted<4>, blocks, dim3(64), lds, stream, gd, P, p, F, Q, dfo, df);
-    else if (G == 2) hipLaunchKernelGGL(k_relax_tilted<2>, blocks, dim3(64), lds, stream, gd, P, p, F, Q, dfo, df);
-    else hipLaunchKernelGGL(k_relax_tilted<1>, blocks, dim3(64), lds, stream, gd, P, p, F, Q, dfo, df);
+    const bool fast = fast_frame(p, gd);
+#define PPM_TILT(GG, FF) hipLaunchKernelGGL((k_relax_tilted<GG, FF>), blocks, dim3(64), lds, stream, gd, P, p, F, Q, dfo, df)
+    if (G == 4) { if (fast) PPM_TILT(4, true); else PPM_TILT(4, false); }
+    else if (G == 2) { if (fast) PPM_TILT(2, true); else PPM_TILT(2, false); }
+    else { if (fast) PPM_TILT(1, true); else PPM_TILT(1, false); }
+#undef PPM_TILT
     return check_launch("relax_strokes_tilted");
 }
 

@@ -56,10 +56,27 @@ __device__ __forceinline__ float4 lerp8(const Cell& C, float a, float b, float c
     return o;
 }
 
+// Texture coordinates of a world point.  AX: the grid axes are the world axes (the off-diagonal
+// entries of dinv are zero), so px*d[0] + py*0 + pz*0 + d[3] = px*d[0] + d[3] for finite
+// coordinates (the dropped terms are signed zeros) and the two multiplies are skipped.
+template <bool AX>
+__device__ __forceinline__ void tex_coords(const GridDev& g, float px, float py, float pz, float& sa, float& sb,
+                                           float& sc) {
+    if (AX) {
+        sa = px * g.da[0] + g.da[3];
+        sb = py * g.db[1] + g.db[3];
+        sc = pz * g.dc[2] + g.dc[3];
+    } else {
+        sa = px * g.da[0] + py * g.da[1] + pz * g.da[2] + g.da[3];
+        sb = px * g.db[0] + py * g.db[1] + pz * g.db[2] + g.db[3];
+        sc = px * g.dc[0] + py * g.dc[1] + pz * g.dc[2] + g.dc[3];
+    }
+}
+
+template <bool AX = false>
 __device__ __forceinline__ float4 sample(const GridDev& g, Cell& C, float px, float py, float pz) {
-    const float sa = px * g.da[0] + py * g.da[1] + pz * g.da[2] + g.da[3];
-    const float sb = px * g.db[0] + py * g.db[1] + pz * g.db[2] + g.db[3];
-    const float sc = px * g.dc[0] + py * g.dc[1] + pz * g.dc[2] + g.dc[3];
+    float sa, sb, sc;
+    tex_coords<AX>(g, px, py, pz, sa, sb, sc);
     int i0, i1, j0, j1, k0, k1;
     float a, b, c;
     axis_repeat(sa, g.nx, i0, i1, a);
@@ -96,13 +113,12 @@ struct GroupCell {
     float c[4 / G][8];
 };
 
-template <int G>
+template <int G, bool AX = false>
 __device__ __forceinline__ float4 sample_group(const GridDev& g, GroupCell<G>& C, int sub, float px, float py,
                                                float pz) {
     constexpr int NC = 4 / G;
-    const float sa = px * g.da[0] + py * g.da[1] + pz * g.da[2] + g.da[3];
-    const float sb = px * g.db[0] + py * g.db[1] + pz * g.db[2] + g.db[3];
-    const float sc = px * g.dc[0] + py * g.dc[1] + pz * g.dc[2] + g.dc[3];
+    float sa, sb, sc;
+    tex_coords<AX>(g, px, py, pz, sa, sb, sc);
     int i0, i1, j0, j1, k0, k1;
     float a, b, c;
     axis_repeat(sa, g.nx, i0, i1, a);
@@ -269,7 +285,11 @@ struct DfOut {
 // STAGE: outputs go through Stage4 runs (8 float4 registers for G = 1); false: one store per
 // state, for kernels whose occupancy those registers would cost (the batched kernel shares its
 // register budget with the grid tiles).
-template <int G, class SAMPLE, bool STAGE = true>
+// IDF: the tip frame is the identity (A, B, C = x, y, z).  The rotations then return their input
+// and the z-only shifts (tip step, surface force) leave x and y alone, for finite values up to
+// the sign of a zero, so they are skipped: the same results with ~30 fewer instructions per
+// FIRE step.
+template <int G, class SAMPLE, bool STAGE = true, bool IDF = false>
 __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, const TiltArgs& p,
                                                       float4* __restrict__ FEs, float4* __restrict__ PPs,
                                                       int sub = 0, const DfOut* dfo = nullptr,
@@ -277,7 +297,7 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
                                                       float* __restrict__ fz_lds = nullptr) {
     const f3 A = ld3(p.A), B = ld3(p.B), C = ld3(p.C);
     const f3 dTip = mk3(C.x * p.dtip, C.y * p.dtip, C.z * p.dtip);
-    const f3 d0r = rotMatT(ld3(p.d0), A, B, C);
+    const f3 d0r = IDF ? ld3(p.d0) : rotMatT(ld3(p.d0), A, B, C);
     f3 tip = start;
     f3 pos = mk3(tip.x + d0r.x, tip.y + d0r.y, tip.z + d0r.z);
     float dt = p.dt0, damp = p.damp0;
@@ -292,9 +312,15 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
         const float4 fe = sample(pos.x, pos.y, pos.z);
         f3 f = mk3(fe.x, fe.y, fe.z);
         const f3 dp = mk3(pos.x - tip.x, pos.y - tip.y, pos.z - tip.z);
-        const f3 ft = rotMatT(tipForce(rotMat(dp, A, B, C), p.k, p.d0), A, B, C);
-        f = mk3(f.x + ft.x, f.y + ft.y, f.z + ft.z);
-        f = mk3(f.x + fsurf.x, f.y + fsurf.y, f.z + fsurf.z);
+        if constexpr (IDF) {
+            const f3 ft = tipForce(dp, p.k, p.d0);
+            f = mk3(f.x + ft.x, f.y + ft.y, f.z + ft.z);
+            f.z = f.z + p.surf;
+        } else {
+            const f3 ft = rotMatT(tipForce(rotMat(dp, A, B, C), p.k, p.d0), A, B, C);
+            f = mk3(f.x + ft.x, f.y + ft.y, f.z + ft.z);
+            f = mk3(f.x + fsurf.x, f.y + fsurf.y, f.z + fsurf.z);
+        }
         const float ff = dot3(f, f);
         bool conv = false, done = false;
         if (p.check_before && ff < p.f2conv) {
@@ -309,7 +335,7 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
         ++it;
         if (done) {
             const float4 fs = (p.check_before && conv) ? fe : sample(pos.x, pos.y, pos.z);
-            const f3 fr = rotMat(mk3(fs.x, fs.y, fs.z), A, B, C);
+            const f3 fr = IDF ? mk3(fs.x, fs.y, fs.z) : rotMat(mk3(fs.x, fs.y, fs.z), A, B, C);
             const float4 feo = make_float4(fr.x, fr.y, fr.z, fs.w);
             const bool fin = finite4(fs) && isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z);
             const float4 ppo = make_float4(pos.x, pos.y, pos.z, (conv && fin) ? (float)it : -(float)it);
@@ -343,8 +369,13 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
                 pend = 0;
             }
             ++o;
-            tip = mk3(tip.x + dTip.x, tip.y + dTip.y, tip.z + dTip.z);
-            pos = mk3(pos.x + dTip.x, pos.y + dTip.y, pos.z + dTip.z);
+            if constexpr (IDF) {
+                tip.z = tip.z + p.dtip;
+                pos.z = pos.z + p.dtip;
+            } else {
+                tip = mk3(tip.x + dTip.x, tip.y + dTip.y, tip.z + dTip.z);
+                pos = mk3(pos.x + dTip.x, pos.y + dTip.y, pos.z + dTip.z);
+            }
             v = mk3(0.f, 0.f, 0.f);
             it = 0;
             if (++iz >= p.nz) break;
@@ -373,15 +404,15 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
     }
 }
 
-template <bool STAGE = true>
+template <bool STAGE = true, bool FAST = false>
 __device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, const TiltArgs& p,
                                                   float4* __restrict__ FEs, float4* __restrict__ PPs,
                                                   const DfOut* dfo = nullptr, float* __restrict__ dfrow = nullptr,
                                                   float* __restrict__ fz_lds = nullptr) {
     Cell cell;
     cell.i = -1; cell.j = -1; cell.k = -1;
-    auto smp = [&](float x, float y, float z) { return sample(g, cell, x, y, z); };
-    relax_tilted_lane_src<1, decltype(smp), STAGE>(smp, start, p, FEs, PPs, 0, dfo, dfrow, fz_lds);
+    auto smp = [&](float x, float y, float z) { return sample<FAST>(g, cell, x, y, z); };
+    relax_tilted_lane_src<1, decltype(smp), STAGE, FAST>(smp, start, p, FEs, PPs, 0, dfo, dfrow, fz_lds);
 }
 
 }  // namespace rc
