@@ -56,7 +56,7 @@ __global__ __launch_bounds__(64) void k_relax_tilted(GridDev g, const float4* __
         relax_tilted_lane<true, FAST>(g, mk3(P.x, P.y, P.z), p, FEs + o, PPs ? PPs + o : nullptr, dp, row, col);
     } else {
         GroupCell<G> cell;
-        cell.i = -1; cell.j = -1; cell.k = -1;
+        cell.reset();
         auto smp = [&](float x, float y, float z) { return sample_group<G, FAST>(g, cell, sub, x, y, z); };
         relax_tilted_lane_src<G, decltype(smp), true, FAST>(smp, mk3(P.x, P.y, P.z), p, FEs + o,
                                                             PPs ? PPs + o : nullptr, sub, dp, row, col);
@@ -155,12 +155,12 @@ __global__ __launch_bounds__(64) void k_relax_strokes(GridDev g, const float4* _
     if (ip < 0) return;                       // whole groups leave together
     if constexpr (G == 1) {
         Cell cell;
-        cell.i = -1; cell.j = -1; cell.k = -1;
+        cell.reset();
         auto smp = [&](float x, float y, float z) { return sample(g, cell, x, y, z); };
         strokes_lane(smp, ip, pts, p, FEs, disps, PPs, true);
     } else {
         GroupCell<G> cell;
-        cell.i = -1; cell.j = -1; cell.k = -1;
+        cell.reset();
         auto smp = [&](float x, float y, float z) { return sample_group<G>(g, cell, sub, x, y, z); };
         strokes_lane(smp, ip, pts, p, FEs, disps, PPs, sub == 0);
     }
@@ -222,12 +222,12 @@ __global__ __launch_bounds__(64) void k_relax_strokes_2d(GridDev g, const float4
     if (ip < 0) return;                       // whole groups leave together
     if constexpr (G == 1) {
         Cell cell;
-        cell.i = -1; cell.j = -1; cell.k = -1;
+        cell.reset();
         auto smp = [&](float x, float y, float z) { return sample(g, cell, x, y, z); };
         strokes2d_lane(smp, ip, pts, p, FEs, disps, PPs, true);
     } else {
         GroupCell<G> cell;
-        cell.i = -1; cell.j = -1; cell.k = -1;
+        cell.reset();
         auto smp = [&](float x, float y, float z) { return sample_group<G>(g, cell, sub, x, y, z); };
         strokes2d_lane(smp, ip, pts, p, FEs, disps, PPs, sub == 0);
     }
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(64) void k_fe_in_strokes(GridDev g, const float4* _
     const float4 P = pts[ip];
     f3 pos = mk3(P.x + p.d0[0], P.y + p.d0[1], P.z + p.d0[2]);
     Cell cell;
-    cell.i = -1; cell.j = -1; cell.k = -1;
+    cell.reset();
     for (int iz = 0; iz < p.nz; ++iz) {
         const float4 fe = sample(g, cell, pos.x, pos.y, pos.z);
         const f3 fr = rotMat(mk3(fe.x, fe.y, fe.z), A, B, C);
@@ -290,7 +290,7 @@ __device__ __forceinline__ f3 tip_anchor(const float4* pts, const StateArgs& p, 
 }
 
 // REPEAT addressing of one axis in FP64: the cell (i0, i1) and the weight a of the upper corner,
-// the texel-centre convention of axis_repeat without its FP32 rounding of the coordinate.
+// the texel-centre convention of axis_frac / axis_index without their FP32 rounding of the coordinate.
 __device__ __forceinline__ void axis_repeat_d(double s, int n, int& i0, int& i1, double& a) {
     const double u = (s - floor(s)) * (double)n;
     const double t = u - 0.5;

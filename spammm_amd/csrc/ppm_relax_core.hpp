@@ -23,17 +23,26 @@ struct GridDev {
     int* flag;
 };
 
+// The cached cell is named by the clamped floor values of the three axes (axis_frac): the corner
+// indices are a function of them (axis_index), so they are only computed when one changes.
 struct Cell {
-    int i, j, k;       // cached cell origin (i0,j0,k0); -1 = empty
+    float fi, fj, fk;  // clamped floors of the cached cell; -2 = empty
     float4 c[8];       // corners 000,100,010,110,001,101,011,111
+    __device__ void reset() { fi = fj = fk = -2.0f; }
 };
 
-__device__ __forceinline__ void axis_repeat(float s, int n, int& i0, int& i1, float& a) {
+// REPEAT addressing of one axis: the interpolation weight a and the clamped floor fl of the
+// texel coordinate (fl in [-1, n]; non-finite coordinates are kept in range).
+__device__ __forceinline__ float axis_frac(float s, int n, float& fl) {
     const float u = (s - floorf(s)) * (float)n;
     const float t = u - 0.5f;
-    float fl = floorf(t);
-    a = t - fl;
-    fl = fminf(fmaxf(fl, -1.0f), (float)n);   // keeps non-finite coordinates in range
+    const float f = floorf(t);
+    fl = fminf(fmaxf(f, -1.0f), (float)n);
+    return t - f;
+}
+
+// The two texel indices of the cell with clamped floor fl.
+__device__ __forceinline__ void axis_index(float fl, int n, int& i0, int& i1) {
     int i = (int)fl;
     int j = i + 1;
     if (i < 0) i += n;
@@ -77,13 +86,14 @@ template <bool AX = false>
 __device__ __forceinline__ float4 sample(const GridDev& g, Cell& C, float px, float py, float pz) {
     float sa, sb, sc;
     tex_coords<AX>(g, px, py, pz, sa, sb, sc);
-    int i0, i1, j0, j1, k0, k1;
-    float a, b, c;
-    axis_repeat(sa, g.nx, i0, i1, a);
-    axis_repeat(sb, g.ny, j0, j1, b);
-    axis_repeat(sc, g.nz, k0, k1, c);
-    if (i0 != C.i || j0 != C.j || k0 != C.k) {
-        C.i = i0; C.j = j0; C.k = k0;
+    float fa, fb, fc;
+    const float a = axis_frac(sa, g.nx, fa), b = axis_frac(sb, g.ny, fb), c = axis_frac(sc, g.nz, fc);
+    if (fa != C.fi || fb != C.fj || fc != C.fk) {
+        C.fi = fa; C.fj = fb; C.fk = fc;
+        int i0, i1, j0, j1, k0, k1;
+        axis_index(fa, g.nx, i0, i1);
+        axis_index(fb, g.ny, j0, j1);
+        axis_index(fc, g.nz, k0, k1);
         k0 -= g.klo;
         k1 -= g.klo;
         if ((unsigned)k0 >= (unsigned)g.nzw || (unsigned)k1 >= (unsigned)g.nzw) {
@@ -109,8 +119,9 @@ __device__ __forceinline__ float4 sample(const GridDev& g, Cell& C, float px, fl
 // they take the same branches and every shuffle reads an active lane.
 template <int G>
 struct GroupCell {
-    int i, j, k;
+    float fi, fj, fk;
     float c[4 / G][8];
+    __device__ void reset() { fi = fj = fk = -2.0f; }
 };
 
 template <int G, bool AX = false>
@@ -119,13 +130,14 @@ __device__ __forceinline__ float4 sample_group(const GridDev& g, GroupCell<G>& C
     constexpr int NC = 4 / G;
     float sa, sb, sc;
     tex_coords<AX>(g, px, py, pz, sa, sb, sc);
-    int i0, i1, j0, j1, k0, k1;
-    float a, b, c;
-    axis_repeat(sa, g.nx, i0, i1, a);
-    axis_repeat(sb, g.ny, j0, j1, b);
-    axis_repeat(sc, g.nz, k0, k1, c);
-    if (i0 != C.i || j0 != C.j || k0 != C.k) {
-        C.i = i0; C.j = j0; C.k = k0;
+    float fa, fb, fc;
+    const float a = axis_frac(sa, g.nx, fa), b = axis_frac(sb, g.ny, fb), c = axis_frac(sc, g.nz, fc);
+    if (fa != C.fi || fb != C.fj || fc != C.fk) {
+        C.fi = fa; C.fj = fb; C.fk = fc;
+        int i0, i1, j0, j1, k0, k1;
+        axis_index(fa, g.nx, i0, i1);
+        axis_index(fb, g.ny, j0, j1);
+        axis_index(fc, g.nz, k0, k1);
         k0 -= g.klo;
         k1 -= g.klo;
         if ((unsigned)k0 >= (unsigned)g.nzw || (unsigned)k1 >= (unsigned)g.nzw) {
@@ -410,7 +422,7 @@ __device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, co
                                                   const DfOut* dfo = nullptr, float* __restrict__ dfrow = nullptr,
                                                   float* __restrict__ fz_lds = nullptr) {
     Cell cell;
-    cell.i = -1; cell.j = -1; cell.k = -1;
+    cell.reset();
     auto smp = [&](float x, float y, float z) { return sample<FAST>(g, cell, x, y, z); };
     relax_tilted_lane_src<1, decltype(smp), STAGE, FAST>(smp, start, p, FEs, PPs, 0, dfo, dfrow, fz_lds);
 }
