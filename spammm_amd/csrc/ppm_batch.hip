@@ -136,8 +136,12 @@ __device__ void do_relax_item(const PpmMolDesc& d, const PpmBatchParams& P, int 
     const size_t o = pix * P.nz;
     const rc::TiltArgs t = tilt_args(P);
     const rc::DfOut dfo{dfW, dfStart, P.df_taps, P.df_nz_out, false};
-    rc::relax_tilted_lane<false>(g, start, t, FEs + o, PPs ? PPs + o : nullptr, df ? &dfo : nullptr,
-                          df ? df + pix * P.df_nz_out : nullptr);
+    float4* pp = PPs ? PPs + o : nullptr;
+    float* dr = df ? df + pix * P.df_nz_out : nullptr;
+    if (rc::fast_frame(t, g))           // the same value in every lane of the item
+        rc::relax_tilted_lane<false, true>(g, start, t, FEs + o, pp, df ? &dfo : nullptr, dr);
+    else
+        rc::relax_tilted_lane<false, false>(g, start, t, FEs + o, pp, df ? &dfo : nullptr, dr);
 }
 
 // OFFQ: the tip has charge sites at a nonzero z offset (compiled out when it has none).

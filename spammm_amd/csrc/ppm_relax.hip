@@ -63,12 +63,6 @@ __global__ __launch_bounds__(64) void k_relax_tilted(GridDev g, const float4* __
     }
 }
 
-// The tip frame is the identity and the grid's inverse cell has no off-diagonal terms.
-bool fast_frame(const TiltArgs& p, const GridDev& g) {
-    return p.A[0] == 1.f && p.A[1] == 0.f && p.A[2] == 0.f && p.B[0] == 0.f && p.B[1] == 1.f && p.B[2] == 0.f &&
-           p.C[0] == 0.f && p.C[1] == 0.f && p.C[2] == 1.f && g.da[1] == 0.f && g.da[2] == 0.f && g.db[0] == 0.f &&
-           g.db[2] == 0.f && g.dc[0] == 0.f && g.dc[1] == 0.f;
-}
 
 // Lanes per scan point of k_relax_tilted / k_relax_strokes: the largest G <= 4 that keeps the
 // launch within about 2 waves per SIMD (1024 SIMDs).  A group replicates the per-point FIRE and

@@ -416,6 +416,14 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
     }
 }
 
+// The tip frame is the identity and the grid's inverse cell has no off-diagonal terms: the
+// relaxation may take its FAST form (same results).
+__host__ __device__ inline bool fast_frame(const TiltArgs& p, const GridDev& g) {
+    return p.A[0] == 1.f && p.A[1] == 0.f && p.A[2] == 0.f && p.B[0] == 0.f && p.B[1] == 1.f && p.B[2] == 0.f &&
+           p.C[0] == 0.f && p.C[1] == 0.f && p.C[2] == 1.f && g.da[1] == 0.f && g.da[2] == 0.f && g.db[0] == 0.f &&
+           g.db[2] == 0.f && g.dc[0] == 0.f && g.dc[1] == 0.f;
+}
+
 template <bool STAGE = true, bool FAST = false>
 __device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, const TiltArgs& p,
                                                   float4* __restrict__ FEs, float4* __restrict__ PPs,
