@@ -23,27 +23,27 @@ struct GridDev {
     int* flag;
 };
 
-// The cached cell is named by the clamped floor values of the three axes (axis_frac): the corner
-// indices are a function of them (axis_index), so they are only computed when one changes.
+// The cached cell is named by the floor values of the three axes (axis_frac): the corner indices
+// are a function of them (axis_index), so they are only computed when one changes.
 struct Cell {
-    float fi, fj, fk;  // clamped floors of the cached cell; -2 = empty
+    float fi, fj, fk;  // floors of the cached cell; -2 = empty
     float4 c[8];       // corners 000,100,010,110,001,101,011,111
     __device__ void reset() { fi = fj = fk = -2.0f; }
 };
 
-// REPEAT addressing of one axis: the interpolation weight a and the clamped floor fl of the
-// texel coordinate (fl in [-1, n]; non-finite coordinates are kept in range).
+// REPEAT addressing of one axis: the interpolation weight a and the floor fl of the texel
+// coordinate (in [-1, n - 1] for a finite coordinate).
 __device__ __forceinline__ float axis_frac(float s, int n, float& fl) {
     const float u = (s - floorf(s)) * (float)n;
     const float t = u - 0.5f;
-    const float f = floorf(t);
-    fl = fminf(fmaxf(f, -1.0f), (float)n);
-    return t - f;
+    fl = floorf(t);
+    return t - fl;
 }
 
-// The two texel indices of the cell with clamped floor fl.
+// The two texel indices of the cell with floor fl (clamped first, which keeps non-finite
+// coordinates in range).
 __device__ __forceinline__ void axis_index(float fl, int n, int& i0, int& i1) {
-    int i = (int)fl;
+    int i = (int)fminf(fmaxf(fl, -1.0f), (float)n);
     int j = i + 1;
     if (i < 0) i += n;
     if (j > n - 1) j -= n;
