@@ -733,7 +733,7 @@ def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False, fd=True):
     from spammm_amd import diff, gradcheck as GC, params as P, workloads as WL
     w = WL.flake_differentiable(z0)
     prob = diff.Problem.from_workload(w, dev, max_iter=1000, f2conv=1e-12, check_before=True, polish=3,
-                                      strict=False, gate_eig_frac=0.5, gate_max_jump=0.3)
+                                      strict=False, gate_eig_frac=0.5, gate_max_jump=0.3, window_margin=(2.0, 1.0))
     R0, E0 = P.mix_R0E0(w.enames, P.ELEMENTS_BENCH, 1.66, 0.0091)
     R0, E0 = torch.tensor(R0, device=dev), torch.tensor(E0, device=dev)
     xyz0 = torch.tensor(w.atoms[:, :3], dtype=torch.float64, device=dev)
@@ -766,9 +766,9 @@ def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False, fd=True):
               "columns": int(cols.numel())}
     with torch.no_grad():
         atoms = torch.cat([xyz0.float(), q0.float()[:, None]], 1).contiguous()
-        _, _, PPs, _ = diff._forward_parts(atoms, diff.lj_cpar(R0, E0), prob)
+        _, _, PPs, _, run_spec = diff._forward_parts(atoms, diff.lj_cpar(R0, E0), prob)
     nimg = (2 * prob.npbc[0] + 1) * (2 * prob.npbc[1] + 1)
-    adj_pairs, adj_octets = grid_adjoint_active_pairs(prob.spec, PPs, xyz0.shape[0] * nimg, cols)
+    adj_pairs, adj_octets = grid_adjoint_active_pairs(run_spec, PPs, xyz0.shape[0] * nimg, cols)
     status = dict(prob.last_status)
     # central differences of the FP64 device forward on 32 random parameters (seed 3), same gated loss
     na = xyz0.shape[0]
@@ -799,6 +799,7 @@ def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False, fd=True):
         "tip_z": [z0, z0 + (w.nz - 1) * float(w.dtip)], "probe_z_rest": [z0 - 4.0, z0 - 4.0 + (w.nz - 1) * float(w.dtip)],
         "forward_ms": tf, "adjoint_ms": tb, "adjoint_over_forward": tb / tf,
         "pixel_z_steps_per_s_forward": w.n_scan * w.nz / (tf * 1e-3),
+        "grid_run": list(run_spec.n), "grid_full": list(prob.spec.n), "window_fallbacks": prob.window_fallbacks,
         "adjoint_status": status, "gating": gating, "fd_fp64": fdres,
         "grid_adjoint_work": {"octets": adj_octets, "pair_evals": adj_pairs,
                               "flop": adj_pairs * ADJ_FLOP_PER_PAIR, "flop_per_pair": ADJ_FLOP_PER_PAIR},

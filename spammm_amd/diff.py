@@ -64,6 +64,15 @@ class Problem:
     last_columns: object = None
     last_gate: dict = field(default_factory=dict)
     last_status: dict = field(default_factory=dict)
+    # Scan window (off when window_margin is None): the forward projects the grid only on the
+    # sub-box of its voxels that holds every probe rest position widened by (lateral, vertical)
+    # margins in A (poses.scan_window); a forward whose probes come within 0.5 A of a cut face is
+    # recomputed on the whole grid (window_fallbacks counts them).  last_spec: the grid the last
+    # forward ran on (its adjoint runs there too).
+    window_margin: tuple = None
+    window_fallbacks: int = 0
+    last_spec: object = None
+    _window: object = None
 
     @property
     def n_scan(self):
@@ -72,6 +81,18 @@ class Problem:
     @property
     def dz(self):
         return float(self.tip.tipC[3])
+
+    def run_spec(self):
+        """(spec, faces) of the grid the forward runs on first: the scan window and its
+        (lo, hi) shared-face flags, or (spec, None) without a window."""
+        if self._window is None:
+            self._window = False
+            if self.window_margin is not None:
+                from .poses import scan_window
+                w = scan_window(self.spec, self.pts.cpu().numpy(), self.nz, self.tip, *self.window_margin)
+                if w is not None:
+                    self._window = (w[0], (w[1], w[2]))
+        return self._window if self._window else (self.spec, None)
 
     @classmethod
     def from_workload(cls, w, device, **kw):
@@ -93,11 +114,12 @@ def lj_cpar(R0, E0):
     return torch.stack([2.0 * E0d * r6, E0d * r6 * r6], dim=1).float().contiguous()
 
 
-def _relax(grid, prob: Problem):
-    FEs, PPs = D.relax_tilted(grid, prob.spec, prob.pts, prob.nz, prob.tip, prob.max_iter, prob.f2conv,
+def _relax(grid, prob: Problem, spec=None):
+    spec = prob.spec if spec is None else spec
+    FEs, PPs = D.relax_tilted(grid, spec, prob.pts, prob.nz, prob.tip, prob.max_iter, prob.f2conv,
                               prob.check_before, scan_ny=int(prob.nxy[1]))
     if prob.polish > 0:
-        D.relax_polish(grid, prob.spec, prob.pts, prob.nz, prob.tip, PPs, FEs, prob.polish)
+        D.relax_polish(grid, spec, prob.pts, prob.nz, prob.tip, PPs, FEs, prob.polish)
     return FEs, PPs
 
 
@@ -125,9 +147,12 @@ def column_gate(grid, spec, pts, nxy, nz, tip, PPs, eig_frac, max_jump):
     return keep.reshape(int(nxy[0]), int(nxy[1])), stats
 
 
-def stable_columns(grid, PPs, prob: Problem):
-    """column_gate for a Problem (its gate_eig_frac, gate_max_jump)."""
-    return column_gate(grid, prob.spec, prob.pts, prob.nxy, prob.nz, prob.tip, PPs, prob.gate_eig_frac,
+def stable_columns(grid, PPs, prob: Problem, spec=None):
+    """column_gate for a Problem (its gate_eig_frac, gate_max_jump) on `grid` of `spec` (default:
+    the grid of the last forward)."""
+    if spec is None:
+        spec = prob.last_spec if prob.last_spec is not None else prob.spec
+    return column_gate(grid, spec, prob.pts, prob.nxy, prob.nz, prob.tip, PPs, prob.gate_eig_frac,
                        prob.gate_max_jump)
 
 
@@ -142,14 +167,27 @@ def masked_l2(df, target, columns=None):
 
 
 def _forward_parts(atoms, cpar, prob: Problem):
-    grid = D.grid_project(atoms, cpar, _lib.PPM_MODEL_LJ, prob.spec, prob.Qs, prob.QZs, prob.npbc, prob.lvA,
-                          prob.lvB, cap=prob.cap, acc64=prob.grid_acc64)
-    FEs, PPs = _relax(grid, prob)
+    """(grid, FEs, PPs, df (nx, ny, nz), spec): the forward on the scan window, or on the whole
+    grid without one or when the probes leave it; spec is the grid it ran on."""
+    def run(spec):
+        grid = D.grid_project(atoms, cpar, _lib.PPM_MODEL_LJ, spec, prob.Qs, prob.QZs, prob.npbc, prob.lvA,
+                              prob.lvB, cap=prob.cap, acc64=prob.grid_acc64)
+        return (grid,) + _relax(grid, prob, spec)
+
+    spec, faces = prob.run_spec()
+    grid, FEs, PPs = run(spec)
+    if faces is not None:
+        from .poses import window_inside
+        if not window_inside(spec, faces, PPs):
+            prob.window_fallbacks += 1
+            spec = prob.spec
+            grid, FEs, PPs = run(spec)
+    prob.last_spec = spec
     if prob.gate_eig_frac > 0:
-        prob.last_columns, prob.last_gate = stable_columns(grid, PPs, prob)
+        prob.last_columns, prob.last_gate = stable_columns(grid, PPs, prob, spec)
     op = DF.z_operator(prob.df_kind, prob.nz, prob.dz, atoms.device, prob.amp, prob.f0, prob.k_cant)
     df = op.apply(FEs, (prob.nz * 4, 4, 0), prob.n_scan, 1, offset=2)
-    return grid, FEs, PPs, df.reshape(int(prob.nxy[0]), int(prob.nxy[1]), prob.nz)
+    return grid, FEs, PPs, df.reshape(int(prob.nxy[0]), int(prob.nxy[1]), prob.nz), spec
 
 
 class DfImage:
@@ -171,8 +209,9 @@ def _make_function():
         def forward(ctx, xyz, q, R0, E0, prob):
             atoms = torch.cat([xyz.detach().float(), q.detach().float()[:, None]], dim=1).contiguous()
             cpar = lj_cpar(R0.detach(), E0.detach())
-            grid, FEs, PPs, df = _forward_parts(atoms, cpar, prob)
+            grid, FEs, PPs, df, spec = _forward_parts(atoms, cpar, prob)
             ctx.prob = prob
+            ctx.spec = spec
             ctx.dtypes = (xyz.dtype, q.dtype, R0.dtype, E0.dtype)
             ctx.save_for_backward(atoms, cpar, grid, R0.detach(), E0.detach(),
                                   PPs if not prob.recompute else torch.empty(0, device=xyz.device))
@@ -183,21 +222,22 @@ def _make_function():
             prob = ctx.prob
             atoms, cpar, grid, R0, E0, PPs = ctx.saved_tensors
             dev = atoms.device
+            spec = ctx.spec
             if prob.recompute:
-                _, PPs = _relax(grid, prob)
+                _, PPs = _relax(grid, prob, spec)
             g = gdf.detach().float().contiguous().reshape(prob.n_scan, prob.nz)
             opT = DF.z_operator(prob.df_kind, prob.nz, prob.dz, dev, prob.amp, prob.f0, prob.k_cant, transpose=True)
             U = torch.zeros((prob.n_scan, prob.nz, 4), dtype=torch.float32, device=dev)
             u_z = opT.apply(g, (prob.nz, 1, 0), prob.n_scan, 1)
             U[..., 2] = u_z[..., 0]
-            gG, lam = D.relax_adjoint(grid, prob.spec, prob.pts, prob.nz, prob.tip, PPs, U)
+            gG, lam = D.relax_adjoint(grid, spec, prob.pts, prob.nz, prob.tip, PPs, U)
             st = lam[..., 3].to(torch.int32)
             prob.last_status = {"states": int(st.numel()), "gated": int((st != 0).sum()),
                                 "residual": int(((st & 1) != 0).sum()), "nonfinite": int(((st & 2) != 0).sum()),
                                 "singular": int(((st & 4) != 0).sum()), "unstable": int(((st & 8) != 0).sum())}
             if prob.strict and prob.last_status["nonfinite"]:
                 raise RuntimeError(f"relax adjoint: {prob.last_status['nonfinite']} states non-finite")
-            g6 = D.grid_adjoint(atoms, cpar, _lib.PPM_MODEL_LJ, prob.spec, prob.Qs, prob.QZs, grid, gG, prob.npbc,
+            g6 = D.grid_adjoint(atoms, cpar, _lib.PPM_MODEL_LJ, spec, prob.Qs, prob.QZs, grid, gG, prob.npbc,
                                 prob.lvA, prob.lvB, cap=prob.cap)
             R0d, E0d = R0.double(), E0.double()
             r5 = R0d ** 5

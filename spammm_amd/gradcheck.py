@@ -36,6 +36,8 @@ class Fp64Forward:
         r6 = R0.double() ** 6
         self.cpar = torch.stack([2.0 * E0.double() * r6, E0.double() * r6 * r6], 1).contiguous()
         self.f32_anchors, self.max_iter, self.f2conv = bool(f32_anchors), int(max_iter), float(f2conv)
+        # the grid the FP32 forward ran on (its scan window, if it has one)
+        self.spec = prob.last_spec if prob.last_spec is not None else prob.run_spec()[0]
         self._grid = None
 
     def df(self, xyz, q):
@@ -43,9 +45,9 @@ class Fp64Forward:
         import torch
         p = self.prob
         atoms = torch.cat([xyz.double(), q.double()[:, None]], 1).contiguous()
-        self._grid = D.grid_project_f64(atoms, self.cpar, 0, p.spec, p.Qs, p.QZs, p.npbc, p.lvA, p.lvB, cap=p.cap,
+        self._grid = D.grid_project_f64(atoms, self.cpar, 0, self.spec, p.Qs, p.QZs, p.npbc, p.lvA, p.lvB, cap=p.cap,
                                         out=self._grid)
-        FEs, _ = D.relax_tilted_f64(self._grid, p.spec, self.pts64, p.nz, p.tip, max_iter=self.max_iter,
+        FEs, _ = D.relax_tilted_f64(self._grid, self.spec, self.pts64, p.nz, p.tip, max_iter=self.max_iter,
                                     f2conv=self.f2conv, check_before=True, want_pps=False,
                                     f32_anchors=self.f32_anchors)
         return FEs[..., 2] @ self.M.T

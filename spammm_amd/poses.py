@@ -132,26 +132,33 @@ class Scene:
         return self.spec, self.full[0], self.full[1], self.full[2]
 
     def inside(self, PPs, guard=0.5, guard_shared=2) -> bool:
-        """True if every probe position of PPs samples the window exactly as it samples the full
-        grid: at least `guard` A inside each window face that cuts the full grid (so the FIRE
-        steps between stored positions stay on the window's voxels), and `guard_shared` voxels
-        inside a face the window shares with the full grid (where both would wrap, each to its
-        own opposite face)."""
+        """window_inside for the scene's window (True without one)."""
         if not self.windowed:
             return True
-        sp = self.spec_run
-        d = np.array([sp.dA[0], sp.dB[1], sp.dC[2]], np.float64)
-        shared_lo, shared_hi = self.window_faces
-        g_lo = np.where(shared_lo, float(guard_shared), np.ceil(guard / d))
-        g_hi = np.where(shared_hi, float(guard_shared), np.ceil(guard / d))
-        import torch
-        mn, mx = torch.aminmax(PPs.reshape(-1, 4), dim=0)
-        lo = mn[:3].cpu().numpy().astype(np.float64)
-        hi = mx[:3].cpu().numpy().astype(np.float64)
-        ulo = (lo - sp.p0.astype(np.float64)) / d
-        uhi = (hi - sp.p0.astype(np.float64)) / d
-        return bool(np.isfinite(ulo).all() and np.isfinite(uhi).all() and (ulo >= g_lo).all()
-                    and (uhi < np.asarray(sp.n, np.float64) - 1 - g_hi).all())
+        return window_inside(self.spec_run, self.window_faces, PPs, guard, guard_shared)
+
+
+def window_inside(spec_run, faces, PPs, guard=0.5, guard_shared=2) -> bool:
+    """True if every probe position of PPs samples the window `spec_run` exactly as it samples the
+    full grid: at least `guard` A inside each window face that cuts the full grid (so the FIRE steps
+    between stored positions stay on the window's voxels), and `guard_shared` voxels inside a face
+    the window shares with the full grid (where both would wrap, each to its own opposite face).
+    An axis whose two faces are both shared is the full grid's axis: both wrap alike, no guard.
+    faces: (lo, hi) bool[3], True where the window face is a face of the full grid."""
+    import torch
+    sp = spec_run
+    d = np.array([sp.dA[0], sp.dB[1], sp.dC[2]], np.float64)
+    shared_lo, shared_hi = (np.asarray(f, bool) for f in faces)
+    whole = shared_lo & shared_hi
+    g_lo = np.where(shared_lo, float(guard_shared), np.ceil(guard / d))
+    g_hi = np.where(shared_hi, float(guard_shared), np.ceil(guard / d))
+    mn, mx = torch.aminmax(PPs.reshape(-1, 4), dim=0)
+    lo = mn[:3].cpu().numpy().astype(np.float64)
+    hi = mx[:3].cpu().numpy().astype(np.float64)
+    ulo = (lo - sp.p0.astype(np.float64)) / d
+    uhi = (hi - sp.p0.astype(np.float64)) / d
+    ok = whole | ((ulo >= g_lo) & (uhi < np.asarray(sp.n, np.float64) - 1 - g_hi))
+    return bool(np.isfinite(ulo).all() and np.isfinite(uhi).all() and ok.all())
 
 
 def scan_window(spec: D.GridSpec, pts, nz: int, tip: D.Tip, margin_xy: float, margin_z: float):

@@ -9,7 +9,13 @@ stable (diff.stable_columns: smallest Hessian eigenvalue >= 0.5 k_lat at every s
 jump >= 0.3 A between slices, no state the adjoint gates) in the forward at the evaluated
 parameters and in the target's forward; the mask is held fixed for the finite differences.
 
+The forward runs on the scan window: the z planes of the probe heights +- 1 A (59 of 240; the
+scan covers the whole periodic x, y cell), the grid's own voxels.
+
 Bars:
+  * the window gives the whole grid's df (rtol 1e-4) and the same gradient (1e-3 of its norm) on
+    the columns stable on both (near a fold the FP32 rounding of the window's origin may put a probe
+    on the other branch);
   * adjoint vs central differences of the FP64 device forward (gradcheck.Fp64Forward) on 32
     random parameters (seed 3; positions and charges): within 1e-3 relative for at least 31 of 32
     by the reference's plateau rule (best-agreeing step of h = 1e-3, 1e-4, 1e-5 A / 1e-4, 1e-5,
@@ -31,11 +37,11 @@ from spammm_amd import params as P        # noqa: E402
 from spammm_amd import workloads as W     # noqa: E402
 
 
-def c3_case(tip_z0=10.0):
+def c3_case(tip_z0=10.0, window=(2.0, 1.0)):
     dev = D.require_gpu()
     w = W.flake_differentiable(tip_z0)
     prob = diff.Problem.from_workload(w, dev, max_iter=1000, f2conv=1e-12, check_before=True, polish=3, strict=False,
-                                      gate_eig_frac=0.5, gate_max_jump=0.3)
+                                      gate_eig_frac=0.5, gate_max_jump=0.3, window_margin=window)
     R0, E0 = P.mix_R0E0(w.enames, P.ELEMENTS_BENCH, 1.66, 0.0091)
     R0, E0 = torch.tensor(R0, device=dev), torch.tensor(E0, device=dev)
     xyz0 = torch.tensor(w.atoms[:, :3], dtype=torch.float64, device=dev)
@@ -45,6 +51,35 @@ def c3_case(tip_z0=10.0):
         target = diff.df_image(xyz_t, q0, R0, E0, prob).double()
     tcols = prob.last_columns.clone()
     return w, prob, R0, E0, xyz0, q0, xyz_t, target, tcols
+
+
+def test_c3_window_equals_full_grid():
+    """The forward on the scan window (z planes of the probe heights +- 1 A) gives the df of the
+    whole grid (the window's voxels are the whole grid's; only the FP32 rounding of its origin
+    differs), and the gradient of a loss over the columns stable on both within 1e-3 of its norm."""
+    out = []
+    for window in ((2.0, 1.0), None):
+        w, prob, R0, E0, xyz0, q0, xyz_t, target, tcols = c3_case(window=window)
+        x = xyz0.clone().requires_grad_(True)
+        q = q0.clone().requires_grad_(True)
+        dfi = diff.df_image(x, q, R0, E0, prob)
+        out.append((prob, dfi.detach(), prob.last_columns & tcols, x, q, target))
+    (pw, dw, cw, _, _, tw), (pf, df_, cf, _, _, tf) = out
+    both = cw & cf
+    m = both[..., None].expand_as(dw)
+    print(f"[c3 window] grid {pw.last_spec.n} of {pf.last_spec.n}, fallbacks {pw.window_fallbacks}; df max |d| "
+          f"{float((dw - df_)[m].abs().max()):.3e} on the stable columns, {float((dw - df_).abs().max()):.3e} on all "
+          f"(max |df| {float(df_.abs().max()):.3e})", flush=True)
+    assert pw.window_fallbacks == 0 and tuple(pw.last_spec.n) != tuple(pf.last_spec.n)
+    assert torch.allclose(dw[m], df_[m], rtol=1e-4, atol=1e-5 * float(df_.abs().max()))
+    grads = []
+    for prob, dfi0, _, x, q, target in out:
+        dfi = diff.df_image(x, q, R0, E0, prob)
+        diff.masked_l2(dfi, tf, both).backward()
+        grads.append(torch.cat([x.grad.flatten(), q.grad]))
+    rel = float((grads[0] - grads[1]).norm() / grads[1].norm())
+    print(f"[c3 window] columns stable on both {int(both.sum())}; gradient |d|/|g| {rel:.2e}", flush=True)
+    assert rel <= 1e-3
 
 
 def test_c3_adjoint_matches_fp64_fd_at_configured_heights():
