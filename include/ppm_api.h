@@ -529,12 +529,13 @@ size_t ppm_grid_adjoint_workspace_size(int na, const PpmGridSpec* spec);
  * with (p0,p1) = (c6,c12) for PPM_MODEL_LJ and (R0,E0) for PPM_MODEL_MORSE, summed over
  * periodic images.  grid is the forward output (used to chain through the |E|>100
  * rescale); the other arguments are those of the forward call.  workspace: device memory
- * of at least ppm_grid_adjoint_workspace_size(na, spec) bytes. */
+ * of at least ppm_grid_adjoint_workspace_size(na, spec) bytes.  params: 1 = all six columns;
+ * 0 = positions and charges only (columns 4, 5 written as 0; ~12 % less arithmetic). */
 int ppm_grid_adjoint(int model, const float* atoms, const float* cpar, int na,
                      const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
                      int npbc_a, int npbc_b, const float lvA[3], const float lvB[3], int cap,
                      const float* grid, const float* grid_cot, void* workspace,
-                     size_t workspace_bytes, double* grad, void* stream);
+                     size_t workspace_bytes, int params, double* grad, void* stream);
 
 /* ppm_grid_adjoint for a grid made by ppm_grid_project_add: gradients of `atoms` only; `base`
  * (the same base grid) enters the chain rule of the |E|>100 rescale of capped voxels. */
@@ -542,7 +543,7 @@ int ppm_grid_adjoint_add(int model, const float* atoms, const float* cpar, int n
                          const PpmGridSpec* spec, const float Qs[4], const float QZs[4],
                          int npbc_a, int npbc_b, const float lvA[3], const float lvB[3], int cap,
                          const float* base, const float* grid, const float* grid_cot,
-                         void* workspace, size_t workspace_bytes, double* grad, void* stream);
+                         void* workspace, size_t workspace_bytes, int params, double* grad, void* stream);
 
 /* Linear operator along z, one output per (pixel, z_out):
  *   out[p*nz_out + j] = sum_{t<taps} W[j*taps + t] * in[(p*nz_in + start[j] + t)*in_stride + in_offset]

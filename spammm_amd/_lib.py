@@ -134,7 +134,7 @@ def load():
     L.ppm_ewald_tip_grid.argtypes = [vp, sz, i, i, P(D2), P(D2), P(PpmGridSpec), _F4, _F4, i, vp, sz, vp, vp]
     L.ppm_grid_adjoint_add.restype = i
     L.ppm_grid_adjoint_add.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
-                                       P(ctypes.c_float * 3), i, vp, vp, vp, vp, ctypes.c_size_t, vp, vp]
+                                       P(ctypes.c_float * 3), i, vp, vp, vp, vp, ctypes.c_size_t, i, vp, vp]
     L.ppm_relax_strokes_tilted.restype = i
     L.ppm_relax_strokes_tilted.argtypes = [vp, P(PpmGridSpec), vp, i, i, P(PpmTip), P(PpmRelaxCtl), vp, vp, vp]
     L.ppm_relax_strokes.restype = i
@@ -183,7 +183,7 @@ def load():
     L.ppm_grid_adjoint_workspace_size.argtypes = [i, P(PpmGridSpec)]
     L.ppm_grid_adjoint.restype = i
     L.ppm_grid_adjoint.argtypes = [i, vp, vp, i, P(PpmGridSpec), _F4, _F4, i, i, P(ctypes.c_float * 3),
-                                   P(ctypes.c_float * 3), i, vp, vp, vp, ctypes.c_size_t, vp, vp]
+                                   P(ctypes.c_float * 3), i, vp, vp, vp, ctypes.c_size_t, i, vp, vp]
     _lib = L
     return L
 

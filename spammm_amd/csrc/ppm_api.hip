@@ -597,7 +597,7 @@ static int grid_adjoint_impl(const char* fn, int model, const float* atoms, cons
                              const PpmGridSpec* spec, const float Qs[4], const float QZs[4], int npbc_a, int npbc_b,
                              const float lvA[3], const float lvB[3], int cap, const float* grid,
                              const float* grid_cot, const float* base, void* workspace, size_t workspace_bytes,
-                             double* grad, void* stream) {
+                             int params, double* grad, void* stream) {
     g_err[0] = 0;
     if (int r = check_spec(fn, spec)) return r;
     if (model != PPM_MODEL_LJ && model != PPM_MODEL_MORSE) { set_error("%s: unknown model %d", fn, model); return PPM_ERR_INVALID_ARG; }
@@ -608,27 +608,28 @@ static int grid_adjoint_impl(const char* fn, int model, const float* atoms, cons
     }
     if (npbc_a < 0 || npbc_b < 0 || npbc_a > 64 || npbc_b > 64) { set_error("%s: image range out of [0,64]", fn); return PPM_ERR_INVALID_ARG; }
     if ((npbc_a > 0 && !lvA) || (npbc_b > 0 && !lvB)) { set_error("%s: images requested without lattice vectors", fn); return PPM_ERR_INVALID_ARG; }
+    if (params != 0 && params != 1) { set_error("%s: params must be 0 or 1, got %d", fn, params); return PPM_ERR_INVALID_ARG; }
     const size_t need = grid_adjoint_workspace(na, *spec);
     if (workspace_bytes < need) {
         set_error("%s: workspace of %zu bytes < required %zu (ppm_grid_adjoint_workspace_size)", fn, workspace_bytes, need);
         return PPM_ERR_INVALID_ARG;
     }
     return launch_grid_adjoint(model, atoms, cpar, na, *spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, grid, grid_cot,
-                               base, workspace, grad, (hipStream_t)stream);
+                               base, workspace, params, grad, (hipStream_t)stream);
 }
 
 int ppm_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
                      const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
                      const float lvB[3], int cap, const float* grid, const float* grid_cot, void* workspace,
-                     size_t workspace_bytes, double* grad, void* stream) {
+                     size_t workspace_bytes, int params, double* grad, void* stream) {
     return grid_adjoint_impl("ppm_grid_adjoint", model, atoms, cpar, na, spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap,
-                             grid, grid_cot, nullptr, workspace, workspace_bytes, grad, stream);
+                             grid, grid_cot, nullptr, workspace, workspace_bytes, params, grad, stream);
 }
 
 int ppm_grid_adjoint_add(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec* spec,
                          const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
                          const float lvB[3], int cap, const float* base, const float* grid, const float* grid_cot,
-                         void* workspace, size_t workspace_bytes, double* grad, void* stream) {
+                         void* workspace, size_t workspace_bytes, int params, double* grad, void* stream) {
     const char* fn = "ppm_grid_adjoint_add";
     if (!base) {
         g_err[0] = 0;
@@ -636,7 +637,7 @@ int ppm_grid_adjoint_add(int model, const float* atoms, const float* cpar, int n
         return PPM_ERR_INVALID_ARG;
     }
     return grid_adjoint_impl(fn, model, atoms, cpar, na, spec, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap, grid, grid_cot,
-                             base, workspace, workspace_bytes, grad, stream);
+                             base, workspace, workspace_bytes, params, grad, stream);
 }
 
 int ppm_z_stencil(const float* in, int in_stride, int in_offset, int n_pix, int nz_in, const float* W,

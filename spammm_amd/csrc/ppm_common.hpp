@@ -102,7 +102,7 @@ size_t grid_adjoint_workspace(int na, const PpmGridSpec& g);
 int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
                         const float lvB[3], int cap, const float* grid, const float* grid_cot, const float* base,
-                        void* workspace, double* grad, hipStream_t stream);
+                        void* workspace, int params, double* grad, hipStream_t stream);
 int launch_fe_in_strokes(const float* grid, const PpmGridSpec& g, const float* pts, int n_scan, int nz,
                          const PpmTip& tip, const float dTip[4], const float dpos0[4], float* FEs,
                          hipStream_t stream);

@@ -156,9 +156,10 @@ __device__ __forceinline__ void pair_fe_raw(float dx, float dy, float dz, float 
 
 // Contributions of one pair to the 6 per-atom accumulators: t[0..2] += d(g.(F,E))/d(voxel - atom)
 // (the caller subtracts them: the atom moves opposite to the voxel offset), t[3] += d/dq,
-// t[4], t[5] += d/d(c6, c12) (LJ) or d/d(R0, E0) (Morse).  OFFS: the tip has charge sites at
+// t[4], t[5] += d/d(c6, c12) (LJ) or d/d(R0, E0) (Morse), only when PAR (the caller wants the
+// pair-parameter gradient; without it they stay zero).  OFFS: the tip has charge sites at
 // nonzero z offsets.
-template <int MODEL, bool OFFS>
+template <int MODEL, bool OFFS, bool PAR>
 __device__ __forceinline__ void pair_adjoint(float dx, float dy, float dz, float dxy2, float q, float p0, float p1,
                                              float K, const GridArgs& a, float4 g, float t[6]) {
 #pragma clang fp contract(fast)
@@ -172,8 +173,10 @@ __device__ __forceinline__ void pair_adjoint(float dx, float dy, float dz, float
         const float t12 = p1 * ir6;
         fr = fmaf(12.f, t12, -6.f * p0) * ir8;
         dfr = fmaf(-84.f, t12, 24.f * p0) * (ir8 * ir2);
-        t[4] = fmaf(-ir6, fmaf(6.f, gdi, g.w), t[4]);                 // d/dc6
-        t[5] = fmaf(ir6 * ir6, fmaf(12.f, gdi, g.w), t[5]);           // d/dc12
+        if (PAR) {
+            t[4] = fmaf(-ir6, fmaf(6.f, gdi, g.w), t[4]);             // d/dc6
+            t[5] = fmaf(ir6 * ir6, fmaf(12.f, gdi, g.w), t[5]);       // d/dc12
+        }
     } else {
         const float r = s * ir;
         const float ex = __expf(K * (r - p0));
@@ -184,8 +187,10 @@ __device__ __forceinline__ void pair_adjoint(float dx, float dy, float dz, float
         dfr = dfr_dr * 0.5f * ir;
         const float dE_dR0 = -2.f * K * A * (ex - 1.f);
         const float dfr_dR0 = 2.f * K * K * A * (2.f * ex - 1.f) * ir;
-        t[4] += gd * dfr_dR0 + g.w * dE_dR0;                                   // d/dR0
-        t[5] += gd * (-2.f * K * ex * (ex - 1.f) * ir) + g.w * ex * (ex - 2.f); // d/dE0
+        if (PAR) {
+            t[4] += gd * dfr_dR0 + g.w * dE_dR0;                                   // d/dR0
+            t[5] += gd * (-2.f * K * ex * (ex - 1.f) * ir) + g.w * ex * (ex - 2.f); // d/dE0
+        }
     }
     if (a.q0 != 0.f) {
         const float ec = q * a.q0 * ir;          // Coulomb at z offset 0 shares d with the pair law
@@ -215,7 +220,7 @@ __device__ __forceinline__ void pair_adjoint(float dx, float dy, float dz, float
     }
 }
 
-template <int MODEL, int V, bool DCZ, bool OFFS>
+template <int MODEL, int V, bool DCZ, bool OFFS, bool PAR>
 __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__ atoms, const float* __restrict__ cpar,
                                                       GridArgs a, const float4* __restrict__ grid,
                                                       const float4* __restrict__ gcot, const float4* __restrict__ base,
@@ -317,6 +322,7 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
     }
 
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int NC = PAR ? 6 : 4;      // accumulated columns: xyz, q (+ the pair parameters)
     for (int j = j_begin; j < j_end; ++j) {
         const float4 at = atoms[j];
         const float p0 = cpar[j * cs], p1 = cpar[j * cs + 1], K = MODEL == PPM_MODEL_LJ ? 0.f : cpar[j * cs + 2];
@@ -335,16 +341,16 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
                 for (int k = 0; k < V; ++k) {
                     const int kk = DCZ ? 0 : k;
                     const float dx = px[kk] - sx, dy = py[kk] - sy;
-                    pair_adjoint<MODEL, OFFS>(dx, dy, pz[k] - sz, fmaf(dx, dx, fmaf(dy, dy, kR2SAFE)), at.w, p0,
-                                              p1, K, a, g[k], t);
+                    pair_adjoint<MODEL, OFFS, PAR>(dx, dy, pz[k] - sz, fmaf(dx, dx, fmaf(dy, dy, kR2SAFE)), at.w,
+                                                   p0, p1, K, a, g[k], t);
                 }
 #pragma unroll
                 for (int c = 0; c < 3; ++c) acc[c] -= (double)t[c];
 #pragma unroll
-                for (int c = 3; c < 6; ++c) acc[c] += (double)t[c];
+                for (int c = 3; c < NC; ++c) acc[c] += (double)t[c];
             }
 #pragma unroll
-        for (int c = 0; c < 6; ++c) {
+        for (int c = 0; c < NC; ++c) {
             double v = acc[c];
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -353,7 +359,9 @@ __global__ __launch_bounds__(256) void k_grid_adjoint(const float4* __restrict__
         __syncthreads();
         if (threadIdx.x < 6)
             partials[((size_t)bid * a.na + j) * 6 + threadIdx.x] =
-                red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+                (int)threadIdx.x < NC
+                    ? red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x]
+                    : 0.0;
         __syncthreads();
     }
 }
@@ -508,7 +516,7 @@ size_t grid_adjoint_workspace(int na, const PpmGridSpec& g) { return adjoint_lay
 int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],
                         const float lvB[3], int cap, const float* grid, const float* grid_cot, const float* base,
-                        void* workspace, double* grad, hipStream_t stream) {
+                        void* workspace, int params, double* grad, hipStream_t stream) {
     const GridArgs a = make_args(na, g, Qs, QZs, npbc_a, npbc_b, lvA, lvB, cap);
     const int V = adjoint_V(g);
     const dim3 blocks = grid_blocks(g, V);
@@ -533,9 +541,14 @@ int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na
     hipLaunchKernelGGL(k_adj_offsets, dim3(1), dim3(1024), 0, stream, w.count, (int)nb, w.off, w.total);
     hipLaunchKernelGGL(k_adj_list, blocks, dim3(256), 0, stream, w.flag, w.off, w.list);
     if (int r = check_launch("grid_adjoint_list")) return r;
-#define PPM_ADJ(M, VV, DCZ, OF)                                                                                  \
-    hipLaunchKernelGGL((k_grid_adjoint<M, VV, DCZ, OF>), adj_blocks, dim3(256), 0, stream, at, cpar, a, G, gc, B, \
-                       w.list, w.total, partials, active)
+#define PPM_ADJ1(M, VV, DCZ, OF, PA)                                                                         \
+    hipLaunchKernelGGL((k_grid_adjoint<M, VV, DCZ, OF, PA>), adj_blocks, dim3(256), 0, stream, at, cpar, a, G, gc, \
+                       B, w.list, w.total, partials, active)
+#define PPM_ADJ(M, VV, DCZ, OF)                                                                              \
+    do {                                                                                                     \
+        if (params) PPM_ADJ1(M, VV, DCZ, OF, true);                                                         \
+        else PPM_ADJ1(M, VV, DCZ, OF, false);                                                               \
+    } while (0)
     if (model == PPM_MODEL_LJ) {
         if (V == 8) { if (offs) PPM_ADJ(PPM_MODEL_LJ, 8, true, true); else PPM_ADJ(PPM_MODEL_LJ, 8, true, false); }
         else { if (offs) PPM_ADJ(PPM_MODEL_LJ, 4, false, true); else PPM_ADJ(PPM_MODEL_LJ, 4, false, false); }
@@ -544,6 +557,7 @@ int launch_grid_adjoint(int model, const float* atoms, const float* cpar, int na
         else { if (offs) PPM_ADJ(PPM_MODEL_MORSE, 4, false, true); else PPM_ADJ(PPM_MODEL_MORSE, 4, false, false); }
     }
 #undef PPM_ADJ
+#undef PPM_ADJ1
     if (int r = check_launch("grid_adjoint")) return r;
     hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)(na * 6)), dim3(256), 0, stream, partials, active, (int)nb,
                        na, grad);

@@ -416,9 +416,10 @@ def relax_polish(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, FEs=None,
 
 
 def grid_adjoint(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, grid, grid_cot, npbc=(0, 0), lvA=(0, 0, 0),
-                 lvB=(0, 0, 0), cap=True, workspace=None, base=None):
+                 lvB=(0, 0, 0), cap=True, workspace=None, base=None, params=True):
     """dL/d(x,y,z,q,p0,p1) per atom (na,6) float64 from dL/dgrid (p0,p1 = c6,c12 for LJ, R0,E0 for Morse).
-    ``base``: the base grid the forward was projected over (grid_project(..., base=))."""
+    ``base``: the base grid the forward was projected over (grid_project(..., base=)).
+    ``params=False``: positions and charges only (columns 4, 5 are zero), less arithmetic."""
     import torch
     _f32dev(atoms, "atoms", 4)
     _f32dev(cpar, "cpar", 2 if model == _lib.PPM_MODEL_LJ else 4)
@@ -437,8 +438,8 @@ def grid_adjoint(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, grid, grid_co
     lvb = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvB, float).reshape(3)])
     head = (int(model), atoms.data_ptr(), cpar.data_ptr(), na, ctypes.byref(s), _lib.f4(Qs), _lib.f4(QZs),
             int(npbc[0]), int(npbc[1]), ctypes.byref(lva), ctypes.byref(lvb), int(bool(cap)))
-    tail = (grid.data_ptr(), grid_cot.data_ptr(), workspace.data_ptr(), workspace.numel(), grad.data_ptr(),
-            _stream(atoms))
+    tail = (grid.data_ptr(), grid_cot.data_ptr(), workspace.data_ptr(), workspace.numel(), int(bool(params)),
+            grad.data_ptr(), _stream(atoms))
     if base is None:
         rc = L.ppm_grid_adjoint(*head, *tail)
     else:

@@ -237,8 +237,10 @@ def _make_function():
                                 "singular": int(((st & 4) != 0).sum()), "unstable": int(((st & 8) != 0).sum())}
             if prob.strict and prob.last_status["nonfinite"]:
                 raise RuntimeError(f"relax adjoint: {prob.last_status['nonfinite']} states non-finite")
+            # the (c6, c12) columns only when R0 or E0 wants a gradient
+            params = bool(ctx.needs_input_grad[2] or ctx.needs_input_grad[3])
             g6 = D.grid_adjoint(atoms, cpar, _lib.PPM_MODEL_LJ, spec, prob.Qs, prob.QZs, grid, gG, prob.npbc,
-                                prob.lvA, prob.lvB, cap=prob.cap)
+                                prob.lvA, prob.lvB, cap=prob.cap, params=params)
             R0d, E0d = R0.double(), E0.double()
             r5 = R0d ** 5
             r6 = r5 * R0d

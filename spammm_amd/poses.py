@@ -304,7 +304,7 @@ def _make_function():
             if scene.strict and scene.last_status["nonfinite"]:
                 raise RuntimeError(f"pose adjoint: {scene.last_status['nonfinite']} states non-finite")
             g6 = D.grid_adjoint(atoms, scene.mol_cpar, _lib.PPM_MODEL_LJ, spec, scene.Qs, scene.QZs,
-                                grid, grid_cot, scene.npbc, scene.lvA, scene.lvB, cap=True, base=base)
+                                grid, grid_cot, scene.npbc, scene.lvA, scene.lvB, cap=True, base=base, params=False)
             gx = g6[:, :3]
             th = pose[3]
             dx = xyz0[:, 0] - ctr[0]

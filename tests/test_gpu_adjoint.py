@@ -144,3 +144,36 @@ def test_adjoint_recompute_equals_stored():
         out.append(np.concatenate([t[k].grad.cpu().numpy().ravel() for k in ("xyz", "q", "R0", "E0")]))
     scale = np.abs(out[0]).max()
     assert np.abs(out[0] - out[1]).max() <= 1e-4 * scale
+
+
+def test_grid_adjoint_without_parameter_columns_equals_full():
+    """ppm_grid_adjoint with params = 0 gives the position and charge columns of the params = 1
+    call (the same formulas; the compiler may fuse multiply-adds differently in the two forms, so
+    to 1e-5 of each column's largest entry) and zero parameter columns; the autograd Function takes
+    that form when neither R0 nor E0 wants a gradient, with the same xyz and q gradients."""
+    w, prob, th = setup_case()
+    dev = prob.pts.device
+    rng = np.random.default_rng(4)
+    atoms = torch.tensor(np.concatenate([th["xyz"], th["q"][:, None]], 1), dtype=torch.float32, device=dev)
+    cpar = diff.lj_cpar(torch.tensor(th["R0"], device=dev), torch.tensor(th["E0"], device=dev))
+    sp = prob.spec
+    grid = D.grid_project(atoms, cpar, 0, sp, prob.Qs, prob.QZs, prob.npbc, prob.lvA, prob.lvB, cap=True)
+    cot = torch.as_tensor(rng.standard_normal(tuple(grid.shape)).astype(np.float32), device=dev)
+    cot[: sp.n[2] // 3] = 0.0                       # some octets without a cotangent
+    full = D.grid_adjoint(atoms, cpar, 0, sp, prob.Qs, prob.QZs, grid, cot, prob.npbc, prob.lvA, prob.lvB, cap=True)
+    part = D.grid_adjoint(atoms, cpar, 0, sp, prob.Qs, prob.QZs, grid, cot, prob.npbc, prob.lvA, prob.lvB, cap=True,
+                          params=False)
+    scale = full[:, :4].abs().amax(dim=0)
+    rel = float(((full[:, :4] - part[:, :4]).abs() / scale).max())
+    print(f"[grid adjoint params=0] max |d| / column scale {rel:.2e}", flush=True)
+    assert rel <= 1e-5
+    assert (part[:, 4:] == 0).all() and (full[:, 4:] != 0).any()
+    grads = []
+    for want in (True, False):
+        t = {k: torch.tensor(v, dtype=torch.float64, device=dev, requires_grad=(want or k in ("xyz", "q")))
+             for k, v in th.items()}
+        df = diff.df_image(t["xyz"], t["q"], t["R0"], t["E0"], prob)
+        (df.double() ** 2).sum().backward()
+        grads.append((t["xyz"].grad, t["q"].grad))
+    for a, b in zip(*grads):
+        assert float((a - b).abs().max() / b.abs().max()) <= 1e-5
