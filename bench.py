@@ -809,7 +809,12 @@ def _diff_case(dev, z0, adam_steps, lr, fd_steps, order4=False, fd=True):
     }
 
 
-ADJ_FLOP_PER_PAIR = 60   # grid->atom adjoint, LJ + tip charge at z offset 0 (pair_adjoint), counted as the forward's 52
+# grid->atom adjoint per pair (pair_adjoint, LJ + tip charge at z offset 0), counted on its formulas
+# like the forward's 52 (add/mul 1, FMA 2, rsqrt 1): d = v - a 3, |d|^2 + eps 6, 1/r and 1/r^2 2,
+# g.d 5, g.d/r^2 1, r^-6, r^-8 3, c12 r^-6 1, fr 3, dfr/ds 4, charge term 6, dq 4, the common factor
+# 4, the three position components 12 = 54; the (c6, c12) columns add 9 (63).  Both legs that report
+# it differentiate positions and charges only.
+ADJ_FLOP_PER_PAIR = 54
 
 
 def grid_adjoint_active_pairs(spec, PPs, n_sites, columns=None, V=8):
