@@ -4,6 +4,8 @@
 // the reference kernels they follow).
 #pragma once
 #pragma clang fp contract(off)
+#include <type_traits>
+
 #include "ppm_common.hpp"
 
 namespace ppm {
@@ -40,15 +42,42 @@ __device__ __forceinline__ float axis_frac(float s, int n, float& fl) {
     return t - fl;
 }
 
-// The two texel indices of the cell with floor fl (clamped first, which keeps non-finite
-// coordinates in range).
+// The two texel indices of the cell with floor fl, REPEAT-wrapped.  fl <= n - 1 for every finite
+// coordinate (the fraction is below 1, or rounds to 1 and gives t = n - 0.5), and fmaxf maps a
+// NaN floor (non-finite coordinate) to -1, so i is in [-1, n - 1] and one wrap per index suffices.
 __device__ __forceinline__ void axis_index(float fl, int n, int& i0, int& i1) {
-    int i = (int)fminf(fmaxf(fl, -1.0f), (float)n);
-    int j = i + 1;
-    if (i < 0) i += n;
-    if (j > n - 1) j -= n;
-    i0 = min(max(i, 0), n - 1);
-    i1 = min(max(j, 0), n - 1);
+    const int i = (int)fmaxf(fl, -1.0f);
+    i0 = i < 0 ? n - 1 : i;
+    i1 = i + 1 > n - 1 ? 0 : i + 1;
+}
+
+// Stored planes hold at most 4 GiB: corner offsets fit 32 bits.
+__host__ __device__ inline bool small_grid(const GridDev& g) {
+    return (unsigned long long)g.nx * (unsigned long long)g.ny * (unsigned long long)g.nzw * 16ull <=
+           0xFFFFFFFFull;
+}
+
+// Element offsets of the 4 corner rows (k0,j0), (k0,j1), (k1,j0), (k1,j1) of the cell with floors
+// (fa, fb, fc) in the stored planes, and its two x indices.  A cell outside the stored planes sets
+// *g.flag (from lane `flag_lane`) and is clamped into them.  O32: 32-bit offsets (small_grid).
+template <bool O32>
+__device__ __forceinline__ void cell_rows(const GridDev& g, float fa, float fb, float fc, bool flag_lane, int& i0,
+                                          int& i1, typename std::conditional<O32, unsigned, size_t>::type (&r)[4]) {
+    using T = typename std::conditional<O32, unsigned, size_t>::type;
+    int j0, j1, k0, k1;
+    axis_index(fa, g.nx, i0, i1);
+    axis_index(fb, g.ny, j0, j1);
+    axis_index(fc, g.nz, k0, k1);
+    k0 -= g.klo;
+    k1 -= g.klo;
+    if ((unsigned)k0 >= (unsigned)g.nzw || (unsigned)k1 >= (unsigned)g.nzw) {
+        if (g.flag && flag_lane) atomicOr(g.flag, 1);
+        k0 = min(max(k0, 0), g.nzw - 1);
+        k1 = min(max(k1, 0), g.nzw - 1);
+    }
+    const T sx = (T)g.nx, sxy = (T)g.nx * (T)g.ny;
+    const T z0 = (T)k0 * sxy, z1 = (T)k1 * sxy, y0 = (T)j0 * sx, y1 = (T)j1 * sx;
+    r[0] = z0 + y0; r[1] = z0 + y1; r[2] = z1 + y0; r[3] = z1 + y1;
 }
 
 __device__ __forceinline__ float4 lerp8(const Cell& C, float a, float b, float c) {
@@ -82,32 +111,28 @@ __device__ __forceinline__ void tex_coords(const GridDev& g, float px, float py,
     }
 }
 
-template <bool AX = false>
+// FAST: the grid axes are the world axes (tex_coords<true>) and the stored planes are a small_grid
+// (32-bit corner offsets from the uniform grid base).
+template <bool FAST = false>
 __device__ __forceinline__ float4 sample(const GridDev& g, Cell& C, float px, float py, float pz) {
     float sa, sb, sc;
-    tex_coords<AX>(g, px, py, pz, sa, sb, sc);
+    tex_coords<FAST>(g, px, py, pz, sa, sb, sc);
     float fa, fb, fc;
     const float a = axis_frac(sa, g.nx, fa), b = axis_frac(sb, g.ny, fb), c = axis_frac(sc, g.nz, fc);
     if (fa != C.fi || fb != C.fj || fc != C.fk) {
         C.fi = fa; C.fj = fb; C.fk = fc;
-        int i0, i1, j0, j1, k0, k1;
-        axis_index(fa, g.nx, i0, i1);
-        axis_index(fb, g.ny, j0, j1);
-        axis_index(fc, g.nz, k0, k1);
-        k0 -= g.klo;
-        k1 -= g.klo;
-        if ((unsigned)k0 >= (unsigned)g.nzw || (unsigned)k1 >= (unsigned)g.nzw) {
-            if (g.flag) atomicOr(g.flag, 1);
-            k0 = min(max(k0, 0), g.nzw - 1);
-            k1 = min(max(k1, 0), g.nzw - 1);
+        int i0, i1;
+        typename std::conditional<FAST, unsigned, size_t>::type r[4];
+        cell_rows<FAST>(g, fa, fb, fc, true, i0, i1, r);
+        if constexpr (FAST) {
+            const char* base = reinterpret_cast<const char*>(g.data);
+#pragma unroll
+            for (int m = 0; m < 8; ++m)
+                C.c[m] = *reinterpret_cast<const float4*>(base + (r[m >> 1] + (unsigned)((m & 1) ? i1 : i0)) * 16u);
+        } else {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) C.c[m] = g.data[r[m >> 1] + (size_t)((m & 1) ? i1 : i0)];
         }
-        const size_t sx = (size_t)g.nx, sxy = (size_t)g.nx * g.ny;
-        const size_t r00 = (size_t)k0 * sxy + (size_t)j0 * sx, r10 = (size_t)k0 * sxy + (size_t)j1 * sx;
-        const size_t r01 = (size_t)k1 * sxy + (size_t)j0 * sx, r11 = (size_t)k1 * sxy + (size_t)j1 * sx;
-        C.c[0] = g.data[r00 + i0]; C.c[1] = g.data[r00 + i1];
-        C.c[2] = g.data[r10 + i0]; C.c[3] = g.data[r10 + i1];
-        C.c[4] = g.data[r01 + i0]; C.c[5] = g.data[r01 + i1];
-        C.c[6] = g.data[r11 + i0]; C.c[7] = g.data[r11 + i1];
     }
     return lerp8(C, a, b, c);
 }
@@ -124,36 +149,43 @@ struct GroupCell {
     __device__ void reset() { fi = fj = fk = -2.0f; }
 };
 
-template <int G, bool AX = false>
+template <int G, bool FAST = false>
 __device__ __forceinline__ float4 sample_group(const GridDev& g, GroupCell<G>& C, int sub, float px, float py,
                                                float pz) {
     constexpr int NC = 4 / G;
     float sa, sb, sc;
-    tex_coords<AX>(g, px, py, pz, sa, sb, sc);
+    tex_coords<FAST>(g, px, py, pz, sa, sb, sc);
     float fa, fb, fc;
     const float a = axis_frac(sa, g.nx, fa), b = axis_frac(sb, g.ny, fb), c = axis_frac(sc, g.nz, fc);
     if (fa != C.fi || fb != C.fj || fc != C.fk) {
         C.fi = fa; C.fj = fb; C.fk = fc;
-        int i0, i1, j0, j1, k0, k1;
-        axis_index(fa, g.nx, i0, i1);
-        axis_index(fb, g.ny, j0, j1);
-        axis_index(fc, g.nz, k0, k1);
-        k0 -= g.klo;
-        k1 -= g.klo;
-        if ((unsigned)k0 >= (unsigned)g.nzw || (unsigned)k1 >= (unsigned)g.nzw) {
-            if (g.flag && sub == 0) atomicOr(g.flag, 1);
-            k0 = min(max(k0, 0), g.nzw - 1);
-            k1 = min(max(k1, 0), g.nzw - 1);
+        int i0, i1;
+        typename std::conditional<FAST, unsigned, size_t>::type r[4];
+        cell_rows<FAST>(g, fa, fb, fc, sub == 0, i0, i1, r);
+        if constexpr (FAST) {
+            const char* base = reinterpret_cast<const char*>(g.data);
+            const unsigned ch = (unsigned)(sub * NC * 4);       // this lane's channels within a texel
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const char* t = base + ((r[m >> 1] + (unsigned)((m & 1) ? i1 : i0)) * 16u + ch);
+                if constexpr (NC == 2) {
+                    const float2 v = *reinterpret_cast<const float2*>(t);
+                    C.c[0][m] = v.x;
+                    C.c[1][m] = v.y;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) C.c[q][m] = reinterpret_cast<const float*>(t)[q];
+                }
+            }
+        } else {
+            const float* base = reinterpret_cast<const float*>(g.data);
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const size_t idx = r[m >> 1] + (size_t)((m & 1) ? i1 : i0);
+#pragma unroll
+                for (int q = 0; q < NC; ++q) C.c[q][m] = base[4 * idx + sub * NC + q];
+            }
         }
-        const size_t sx = (size_t)g.nx, sxy = (size_t)g.nx * g.ny;
-        const size_t r00 = (size_t)k0 * sxy + (size_t)j0 * sx, r10 = (size_t)k0 * sxy + (size_t)j1 * sx;
-        const size_t r01 = (size_t)k1 * sxy + (size_t)j0 * sx, r11 = (size_t)k1 * sxy + (size_t)j1 * sx;
-        const size_t idx[8] = {r00 + i0, r00 + i1, r10 + i0, r10 + i1, r01 + i0, r01 + i1, r11 + i0, r11 + i1};
-        const float* base = reinterpret_cast<const float*>(g.data);
-#pragma unroll
-        for (int m = 0; m < 8; ++m)
-#pragma unroll
-            for (int q = 0; q < NC; ++q) C.c[q][m] = base[4 * idx[m] + sub * NC + q];
     }
     const float ma = 1.0f - a, mb = 1.0f - b, mc = 1.0f - c;
     float v[NC];
@@ -416,12 +448,12 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
     }
 }
 
-// The tip frame is the identity and the grid's inverse cell has no off-diagonal terms: the
-// relaxation may take its FAST form (same results).
+// The tip frame is the identity, the grid's inverse cell has no off-diagonal terms and its stored
+// planes are a small_grid: the relaxation may take its FAST form (same results).
 __host__ __device__ inline bool fast_frame(const TiltArgs& p, const GridDev& g) {
     return p.A[0] == 1.f && p.A[1] == 0.f && p.A[2] == 0.f && p.B[0] == 0.f && p.B[1] == 1.f && p.B[2] == 0.f &&
            p.C[0] == 0.f && p.C[1] == 0.f && p.C[2] == 1.f && g.da[1] == 0.f && g.da[2] == 0.f && g.db[0] == 0.f &&
-           g.db[2] == 0.f && g.dc[0] == 0.f && g.dc[1] == 0.f;
+           g.db[2] == 0.f && g.dc[0] == 0.f && g.dc[1] == 0.f && small_grid(g);
 }
 
 template <bool STAGE = true, bool FAST = false>
