@@ -352,37 +352,51 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
     int o = 0;
     Stage4 s1, s2;       // G = 1: FEs, PPs; G > 1: the lane's own output
     int pend = 0;
+    // The sample that gives a finished z-step's output (the resample at the final position) is
+    // taken by the loop's own sample at the top of the next iteration (emit_next), so a wave whose
+    // lanes finish their steps at different iterations runs one sample path, not a second one in
+    // the branch of the finishing lanes; each lane's sequence of operations is unchanged.
+    bool emit_next = false, conv_out = false;
     while (true) {
         const float4 fe = sample(pos.x, pos.y, pos.z);
-        f3 f = mk3(fe.x, fe.y, fe.z);
-        const f3 dp = mk3(pos.x - tip.x, pos.y - tip.y, pos.z - tip.z);
-        if constexpr (IDF) {
-            const f3 ft = tipForce(dp, p.k, p.d0);
-            f = mk3(f.x + ft.x, f.y + ft.y, f.z + ft.z);
-            f.z = f.z + p.surf;
-        } else {
-            const f3 ft = rotMatT(tipForce(rotMat(dp, A, B, C), p.k, p.d0), A, B, C);
-            f = mk3(f.x + ft.x, f.y + ft.y, f.z + ft.z);
-            f = mk3(f.x + fsurf.x, f.y + fsurf.y, f.z + fsurf.z);
+        bool emit = emit_next;
+        emit_next = false;
+        if (!emit) {
+            f3 f = mk3(fe.x, fe.y, fe.z);
+            const f3 dp = mk3(pos.x - tip.x, pos.y - tip.y, pos.z - tip.z);
+            if constexpr (IDF) {
+                const f3 ft = tipForce(dp, p.k, p.d0);
+                f = mk3(f.x + ft.x, f.y + ft.y, f.z + ft.z);
+                f.z = f.z + p.surf;
+            } else {
+                const f3 ft = rotMatT(tipForce(rotMat(dp, A, B, C), p.k, p.d0), A, B, C);
+                f = mk3(f.x + ft.x, f.y + ft.y, f.z + ft.z);
+                f = mk3(f.x + fsurf.x, f.y + fsurf.y, f.z + fsurf.z);
+            }
+            const float ff = dot3(f, f);
+            bool conv = false, done = false;
+            if (p.check_before && ff < p.f2conv) {
+                conv = done = true;
+            } else {
+                v = integrate(p.damped, f, v, dt, damp, dtmin, dtmax, damp0);
+                v = mk3(v.x + f.x * dt, v.y + f.y * dt, v.z + f.z * dt);
+                pos = mk3(pos.x + v.x * dt, pos.y + v.y * dt, pos.z + v.z * dt);
+                if (!p.check_before && ff < p.f2conv) conv = done = true;
+                else if (it + 1 >= p.max_iter) done = true;
+            }
+            ++it;
+            if (done) {
+                conv_out = conv;
+                if (p.check_before && conv) emit = true;   // the output is this sample (no update)
+                else emit_next = true;                     // the output is the sample at the new position
+            }
         }
-        const float ff = dot3(f, f);
-        bool conv = false, done = false;
-        if (p.check_before && ff < p.f2conv) {
-            conv = done = true;
-        } else {
-            v = integrate(p.damped, f, v, dt, damp, dtmin, dtmax, damp0);
-            v = mk3(v.x + f.x * dt, v.y + f.y * dt, v.z + f.z * dt);
-            pos = mk3(pos.x + v.x * dt, pos.y + v.y * dt, pos.z + v.z * dt);
-            if (!p.check_before && ff < p.f2conv) conv = done = true;
-            else if (it + 1 >= p.max_iter) done = true;
-        }
-        ++it;
-        if (done) {
-            const float4 fs = (p.check_before && conv) ? fe : sample(pos.x, pos.y, pos.z);
+        if (emit) {
+            const float4 fs = fe;
             const f3 fr = IDF ? mk3(fs.x, fs.y, fs.z) : rotMat(mk3(fs.x, fs.y, fs.z), A, B, C);
             const float4 feo = make_float4(fr.x, fr.y, fr.z, fs.w);
             const bool fin = finite4(fs) && isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z);
-            const float4 ppo = make_float4(pos.x, pos.y, pos.z, (conv && fin) ? (float)it : -(float)it);
+            const float4 ppo = make_float4(pos.x, pos.y, pos.z, (conv_out && fin) ? (float)it : -(float)it);
             if (fz_lds) fz_lds[iz * 64] = feo.z;
             if (!STAGE) {
                 if (G == 1) {
