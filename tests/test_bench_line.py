@@ -28,8 +28,10 @@ def _head():
                      "compute_unit": "FP32 vector ALU (no matrix instructions); peak = published MI355X FP32",
                      "achieved": 151.66123, "peak": 157.3, "unit": "TFLOP/s", "frac": 0.9641234,
                      "traffic": 345794965.3, "work_unit": "38880000000 pair evaluations x 52 FLOP per launch",
-                     "avg_launch_ms": 13.331},
-        "stages": {"grid_points_per_s": 1620300000.0, "grid_ms": 13.331, "relax_ms": 3.0272,
+                     "avg_launch_ms": 14.401, "avg_launch_ms_one_stream": 13.331, "frac_one_stream": 0.9641234},
+        "stages": {"one_stream": {"ms_per_step": 15.546123, "value": 252940000.123},
+                   "two_streams": {"grid_ms": 14.40123, "relax_ms": 3.512345, "outputs_equal_one_stream": True},
+                   "grid_points_per_s": 1620300000.0, "grid_ms": 13.331, "relax_ms": 3.0272,
                    "relax_pixel_z_steps_per_s": 1298900000.0, "relax_mean_fire_iters": 25.96,
                    "relax_converged_fraction": 0.99876, "relax_algorithmic_GBps": 4524.0,
                    "relax_algorithmic_frac_of_hbm_peak": 0.56551, "relax_hbm_bytes_per_launch": 286130747.1,
