@@ -67,8 +67,8 @@ __global__ __launch_bounds__(64) void k_relax_tilted(GridDev g, const float4* __
 // Lanes per scan point of k_relax_tilted / k_relax_strokes: the largest G <= 4 that keeps the
 // launch within about 2 waves per SIMD (1024 SIMDs).  A group replicates the per-point FIRE and
 // spring arithmetic in its G lanes, so past that occupancy the extra waves cost VALU issue slots
-// instead of hiding latency (measured on the 256x256-pixel flake scan: G = 4, 2, 1 take 3.38,
-// 2.79, 2.97 ms; on 16384 pixels 1.86, 2.31, 2.57 ms; the results are bit-identical).
+// instead of hiding latency (measured on the 256x256-pixel flake scan: G = 4, 2, 1 take 3.76,
+// 2.43, 2.71 ms; on 16384 pixels 1.54, 1.96, 2.35 ms; the results are bit-identical).
 int relax_group(int n_scan) {
     int g = 1;
     while (g < 4 && (long long)n_scan * g * 2 <= 2LL * 1024 * 64) g *= 2;
