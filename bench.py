@@ -257,8 +257,8 @@ SUMMARY_KEYS = {
                        "fd_fp64_smooth_heights", "gated_columns_configured_heights", "adam_configured_heights",
                        "adam_smooth_heights"),
     "substrate_poses": ("forward_ms_median", "adjoint_ms_median", "adjoint_over_forward",
-                        "placements_per_s_forward_plus_adjoint", "window_fallbacks", "gated_states_total",
-                        "gated_columns_total", "directional_fd_rel_err"),
+                        "placements_per_s_forward_plus_adjoint", "batched_placements_per_s", "window_fallbacks",
+                        "gated_states_total", "gated_columns_total", "directional_fd_rel_err"),
     "direct_morse": ("forward_ms", "vjp_ms", "vjp_over_forward", "pixel_z_steps_per_s"),
     "fdbm": ("stage3_device_ms", "grid_points_per_s_stage3"),
     "ewald": ("tip_grid_ms", "direct_3x3_coulomb_ms", "kept_g_pairs"),
@@ -431,7 +431,7 @@ def substrate_leg(dev, n_poses=64, fd=True):
     q0 = torch.tensor(sc["mol_atoms"][:, 3], dtype=torch.float64, device=dev)
     dpose = torch.tensor([0.1, -0.1, 0.0, 0.02], dtype=torch.float64, device=dev)
     t_f, t_b, gnorm, gated, cols_out = [], [], [], 0, 0
-    checks = []
+    checks, dfs, pgrads = [], [], []
     for k in range(n_poses):
         pose_k = torch.tensor(sc["poses"][k], dtype=torch.float64, device=dev)
         with torch.no_grad():
@@ -455,6 +455,8 @@ def substrate_leg(dev, n_poses=64, fd=True):
         gnorm.append(float(pose.grad.norm()))
         if k == 0:
             cols0 = cols.clone()
+        dfs.append(dfi.detach())
+        pgrads.append(pose.grad.detach().clone())
         if fd and k < 2:    # directional finite difference along -grad(pose) on two placements
             gp = pose.grad.detach()
             d = -gp / gp.norm()
@@ -471,6 +473,8 @@ def substrate_leg(dev, n_poses=64, fd=True):
             checks.append(row)
         if (k + 1) % 16 == 0:
             log(f"[bench] substrate placements {k + 1}/{n_poses}")
+    batched = _substrate_batched(scene, sc, dev, n_poses, q0, dpose, dfs, pgrads)
+    del dfs, pgrads
     states = scene.n_scan * scene.nz
     with torch.no_grad():       # work of the grid->atom adjoint of placement 0 (its active workgroups)
         x0 = PS.place(torch.as_tensor(scene.mol_xyz0, device=dev), torch.as_tensor(scene.center, device=dev),
@@ -494,9 +498,52 @@ def substrate_leg(dev, n_poses=64, fd=True):
         "directional_fd": checks, "directional_fd_rel_err": [r["rel_err_best"] for r in checks],
         "scan_window": list(scene.spec_run.n), "window_fallbacks": int(scene.window_fallbacks),
         "grid_bytes": int(scene.base.numel() * 4), "peak_mem_GB": torch.cuda.max_memory_allocated(dev) / 1e9,
+        "batched": batched,
+        "batched_placements_per_s": batched["placements_per_s_forward_plus_adjoint"],
     }
     log(f"[bench] substrate: {res}")
     del scene
+    torch.cuda.empty_cache()
+    return res
+
+
+def _substrate_batched(scene, sc, dev, n_poses, q0, dpose, dfs, pgrads):
+    """The substrate placements again through poses.pose_df_batch (scene.batch_depth placements in
+    flight, each on its own stream; the grids kept for the backward): the same L2 loss per placement,
+    summed; its df images against the one-at-a-time images (bit for bit) and its pose gradients
+    against theirs."""
+    import torch
+    from spammm_amd import poses as PS
+    P = torch.tensor(np.asarray(sc["poses"][:n_poses]), dtype=torch.float64, device=dev)
+    with torch.no_grad():
+        target = PS.pose_df_batch(P + dpose, q0, scene).double()
+    tcols = scene.last_columns.clone()
+    stream = torch.cuda.current_stream(dev)
+    for rep in range(2):        # the first pass allocates the kept grids and the batch buffers
+        poses = P.clone().requires_grad_(True)
+        q = q0.clone().requires_grad_(True)
+        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        torch.cuda.synchronize(dev)
+        a.record(stream)
+        df = PS.pose_df_batch(poses, q, scene)
+        cols = scene.last_columns & tcols
+        loss = 0.5 * (((df.double() - target) * cols[..., None]) ** 2).sum()
+        b.record(stream)
+        loss.backward()
+        c.record(stream)
+        torch.cuda.synchronize(dev)
+    t_f, t_b = a.elapsed_time(b), b.elapsed_time(c)
+    same = sum(bool(torch.equal(df[k].detach(), dfs[k])) for k in range(n_poses))
+    rel = max(float((poses.grad[k] - pgrads[k]).norm() / pgrads[k].norm()) for k in range(n_poses))
+    kept = min(n_poses, int(scene.batch_grid_budget // (scene.base.numel() * 4)))
+    res = {"placements": n_poses, "in_flight": scene.batch_depth, "grids_kept": kept,
+           "forward_ms": t_f, "adjoint_ms": t_b, "forward_ms_per_placement": t_f / n_poses,
+           "adjoint_ms_per_placement": t_b / n_poses,
+           "placements_per_s_forward_plus_adjoint": 1e3 * n_poses / (t_f + t_b),
+           "df_bitequal_one_at_a_time": [same, n_poses], "pose_grad_max_rel_diff_one_at_a_time": rel,
+           "gated_states": scene.last_status.get("gated", 0)}
+    log(f"[bench] substrate batched: {res}")
+    del target, df, loss
     torch.cuda.empty_cache()
     return res
 

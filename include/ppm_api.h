@@ -129,6 +129,15 @@ int ppm_version(void);
 /* Message of the last failed call on this thread ("" if none). */
 const char* ppm_last_error(void);
 
+/* A non-blocking HIP stream on the current device (*stream), and its release.  No counterpart in
+ * the reference (it runs one in-order OpenCL queue per AFMulator); for callers that keep several
+ * scans in flight at once.  A stream created here takes the least used hardware queue of the
+ * process when it is created, whereas streams handed out from a pool that was created in one go
+ * can share a hardware queue and then run one after another.  ppm_stream_destroy may be called
+ * with work still queued on the stream; its resources are released when that work is done. */
+int ppm_stream_create(void** stream);
+int ppm_stream_destroy(void* stream);
+
 /* Force-field grid projection.
  * Replaces evalLJC_QZs_toImg (kernels/AFM.cl:916-991) and evalMorseC_QZs_toImg
  * (kernels/AFM.cl:996-1056) as launched by AFMulator.make_forcefield

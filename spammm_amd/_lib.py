@@ -56,7 +56,8 @@ EXPORTS = (
     "ppm_ewald_num_g", "ppm_ewald_system_bytes", "ppm_ewald_prepare", "ppm_ewald_eval", "ppm_ewald_brute",
     "ppm_ewald_cluster", "ppm_ewald_grid_workspace_size", "ppm_ewald_tip_grid", "ppm_ewald_kept_count_offset",
     "ppm_relax_state_diag", "ppm_relax_strokes_2d", "ppm_grid_project_window", "ppm_relax_strokes_tilted_window",
-    "ppm_batch_forward_noalloc", "ppm_fdbm_release_plans", "ppm_relax_strokes_tilted_df",
+    "ppm_batch_forward_noalloc", "ppm_fdbm_release_plans", "ppm_relax_strokes_tilted_df", "ppm_stream_create",
+    "ppm_stream_destroy",
 )
 
 _lib = None
@@ -168,6 +169,10 @@ def load():
                                               vp, i, i, vp, vp]
     L.ppm_fdbm_release_plans.restype = i
     L.ppm_fdbm_release_plans.argtypes = []
+    L.ppm_stream_create.restype = i
+    L.ppm_stream_create.argtypes = [P(vp)]
+    L.ppm_stream_destroy.restype = i
+    L.ppm_stream_destroy.argtypes = [vp]
     L.ppm_batch_forward_noalloc.restype = i
     L.ppm_batch_forward_noalloc.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i, i, vp]
     L.ppm_relax_direct_morse.restype = i

@@ -85,6 +85,24 @@ int ppm_version(void) { return PPM_API_VERSION; }
 
 const char* ppm_last_error(void) { return g_err; }
 
+int ppm_stream_create(void** stream) {
+    g_err[0] = 0;
+    if (!stream) { set_error("ppm_stream_create: stream is NULL"); return PPM_ERR_INVALID_ARG; }
+    hipStream_t s = nullptr;
+    const hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) { set_error("ppm_stream_create: %s", hipGetErrorString(e)); return PPM_ERR_HIP; }
+    *stream = s;
+    return PPM_OK;
+}
+
+int ppm_stream_destroy(void* stream) {
+    g_err[0] = 0;
+    if (!stream) { set_error("ppm_stream_destroy: stream is NULL"); return PPM_ERR_INVALID_ARG; }
+    const hipError_t e = hipStreamDestroy(static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) { set_error("ppm_stream_destroy: %s", hipGetErrorString(e)); return PPM_ERR_HIP; }
+    return PPM_OK;
+}
+
 static int grid_project_impl(const char* fn, int model, const float* atoms, const float* cpar, int na,
                              const PpmGridSpec* spec, const float Qs[4], const float QZs[4], int npbc_a, int npbc_b,
                              const float lvA[3], const float lvB[3], int cap, const float* base, float* grid,

@@ -28,6 +28,37 @@ def _stream(t):
     return torch.cuda.current_stream(t.device).cuda_stream
 
 
+class Stream:
+    """A non-blocking HIP stream of the library's own (ppm_stream_create) and its torch handle
+    ``.torch`` (torch.cuda.ExternalStream).  Each one takes the least used hardware queue when it is
+    created, so streams created one after another run side by side; torch's pooled streams can
+    share a queue.  Buffers used on it are allocated on the caller's stream and must stay alive
+    until that stream has waited for this one (close() makes the current stream wait, then releases
+    the stream; so does garbage collection)."""
+
+    def __init__(self, device=None):
+        import torch
+        dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(dev):
+            _lib.check(_lib.load().ppm_stream_create(ctypes.byref(h)))
+        self.handle = h.value
+        self.torch = torch.cuda.ExternalStream(self.handle, device=dev)
+
+    def close(self):
+        if self.handle:
+            import torch
+            torch.cuda.current_stream(self.torch.device).wait_stream(self.torch)
+            h, self.handle = self.handle, None
+            _lib.check(_lib.load().ppm_stream_destroy(ctypes.c_void_p(h)))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _f32dev(t, name, shape_last=None):
     import torch
     if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float32:
@@ -352,9 +383,9 @@ def fe_in_strokes_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, dTip, 
     return FEs
 
 
-def relax_adjoint(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, dL_dFEs, grid_cot=None):
+def relax_adjoint(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, dL_dFEs, grid_cot=None, lam=None):
     """Implicit-function adjoint of relaxed states: accumulates dL/dgrid into ``grid_cot`` (zeros if None).
-    Returns (grid_cot, lambdas (n_scan,nz,4) with the status word in .w)."""
+    Returns (grid_cot, lambdas (n_scan,nz,4) with the status word in .w; into ``lam`` if given)."""
     import torch
     _f32dev(grid, "grid", 4)
     _f32dev(points, "points", 4)
@@ -369,7 +400,12 @@ def relax_adjoint(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, dL_dFEs,
         _f32dev(grid_cot, "grid_cot", 4)
         if grid_cot.numel() != grid.numel():
             raise ValueError("grid_cot must have the grid's shape")
-    lam = torch.empty((ns, nz, 4), dtype=torch.float32, device=grid.device)
+    if lam is None:
+        lam = torch.empty((ns, nz, 4), dtype=torch.float32, device=grid.device)
+    else:
+        _f32dev(lam, "lam", 4)
+        if lam.numel() != ns * nz * 4:
+            raise ValueError("lam must be (n_scan, nz, 4)")
     L = _lib.load()
     s, tp = spec.c(), tip.c()
     rc = L.ppm_relax_adjoint(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz), ctypes.byref(tp),
@@ -378,9 +414,9 @@ def relax_adjoint(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, dL_dFEs,
     return grid_cot, lam
 
 
-def relax_state_diag(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs):
+def relax_state_diag(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, out=None):
     """Per-state (|G|, smallest eigenvalue of sym(-J), condition number, status bits) of relaxed
-    states, (n_scan, nz, 4) float32; status bits as relax_adjoint's."""
+    states, (n_scan, nz, 4) float32 (into ``out`` if given); status bits as relax_adjoint's."""
     import torch
     _f32dev(grid, "grid", 4)
     _f32dev(points, "points", 4)
@@ -388,7 +424,13 @@ def relax_state_diag(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs):
     ns = int(points.shape[0])
     if PPs.numel() != ns * nz * 4:
         raise ValueError("PPs must be (n_scan, nz, 4)")
-    diag = torch.empty((ns, nz, 4), dtype=torch.float32, device=grid.device)
+    if out is None:
+        diag = torch.empty((ns, nz, 4), dtype=torch.float32, device=grid.device)
+    else:
+        _f32dev(out, "out", 4)
+        if out.numel() != ns * nz * 4:
+            raise ValueError("out must be (n_scan, nz, 4)")
+        diag = out
     L = _lib.load()
     s, tp = spec.c(), tip.c()
     rc = L.ppm_relax_state_diag(grid.data_ptr(), ctypes.byref(s), points.data_ptr(), ns, int(nz), ctypes.byref(tp),
@@ -416,10 +458,11 @@ def relax_polish(grid, spec: GridSpec, points, nz: int, tip: Tip, PPs, FEs=None,
 
 
 def grid_adjoint(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, grid, grid_cot, npbc=(0, 0), lvA=(0, 0, 0),
-                 lvB=(0, 0, 0), cap=True, workspace=None, base=None, params=True):
-    """dL/d(x,y,z,q,p0,p1) per atom (na,6) float64 from dL/dgrid (p0,p1 = c6,c12 for LJ, R0,E0 for Morse).
-    ``base``: the base grid the forward was projected over (grid_project(..., base=)).
-    ``params=False``: positions and charges only (columns 4, 5 are zero), less arithmetic."""
+                 lvB=(0, 0, 0), cap=True, workspace=None, base=None, params=True, out=None):
+    """dL/d(x,y,z,q,p0,p1) per atom (na,6) float64 from dL/dgrid (p0,p1 = c6,c12 for LJ, R0,E0 for Morse),
+    into ``out`` if given.  ``base``: the base grid the forward was projected over
+    (grid_project(..., base=)).  ``params=False``: positions and charges only (columns 4, 5 are zero),
+    less arithmetic."""
     import torch
     _f32dev(atoms, "atoms", 4)
     _f32dev(cpar, "cpar", 2 if model == _lib.PPM_MODEL_LJ else 4)
@@ -433,7 +476,12 @@ def grid_adjoint(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, grid, grid_co
     need = int(L.ppm_grid_adjoint_workspace_size(na, ctypes.byref(s)))
     if workspace is None or workspace.numel() < need:
         workspace = torch.empty(need, dtype=torch.uint8, device=atoms.device)
-    grad = torch.empty((na, 6), dtype=torch.float64, device=atoms.device)
+    if out is None:
+        grad = torch.empty((na, 6), dtype=torch.float64, device=atoms.device)
+    else:
+        if out.dtype != torch.float64 or not out.is_cuda or not out.is_contiguous() or out.numel() != na * 6:
+            raise ValueError("out must be a contiguous (na, 6) float64 device tensor")
+        grad = out
     lva = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvA, float).reshape(3)])
     lvb = (ctypes.c_float * 3)(*[float(x) for x in np.asarray(lvB, float).reshape(3)])
     head = (int(model), atoms.data_ptr(), cpar.data_ptr(), na, ctypes.byref(s), _lib.f4(Qs), _lib.f4(QZs),

@@ -129,22 +129,45 @@ def column_gate(grid, spec, pts, nxy, nz, tip, PPs, eig_frac, max_jump):
     eigenvalue (of the symmetric part of -J, ppm_relax_state_diag) of at least eig_frac * k_lat, and
     the probe moves by less than max_jump A relative to the tip between consecutive slices -- and a
     dict of counts."""
-    import torch
+    keep, parts = column_gate_parts(grid, spec, pts, nxy, nz, tip, PPs, eig_frac, max_jump)
+    return keep, gate_stats(parts, eig_frac, max_jump)
+
+
+def column_gate_parts(grid, spec, pts, nxy, nz, tip, PPs, eig_frac, max_jump):
+    """column_gate without host synchronisation, all on the current stream: ((nx, ny) bool keep,
+    (n_scan,) bool device tensors (adjoint_gated, near_fold, jumped))."""
     dg = D.relax_state_diag(grid, spec, pts, nz, tip, PPs)
+    keep, parts = gate_from_diag(dg, PPs, pts, nz, tip, eig_frac, max_jump)
+    return keep.reshape(int(nxy[0]), int(nxy[1])), parts
+
+
+def gate_from_diag(dg, PPs, pts, nz, tip, eig_frac, max_jump):
+    """The column tests of column_gate from the state diagnostics dg and probe positions PPs, both
+    (..., n_scan, nz, 4) (any leading image axes): ((..., n_scan) bool keep, (adjoint_gated,
+    near_fold, jumped))."""
+    import torch
     k_lat = float(np.min(np.abs(np.asarray(tip.stiffness[:3], np.float64))))
-    ns = int(nxy[0]) * int(nxy[1])
     dev = PPs.device
     iz = torch.arange(nz, device=dev, dtype=torch.float32)
     C = torch.as_tensor(np.asarray(tip.tipC[:3], np.float32) * np.float32(tip.tipC[3]), device=dev)
     rel = PPs[..., :3] - (pts[:, None, :3] + iz[None, :, None] * C)       # probe - tip anchor
-    jump = (rel[:, 1:] - rel[:, :-1]).norm(dim=-1).amax(dim=1) if nz > 1 else torch.zeros(ns, device=dev)
-    gated = (dg[..., 3] != 0).any(dim=1)
-    soft = dg[..., 1].amin(dim=1) < eig_frac * k_lat
+    if nz > 1:
+        jump = (rel[..., 1:, :] - rel[..., :-1, :]).norm(dim=-1).amax(dim=-1)
+    else:
+        jump = torch.zeros(PPs.shape[:-2], device=dev)
+    gated = (dg[..., 3] != 0).any(dim=-1)
+    soft = dg[..., 1].amin(dim=-1) < eig_frac * k_lat
     jumped = jump >= max_jump
     keep = ~(gated | soft | jumped)
-    stats = {"columns": ns, "kept": int(keep.sum()), "adjoint_gated": int(gated.sum()), "near_fold": int(soft.sum()),
-             "jumped": int(jumped.sum()), "eig_frac": eig_frac, "max_jump_A": max_jump}
-    return keep.reshape(int(nxy[0]), int(nxy[1])), stats
+    return keep, (gated, soft, jumped)
+
+
+def gate_stats(parts, eig_frac, max_jump):
+    """Counts of column_gate_parts (one or several images' parts stacked along a leading axis)."""
+    gated, soft, jumped = parts
+    keep = ~(gated | soft | jumped)
+    return {"columns": int(keep.numel()), "kept": int(keep.sum()), "adjoint_gated": int(gated.sum()),
+            "near_fold": int(soft.sum()), "jumped": int(jumped.sum()), "eig_frac": eig_frac, "max_jump_A": max_jump}
 
 
 def stable_columns(grid, PPs, prob: Problem, spec=None):

@@ -43,6 +43,10 @@ class ScanSlot:
         import torch
         (stream or torch.cuda.current_stream(self.grid.device)).wait_event(self.done)
 
+    @property
+    def torch_stream(self):
+        return self.stream.torch
+
 
 class ScanPipeline:
     """Plain scans of a sequence of atom sets that share one grid geometry, scan raster and tip.
@@ -72,15 +76,17 @@ class ScanPipeline:
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.slots = []
         for i in range(depth):
-            # private streams (default priority): a slot on the caller's stream would make the next
-            # image's inputs wait for this image's relaxation
+            # streams of the library's own (each on its own hardware queue; default priority): a slot
+            # on the caller's stream would make the next image's inputs wait for this image's
+            # relaxation
             self.slots.append(ScanSlot(
-                stream=torch.cuda.Stream(self.dev), grid=torch.empty((ng, ny, nx, 4), **f32),
+                stream=D.Stream(self.dev), grid=torch.empty((ng, ny, nx, 4), **f32),
                 FEs=torch.empty((ns, self.nz, 4), **f32),
                 PPs=torch.empty((ns, self.nz, 4), **f32) if want_pps else None,
                 df=torch.empty((ns, int(df_op.n_out)), **f32) if df_op is not None else None))
         self.count = 0
         self._last_grid = None
+        self._inputs = [None] * depth
 
     def submit(self, atoms, cpar, record=False) -> ScanSlot:
         """Queue the scan of one atom set (atoms (na,4) x,y,z,q and cpar on the device, produced on the
@@ -89,10 +95,12 @@ class ScanPipeline:
         import torch
         slot = self.slots[self.count % len(self.slots)]
         cur = torch.cuda.current_stream(self.dev)
-        s = slot.stream
+        s = slot.stream.torch
+        i = self.count % len(self.slots)
+        if self._inputs[i] is not None:
+            cur.wait_event(slot.done)              # the previous inputs of this slot may be freed after it
+        self._inputs[i] = (atoms, cpar)            # kept alive while their image is in flight
         s.wait_stream(cur)                         # inputs were written on the caller's stream
-        atoms.record_stream(s)
-        cpar.record_stream(s)
         with torch.cuda.stream(s):
             if self._last_grid is not None:
                 s.wait_event(self._last_grid)      # grid projections run one at a time
@@ -119,3 +127,16 @@ class ScanPipeline:
         for sl in self.slots:
             if sl.done is not None:
                 cur.wait_event(sl.done)
+
+    def close(self):
+        """Make the current stream wait for every queued image and release the streams (the slots'
+        buffers stay valid)."""
+        for sl in self.slots:
+            sl.stream.close()
+        self._inputs = [None] * len(self.slots)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

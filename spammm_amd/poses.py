@@ -13,15 +13,19 @@ relaxes the scan and applies the df operator.  The backward takes the probe posi
 forward kept (16 B per state; with keep_states = False it recomputes the relaxation instead)
 and, unless the shared grid buffer still holds this placement's grid (no other placement's
 forward ran in between), recomputes the grid (one grid buffer is shared by all placements);
-relaxation trajectories are never stored.  It then runs the implicit-function relax adjoint, the grid->atom adjoint of the molecule's atoms over
-the substrate base (ppm_grid_adjoint_add) and the rigid-motion chain rule:
+relaxation trajectories are never stored.  It then runs the implicit-function relax adjoint, the
+grid->atom adjoint of the molecule's atoms over the substrate base (ppm_grid_adjoint_add) and the
+rigid-motion chain rule:
     x_i = R(theta) (x0_i - c) + c + s
     dL/ds = sum_i dL/dx_i,   dL/dtheta = sum_i dL/dx_i . R'(theta) (x0_i - c).
+pose_df_batch runs a batch of placements the same way with several in flight at once, each on a
+stream of its own, and keeps their grids for the backward within a memory budget.
 There is no reference counterpart (the reference relaxes one molecule per image without a
 substrate grid); the df, loss and adjoint conventions are those of spammm_amd.diff.
 """
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -75,6 +79,12 @@ class Scene:
     gate_max_jump: float = 0.3
     last_columns: object = None
     last_gate: dict = field(default_factory=dict)
+    # pose_df_batch: placements in flight at once (one stream and one set of scratch buffers each),
+    # and the bytes of placement grids a batched forward may keep for its backward (others are
+    # recomputed there)
+    batch_depth: int = 2
+    batch_grid_budget: float = 64 * 2 ** 30
+    batch_slots: list = field(default=None, repr=False)
 
     @property
     def n_scan(self):
@@ -146,15 +156,20 @@ def window_inside(spec_run, faces, PPs, guard=0.5, guard_shared=2) -> bool:
     An axis whose two faces are both shared is the full grid's axis: both wrap alike, no guard.
     faces: (lo, hi) bool[3], True where the window face is a face of the full grid."""
     import torch
+    mn, mx = torch.aminmax(PPs.reshape(-1, 4), dim=0)
+    return _inside_bounds(spec_run, faces, mn[:3].cpu().numpy(), mx[:3].cpu().numpy(), guard, guard_shared)
+
+
+def _inside_bounds(spec_run, faces, lo, hi, guard=0.5, guard_shared=2) -> bool:
+    """window_inside from the smallest and largest probe coordinates lo, hi (3,)."""
     sp = spec_run
     d = np.array([sp.dA[0], sp.dB[1], sp.dC[2]], np.float64)
     shared_lo, shared_hi = (np.asarray(f, bool) for f in faces)
     whole = shared_lo & shared_hi
     g_lo = np.where(shared_lo, float(guard_shared), np.ceil(guard / d))
     g_hi = np.where(shared_hi, float(guard_shared), np.ceil(guard / d))
-    mn, mx = torch.aminmax(PPs.reshape(-1, 4), dim=0)
-    lo = mn[:3].cpu().numpy().astype(np.float64)
-    hi = mx[:3].cpu().numpy().astype(np.float64)
+    lo = np.asarray(lo, np.float64)
+    hi = np.asarray(hi, np.float64)
     ulo = (lo - sp.p0.astype(np.float64)) / d
     uhi = (hi - sp.p0.astype(np.float64)) / d
     ok = whole | ((ulo >= g_lo) & (uhi < np.asarray(sp.n, np.float64) - 1 - g_hi))
@@ -241,6 +256,23 @@ def _forward(scene: Scene, atoms, grid_valid=False, full=False):
     return FEs, PPs
 
 
+def _atoms(xyz0, ctr, pose, q):
+    import torch
+    return torch.cat([place(xyz0, ctr, pose).float(), q.float()[:, None]], 1).contiguous()
+
+
+def _pose_grad(g6, xyz0, ctr, pose):
+    """(4,) gradient w.r.t. (sx, sy, sz, theta) from the atoms' position gradients g6[:, :3]."""
+    import torch
+    gx = g6[:, :3]
+    th = pose[3]
+    dx = xyz0[:, 0] - ctr[0]
+    dy = xyz0[:, 1] - ctr[1]
+    dxr = -torch.sin(th) * dx - torch.cos(th) * dy       # d x_i / d theta
+    dyr = torch.cos(th) * dx - torch.sin(th) * dy        # d y_i / d theta
+    return torch.stack([gx[:, 0].sum(), gx[:, 1].sum(), gx[:, 2].sum(), (gx[:, 0] * dxr + gx[:, 1] * dyr).sum()])
+
+
 def _make_function():
     import torch
 
@@ -248,8 +280,7 @@ def _make_function():
         @staticmethod
         def forward(ctx, pose, q, scene):
             xyz0 = torch.as_tensor(scene.mol_xyz0, dtype=torch.float64, device=pose.device)
-            xyz = place(xyz0, torch.as_tensor(scene.center, device=pose.device), pose.detach())
-            atoms = torch.cat([xyz.float(), q.detach().float()[:, None]], 1).contiguous()
+            atoms = _atoms(xyz0, torch.as_tensor(scene.center, device=pose.device), pose.detach(), q.detach())
             FEs, PPs = _forward(scene, atoms)
             full = False
             if not scene.inside(PPs):
@@ -278,8 +309,7 @@ def _make_function():
             dev = pose.device
             xyz0 = torch.as_tensor(scene.mol_xyz0, dtype=torch.float64, device=dev)
             ctr = torch.as_tensor(scene.center, device=dev)
-            xyz = place(xyz0, ctr, pose)
-            atoms = torch.cat([xyz.float(), q.float()[:, None]], 1).contiguous()
+            atoms = _atoms(xyz0, ctr, pose, q)
             # the grid if another placement has overwritten it; the probe positions unless kept
             full = ctx.full
             spec, base, grid, grid_cot = scene.buffers(full)
@@ -305,15 +335,7 @@ def _make_function():
                 raise RuntimeError(f"pose adjoint: {scene.last_status['nonfinite']} states non-finite")
             g6 = D.grid_adjoint(atoms, scene.mol_cpar, _lib.PPM_MODEL_LJ, spec, scene.Qs, scene.QZs,
                                 grid, grid_cot, scene.npbc, scene.lvA, scene.lvB, cap=True, base=base, params=False)
-            gx = g6[:, :3]
-            th = pose[3]
-            dx = xyz0[:, 0] - ctr[0]
-            dy = xyz0[:, 1] - ctr[1]
-            dxr = -torch.sin(th) * dx - torch.cos(th) * dy       # d x_i / d theta
-            dyr = torch.cos(th) * dx - torch.sin(th) * dy        # d y_i / d theta
-            gpose = torch.stack([gx[:, 0].sum(), gx[:, 1].sum(), gx[:, 2].sum(),
-                                 (gx[:, 0] * dxr + gx[:, 1] * dyr).sum()])
-            return gpose, g6[:, 3].to(q.dtype), None
+            return _pose_grad(g6, xyz0, ctr, pose), g6[:, 3].to(q.dtype), None
 
     return _PoseDf
 
@@ -332,3 +354,273 @@ def pose_df(pose, q, scene: Scene):
     """Differentiable df image (nx, ny, nz) of the molecule placed by pose = (sx, sy, sz, theta)
     (float64 tensor) with charges q over the scene's substrate."""
     return PoseDf.fn().apply(pose, q, scene)
+
+
+# ---------------------------------------------------------------------------------------------
+# Batched placements: several placements in flight on their own streams.  One placement's
+# relaxation (one chain of dependent FIRE steps per scan point, ~1 wave per SIMD on a 192x192
+# scan) and implicit-function adjoint leave most VALU issue slots idle; the next placement's grid
+# projection or grid->atom adjoint fills them.  Each placement runs the same launches on the same
+# inputs as pose_df, so its df image is the same bits.  The per-placement work on a stream is
+# only the library's launches: a small torch kernel queued behind another stream's large launch
+# waits for free compute units, so the atom positions, the df cotangents, the column gate and the
+# pose gradients are computed for the whole batch before or after the streams run.
+
+@dataclass
+class _Slot:
+    """One placement in flight: a stream of the library's own (device.Stream) and buffers allocated
+    on the caller's stream, which waits for this stream at the end of every batched call (and when
+    the slot is dropped) before any of them can be freed."""
+    stream: object
+    FEs: object                 # (n_scan, nz, 4) relaxation output of the placement in flight
+    grid_cot: object            # adjoint accumulation buffer
+    lam: object                 # (n_scan, nz, 4) relax adjoint multipliers + status word
+    grid: object = None         # grid of a placement whose grid the forward does not keep
+    workspace: object = None    # grid->atom adjoint workspace
+
+    def __del__(self):
+        try:
+            self.stream.close()
+        except Exception:
+            pass
+
+
+def _batch_slots(scene: Scene, depth: int):
+    import torch
+    if scene.batch_slots is None or len(scene.batch_slots) != depth:
+        dev = scene.base.device
+        slots = []
+        for sl in scene.batch_slots or ():
+            sl.stream.close()
+        for _ in range(depth):
+            f32 = dict(dtype=torch.float32, device=dev)
+            sl = _Slot(stream=D.Stream(dev), FEs=torch.empty((scene.n_scan, scene.nz, 4), **f32),
+                       grid_cot=torch.empty_like(scene.base), lam=torch.empty((scene.n_scan, scene.nz, 4), **f32))
+            slots.append(sl)
+        scene.batch_slots = slots
+    return scene.batch_slots
+
+
+def _slot_grid(sl: _Slot, base):
+    import torch
+    if sl.grid is None:
+        sl.grid = torch.empty_like(base)
+    return sl.grid
+
+
+class _Chain:
+    """Runs the VALU-bound launches of a batch (grid projections, grid->atom adjoints) one at a
+    time across the streams: each waits for the previous one, so a placement's latency-bound
+    relaxation or relax adjoint runs beside the next placement's VALU-bound kernel rather than
+    beside another relaxation."""
+
+    def __init__(self):
+        self.last = None
+
+    def __call__(self, stream):
+        import contextlib
+        import torch
+
+        @contextlib.contextmanager
+        def ctx():
+            if self.last is not None:
+                stream.wait_event(self.last)
+            yield
+            self.last = torch.cuda.Event()
+            self.last.record(stream)
+        return ctx()
+
+
+def _atoms_batch(xyz0, ctr, poses, qs):
+    """(B, na, 4) float32 atoms of B placements: row b equals _atoms(xyz0, ctr, poses[b], qs[b])
+    bit for bit (the same elementwise operations)."""
+    import torch
+    th = poses[:, 3:4]
+    c, s = torch.cos(th), torch.sin(th)
+    dx = (xyz0[:, 0] - ctr[0])[None, :]
+    dy = (xyz0[:, 1] - ctr[1])[None, :]
+    x = c * dx - s * dy + ctr[0] + poses[:, 0:1]
+    y = s * dx + c * dy + ctr[1] + poses[:, 1:2]
+    z = xyz0[None, :, 2] + poses[:, 2:3]
+    return torch.stack([x.float(), y.float(), z.float(), qs.float()], 2).contiguous()
+
+
+def _pose_grads(g6, xyz0, ctr, poses):
+    """(B, 4) gradients w.r.t. (sx, sy, sz, theta) from the atoms' position gradients g6 (B, na, 6)."""
+    import torch
+    gx = g6[..., :3]
+    th = poses[:, 3:4]
+    dx = (xyz0[:, 0] - ctr[0])[None, :]
+    dy = (xyz0[:, 1] - ctr[1])[None, :]
+    dxr = -torch.sin(th) * dx - torch.cos(th) * dy       # d x_i / d theta
+    dyr = torch.cos(th) * dx - torch.sin(th) * dy        # d y_i / d theta
+    return torch.stack([gx[..., 0].sum(1), gx[..., 1].sum(1), gx[..., 2].sum(1),
+                        (gx[..., 0] * dxr + gx[..., 1] * dyr).sum(1)], 1)
+
+
+def _make_batch_function():
+    import torch
+    from .diff import gate_from_diag, gate_stats
+
+    class _PoseDfBatch(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, poses, q, scene):
+            if scene.base is None:
+                raise RuntimeError("Scene.prepare() must run before pose_df_batch")
+            dev = poses.device
+            B = int(poses.shape[0])
+            poses_d, q_d = poses.detach(), q.detach()
+            qs = q_d if q_d.dim() == 2 else q_d[None].expand(B, -1)
+            xyz0 = torch.as_tensor(scene.mol_xyz0, dtype=torch.float64, device=dev)
+            ctr = torch.as_tensor(scene.center, device=dev)
+            spec, base, pts, tip = scene.spec_run, scene.base, scene.pts, scene.tip
+            nx, ny = (int(v) for v in scene.nxy)
+            ns, nz = scene.n_scan, scene.nz
+            depth = max(1, int(scene.batch_depth))
+            slots = _batch_slots(scene, depth)
+            keep = bool(ctx.needs_input_grad[0] or ctx.needs_input_grad[1])
+            n_keep = min(B, int(scene.batch_grid_budget // (base.numel() * 4))) if keep else 0
+            gate = scene.gate_eig_frac > 0
+            op = DF.z_operator(scene.df_kind, nz, scene.dz, dev, scene.amp, scene.f0, scene.k_cant)
+            cur = torch.cuda.current_stream(dev)
+            atoms = _atoms_batch(xyz0, ctr, poses_d, qs)
+            f32 = dict(dtype=torch.float32, device=dev)
+            PPs = torch.empty((B, ns, nz, 4), **f32)
+            df = torch.empty((B, nx, ny, nz), **f32)
+            dg = torch.empty((B, ns, nz, 4), **f32) if gate else None
+            grids = torch.empty((n_keep, *base.shape), dtype=base.dtype, device=dev)   # one allocation
+            for sl in slots:
+                if n_keep < B:
+                    _slot_grid(sl, base)               # on the caller's stream, before the loop
+                sl.stream.torch.wait_stream(cur)
+            chain = _Chain()
+            for b in range(B):
+                sl = slots[b % depth]
+                with torch.cuda.stream(sl.stream.torch):
+                    g = grids[b] if b < n_keep else _slot_grid(sl, base)
+                    with chain(sl.stream.torch):
+                        D.grid_project(atoms[b], scene.mol_cpar, _lib.PPM_MODEL_LJ, spec, scene.Qs, scene.QZs,
+                                       scene.npbc, scene.lvA, scene.lvB, cap=True, out=g, base=base)
+                    D.relax_tilted(g, spec, pts, nz, tip, scene.max_iter, scene.f2conv, True, scan_ny=ny, FEs=sl.FEs,
+                                   PPs=PPs[b])
+                    if scene.polish > 0:
+                        D.relax_polish(g, spec, pts, nz, tip, PPs[b], sl.FEs, scene.polish)
+                    if gate:
+                        D.relax_state_diag(g, spec, pts, nz, tip, PPs[b], out=dg[b])
+                    op.apply(sl.FEs, (nz * 4, 4, 0), ns, 1, offset=2, out=df[b])
+            for sl in slots:
+                cur.wait_stream(sl.stream.torch)
+            # probes that left the scan window: that placement again on the full grid
+            full = np.zeros(B, bool)
+            if scene.windowed:
+                mn, mx = torch.aminmax(PPs.reshape(B, -1, 4), dim=1)
+                mn, mx = mn[:, :3].cpu().numpy(), mx[:, :3].cpu().numpy()
+                for b in range(B):
+                    if _inside_bounds(spec, scene.window_faces, mn[b], mx[b]):
+                        continue
+                    scene.window_fallbacks += 1
+                    full[b] = True
+                    FEs_b, PPs_b = _forward(scene, atoms[b], full=True)
+                    PPs[b] = PPs_b
+                    op.apply(FEs_b, (nz * 4, 4, 0), ns, 1, offset=2, out=df[b])
+                    if gate:
+                        spec_f, _, grid_f, _ = scene.buffers(True)
+                        D.relax_state_diag(grid_f, spec_f, pts, nz, tip, PPs[b], out=dg[b])
+            if gate:
+                keep_cols, parts = gate_from_diag(dg, PPs, pts, nz, tip, scene.gate_eig_frac, scene.gate_max_jump)
+                del dg
+                scene.last_columns = keep_cols.reshape(B, nx, ny)
+                scene.last_gate = gate_stats(parts, scene.gate_eig_frac, scene.gate_max_jump)
+            ctx.scene, ctx.grids, ctx.full, ctx.depth, ctx.PPs, ctx.atoms = scene, grids, full, depth, PPs, atoms
+            ctx.q_shared = q.dim() == 1
+            ctx.save_for_backward(poses_d, q_d)
+            return df
+
+        @staticmethod
+        def backward(ctx, gdf):
+            scene = ctx.scene
+            poses, q = ctx.saved_tensors
+            dev = poses.device
+            B = int(poses.shape[0])
+            xyz0 = torch.as_tensor(scene.mol_xyz0, dtype=torch.float64, device=dev)
+            ctr = torch.as_tensor(scene.center, device=dev)
+            spec, base, pts, tip = scene.spec_run, scene.base, scene.pts, scene.tip
+            ns, nz = scene.n_scan, scene.nz
+            slots = _batch_slots(scene, ctx.depth)
+            opT = DF.z_operator(scene.df_kind, nz, scene.dz, dev, scene.amp, scene.f0, scene.k_cant, transpose=True)
+            cur = torch.cuda.current_stream(dev)
+            grids, PPs, atoms = ctx.grids, ctx.PPs, ctx.atoms
+            # dL/dFEs of every placement (Fz channel only)
+            g_all = gdf.detach().float().contiguous().reshape(B * ns, nz)
+            U = torch.zeros((B, ns, nz, 4), dtype=torch.float32, device=dev)
+            U[..., 2] = opT.apply(g_all, (nz, 1, 0), B * ns, 1).reshape(B, ns, nz)
+            g6 = torch.empty((B, atoms.shape[1], 6), dtype=torch.float64, device=dev)
+            status = torch.empty((B, ns, nz), dtype=torch.float32, device=dev)
+            chain = _Chain()
+
+            def adjoint(b, g, spec_b, base_b, grid_cot, lam, workspace, stream):
+                grid_cot.zero_()
+                D.relax_adjoint(g, spec_b, pts, nz, tip, PPs[b], U[b], grid_cot=grid_cot, lam=lam)
+                status[b].copy_(lam[..., 3])
+                with chain(stream):
+                    D.grid_adjoint(atoms[b], scene.mol_cpar, _lib.PPM_MODEL_LJ, spec_b, scene.Qs, scene.QZs, g,
+                                   grid_cot, scene.npbc, scene.lvA, scene.lvB, cap=True, workspace=workspace,
+                                   base=base_b, params=False, out=g6[b])
+
+            need = int(_lib.load().ppm_grid_adjoint_workspace_size(int(atoms.shape[1]), ctypes.byref(spec.c())))
+            for sl in slots:
+                if sl.workspace is None or sl.workspace.numel() < need:
+                    sl.workspace = torch.empty(need, dtype=torch.uint8, device=dev)
+                if len(grids) < B:
+                    _slot_grid(sl, base)
+                sl.stream.torch.wait_stream(cur)
+            for b in range(B):
+                if ctx.full[b]:
+                    continue
+                sl = slots[b % ctx.depth]
+                with torch.cuda.stream(sl.stream.torch):
+                    if b < len(grids):
+                        g = grids[b]
+                    else:
+                        g = _slot_grid(sl, base)
+                        with chain(sl.stream.torch):
+                            D.grid_project(atoms[b], scene.mol_cpar, _lib.PPM_MODEL_LJ, spec, scene.Qs, scene.QZs,
+                                           scene.npbc, scene.lvA, scene.lvB, cap=True, out=g, base=base)
+                    adjoint(b, g, spec, base, sl.grid_cot, sl.lam, sl.workspace, sl.stream.torch)
+            for sl in slots:
+                cur.wait_stream(sl.stream.torch)
+            for b in np.nonzero(ctx.full)[0]:        # full-grid placements, one at a time
+                _project(scene, atoms[b], full=True)
+                spec_f, base_f, grid_f, cot_f = scene.buffers(True)
+                adjoint(int(b), grid_f, spec_f, base_f, cot_f, slots[0].lam.new_empty((ns, nz, 4)), None, cur)
+            ctx.grids = ctx.PPs = ctx.atoms = None
+            st = status.to(torch.int32)
+            scene.last_status = {"states": B * ns * nz, "gated": int((st != 0).sum()),
+                                 "nonfinite": int(((st & 2) != 0).sum())}
+            if scene.strict and scene.last_status["nonfinite"]:
+                raise RuntimeError(f"pose adjoint: {scene.last_status['nonfinite']} states non-finite")
+            gpose = _pose_grads(g6, xyz0, ctr, poses)
+            gq = g6[..., 3]
+            return gpose.to(poses.dtype), (gq.sum(0) if ctx.q_shared else gq).to(q.dtype), None
+
+    return _PoseDfBatch
+
+
+class PoseDfBatch:
+    _fn = None
+
+    @classmethod
+    def fn(cls):
+        if cls._fn is None:
+            cls._fn = _make_batch_function()
+        return cls._fn
+
+
+def pose_df_batch(poses, q, scene: Scene):
+    """Differentiable df images (B, nx, ny, nz) of B placements, poses (B, 4) = (sx, sy, sz, theta)
+    (float64 tensor), charges q (na,) shared by all placements or (B, na).  Placement b gives the
+    bits of pose_df(poses[b], q, scene); up to scene.batch_depth placements run at once, each on a
+    stream of its own.  The forward keeps the placements' grids for the backward while they fit in
+    scene.batch_grid_budget bytes (the backward recomputes the others) and their probe positions
+    (16 B per state).  With column gating on, scene.last_columns is (B, nx, ny)."""
+    return PoseDfBatch.fn().apply(poses, q, scene)

@@ -73,7 +73,8 @@ def _legs():
         "direct_morse": {"forward_ms": 3.6843, "vjp_ms": 0.56994, "vjp_over_forward": 0.15469,
                          "pixel_z_steps_per_s": 66704000.0},
         "substrate_poses": {"forward_ms_median": 15.285, "adjoint_ms_median": 13.981, "adjoint_over_forward": 0.92071,
-                            "placements_per_s_forward_plus_adjoint": 33.98, "window_fallbacks": 0,
+                            "placements_per_s_forward_plus_adjoint": 33.98, "batched_placements_per_s": 52.123,
+                            "window_fallbacks": 0,
                             "gated_states_total": 39059, "gated_columns_total": [140093, 2359296],
                             "directional_fd_rel_err": [4.9427e-05, 3.7668e-05],
                             "directional_fd": [{"pose": k, "adjoint": -68241.49, "fd_eps_0.001": -68244.86}
