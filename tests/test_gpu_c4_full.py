@@ -135,3 +135,29 @@ def test_c4_pose_gradient_directional_fd(c4, k):
           f"FD {fds}, rel {rel}", flush=True)
     assert cols.sum() >= 0.8 * cols.numel()
     assert min(rel) <= 1e-3
+
+
+def test_c4_batch_equals_one_at_a_time(c4):
+    """Four placements through pose_df_batch (two in flight, grids kept) at full size: each df
+    image equals pose_df's bit for bit and the pose gradients agree to the rounding of the relax
+    adjoint's atomic scatter (rtol 1e-5)."""
+    sc, scene = c4
+    dev = scene.pts.device
+    P = torch.tensor(np.asarray(sc["poses"][4:8]), dtype=torch.float64, device=dev)
+    q0 = torch.tensor(sc["mol_atoms"][:, 3], dtype=torch.float64, device=dev)
+    v = torch.as_tensor(np.random.default_rng(10).standard_normal((4, *sc["nxy"], sc["nz"])), device=dev)
+    ref = []
+    for k in range(4):
+        pose = P[k].clone().requires_grad_(True)
+        df = PS.pose_df(pose, q0, scene)
+        (df.double() * v[k]).sum().backward()
+        ref.append((df.detach(), pose.grad))
+    poses = P.clone().requires_grad_(True)
+    df = PS.pose_df_batch(poses, q0, scene)
+    (df.double() * v).sum().backward()
+    for k, (d, g) in enumerate(ref):
+        rel = float((poses.grad[k] - g).norm() / g.norm())
+        print(f"[c4] batch placement {k}: df equal {bool(torch.equal(df[k].detach(), d))}, pose grad rel {rel:.2e}",
+              flush=True)
+        assert torch.equal(df[k].detach(), d)
+        assert torch.allclose(poses.grad[k], g, rtol=1e-5, atol=1e-6 * float(g.abs().max()))
