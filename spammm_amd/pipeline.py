@@ -5,8 +5,7 @@ by the relaxed scan over that grid (relaxStrokesTilted, kernels/AFM.cl:478-562) 
 applied to each Fz column in the same launch.  On one stream the relaxation runs alone on the
 machine: one scan point is a serial chain of FIRE steps, so a 256x256 raster (two lanes per point)
 fills about two waves per SIMD, and most VALU issue slots sit idle while those chains wait on their
-own results.  The
-grid projection of the next image has no such wait.  ``ScanPipeline`` runs the images of a sequence
+own results.  The grid projection of the next image has no such wait.  ``ScanPipeline`` runs the images of a sequence
 on ``depth`` streams, image k on stream k % depth with its own grid and output buffers: the grid
 projections run one after another (each waits for the previous image's projection), and image k's
 relaxation runs beside image k+1's projection.  Every image gives the same bits as a scan of it
@@ -25,9 +24,9 @@ from . import device as D
 
 @dataclass
 class ScanSlot:
-    """The buffers and events of one in-flight image: ``grid`` (nz,ny,nx,4), ``FEs``/``PPs``
-    (n_scan,nz,4), ``df`` (n_scan,n_out) or None.  Valid once ``done`` has completed; reused by the
-    image submitted ``depth`` images later."""
+    """The stream (device.Stream), buffers and events of one in-flight image: ``grid`` (nz,ny,nx,4),
+    ``FEs``/``PPs`` (n_scan,nz,4), ``df`` (n_scan,n_out) or None.  Valid once ``done`` has completed;
+    reused by the image submitted ``depth`` images later."""
     stream: object
     grid: object
     FEs: object
@@ -42,10 +41,6 @@ class ScanSlot:
         """Make ``stream`` (default: the current stream) wait for this image's outputs."""
         import torch
         (stream or torch.cuda.current_stream(self.grid.device)).wait_event(self.done)
-
-    @property
-    def torch_stream(self):
-        return self.stream.torch
 
 
 class ScanPipeline:
@@ -76,9 +71,9 @@ class ScanPipeline:
         f32 = dict(dtype=torch.float32, device=self.dev)
         self.slots = []
         for i in range(depth):
-            # streams of the library's own (each on its own hardware queue; default priority): a slot
-            # on the caller's stream would make the next image's inputs wait for this image's
-            # relaxation
+            # streams of the library's own (device.Stream: each takes the least used hardware queue;
+            # default priority); a slot on the caller's stream would make the next image's inputs
+            # wait for this image's relaxation
             self.slots.append(ScanSlot(
                 stream=D.Stream(self.dev), grid=torch.empty((ng, ny, nx, 4), **f32),
                 FEs=torch.empty((ns, self.nz, 4), **f32),
