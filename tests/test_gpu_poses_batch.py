@@ -93,3 +93,30 @@ def test_batch_with_full_grid_fallback():
     got = _batched(tight, sc, P, v)
     _check(ref, got, "full-grid fallback")
     assert n_ref >= 1 and tight.window_fallbacks == 2 * n_ref
+
+
+def test_batch_with_per_placement_charges():
+    """Charges (B, na), one row per placement: each row's gradient is that placement's q.grad."""
+    sc = W.small_substrate_scene()
+    scene, _ = make_scene(sc)
+    dev = scene.pts.device
+    P = _poses(sc)[:4]
+    q0 = sc["mol_atoms"][:, 3]
+    Q = np.stack([q0 * (1.0 + 0.1 * k) for k in range(P.shape[0])])
+    v = torch.as_tensor(np.random.default_rng(14).standard_normal((P.shape[0], *sc["nxy"], sc["nz"])), device=dev)
+    ref = []
+    for k in range(P.shape[0]):
+        pose = torch.tensor(P[k], dtype=torch.float64, device=dev, requires_grad=True)
+        q = torch.tensor(Q[k], dtype=torch.float64, device=dev, requires_grad=True)
+        df = PS.pose_df(pose, q, scene)
+        (df.double() * v[k]).sum().backward()
+        ref.append((df.detach(), pose.grad, q.grad))
+    poses = torch.tensor(P, dtype=torch.float64, device=dev, requires_grad=True)
+    qb = torch.tensor(Q, dtype=torch.float64, device=dev, requires_grad=True)
+    df = PS.pose_df_batch(poses, qb, scene)
+    (df.double() * v).sum().backward()
+    assert qb.grad.shape == qb.shape
+    for k, (d, gp, gq) in enumerate(ref):
+        assert torch.equal(df[k].detach(), d), k
+        assert torch.allclose(poses.grad[k], gp, rtol=1e-5, atol=1e-6 * float(gp.abs().max())), k
+        assert torch.allclose(qb.grad[k], gq, rtol=1e-5, atol=1e-6 * float(gq.abs().max())), k
