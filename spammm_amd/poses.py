@@ -386,8 +386,12 @@ class _Slot:
 
 
 def _batch_slots(scene: Scene, depth: int):
+    """The scene's `depth` slots, made again when the depth or the grid / scan shapes changed (a
+    later prepare() with other window margins)."""
     import torch
-    if scene.batch_slots is None or len(scene.batch_slots) != depth:
+    old = scene.batch_slots
+    if (old is None or len(old) != depth or old[0].grid_cot.shape != scene.base.shape
+            or old[0].FEs.shape != (scene.n_scan, scene.nz, 4)):
         dev = scene.base.device
         slots = []
         for sl in scene.batch_slots or ():
