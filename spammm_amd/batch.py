@@ -335,12 +335,16 @@ class BatchForward:
 
     # ── the plain per-image path (baseline) ───────────────────────────────────
 
-    def plain_loop(self, FEs_out=None, df_out=None, windowed=False):
+    def plain_loop(self, FEs_out=None, df_out=None, windowed=False, streams=1):
         """The same work as a loop of plain per-image launches: returns (FEs, df).  windowed=False:
         whole grids; True: each molecule's grid projected and relaxed on the z window of its batch
         descriptor (ppm_grid_project_window + ppm_relax_strokes_tilted_window), the same pair
         evaluations as the batched launch; a molecule whose probes leave the window is recomputed
-        with its whole grid (as the batched path does)."""
+        with its whole grid (as the batched path does).  streams > 1: molecule i on stream
+        i % streams (device.Stream) with a grid buffer of its own, the grid projections one at a
+        time across the streams, so a molecule's relaxation runs beside the next one's projection
+        (the two-stream form of pipeline.ScanPipeline for molecules of different grids)."""
+        import contextlib
         import torch
         FEs = FEs_out if FEs_out is not None else torch.empty_like(self.FEs)
         df = df_out if df_out is not None else (torch.empty_like(self.df) if self.df is not None else None)
@@ -349,27 +353,50 @@ class BatchForward:
                          for w in self.molecules]
         if windowed:
             flags = torch.zeros(len(self.molecules), dtype=torch.int32, device=self.device)
+        shapes = [(int(d.nzw) if windowed else int(sp.n[2]), int(sp.n[1]), int(sp.n[0]), 4)
+                  for sp, d in zip(self.specs, self.descs_host)]
+        cur = torch.cuda.current_stream(self.device)
+        if streams > 1:
+            lanes = [D.Stream(self.device) for _ in range(streams)]
+            big = max(int(np.prod(s)) for s in shapes)
+            bufs = [torch.empty(big, dtype=torch.float32, device=self.device) for _ in range(streams)]
+            for ln in lanes:
+                ln.torch.wait_stream(cur)
+        last = None
         cache = {}
         for i, (w, sp, d) in enumerate(zip(self.molecules, self.specs, self.descs_host)):
-            nzg = int(d.nzw) if windowed else int(sp.n[2])
-            key = (int(sp.n[0]), int(sp.n[1]), nzg)
-            if key not in cache:
-                cache.clear()
-                cache[key] = torch.empty((nzg, int(sp.n[1]), int(sp.n[0]), 4), dtype=torch.float32,
-                                         device=self.device)
-            G = cache[key]
+            if streams > 1:
+                s = lanes[i % streams].torch
+                G = bufs[i % streams][:int(np.prod(shapes[i]))].view(shapes[i])
+                ctx = torch.cuda.stream(s)
+            else:
+                if shapes[i] not in cache:
+                    cache.clear()
+                    cache[shapes[i]] = torch.empty(shapes[i], dtype=torch.float32, device=self.device)
+                G = cache[shapes[i]]
+                ctx = contextlib.nullcontext()
             a0, a1 = d.atom_off, d.atom_off + d.na
             win = (int(d.klo), int(d.nzw)) if windowed else None
-            D.grid_project(self.atoms[a0:a1], self.cpar[a0:a1], self.model, sp, self.Qs, self.QZs, w.npbc, w.lvA,
-                           w.lvB, cap=self.cap, out=G, window=win)
-            pts = self._pts[i]
-            p0, p1 = d.pix_off, d.pix_off + d.scan_nx * d.scan_ny
-            D.relax_tilted(G, sp, pts, self.nz, self.tip, self.max_iter, self.f2conv, self.check_before,
-                           scan_ny=d.scan_ny, FEs=FEs[p0:p1], want_pps=False, window=win,
-                           flag=flags[i:i + 1] if windowed else None)
-            if df is not None:
-                self.dfop.apply(FEs, (self.nz * 4, 4, 0), p1 - p0, 1, offset=2 + p0 * self.nz * 4,
-                                out=df[p0:p1].view(p1 - p0, self.dfop.n_out, 1))
+            with ctx:
+                if streams > 1 and last is not None:
+                    s.wait_event(last)                 # projections one at a time
+                D.grid_project(self.atoms[a0:a1], self.cpar[a0:a1], self.model, sp, self.Qs, self.QZs, w.npbc,
+                               w.lvA, w.lvB, cap=self.cap, out=G, window=win)
+                if streams > 1:
+                    last = torch.cuda.Event()
+                    last.record(s)
+                pts = self._pts[i]
+                p0, p1 = d.pix_off, d.pix_off + d.scan_nx * d.scan_ny
+                D.relax_tilted(G, sp, pts, self.nz, self.tip, self.max_iter, self.f2conv, self.check_before,
+                               scan_ny=d.scan_ny, FEs=FEs[p0:p1], want_pps=False, window=win,
+                               flag=flags[i:i + 1] if windowed else None)
+                if df is not None:
+                    self.dfop.apply(FEs, (self.nz * 4, 4, 0), p1 - p0, 1, offset=2 + p0 * self.nz * 4,
+                                    out=df[p0:p1].view(p1 - p0, self.dfop.n_out, 1))
+        if streams > 1:
+            for ln in lanes:
+                ln.close()                             # the current stream waits for it
+            del bufs
         if windowed:
             bad = np.flatnonzero(flags.cpu().numpy())
             self.last_plain_fallback = bad
