@@ -14,7 +14,7 @@ Default workload (flake): 200-atom flat polycyclic flake, 30x30x24 A cell on a
 0.1 A grid (300x300x240 float4), 256x256 pixels x 60 z-steps.
 
 Prints ONE compact JSON line (under 4 KB: the headline, roofline, CPU baseline and a summary
-per leg) and writes the full per-leg record to --detail (default gpurun_out/bench_detail.json).
+per leg) and writes the full per-leg record to --detail (default bench_detail.json).
 Roofline of the dominant kernel (grid projection) is
 measured live with HIP events on the stream the kernels run on; the CPU baseline
 times the C++ oracle (FP32 mode, OpenMP) on a bounded sample of the same workload.
@@ -59,7 +59,7 @@ def main():
     ap.add_argument("--no-ewald", action="store_true", help="skip the 2D Ewald substrate electrostatics leg")
     ap.add_argument("--z-margin", default=None,
                     help="z window margins of the batched leg in A, 'below,above' (default: the library's)")
-    ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
+    ap.add_argument("--detail", default="bench_detail.json",
                     help="file for the full per-leg record (the printed line carries summaries)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 1 GiB device-to-device copy (scale of the HBM byte counters under rocprofv3 --pmc)")

@@ -41,7 +41,7 @@ def _head():
                          "sample": "grid: 10 of 240 z-planes; relax: 4096 of 65536 pixels x 60 z; scaled to the "
                                    "full step (grid 17.25 s + relax 0.40 s)",
                          "grid_points_per_s": 1252000.0, "relax_pixel_z_steps_per_s": 9905300.0},
-        "detail_file": "gpurun_out/bench_detail.json",
+        "detail_file": "bench_detail.json",
     }
 
 
