@@ -113,3 +113,36 @@ def tune(batch: BatchForward, candidates=None, reps: int = 2, use_cache: bool = 
         data[sig] = {"best": best.__dict__, "best_ms": best_ms, "timings": results}
         _store(data)
     return best, results
+
+
+def refine(batch: BatchForward, cfg: LaunchConfig, lags=(32, 64), reps: int = 2, log=None):
+    """The tuned shape ``cfg`` checked on the whole ``batch``: the same vz, tile and workgroups per
+    CU as two-phase and as fused with each queue lag in ``lags`` (lags longer than the tuning
+    sample's molecule count; a relaxation block then finds its tiles done instead of waiting), each
+    timed on the whole batch; returns (the fastest, {key: ms}).  Every variant must give the first
+    one's results bit for bit."""
+    import torch
+    variants = [cfg, LaunchConfig(cfg.vz, cfg.tile_w, cfg.workers_per_cu, "two_phase", 0)]
+    variants += [LaunchConfig(cfg.vz, cfg.tile_w, cfg.workers_per_cu, "fused", int(lag)) for lag in lags
+                 if int(lag) < len(batch.molecules)]
+    seen, results, ref = set(), {}, None
+    best, best_ms = cfg, float("inf")
+    for v in variants:
+        if v.key() in seen:
+            continue
+        seen.add(v.key())
+        ms = time_config(batch, v, reps)
+        err, _ = batch.status()
+        if err:
+            raise RuntimeError(f"autotune: launch {v.key()} reported a wait timeout")
+        out = batch.df.clone() if batch.df is not None else batch.FEs.clone()
+        if ref is None:
+            ref = out
+        elif not torch.equal(ref, out):
+            raise RuntimeError(f"autotune: launch {v.key()} changed the results")
+        results[v.key()] = ms
+        if log:
+            log(f"[autotune] whole batch {v.key()}: {ms:.3f} ms")
+        if ms < best_ms:
+            best, best_ms = v, ms
+    return best, results

@@ -102,6 +102,10 @@ def test_autotune_picks_a_config_with_identical_results(tmp_path, monkeypatch):
     assert len(res) == 8 and best.key() in res
     best2, _ = autotune.tune(bf, cand, reps=1)      # served from the cache
     assert best2.key() == best.key()
+    ref = bf.forward(best).clone()
+    best3, whole = autotune.refine(bf, best, lags=(2, 3), reps=1)   # every variant checked bit for bit inside
+    assert best3.key() in whole and len(whole) >= 3
+    assert torch.equal(bf.forward(best3), ref)
 
 
 def test_grid_and_relax_windows_equal_the_whole_grid():
@@ -140,6 +144,10 @@ def test_windowed_plain_loop_equals_batch_and_whole_grids():
     torch.cuda.synchronize()
     assert bf.last_plain_fallback.size == 0
     assert torch.equal(FEw, bf.FEs) and torch.equal(dfw, df)
+    for windowed in (False, True):         # a molecule's relaxation beside the next one's projection
+        FE2, df2 = bf.plain_loop(windowed=windowed, streams=2)
+        torch.cuda.synchronize()
+        assert torch.equal(FE2, bf.FEs) and torch.equal(df2, df), windowed
     bf2 = make_batch(4, z_margin=-0.5)     # thin windows: the windowed plain loop must fall back as well
     FEw2, dfw2 = bf2.plain_loop(windowed=True)
     torch.cuda.synchronize()
