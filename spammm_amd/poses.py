@@ -471,6 +471,11 @@ def _make_batch_function():
         def forward(ctx, poses, q, scene):
             if scene.base is None:
                 raise RuntimeError("Scene.prepare() must run before pose_df_batch")
+            na = int(scene.mol_xyz0.shape[0])
+            if poses.dim() != 2 or poses.shape[1] != 4 or poses.shape[0] < 1:
+                raise ValueError(f"poses must be (B >= 1, 4), got {tuple(poses.shape)}")
+            if tuple(q.shape) not in ((na,), (int(poses.shape[0]), na)):
+                raise ValueError(f"q must be ({na},) or ({int(poses.shape[0])}, {na}), got {tuple(q.shape)}")
             dev = poses.device
             B = int(poses.shape[0])
             poses_d, q_d = poses.detach(), q.detach()
