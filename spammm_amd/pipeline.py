@@ -89,6 +89,8 @@ class ScanPipeline:
         relaxation end) in ``slot.events``."""
         import torch
         slot = self.slots[self.count % len(self.slots)]
+        if slot.stream.handle is None:
+            raise RuntimeError("the pipeline is closed")
         cur = torch.cuda.current_stream(self.dev)
         s = slot.stream.torch
         i = self.count % len(self.slots)

@@ -62,6 +62,10 @@ def test_pipeline_images_equal_scans_alone(depth):
             assert torch.equal(x, y), (k, name)
     # the images differ from each other (a pipeline that mixed up its buffers could not pass)
     assert not torch.equal(want[0][2], want[1][2])
+    pipe.close()
+    assert torch.equal(pipe.slots[(len(sets) - 1) % depth].df, want[-1][2])    # buffers outlive the streams
+    with pytest.raises(RuntimeError):
+        pipe.submit(sets[0], cpar)
 
 
 def test_pipeline_rejects_one_buffer_set():
