@@ -77,6 +77,12 @@ typedef struct PpmRelaxCtl {
                              reference's default OPT_FIRE=1 build); 1 = damped velocity
                              v *= (1 - damp) with constant dt, damp (its OPT_FIRE=0 build,
                              AFMulator(use_fire=False), kernels/AFM.cl:378-382, :536-540) */
+    int   lanes;          /* lanes per scan point of ppm_relax_strokes_tilted*, ppm_relax_strokes and
+                             ppm_relax_strokes_2d:
+                             1, 2 or 4, or 0 = the library's choice (the fastest for a launch that has
+                             the GPU to itself: about two waves per SIMD).  Results do not depend on
+                             it; 1 issues the fewest instructions, for a relaxation that shares the
+                             GPU with other work (consecutive scans on two streams). */
 } PpmRelaxCtl;
 
 /* One molecule of a batch (an array of these lives in device memory).  The molecule's grid

@@ -42,7 +42,8 @@ class PpmTip(ctypes.Structure):
 
 class PpmRelaxCtl(ctypes.Structure):
     _fields_ = [("max_iter", ctypes.c_int), ("f2conv", ctypes.c_float), ("check_before", ctypes.c_int),
-                ("scan_ny", ctypes.c_int), ("f32_anchors", ctypes.c_int), ("damped", ctypes.c_int)]
+                ("scan_ny", ctypes.c_int), ("f32_anchors", ctypes.c_int), ("damped", ctypes.c_int),
+                ("lanes", ctypes.c_int)]
 
 
 EXPORTS = (

@@ -68,6 +68,10 @@ static int check_ctl(const char* fn, const PpmRelaxCtl* c, int n_scan) {
     }
     if (!std::isfinite(c->f2conv) || c->f2conv < 0.f) { set_error("%s: invalid f2conv", fn); return PPM_ERR_INVALID_ARG; }
     if (c->damped != 0 && c->damped != 1) { set_error("%s: damped=%d is not 0 or 1", fn, c->damped); return PPM_ERR_INVALID_ARG; }
+    if (c->lanes != 0 && c->lanes != 1 && c->lanes != 2 && c->lanes != 4) {
+        set_error("%s: lanes=%d is not 0, 1, 2 or 4", fn, c->lanes);
+        return PPM_ERR_INVALID_ARG;
+    }
     if (c->scan_ny < 0 || (c->scan_ny > 0 && n_scan % c->scan_ny != 0)) {
         set_error("%s: scan_ny=%d does not divide n_scan=%d", fn, c->scan_ny, n_scan);
         return PPM_ERR_INVALID_ARG;

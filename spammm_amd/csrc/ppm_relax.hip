@@ -639,7 +639,7 @@ int launch_relax_tilted(const float* grid, const PpmGridSpec& s, const float* pt
     p.dtip = tip.tipC[3];
     for (int q = 0; q < 4; ++q) { p.k[q] = tip.stiffness[q]; p.d0[q] = tip.dpos0[q]; }
     p.dt0 = tip.relax[0]; p.damp0 = tip.relax[1]; p.surf = tip.surfFF[0];
-    const int G = relax_group(n_scan);
+    const int G = ctl.lanes > 0 ? ctl.lanes : relax_group(n_scan);
     const dim3 blocks((unsigned)group_waves(n_scan, ctl.scan_ny, G));
     GridDev gd = make_grid(grid, s);
     if (nzw > 0) { gd.klo = klo; gd.nzw = nzw; gd.flag = flag; }
@@ -667,7 +667,7 @@ int launch_relax_strokes(const float* grid, const PpmGridSpec& s, const float* p
     for (int q = 0; q < 3; ++q) p.dTip[q] = dTip[q];
     for (int q = 0; q < 4; ++q) { p.k[q] = stiffness[q]; p.d0[q] = dpos0[q]; }
     p.dt0 = relax[0]; p.damp0 = relax[1];
-    const int G = relax_group(n_scan);
+    const int G = ctl.lanes > 0 ? ctl.lanes : relax_group(n_scan);
     const dim3 blocks((unsigned)group_waves(n_scan, ctl.scan_ny, G));
     const GridDev gd = make_grid(grid, s);
     const float4* P = reinterpret_cast<const float4*>(pts);
@@ -685,7 +685,7 @@ int launch_relax_strokes_2d(const float* grid, const PpmGridSpec& s, const float
     Strokes2DArgs p{};
     p.n_scan = n_scan; p.nz = nz; p.max_iter = ctl.max_iter; p.scan_ny = ctl.scan_ny; p.f2conv = ctl.f2conv;
     p.k_lat = k_lat; p.dh = dh; p.dt = dt; p.damp = damp;
-    const int G = relax_group(n_scan);
+    const int G = ctl.lanes > 0 ? ctl.lanes : relax_group(n_scan);
     const dim3 blocks((unsigned)group_waves(n_scan, ctl.scan_ny, G));
     const GridDev gd = make_grid(grid, s);
     const float4* P = reinterpret_cast<const float4*>(pts);

@@ -145,9 +145,10 @@ class Tip:
 
 
 def relax_ctl(max_iter=128, f2conv=1e-8, check_before=False, scan_ny=0, f32_anchors=False,
-              damped=False) -> _lib.PpmRelaxCtl:
+              damped=False, lanes=0) -> _lib.PpmRelaxCtl:
     """Relaxation control record; ``damped``: the damped-velocity integrator of the reference's
-    OPT_FIRE=0 build instead of FIRE."""
+    OPT_FIRE=0 build instead of FIRE; ``lanes``: lanes per scan point (1, 2, 4; 0 = the library's
+    choice for a launch that has the GPU to itself; results do not depend on it)."""
     c = _lib.PpmRelaxCtl()
     c.max_iter = int(max_iter)
     c.f2conv = float(f2conv)
@@ -155,6 +156,7 @@ def relax_ctl(max_iter=128, f2conv=1e-8, check_before=False, scan_ny=0, f32_anch
     c.scan_ny = int(scan_ny)
     c.f32_anchors = int(bool(f32_anchors))
     c.damped = int(bool(damped))
+    c.lanes = int(lanes)
     return c
 
 
@@ -244,13 +246,13 @@ def grid_project_f64(atoms, cpar, model: int, spec: GridSpec, Qs, QZs, npbc=(0, 
 
 def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, f2conv=1e-8, check_before=False,
                  scan_ny=0, FEs=None, PPs=None, want_pps=True, damped=False, window=None, flag=None, df_op=None,
-                 df=None):
+                 df=None, lanes=0):
     """relaxStrokesTilted on the device: returns (FEs (n_scan,nz,4), PPs (n_scan,nz,4) or None).
     ``window``: (klo, nzw) -> ``grid`` holds only planes [klo, klo + nzw) of the spec's grid
     (ppm_relax_strokes_tilted_window); ``flag`` (int32 device tensor, zeroed by the caller) turns
     nonzero if a probe sampled outside the window.  ``df_op`` (a df.AxisOperator over nz slices) with
     ``df`` ((n_scan, n_out) float32 device tensor): the operator applied to each Fz column in the
-    same launch (ppm_relax_strokes_tilted_df)."""
+    same launch (ppm_relax_strokes_tilted_df).  ``lanes``: lanes per scan point (relax_ctl)."""
     import torch
     _f32dev(grid, "grid", 4)
     _f32dev(points, "points", 4)
@@ -270,7 +272,7 @@ def relax_tilted(grid, spec: GridSpec, points, nz: int, tip: Tip, max_iter=128, 
             if t.numel() != ns * nz * 4:
                 raise ValueError(f"{nm} has the wrong size")
     L = _lib.load()
-    s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, scan_ny, damped=damped)
+    s, tp, ctl = spec.c(), tip.c(), relax_ctl(max_iter, f2conv, check_before, scan_ny, damped=damped, lanes=lanes)
     if df_op is not None:
         if window is not None:
             raise ValueError("the fused df takes no z window")

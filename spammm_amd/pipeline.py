@@ -49,12 +49,15 @@ class ScanPipeline:
     spec: device.GridSpec; points: (n_scan,4) float32 device tensor; nz: z-steps per point;
     tip: device.Tip; model: _lib.PPM_MODEL_LJ or _lib.PPM_MODEL_MORSE; Qs, QZs, npbc, lvA, lvB, cap:
     as device.grid_project; max_iter, f2conv, check_before, scan_ny, damped: as device.relax_tilted;
-    df_op: a df.AxisOperator over nz slices (or None: no df); depth: buffer sets and streams (>= 2).
+    df_op: a df.AxisOperator over nz slices (or None: no df); depth: buffer sets and streams (>= 2);
+    lanes: lanes per scan point of the relaxation (default 1: beside a grid projection the
+    relaxation's latency is hidden and only its instruction count matters, and one lane per point
+    issues the fewest; 0 = the library's choice for a launch alone on the GPU).
     """
 
     def __init__(self, spec, points, nz, tip, model=_lib.PPM_MODEL_LJ, Qs=(0, 0, 0, 0), QZs=(0, 0, 0, 0),
                  npbc=(0, 0), lvA=(0, 0, 0), lvB=(0, 0, 0), cap=True, max_iter=128, f2conv=1e-8, check_before=False,
-                 scan_ny=0, damped=False, df_op=None, depth=2, want_pps=True):
+                 scan_ny=0, damped=False, df_op=None, depth=2, want_pps=True, lanes=1):
         import torch
         if depth < 2:
             raise ValueError("a pipeline needs at least two buffer sets")
@@ -62,7 +65,8 @@ class ScanPipeline:
         self.dev = points.device
         self.spec, self.points, self.nz, self.tip = spec, points, int(nz), tip
         self.model, self.Qs, self.QZs, self.npbc, self.lvA, self.lvB, self.cap = model, Qs, QZs, npbc, lvA, lvB, cap
-        self.relax = dict(max_iter=max_iter, f2conv=f2conv, check_before=check_before, scan_ny=scan_ny, damped=damped)
+        self.relax = dict(max_iter=max_iter, f2conv=f2conv, check_before=check_before, scan_ny=scan_ny, damped=damped,
+                          lanes=lanes)
         self.df_op = df_op
         if df_op is not None and int(df_op.n_in) != self.nz:
             raise ValueError("df_op must act on nz slices")

@@ -49,12 +49,16 @@ def test_invalid_arguments_rejected_on_host():
     assert rc == _lib.PPM_ERR_INVALID_ARG and b"max_iter" in L.ppm_last_error()
     with pytest.raises(ValueError):
         _lib.check(rc)
+    ctl.max_iter = 10
+    ctl.lanes = 3
+    rc = L.ppm_relax_strokes_tilted(1, ctypes.byref(s), 1, 4, 3, ctypes.byref(tip), ctypes.byref(ctl), 1, None, None)
+    assert rc == _lib.PPM_ERR_INVALID_ARG and b"lanes" in L.ppm_last_error()
 
 
 def test_struct_layouts_match_header():
     assert ctypes.sizeof(_lib.PpmGridSpec) == 16 + 7 * 16
     assert ctypes.sizeof(_lib.PpmTip) == 7 * 16
-    assert ctypes.sizeof(_lib.PpmRelaxCtl) == 24
+    assert ctypes.sizeof(_lib.PpmRelaxCtl) == 28
 
 
 def test_scan_points_match_reference_loop():
@@ -103,7 +107,7 @@ def test_batch_struct_layouts_match_header():
     from spammm_amd import batch as B
     assert ctypes.sizeof(B.PpmMolDesc) == 16 * 4 + 8 + 12 * 16
     assert B.PpmMolDesc.grid_off.offset == 64
-    assert ctypes.sizeof(B.PpmBatchParams) == 12 * 4 + 2 * 16 + 7 * 16 + 24
+    assert ctypes.sizeof(B.PpmBatchParams) == 12 * 4 + 2 * 16 + 7 * 16 + 28
 
 
 def test_batch_z_window_covers_probe_heights():
