@@ -84,6 +84,9 @@ class Scene:
     # recomputed there)
     batch_depth: int = 2
     batch_grid_budget: float = 64 * 2 ** 30
+    # lanes per scan point of the batch's relaxations (device.relax_ctl): 1, the fewest instructions,
+    # since they run beside another placement's grid projection (0: the choice for a lone launch)
+    batch_lanes: int = 1
     batch_slots: list = field(default=None, repr=False)
 
     @property
@@ -511,7 +514,7 @@ def _make_batch_function():
                         D.grid_project(atoms[b], scene.mol_cpar, _lib.PPM_MODEL_LJ, spec, scene.Qs, scene.QZs,
                                        scene.npbc, scene.lvA, scene.lvB, cap=True, out=g, base=base)
                     D.relax_tilted(g, spec, pts, nz, tip, scene.max_iter, scene.f2conv, True, scan_ny=ny, FEs=sl.FEs,
-                                   PPs=PPs[b])
+                                   PPs=PPs[b], lanes=scene.batch_lanes)
                     if scene.polish > 0:
                         D.relax_polish(g, spec, pts, nz, tip, PPs[b], sl.FEs, scene.polish)
                     if gate:
