@@ -10,15 +10,16 @@
 //   images atom shifted by ix*lvA + iy*lvB, ix,iy in [-npbc,npbc] (kernels/gridFF.cl:960-996)
 //
 // Design (FP32 VALU-bound):
-//   * one workgroup = 256 columns (ia,ib) x V consecutive z voxels per thread (V = 16 for
-//     LJ, 8 for Morse, 4 for a tilted dC); voxels of one thread share dx, dy and dx^2+dy^2
-//     for every site;
+//   * one workgroup = 256 columns (ia,ib) x V consecutive z voxels per thread (V = 16 or 8
+//     for LJ, chosen per launch shape; 8 for Morse, 4 for a tilted dC); voxels of one thread
+//     share dx, dy and dx^2+dy^2 for every site;
 //   * the atom x image "sites" are expanded once per 256-site tile into LDS by the
 //     workgroup (one site per thread), then read as broadcasts by every lane;
 //   * the tip charge at z offset 0 shares r^2 with the pair law (one rsqrt for both);
 //     charge sites with a nonzero offset are evaluated separately;
 //   * stores are float4 per voxel, consecutive lanes -> consecutive x (1 KiB per wave).
 #include <algorithm>
+#include <atomic>
 
 #include "ppm_grid_core.hpp"
 
@@ -384,6 +385,45 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
     if (threadIdx.x == 0) grad[j * 6 + c] = red[0];
 }
 
+constexpr int kMaxDevices = 64;
+std::atomic<int> g_cus[kMaxDevices];
+
+// Compute units of the current device, looked up once per device (256, MI355X's count, if the
+// query fails).
+int device_cus() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return 256;
+    }
+    int n = g_cus[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) {
+            (void)hipGetLastError();
+            n = 256;
+        }
+        g_cus[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+
+// 16 or 8 voxels per thread for an LJ launch of ncol columns x nzw planes.  Per voxel, 16 is ~7 %
+// cheaper once every compute unit holds several workgroups, but its workgroups take twice as long,
+// pad z to a multiple of 16, and, one per compute unit (one wave per SIMD), run latency-bound.
+// With k = ceil(workgroups / compute units), the busiest unit's share, the launch time of either
+// form, measured on MI355X over 128^2..480^2 columns x 16..256 planes, fits
+//   16 voxels: 62 k + 25 (+ 15 when k = 1),   8 voxels: 33 k + 25   (units of 10 us, 200 atoms x 9 images)
+// and both scale with the pair count alike, so the smaller prediction picks the form (within 1.5 %
+// of the faster one on every measured shape).  The two forms sum each voxel's pair terms in the
+// same order: their grids are bit-identical.
+bool lj_16_voxels(int ncol, int nzw) {
+    const long long cus = device_cus();
+    const long long cb = (ncol + 255) / 256;
+    const long long k16 = (cb * ((nzw + 15) / 16) + cus - 1) / cus;
+    const long long k8 = (cb * ((nzw + 7) / 8) + cus - 1) / cus;
+    return 62 * k16 + (k16 == 1 ? 15 : 0) <= 33 * k8;
+}
+
 template <int MODEL, bool HASQ0>
 int dispatch(const float4* atoms, const float* cpar, const GridArgs& a, float4* out, const float4* base,
              bool acc64, hipStream_t st, int klo, int nzw) {
@@ -402,7 +442,7 @@ int dispatch(const float4* atoms, const float* cpar, const GridArgs& a, float4* 
         return check_launch("grid_project_acc64");
     }
     if (dcz) {
-        if (MODEL == PPM_MODEL_LJ && nzw >= 16) {   // 16 voxels per thread: -10 % on LJ, +2 % on Morse
+        if (MODEL == PPM_MODEL_LJ && nzw >= 16 && lj_16_voxels(a.ncol, nzw)) {   // 16 on Morse: +2 %
             dim3 grd((a.ncol + 255) / 256, (nzw + 15) / 16);
             hipLaunchKernelGGL((k_grid_project<MODEL, HASQ0, 16, true>), grd, blk, 0, st, atoms, cpar, a, out, base,
                                klo, kend);

@@ -137,6 +137,22 @@ def test_grid_and_relax_windows_equal_the_whole_grid():
     assert int(flag.item()) != 0
 
 
+def test_flake_grid_windows_of_any_depth_equal_the_whole_grid():
+    """The LJ grid takes 16 or 8 voxels per thread by launch shape (on 300^2 columns: 8 for windows of
+    16, 40 and 62 planes, 16 for 96, 149 and the whole 240); every form stores the planes of the whole
+    grid bit for bit."""
+    w = W.flake()
+    d = D.require_gpu()
+    sp = D.axis_aligned_spec(w.p0, w.L, w.n)
+    atoms, cpar = torch.as_tensor(w.atoms, device=d), torch.as_tensor(w.cLJ, device=d)
+    Gf = D.grid_project(atoms, cpar, 0, sp, w.Qs, w.QZs, w.npbc, w.lvA, w.lvB)
+    for klo, nzw in ((0, 16), (57, 40), (100, 62), (5, 96), (91, 149)):
+        Gw = D.grid_project(atoms, cpar, 0, sp, w.Qs, w.QZs, w.npbc, w.lvA, w.lvB, window=(klo, nzw))
+        print(f"[window] flake planes [{klo}, {klo + nzw}): max |d| {(Gw - Gf[klo:klo + nzw]).abs().max().item():.3e}",
+              flush=True)
+        assert torch.equal(Gw, Gf[klo:klo + nzw]), (klo, nzw)
+
+
 def test_windowed_plain_loop_equals_batch_and_whole_grids():
     bf = make_batch(6)
     df = bf.forward(B.LaunchConfig(8, 8, 2, "fused"))
