@@ -407,21 +407,22 @@ int device_cus() {
     return n;
 }
 
-// 16 or 8 voxels per thread for an LJ launch of ncol columns x nzw planes.  Per voxel, 16 is ~7 %
+// 16 or 8 voxels per thread for an LJ launch of ncol columns x nzw planes.  Per voxel, 16 is ~8 %
 // cheaper once every compute unit holds several workgroups, but its workgroups take twice as long,
 // pad z to a multiple of 16, and, one per compute unit (one wave per SIMD), run latency-bound.
 // With k = ceil(workgroups / compute units), the busiest unit's share, the launch time of either
-// form, measured on MI355X over 128^2..480^2 columns x 16..256 planes, fits
-//   16 voxels: 62 k + 25 (+ 15 when k = 1),   8 voxels: 33 k + 25   (units of 10 us, 200 atoms x 9 images)
-// and both scale with the pair count alike, so the smaller prediction picks the form (within 1.5 %
-// of the faster one on every measured shape).  The two forms sum each voxel's pair terms in the
-// same order: their grids are bit-identical.
+// form, measured on MI355X over 128^2..480^2 columns x 16..256 planes, fits (least squares)
+//   16 voxels: 61.0 k + 29 (+ 12 when k = 1),   8 voxels: 33.4 k + 18
+// in units of 10 us for 1800 sites (200 atoms x 9 images); the ratio of the two forms does not
+// depend on the site count (20..600 atoms measured), so the smaller prediction picks the form
+// (within 1.4 % of the faster one on every measured shape).  The two forms sum each voxel's pair
+// terms in the same order: their grids are bit-identical.
 bool lj_16_voxels(int ncol, int nzw) {
     const long long cus = device_cus();
     const long long cb = (ncol + 255) / 256;
     const long long k16 = (cb * ((nzw + 15) / 16) + cus - 1) / cus;
     const long long k8 = (cb * ((nzw + 7) / 8) + cus - 1) / cus;
-    return 62 * k16 + (k16 == 1 ? 15 : 0) <= 33 * k8;
+    return 610 * k16 + 290 + (k16 == 1 ? 120 : 0) <= 334 * k8 + 180;
 }
 
 template <int MODEL, bool HASQ0>
