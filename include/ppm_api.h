@@ -15,7 +15,8 @@
  *     device.  Calls on different streams may run concurrently.
  *   - Return value: PPM_OK (0) or a PPM_ERR_* code; ppm_last_error() returns a
  *     thread-local message describing the last failure on the calling thread.
- *   - float4 arrays are packed as 4 consecutive floats (x,y,z,w), 16-byte aligned.
+ *   - float4 arrays are packed as 4 consecutive floats (x,y,z,w), 16-byte aligned
+ *     (the relaxation entry points reject misaligned points and outputs).
  *   - Grid layout: (nz, ny, nx, 4) float, x fastest — the texel order of the
  *     reference's RGBA32F image3d (kernels/AFM.cl:989, write_imagef at (ia,ib,ic)).
  *   - Scan layout: point index k = ix*ny + iy (ix-major, spammm/SPM/AFM.py:977-980);

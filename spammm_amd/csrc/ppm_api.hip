@@ -2,6 +2,7 @@
 //
 // Each entry point validates its arguments on the host, then enqueues the kernels on
 // the caller's stream.  Nothing is allocated and nothing is synchronized.
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -12,6 +13,33 @@
 namespace ppm {
 
 static thread_local char g_err[512] = "";
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::atomic<int> g_cus[kMaxDevices], g_lds[kMaxDevices];
+
+int device_attr(std::atomic<int>* cache, hipDeviceAttribute_t attr, int fallback) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        (void)hipGetLastError();
+        return fallback;
+    }
+    int n = cache[dev].load(std::memory_order_relaxed);
+    if (n <= 0) {
+        if (hipDeviceGetAttribute(&n, attr, dev) != hipSuccess || n <= 0) {
+            (void)hipGetLastError();
+            n = fallback;
+        }
+        cache[dev].store(n, std::memory_order_relaxed);
+    }
+    return n;
+}
+}  // namespace
+
+int device_cus() { return device_attr(g_cus, hipDeviceAttributeMultiprocessorCount, 256); }
+int device_lds_per_cu() {
+    return device_attr(g_lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, 160 * 1024);
+}
 
 void set_error(const char* fmt, ...) {
     va_list ap;
@@ -57,6 +85,19 @@ static int check_scan(const char* fn, const float* pts, int n_scan, int nz, cons
     if (n_scan < 1 || nz < 1) { set_error("%s: invalid n_scan=%d nz=%d", fn, n_scan, nz); return PPM_ERR_INVALID_ARG; }
     if ((long long)n_scan * nz > (1LL << 31)) { set_error("%s: n_scan*nz too large", fn); return PPM_ERR_INVALID_ARG; }
     if (!pts || !FEs) { set_error("%s: NULL points/FEs", fn); return PPM_ERR_INVALID_ARG; }
+    if ((reinterpret_cast<uintptr_t>(pts) | reinterpret_cast<uintptr_t>(FEs)) & 15) {
+        set_error("%s: points and FEs must be 16-byte aligned (float4 rows)", fn);
+        return PPM_ERR_INVALID_ARG;
+    }
+    return PPM_OK;
+}
+
+// An optional float4 output (PPs, displacements): NULL or 16-byte aligned.
+static int check_out4(const char* fn, const float* out, const char* name) {
+    if (reinterpret_cast<uintptr_t>(out) & 15) {
+        set_error("%s: %s must be 16-byte aligned (float4 rows)", fn, name);
+        return PPM_ERR_INVALID_ARG;
+    }
     return PPM_OK;
 }
 
@@ -388,6 +429,7 @@ int ppm_relax_strokes_tilted(const float* grid, const PpmGridSpec* spec, const f
     g_err[0] = 0;
     if (int r = check_spec(fn, spec)) return r;
     if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (int r = check_out4(fn, PPs, "PPs")) return r;
     if (int r = check_ctl(fn, ctl, n_scan)) return r;
     if (!grid || !tip) { set_error("%s: NULL grid/tip", fn); return PPM_ERR_INVALID_ARG; }
     return launch_relax_tilted(grid, *spec, points, n_scan, nz, *tip, *ctl, FEs, PPs, (hipStream_t)stream);
@@ -400,6 +442,7 @@ int ppm_relax_strokes_tilted_df(const float* grid, const PpmGridSpec* spec, cons
     g_err[0] = 0;
     if (int r = check_spec(fn, spec)) return r;
     if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (int r = check_out4(fn, PPs, "PPs")) return r;
     if (int r = check_ctl(fn, ctl, n_scan)) return r;
     if (!grid || !tip) { set_error("%s: NULL grid/tip", fn); return PPM_ERR_INVALID_ARG; }
     if (!W || !start || !df || taps < 1 || taps > nz || nz_out < 1) {
@@ -421,6 +464,7 @@ int ppm_relax_strokes_tilted_window(const float* grid, const PpmGridSpec* spec, 
         return PPM_ERR_INVALID_ARG;
     }
     if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (int r = check_out4(fn, PPs, "PPs")) return r;
     if (int r = check_ctl(fn, ctl, n_scan)) return r;
     if (!grid || !tip || !flag) { set_error("%s: NULL grid/tip/flag", fn); return PPM_ERR_INVALID_ARG; }
     return launch_relax_tilted(grid, *spec, points, n_scan, nz, *tip, *ctl, FEs, PPs, (hipStream_t)stream, klo, nzw,
@@ -434,6 +478,8 @@ int ppm_relax_strokes(const float* grid, const PpmGridSpec* spec, const float* p
     g_err[0] = 0;
     if (int r = check_spec(fn, spec)) return r;
     if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (int r = check_out4(fn, disps, "disps")) return r;
+    if (int r = check_out4(fn, PPs, "PPs")) return r;
     if (int r = check_ctl(fn, ctl, n_scan)) return r;
     if (!grid || !dTip || !stiffness || !dpos0 || !relax) { set_error("%s: NULL argument", fn); return PPM_ERR_INVALID_ARG; }
     return launch_relax_strokes(grid, *spec, points, n_scan, nz, dTip, stiffness, dpos0, relax, *ctl, FEs, disps, PPs,
@@ -447,6 +493,8 @@ int ppm_relax_strokes_2d(const float* grid, const PpmGridSpec* spec, const float
     g_err[0] = 0;
     if (int r = check_spec(fn, spec)) return r;
     if (int r = check_scan(fn, points, n_scan, nz, FEs)) return r;
+    if (int r = check_out4(fn, disps, "disps")) return r;
+    if (int r = check_out4(fn, PPs, "PPs")) return r;
     if (int r = check_ctl(fn, ctl, n_scan)) return r;
     if (!grid) { set_error("%s: NULL grid", fn); return PPM_ERR_INVALID_ARG; }
     if (!std::isfinite(k_lat) || !std::isfinite(dh) || !std::isfinite(dt) || !std::isfinite(damp)) {

@@ -32,6 +32,11 @@ __host__ __device__ inline f3 ld3(const float* p) { return f3{p[0], p[1], p[2]};
 void set_error(const char* fmt, ...);
 int check_launch(const char* what);
 
+// Compute units and LDS bytes per compute unit of the current device, looked up once per device
+// (MI355X's 256 and 160 KiB if a query fails).
+int device_cus();
+int device_lds_per_cu();
+
 // Per-translation-unit launchers (host side).
 int launch_grid_project(int model, const float* atoms, const float* cpar, int na, const PpmGridSpec& g,
                         const float Qs[4], const float QZs[4], int npbc_a, int npbc_b, const float lvA[3],

@@ -19,7 +19,6 @@
 //     charge sites with a nonzero offset are evaluated separately;
 //   * stores are float4 per voxel, consecutive lanes -> consecutive x (1 KiB per wave).
 #include <algorithm>
-#include <atomic>
 
 #include "ppm_grid_core.hpp"
 
@@ -383,28 +382,6 @@ __global__ __launch_bounds__(256) void k_reduce_partials(const double* __restric
         __syncthreads();
     }
     if (threadIdx.x == 0) grad[j * 6 + c] = red[0];
-}
-
-constexpr int kMaxDevices = 64;
-std::atomic<int> g_cus[kMaxDevices];
-
-// Compute units of the current device, looked up once per device (256, MI355X's count, if the
-// query fails).
-int device_cus() {
-    int dev = -1;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
-        (void)hipGetLastError();
-        return 256;
-    }
-    int n = g_cus[dev].load(std::memory_order_relaxed);
-    if (n <= 0) {
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) {
-            (void)hipGetLastError();
-            n = 256;
-        }
-        g_cus[dev].store(n, std::memory_order_relaxed);
-    }
-    return n;
 }
 
 // 16 or 8 voxels per thread for an LJ launch of ncol columns x nzw planes.  Per voxel, 16 is ~8 %

@@ -329,16 +329,23 @@ struct DfOut {
 // STAGE: outputs go through Stage4 runs (8 float4 registers for G = 1); false: one store per
 // state, for kernels whose occupancy those registers would cost (the batched kernel shares its
 // register budget with the grid tiles).
+// LDSST (with STAGE, G = 1): the runs are staged in LDS instead of registers: `stage` is this lane's
+// slot 0 of an 8 x 64 float4 area (slot k at stage[k * 64]; slots 0-3 FEs, 4-7 PPs), state k of a run
+// in slot k (runs hold at most 4 states: they end at each 64 B boundary of the 16-byte aligned FEs).  Shifting registers through Stage4 made the compiler copy all 8 float4 at the
+// loop's back edge on every FIRE step, and the store branch runs on nearly every step of a wave
+// (some lane finishes a z-step on most of them).
 // IDF: the tip frame is the identity (A, B, C = x, y, z).  The rotations then return their input
 // and the z-only shifts (tip step, surface force) leave x and y alone, for finite values up to
 // the sign of a zero, so they are skipped: the same results with ~30 fewer instructions per
 // FIRE step.
-template <int G, class SAMPLE, bool STAGE = true, bool IDF = false>
+template <int G, class SAMPLE, bool STAGE = true, bool IDF = false, bool LDSST = false>
 __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, const TiltArgs& p,
                                                       float4* __restrict__ FEs, float4* __restrict__ PPs,
                                                       int sub = 0, const DfOut* dfo = nullptr,
                                                       float* __restrict__ dfrow = nullptr,
-                                                      float* __restrict__ fz_lds = nullptr) {
+                                                      float* __restrict__ fz_lds = nullptr,
+                                                      float4* __restrict__ stage = nullptr) {
+    static_assert(!LDSST || (STAGE && G == 1), "LDS staging is built for one lane per point");
     const f3 A = ld3(p.A), B = ld3(p.B), C = ld3(p.C);
     const f3 dTip = mk3(C.x * p.dtip, C.y * p.dtip, C.z * p.dtip);
     const f3 d0r = IDF ? ld3(p.d0) : rotMatT(ld3(p.d0), A, B, C);
@@ -407,6 +414,9 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
                 } else if (sub == 1 && PPs) {
                     PPs[o] = ppo;
                 }
+            } else if constexpr (LDSST) {
+                stage[pend * 64] = feo;
+                stage[(4 + pend) * 64] = ppo;
             } else if (G == 1) {
                 s1.push(feo);
                 s2.push(ppo);
@@ -415,7 +425,19 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
             }
             ++pend;
             const bool last = iz + 1 >= p.nz;
-            if (STAGE && (last || (reinterpret_cast<uintptr_t>(FEs + o + 1) & 63) == 0)) {
+            if (STAGE && LDSST && (last || (reinterpret_cast<uintptr_t>(FEs + o + 1) & 63) == 0)) {
+                float4* df4 = FEs + o - (pend - 1);
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (k < pend) df4[k] = stage[k * 64];
+                if (PPs) {
+                    float4* dq = PPs + o - (pend - 1);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        if (k < pend) dq[k] = stage[(4 + k) * 64];
+                }
+                pend = 0;
+            } else if (STAGE && !LDSST && (last || (reinterpret_cast<uintptr_t>(FEs + o + 1) & 63) == 0)) {
                 if (G == 1) {
                     s1.flush(FEs + o, pend);
                     if (PPs) s2.flush(PPs + o, pend);
@@ -470,15 +492,17 @@ __host__ __device__ inline bool fast_frame(const TiltArgs& p, const GridDev& g) 
            g.db[2] == 0.f && g.dc[0] == 0.f && g.dc[1] == 0.f && small_grid(g);
 }
 
-template <bool STAGE = true, bool FAST = false>
+template <bool STAGE = true, bool FAST = false, bool LDSST = false>
 __device__ __forceinline__ void relax_tilted_lane(const GridDev& g, f3 start, const TiltArgs& p,
                                                   float4* __restrict__ FEs, float4* __restrict__ PPs,
                                                   const DfOut* dfo = nullptr, float* __restrict__ dfrow = nullptr,
-                                                  float* __restrict__ fz_lds = nullptr) {
+                                                  float* __restrict__ fz_lds = nullptr,
+                                                  float4* __restrict__ stage = nullptr) {
     Cell cell;
     cell.reset();
     auto smp = [&](float x, float y, float z) { return sample<FAST>(g, cell, x, y, z); };
-    relax_tilted_lane_src<1, decltype(smp), STAGE, FAST>(smp, start, p, FEs, PPs, 0, dfo, dfrow, fz_lds);
+    relax_tilted_lane_src<1, decltype(smp), STAGE, FAST, LDSST>(smp, start, p, FEs, PPs, 0, dfo, dfrow, fz_lds,
+                                                                stage);
 }
 
 }  // namespace rc

@@ -34,25 +34,34 @@ def test_version_and_error_string_without_gpu():
 
 
 def test_invalid_arguments_rejected_on_host():
-    """Argument validation runs before any device work, so it is testable without a GPU."""
+    """Argument validation runs before any device work, so it is testable without a GPU (the device
+    addresses below are never dereferenced)."""
     L = _lib.load()
     s = _lib.PpmGridSpec()
     s.nx, s.ny, s.nz = 0, 4, 4
     ctl = D.relax_ctl()
     tip = D.Tip().c()
-    rc = L.ppm_relax_strokes_tilted(1, ctypes.byref(s), 1, 4, 3, ctypes.byref(tip), ctypes.byref(ctl), 1, None, None)
+    A, B = 0x10000, 0x20000          # 16-byte aligned stand-ins for device buffers
+    call = lambda pts, fes, pps: L.ppm_relax_strokes_tilted(A, ctypes.byref(s), pts, 4, 3, ctypes.byref(tip),
+                                                            ctypes.byref(ctl), fes, pps, None)
+    rc = call(A, B, None)
     assert rc == _lib.PPM_ERR_INVALID_ARG
     assert b"grid dims" in L.ppm_last_error()
     s.nx = 4
     ctl.max_iter = 0
-    rc = L.ppm_relax_strokes_tilted(1, ctypes.byref(s), 1, 4, 3, ctypes.byref(tip), ctypes.byref(ctl), 1, None, None)
+    rc = call(A, B, None)
     assert rc == _lib.PPM_ERR_INVALID_ARG and b"max_iter" in L.ppm_last_error()
     with pytest.raises(ValueError):
         _lib.check(rc)
     ctl.max_iter = 10
     ctl.lanes = 3
-    rc = L.ppm_relax_strokes_tilted(1, ctypes.byref(s), 1, 4, 3, ctypes.byref(tip), ctypes.byref(ctl), 1, None, None)
+    rc = call(A, B, None)
     assert rc == _lib.PPM_ERR_INVALID_ARG and b"lanes" in L.ppm_last_error()
+    ctl.lanes = 0
+    # float4 rows must be 16-byte aligned: misaligned points, FEs or PPs are rejected
+    for args in ((A + 4, B, None), (A, B + 8, None), (A, B, B + 0x1000 + 12)):
+        rc = call(*args)
+        assert rc == _lib.PPM_ERR_INVALID_ARG and b"16-byte aligned" in L.ppm_last_error(), args
 
 
 def test_struct_layouts_match_header():
