@@ -43,7 +43,7 @@ def log(msg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="flake", choices=["flake", "naphthalene"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
