@@ -76,13 +76,16 @@ __global__ __launch_bounds__(64) void k_relax_tilted(GridDev g, const float4* __
 
 
 // Lanes per scan point of k_relax_tilted / k_relax_strokes: the largest G <= 4 that keeps the
-// launch within about 2 waves per SIMD (1024 SIMDs).  A group replicates the per-point FIRE and
-// spring arithmetic in its G lanes, so past that occupancy the extra waves cost VALU issue slots
-// instead of hiding latency (measured on the 256x256-pixel flake scan: G = 4, 2, 1 take 3.76,
-// 2.43, 2.71 ms; on 16384 pixels 1.54, 1.96, 2.35 ms; the results are bit-identical).
+// launch within 2.5 waves per SIMD (10 per compute unit).  A group replicates the per-point FIRE
+// and spring arithmetic in its G lanes, so past that occupancy the extra waves cost VALU issue
+// slots instead of hiding latency.  Measured on the flake grid (G = 4, 2, 1; bit-identical
+// results): 16,384 pixels 1.92, 2.31, 2.64 ms; 40,000 pixels (2500 waves at G = 4) 2.16, 2.35,
+// 2.76; 43,264 (2704) 2.72, 2.40, 2.83; 65,536 3.72, 2.46, 2.73; 73,984 (2312 waves at G = 2)
+// 4.06, 2.53, 2.70; 82,944 (2592) 4.39, 2.89, 2.71.
 int relax_group(int n_scan) {
+    const long long max_lanes = 10LL * device_cus() * 64;
     int g = 1;
-    while (g < 4 && (long long)n_scan * g * 2 <= 2LL * 1024 * 64) g *= 2;
+    while (g < 4 && (long long)n_scan * g * 2 <= max_lanes) g *= 2;
     return g;
 }
 

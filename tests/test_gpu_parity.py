@@ -280,14 +280,15 @@ def test_full_naphthalene_scan_subsample_bitexact():
     np.testing.assert_array_equal(FEs[sel], FEo)
 
 
-@pytest.mark.parametrize("nxy", [(12, 10), (200, 200), (280, 260), (480, 480)])
+@pytest.mark.parametrize("nxy", [(12, 10), (200, 200), (230, 230), (300, 290), (480, 480)])
 def test_relax_with_fused_df_equals_separate_operator(nxy):
     """ppm_relax_strokes_tilted_df: FEs and PPs equal ppm_relax_strokes_tilted's and df equals the
     separate operator launch on its FEs, bit for bit, for each lane-group width (these scans run 4,
-    2, 1 and 1 lanes per point); FEs also bit-exact against the FP32 oracle on a subsample.  One lane
-    per point stages its output runs in LDS while the launch's waves stay resident with it: both
-    runs of 280x260 points; of 480x480 (3600 waves, 15 per compute unit) the one without df, while
-    the df run (12 Fz floats per lane in LDS as well: 14 waves per unit) stages in registers."""
+    4, 2, 1 and 1 lanes per point: up to 2.5 waves per SIMD); FEs also bit-exact against the FP32
+    oracle on a subsample.  One lane per point stages its output runs in LDS while the launch's
+    waves stay resident with it: both runs of 300x290 points (1406 waves, 6 per compute unit); of
+    480x480 (3600 waves, 15 per unit) the one without df, while the df run (12 Fz floats per lane
+    in LDS as well: 14 waves per unit) stages in registers."""
     from spammm_amd import df as DF
     w = W.small()
     G, sp = grid_gpu(w)
