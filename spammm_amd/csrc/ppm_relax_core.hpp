@@ -330,10 +330,11 @@ struct DfOut {
 // state, for kernels whose occupancy those registers would cost (the batched kernel shares its
 // register budget with the grid tiles).
 // LDSST (with STAGE, G = 1): the runs are staged in LDS instead of registers: `stage` is this lane's
-// slot 0 of an 8 x 64 float4 area (slot k at stage[k * 64]; slots 0-3 FEs, 4-7 PPs), state k of a run
-// in slot k (runs hold at most 4 states: they end at each 64 B boundary of the 16-byte aligned FEs).  Shifting registers through Stage4 made the compiler copy all 8 float4 at the
-// loop's back edge on every FIRE step, and the store branch runs on nearly every step of a wave
-// (some lane finishes a z-step on most of them).
+// slot 0 of an 8 x 64 float4 area (slot k at stage[k * 64]; slots 0-3 FEs, 4-7 PPs), state k of a
+// run in slot k (runs hold at most 4 states: they end at each 64 B boundary of the 16-byte aligned
+// FEs).  Shifting registers through Stage4 made the compiler copy all 8 float4 at the loop's back
+// edge on every FIRE step, and the store branch runs on nearly every step of a wave (some lane
+// finishes a z-step on most of them).
 // IDF: the tip frame is the identity (A, B, C = x, y, z).  The rotations then return their input
 // and the z-only shifts (tip step, surface force) leave x and y alone, for finite values up to
 // the sign of a zero, so they are skipped: the same results with ~30 fewer instructions per
@@ -426,10 +427,10 @@ __device__ __forceinline__ void relax_tilted_lane_src(SAMPLE& sample, f3 start, 
             ++pend;
             const bool last = iz + 1 >= p.nz;
             if (STAGE && LDSST && (last || (reinterpret_cast<uintptr_t>(FEs + o + 1) & 63) == 0)) {
-                float4* df4 = FEs + o - (pend - 1);
+                float4* dst = FEs + o - (pend - 1);
 #pragma unroll
                 for (int k = 0; k < 4; ++k)
-                    if (k < pend) df4[k] = stage[k * 64];
+                    if (k < pend) dst[k] = stage[k * 64];
                 if (PPs) {
                     float4* dq = PPs + o - (pend - 1);
 #pragma unroll
